@@ -1,0 +1,194 @@
+"""Benchmark of the spectral SWE hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one SHT round trip (synthesis + analysis) of 3 fields (phi, u, v) at
+T511 on the 768 x 1536 Gaussian grid, FP64, spectral inputs already resident
+in HBM, seeded coefficients with power ~ l^-3.  The JSON line also carries the
+nonlinear SWE RK4 rate at T511 (Gaussian dam, dt = 0.001 a / nlat = 8.296 s),
+the roofline of the dominant kernel (the Legendre synthesis) measured with HIP
+events on the library's stream, and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from sph_swe_amd import capi  # noqa: E402
+
+L_BENCH = 511
+NFIELDS = 3
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def seeded_spec(L, nf, seed=1234):
+    rng = np.random.default_rng(seed)
+    out = []
+    for f in range(nf):
+        qs = []
+        for m in range(L + 1):
+            l = np.arange(m, L + 1)
+            amp = np.where(l > 0, np.maximum(l, 1) ** -1.5, 0.0)
+            q = amp * np.exp(1j * rng.uniform(0, 2 * np.pi, l.size))
+            if m == 0:
+                q = q.real.astype(complex)
+            qs.append(q)
+        out.append(np.concatenate(qs))
+    return np.stack(out)
+
+
+def gaussian_dam(lon, ct, lat_c=np.pi / 4, lon_c=np.pi, amp=500.0, width=0.1, h0=10000.0):
+    phi1 = np.arcsin(ct)[None, :]
+    c = np.sin(phi1) * np.sin(lat_c) + np.cos(phi1) * np.cos(lat_c) * np.cos(np.asarray(lon)[:, None] - lon_c)
+    d = np.arccos(np.clip(c, -1, 1))
+    return (h0 + amp * np.exp(-(d / width) ** 2)).reshape(-1)
+
+
+def bench_sht(ctx, steps, warmup):
+    lib, h = ctx.lib, ctx.h
+    spec = ctx.to_device(seeded_spec(ctx.L, NFIELDS))
+    spat = ctx.empty(NFIELDS * ctx.nspat * 8)
+    spec2 = ctx.empty(NFIELDS * ctx.nlm * 16)
+
+    def round_trip():
+        ctx._check(lib.sphswe_sht_synthesis(h, spec.ptr, spat.ptr, NFIELDS, capi.PREC_FP64))
+        ctx._check(lib.sphswe_sht_analysis(h, spat.ptr, spec2.ptr, NFIELDS, capi.PREC_FP64))
+
+    for _ in range(warmup):
+        round_trip()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.mark(0)
+    for _ in range(steps):
+        round_trip()
+    ctx.mark(1)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    dev_ms = ctx.elapsed_ms(0, 1)
+    # accuracy of the round trip (band-limited input -> identity)
+    q_in = ctx.to_host(spec, np.complex128, (NFIELDS, ctx.nlm))
+    q_out = ctx.to_host(spec2, np.complex128, (NFIELDS, ctx.nlm))
+    err = max(np.linalg.norm(q_out[k] - q_in[k]) / np.linalg.norm(q_in[k]) for k in range(NFIELDS))
+    return wall, dev_ms, err
+
+
+def bench_legendre_kernel(ctx, reps=20):
+    """Average duration of one Legendre synthesis launch (3 fields), HIP events
+    on the library stream, and its algorithmic FLOP count."""
+    lib, h = ctx.lib, ctx.h
+    spec = ctx.to_device(seeded_spec(ctx.L, NFIELDS, seed=7))
+    four = ctx.empty(NFIELDS * (ctx.L + 1) * ctx.nlat * 16)
+    for _ in range(3):
+        ctx._check(lib.sphswe_legendre_synthesis(h, spec.ptr, four.ptr, NFIELDS, capi.PREC_FP64))
+    ctx.mark(2)
+    for _ in range(reps):
+        ctx._check(lib.sphswe_legendre_synthesis(h, spec.ptr, four.ptr, NFIELDS, capi.PREC_FP64))
+    ctx.mark(3)
+    ms = ctx.elapsed_ms(2, 3) / reps
+    flops = 4.0 * ctx.nlm * (ctx.nlat // 2) * NFIELDS
+    # analysis kernel as well
+    ctx.mark(4)
+    for _ in range(reps):
+        ctx._check(lib.sphswe_legendre_analysis(h, four.ptr, spec.ptr, NFIELDS, capi.PREC_FP64))
+    ctx.mark(5)
+    ms_a = ctx.elapsed_ms(4, 5) / reps
+    return ms, ms_a, flops
+
+
+def bench_swe(ctx, steps, warmup):
+    lon = np.arange(ctx.nphi) / ctx.nphi * 2 * np.pi
+    hgt = gaussian_dam(lon, ctx.ct)
+    z = np.zeros_like(hgt)
+    spec = ctx.analysis(np.stack([hgt, z, z]))
+    state = ctx.to_device(spec)
+    dt = 0.001 * 6.37122e6 / ctx.nlat
+    prm = capi.swe_params(dt=dt)
+    ctx.swe_rk4(prm, state, 1, warmup)
+    ctx.sync()
+    ctx.mark(6)
+    ctx.swe_rk4(prm, state, 1, steps)
+    ctx.mark(7)
+    ms = ctx.elapsed_ms(6, 7)
+    out = ctx.to_host(state, np.complex128, (3, ctx.nlm))
+    if not np.all(np.isfinite(out)):
+        raise RuntimeError("SWE state became non-finite")
+    return steps / (ms / 1000.0), dt
+
+
+def cpu_baseline(L, ctx_nlat, ctx_nphi):
+    """The oracle (single-threaded C restatement) on a bounded sample: one
+    round trip of 3 fields at T511."""
+    from oracle import oracle as O
+    orc = O.Oracle(L, ctx_nlat, ctx_nphi)
+    spec = seeded_spec(L, NFIELDS)
+    t0 = time.perf_counter()
+    for k in range(NFIELDS):
+        f = orc.synthesis(spec[k])
+        orc.analysis(f)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "round_trips/s", "cores": 1, "kind": "port",
+            "sample": "1 round trip of 3 fields at T511 (768x1536), oracle/sphswe_oracle.c, %.1f s" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--swe-steps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    ctx = capi.Context(L_BENCH)
+    wall, dev_ms, rt_err = bench_sht(ctx, args.steps, args.warmup)
+    ms_step = dev_ms / args.steps
+    value = args.steps / (dev_ms / 1000.0)
+    ms_syn, ms_ana, flops = bench_legendre_kernel(ctx)
+    swe_rate, swe_dt = bench_swe(ctx, args.swe_steps, 2)
+    achieved = flops / (ms_syn * 1e-3) / 1e12
+    cpu = None if args.no_cpu else cpu_baseline(L_BENCH, ctx.nlat, ctx.nphi)
+    line = {
+        "metric": "SHT round-trips/sec at T511 (3 fields, FP64); SWE steps/sec at T511 in swe_steps_per_s",
+        "value": value,
+        "unit": "round_trips/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded spectra, power ~ l^-3; Gaussian dam for the SWE rate)",
+        "config": {"workload": "sht_round_trip_T511_3fields", "lmax": L_BENCH, "nlat": ctx.nlat,
+                   "nphi": ctx.nphi, "fields": NFIELDS, "precision": "fp64"},
+        "wall_s": wall,
+        "round_trip_rel_l2": rt_err,
+        "swe_steps_per_s": swe_rate,
+        "swe_config": {"lmax": L_BENCH, "dt": swe_dt, "scheme": "RK4 nonlinear", "steps": args.swe_steps,
+                       "viscosity": 1e5},
+        "roofline": {"bound": "mfma", "engine": "fp64 VALU (fp64 vector peak = fp64 matrix peak on MI355X)",
+                     "kernel": "k_leg_synth (Legendre synthesis, 3 fields)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "avg_ms": ms_syn, "analysis_avg_ms": ms_ana,
+                     "analysis_achieved": flops / (ms_ana * 1e-3) / 1e12},
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

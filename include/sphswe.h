@@ -1,0 +1,163 @@
+/*
+ * sphswe.h -- C ABI of libsphswe.so, the gfx950 (MI355X) spectral shallow-water
+ * hot path.  Plain pointers and sizes only.  Every call returns an int status
+ * (SPHSWE_OK = 0); the message of the last failure on a context is available
+ * from sphswe_last_error().  Nothing here calls exit() or throws.
+ *
+ * All "d_" pointers are device pointers (from sphswe_malloc or any HIP
+ * allocation on the context's device).  Calls are asynchronous on the
+ * context's stream unless stated otherwise and are not thread-safe per
+ * context, as the reference's SHTNS plan and solvers are not.
+ *
+ * Data layouts (identical to the reference):
+ *   physical field   double[nphi][nlat], index i*nlat + j, j = 0 northernmost
+ *                    ring (SHT_THETA_CONTIGUOUS), longitude 2*pi*i/nphi
+ *   real spectrum    complex double[nlm], index m*(2L-m+1)/2 + l (m-major)
+ *   complex spectrum complex double[(L+1)^2], index l*l + l + m
+ * Complex numbers are interleaved (re, im) doubles.
+ * Normalisation: orthonormal with Condon-Shortley phase (sht_orthonormal).
+ */
+#ifndef SPHSWE_H
+#define SPHSWE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sphswe_ctx sphswe_ctx;
+typedef struct sphswe_rexi sphswe_rexi;
+
+enum {
+	SPHSWE_OK = 0,
+	SPHSWE_ERR_ARG = 1,       /* invalid argument or size */
+	SPHSWE_ERR_HIP = 2,       /* HIP runtime error */
+	SPHSWE_ERR_NOMEM = 3,     /* device allocation failed */
+	SPHSWE_ERR_SINGULAR = 4,  /* banded factorisation met a zero pivot (zgbsv INFO > 0) */
+	SPHSWE_ERR_NAN = 5        /* NaN/Inf detected */
+};
+
+/* precision of the Legendre contraction; the recurrence is always FP64 */
+enum {
+	SPHSWE_PREC_FP64 = 0,
+	SPHSWE_PREC_FP32 = 1
+};
+
+/* ---- configuration (replaces shtns_create / shtns_set_grid{,_auto} and
+ *      SPHConfig::setup / setupAutoPhysicalSpace, src/include/sph/SPHConfig.hpp:324-399) */
+
+/* grid of shtns_set_grid_auto(..., nl_order, &nlat, &nphi) (SPHConfig.hpp:387-396) */
+int sphswe_grid_auto(int lmax, int nl_order, int *nlat, int *nphi);
+
+/* lmax must equal mmax (SPHConfig.hpp:171-176); nlat even, nphi >= 2*lmax+1 with
+ * prime factors in {2,3,5,7}; device = HIP device ordinal */
+int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device, sphswe_ctx **ctx);
+int sphswe_destroy(sphswe_ctx *ctx);
+
+/* shtns->nlat, nphi, nlm, (L+1)^2 (SPHConfig.hpp:160-168) */
+int sphswe_get_sizes(const sphswe_ctx *ctx, int *lmax, int *nlat, int *nphi, int *nlm, int *ncplx);
+/* host copies of shtns->ct, shtns->st and the Gauss weights (SPHConfig.hpp:303-319) */
+int sphswe_get_grid(const sphswe_ctx *ctx, double *ct, double *st, double *wg);
+const char *sphswe_last_error(const sphswe_ctx *ctx);
+/* wait for all work on the context stream */
+int sphswe_sync(sphswe_ctx *ctx);
+/* the context stream (hipStream_t) for callers that order their own work */
+void *sphswe_stream(sphswe_ctx *ctx);
+
+/* ---- device memory (replaces MemBlockAlloc::alloc/free, MemBlockAlloc.hpp:398-513;
+ *      fftw_malloc of SHTNS buffers) */
+int sphswe_malloc(sphswe_ctx *ctx, size_t bytes, void **d_ptr);
+int sphswe_free(sphswe_ctx *ctx, void *d_ptr);
+/* synchronous copies */
+int sphswe_memcpy_h2d(sphswe_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int sphswe_memcpy_d2h(sphswe_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+/* asynchronous on the context stream */
+int sphswe_memcpy_d2d(sphswe_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+int sphswe_fill_zero(sphswe_ctx *ctx, void *d_ptr, size_t bytes);
+
+/* ---- transforms.  nfields fields stored back to back (stride nspat doubles,
+ *      2*nlm doubles or 2*(L+1)^2 doubles).  spat_to_SH / SH_to_spat /
+ *      spat_cplx_to_SH / SH_to_spat_cplx (call sites SPHData.hpp:201,221;
+ *      SPHDataComplex.hpp:190,210).  Unlike SHTNS the input is not destroyed. */
+int sphswe_sht_analysis(sphswe_ctx *ctx, const double *d_spat, double *d_spec, int nfields, int precision);
+int sphswe_sht_synthesis(sphswe_ctx *ctx, const double *d_spec, double *d_spat, int nfields, int precision);
+int sphswe_sht_analysis_cplx(sphswe_ctx *ctx, const double *d_spat, double *d_spec, int nfields, int precision);
+int sphswe_sht_synthesis_cplx(sphswe_ctx *ctx, const double *d_spec, double *d_spat, int nfields, int precision);
+
+/* the two stages of the real pair on their own (for measurement and fusion).
+ * Fourier layout: complex double[nfields][L+1][nlat], entry (m, j) is
+ * (2 pi / nphi) sum_i f(i,j) e^{-2 pi i m i / nphi}. */
+int sphswe_fft_analysis(sphswe_ctx *ctx, const double *d_spat, double *d_four, int nfields);
+int sphswe_fft_synthesis(sphswe_ctx *ctx, const double *d_four, double *d_spat, int nfields);
+int sphswe_legendre_analysis(sphswe_ctx *ctx, const double *d_four, double *d_spec, int nfields, int precision);
+int sphswe_legendre_synthesis(sphswe_ctx *ctx, const double *d_spec, double *d_four, int nfields, int precision);
+
+/* ---- spectral operators on nfields fields: out = s * op(in), s complex
+ *      (SPHOperators.hpp:30-498 and SPHOperatorsComplex.hpp:27-470; identities
+ *      SPHIdentities.hpp:17-104).  in and out must not alias. */
+enum {
+	SPHSWE_OP_IDENTITY = 0,      /* s * x */
+	SPHSWE_OP_DIFF_LON = 1,      /* diff_lon: i m x */
+	SPHSWE_OP_ONE_MINUS_MU2_DMU = 2, /* spec_one_minus_mu_squared_diff_lat_mu */
+	SPHSWE_OP_MU = 3,            /* mu */
+	SPHSWE_OP_MU2 = 4,           /* mu2 (A/B/C band, not mu o mu) */
+	SPHSWE_OP_LAPLACE = 5,       /* -n(n+1) x */
+	SPHSWE_OP_HELMHOLTZ = 6      /* x / (a - (b/r^2) n(n+1)), spec_solve_helmholtz; s = a, (p0,p1)=(b, r) */
+};
+int sphswe_spec_op(sphswe_ctx *ctx, int op, const double *d_in, double *d_out, int nfields,
+		   double s_re, double s_im, double p0, double p1);
+int sphswe_spec_op_cplx(sphswe_ctx *ctx, int op, const double *d_in, double *d_out, int nfields,
+			double s_re, double s_im, double p0, double p1, double p2);
+/* out = a*x + b*y over n complex entries (a, b complex); y may be NULL */
+int sphswe_spec_axpby(sphswe_ctx *ctx, size_t n, double a_re, double a_im, const double *d_x,
+		      double b_re, double b_im, const double *d_y, double *d_out);
+/* spectral (0,0) += v*sqrt(4 pi) (SPHData.hpp:427-440) */
+int sphswe_spec_add_scalar(sphswe_ctx *ctx, double *d_spec, double v_re, double v_im);
+
+/* ---- physical pointwise operators on real fields (spat_update_lambda*,
+ *      operator* , SPHData.hpp:334-353,645-746) */
+enum {
+	SPHSWE_SPAT_MUL = 0,          /* out = a * b */
+	SPHSWE_SPAT_DIV_COSPHI = 1,   /* out = a / sqrt(1 - mu^2) */
+	SPHSWE_SPAT_MUL_ST = 2,       /* out = a * st */
+	SPHSWE_SPAT_DIV_ST = 3,       /* out = a / st */
+	SPHSWE_SPAT_MUL_MU = 4,       /* out = a * mu */
+	SPHSWE_SPAT_SCALE = 5,        /* out = s * a */
+	SPHSWE_SPAT_DIV_COSLAT = 6    /* out = a / cos(lat[j]) */
+};
+int sphswe_spat_op(sphswe_ctx *ctx, int op, const double *d_a, const double *d_b, double *d_out,
+		   double s, int nfields);
+int sphswe_spat_op_cplx(sphswe_ctx *ctx, int op, const double *d_a, const double *d_b, double *d_out,
+			double s, int nfields);
+/* host flag: 1 if any entry is NaN/Inf (SPHData.hpp:560-569); synchronous */
+int sphswe_any_nan(sphswe_ctx *ctx, const double *d_x, size_t n, int *h_flag);
+/* min / max / max|x| of a physical field (SPHData.hpp:800-875); synchronous */
+int sphswe_reduce_minmax(sphswe_ctx *ctx, const double *d_x, size_t n, double *h_min, double *h_max,
+			 double *h_absmax);
+
+/* ---- nonlinear SWE, RK4 (AppSWE.hpp:574-659 + TimesteppingRK.hpp:310-392) */
+typedef struct {
+	double earth_radius;     /* a */
+	double gravitation;      /* g */
+	double coriolis_omega;   /* Omega */
+	double h0;               /* average height */
+	double viscosity;        /* nu (explicit nu*laplace/a^2) */
+	double dt;
+	int nonlinear;           /* 1: nonlinear equations, 0: linear */
+	int precision;           /* SPHSWE_PREC_* for the Legendre stage */
+} sphswe_swe_params;
+
+/* Advance nmembers independent states by nsteps RK4 steps.  d_state holds, per
+ * member, the spectral fields h, u, v back to back (3*nlm complex).  The state
+ * must be spectral (the caller analyses a physical initial condition first). */
+int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmembers, int nsteps);
+
+/* ---- timing on the context stream: ms between two sphswe_timer_mark calls */
+int sphswe_timer_mark(sphswe_ctx *ctx, int slot);
+int sphswe_timer_elapsed(sphswe_ctx *ctx, int slot0, int slot1, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

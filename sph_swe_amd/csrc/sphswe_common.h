@@ -1,0 +1,137 @@
+// Internal declarations shared by the libsphswe translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include <unordered_map>
+
+#include "../../include/sphswe.h"
+
+#define SPHSWE_MAX_TIMERS 16
+#define SPHSWE_MAX_RADIX 16
+
+struct FftPlan {
+	int n;
+	int npass;
+	int radix[SPHSWE_MAX_RADIX];
+};
+
+struct sphswe_ctx {
+	int L, nlat, nphi, nlm, ncplx, nspat, NJ;
+	int device;
+	hipStream_t stream;
+	std::string err;
+
+	std::vector<double> ct, st, wg;
+
+	// device tables
+	double *d_ct = nullptr;       // mu = cos(theta) [nlat]
+	double *d_st = nullptr;       // sin(theta) [nlat]
+	double *d_sq = nullptr;       // sqrt(1 - mu^2) [nlat]
+	double *d_coslat = nullptr;   // cos(lat[j]) [nlat]
+	double *d_wg = nullptr;       // Gauss weights [nlat]
+	double2 *d_coef = nullptr;    // recurrence (a_lm, b_lm) [nlm], index of the real spectrum
+	int *d_l0 = nullptr;          // first significant degree [(L+1) * NJ]
+	double2 *d_seed = nullptr;    // (Pbar_l0, Pbar_l0+1) [(L+1) * NJ]
+	int *d_mofidx = nullptr;      // m of real spectral index [nlm]
+	int *d_lofcidx = nullptr;     // l of complex spectral index [(L+1)^2]
+	double2 *d_tw = nullptr;      // e^{-2 pi i k / nphi} [nphi]
+	FftPlan plan;
+
+	// scratch (grown on demand, never shrunk)
+	void *d_scratch = nullptr;
+	size_t scratch_bytes = 0;
+	double *d_flag = nullptr;     // small reduction scratch (64 doubles)
+
+	std::unordered_map<void *, size_t> allocs;
+
+	hipEvent_t ev[SPHSWE_MAX_TIMERS];
+};
+
+// error helpers ---------------------------------------------------------------
+
+int sphswe_set_err(sphswe_ctx *ctx, int code, const char *fmt, ...);
+
+#define SPHSWE_HIP(ctx, call)                                                              \
+	do {                                                                               \
+		hipError_t _e = (call);                                                    \
+		if (_e != hipSuccess)                                                      \
+			return sphswe_set_err((ctx), SPHSWE_ERR_HIP, "%s: %s (%s:%d)", #call,  \
+					      hipGetErrorString(_e), __FILE__, __LINE__);  \
+	} while (0)
+
+#define SPHSWE_LAUNCH_CHECK(ctx)                                                           \
+	do {                                                                               \
+		hipError_t _e = hipGetLastError();                                         \
+		if (_e != hipSuccess)                                                      \
+			return sphswe_set_err((ctx), SPHSWE_ERR_HIP, "kernel launch: %s (%s:%d)", \
+					      hipGetErrorString(_e), __FILE__, __LINE__);  \
+	} while (0)
+
+// scratch of at least `bytes`, valid until the next call that asks for scratch
+int sphswe_scratch(sphswe_ctx *ctx, size_t bytes, void **p);
+
+// internal launchers ----------------------------------------------------------
+
+int sphswe_fft_r2c(sphswe_ctx *ctx, const double *spat, double2 *four, int nf);
+int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf);
+int sphswe_fft_c2c_fwd(sphswe_ctx *ctx, const double2 *spat, double2 *four, int nf);
+int sphswe_fft_c2c_bwd(sphswe_ctx *ctx, const double2 *four, double2 *spat, int nf);
+int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
+int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
+int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
+int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
+
+// device helpers --------------------------------------------------------------
+
+__host__ __device__ inline double2 cmul(double2 a, double2 b)
+{
+	return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// real spectral index (SPHConfig.hpp:106-116)
+__host__ __device__ inline int lm_index(int L, int l, int m) { return (m * (2 * L - m + 1)) / 2 + l; }
+
+// SPHIdentities (SPHIdentities.hpp:17-104) -------------------------------------
+__host__ __device__ inline double idR(double k, double m)
+{
+	double n = k + 1;
+	if (n < 0) return 0;
+	if (n < fabs(m)) return 0;
+	if (n * n - m * m < 0) return 0;
+	return sqrt((n * n - m * m) / (4.0 * n * n - 1.0));
+}
+__host__ __device__ inline double idS(double k, double m)
+{
+	double n = k - 1;
+	if (n < 0) return 0;
+	if (n < fabs(m)) return 0;
+	if (n * n - m * m < 0) return 0;
+	return sqrt(((n + 1.0) * (n + 1.0) - m * m) / ((2.0 * n + 1.0) * (2.0 * n + 3.0)));
+}
+__host__ __device__ inline double idA(double k, double m) { double n = k + 2.0; return idR(n - 1, m) * idR(n - 2, m); }
+__host__ __device__ inline double idB(double n, double m) { return idR(n - 1, m) * idS(n, m) + idS(n + 1, m) * idR(n, m); }
+__host__ __device__ inline double idC(double k, double m) { double n = k - 2.0; return idS(n + 1, m) * idS(n + 2, m); }
+__host__ __device__ inline double idD(double k, double m)
+{
+	double n = k + 1;
+	return (2.0 * n + 1.0) * sqrt((n * n - m * m) / (4.0 * n * n - 1.0));
+}
+__host__ __device__ inline double idE(double n, double) { return -n; }
+
+// XCD-aware block remap: hardware places block b on XCD (b mod 8); consecutive
+// logical blocks are gathered on one XCD so that neighbours share its L2.
+__device__ inline int xcd_remap(int b, int nblocks_padded)
+{
+	int per = nblocks_padded >> 3;
+	return (b & 7) * per + (b >> 3);
+}

@@ -1,0 +1,265 @@
+// Fused nonlinear SWE tendency + classic RK4 over nmembers independent states.
+//
+// Follows AppSWE::p_run_euler_timestep_update (src/AppSWE.hpp:574-659) and
+// TimesteppingRK::run_rk_timestep order 4 (src/include/sweet/TimesteppingRK.hpp:
+// 310-392) with the same projection points: every field the reference analyses
+// (products, metric-divided gradients) is analysed here too, every operator is
+// applied in the same space.  The difference is batching: the reference runs
+// about 30 single-field transforms per tendency through a lazy graph; here the
+// fields of one projection level go through one batched transform:
+//
+//   level 0  synthesis of 9 fields:  h, u, v, im*h, D'h, im*u, D'u, im*v, D'v
+//            (D' = (1-mu^2) d/dmu, SPHOperators.hpp:234-260)
+//   pointwise h*u, h*v*st (div_lat, SPHOperators.hpp:411-439) and the u, v
+//            tendencies (grad_lon/grad_lat divide by sqrt(1-mu^2), f = 2 Omega mu)
+//   level 1  analysis of 4 fields, im* and D' on the two flux fields
+//   level 2  synthesis of 2 fields, divide by sqrt(1-mu^2) (and st)
+//   level 3  analysis of 1 field (the h tendency)
+//   spectral viscosity nu/a^2 laplace and the RK4 combinations.
+//
+// Sums that the reference forms in spectral space after separate analyses
+// (e.g. div_lon(hu) + div_lat(hv)) are formed here in physical space before
+// one analysis; the analysis is linear, so the two agree to round-off.
+#include "sphswe_common.h"
+
+namespace {
+
+struct SweWs {
+	size_t bytes = 0;
+	void *base = nullptr;
+};
+std::unordered_map<const sphswe_ctx *, SweWs> g_ws;
+
+// spectral prep: 9 fields per member from (h, u, v)
+__global__ void k_prep9(const double2 *__restrict__ X, double2 *__restrict__ S9, const int *__restrict__ mof, int L,
+			int nlm, int nmem)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= (long)nlm * nmem) return;
+	int mem = (int)(i / nlm);
+	int idx = (int)(i - (long)mem * nlm);
+	int m = mof[idx];
+	int base = lm_index(L, 0, m);
+	int l = idx - base;
+	const double dl = l, dm = m;
+	const double cm = (-dl + 1.0) * idR(dl - 1, dm), cp = (dl + 2.0) * idS(dl + 1, dm);
+	const double2 *x = X + (long)mem * 3 * nlm;
+	double2 *o = S9 + (long)mem * 9 * nlm;
+#pragma unroll
+	for (int f = 0; f < 3; f++) {
+		const double2 *xf = x + (long)f * nlm + base;
+		double2 v = xf[l];
+		double2 a = (l - 1 >= m) ? xf[l - 1] : make_double2(0, 0);
+		double2 b = (l + 1 <= L) ? xf[l + 1] : make_double2(0, 0);
+		o[(long)f * nlm + idx] = v;
+		o[(long)(3 + 2 * f) * nlm + idx] = make_double2(-dm * v.y, dm * v.x);
+		o[(long)(4 + 2 * f) * nlm + idx] = make_double2(cm * a.x + cp * b.x, cm * a.y + cp * b.y);
+	}
+}
+
+// pointwise level 0 -> 4 fields per member
+__global__ void k_phys1(const double *__restrict__ P9, double *__restrict__ P4, const double *__restrict__ ct,
+			const double *__restrict__ st, const double *__restrict__ sq, int nlat, long nspat, int nmem,
+			double ga, double two_omega, double ia, int nonlinear, double h0a)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= nspat * nmem) return;
+	int mem = (int)(i / nspat);
+	long k = i - (long)mem * nspat;
+	int j = (int)(k % nlat);
+	const double *p = P9 + (long)mem * 9 * nspat + k;
+	double H = p[0], U = p[nspat], V = p[2 * nspat];
+	double Hl = p[3 * nspat], Hm = p[4 * nspat];
+	double Ul = p[5 * nspat], Um = p[6 * nspat], Vl = p[7 * nspat], Vm = p[8 * nspat];
+	const double c = sq[j], mu = ct[j];
+	double *o = P4 + (long)mem * 4 * nspat + k;
+	if (nonlinear) {
+		o[0] = U * H;                         // h*u
+		o[nspat] = (V * H) * st[j];           // (h*v)*cos(phi)
+		double tu = -ga * (Hl / c) + two_omega * (V * mu) - ((Ul / c) * U + (Um / c) * V) * ia;
+		double tv = -ga * (Hm / c) - two_omega * (U * mu) - ((Vl / c) * U + (Vm / c) * V) * ia;
+		o[2 * nspat] = tu;
+		o[3 * nspat] = tv;
+	} else {
+		o[0] = U;
+		o[nspat] = V * st[j];
+		o[2 * nspat] = -ga * (Hl / c) + two_omega * (V * mu);
+		o[3 * nspat] = -ga * (Hm / c) - two_omega * (U * mu);
+	}
+	(void)h0a;
+}
+
+// spectral level 1: im * A(hu), D' A(hv st)
+__global__ void k_prep2(const double2 *__restrict__ S4, double2 *__restrict__ S2, const int *__restrict__ mof, int L,
+			int nlm, int nmem)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= (long)nlm * nmem) return;
+	int mem = (int)(i / nlm);
+	int idx = (int)(i - (long)mem * nlm);
+	int m = mof[idx];
+	int base = lm_index(L, 0, m);
+	int l = idx - base;
+	const double dl = l, dm = m;
+	const double2 *a1 = S4 + (long)mem * 4 * nlm;
+	const double2 *a2 = a1 + nlm + base;
+	double2 v = a1[idx];
+	double2 *o = S2 + (long)mem * 2 * nlm;
+	o[idx] = make_double2(-dm * v.y, dm * v.x);
+	const double cm = (-dl + 1.0) * idR(dl - 1, dm), cp = (dl + 2.0) * idS(dl + 1, dm);
+	double2 a = (l - 1 >= m) ? a2[l - 1] : make_double2(0, 0);
+	double2 b = (l + 1 <= L) ? a2[l + 1] : make_double2(0, 0);
+	o[nlm + idx] = make_double2(cm * a.x + cp * b.x, cm * a.y + cp * b.y);
+}
+
+// pointwise level 2: h tendency in physical space
+__global__ void k_phys2(const double *__restrict__ P2, double *__restrict__ P1, const double *__restrict__ st,
+			const double *__restrict__ sq, int nlat, long nspat, int nmem, double scale)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= nspat * nmem) return;
+	int mem = (int)(i / nspat);
+	long k = i - (long)mem * nspat;
+	int j = (int)(k % nlat);
+	const double *p = P2 + (long)mem * 2 * nspat + k;
+	double c = sq[j];
+	double dlon = p[0] / c;
+	double dlat = (p[nspat] / c) / st[j];
+	P1[(long)mem * nspat + k] = -(dlon + dlat) * scale;
+}
+
+// tendencies k = (T_h, T_u, T_v) + nu/a^2 laplace(Y) and the RK4 bookkeeping
+//   stage < 3: acc (=|+=) b_s k ; Ynext = X + dt*(a_{s+1} k)
+//   stage 3:   X += dt*(acc + b_3 k)
+__global__ void k_stage(const double2 *__restrict__ TH, const double2 *__restrict__ S4, const double2 *__restrict__ Y,
+			double2 *__restrict__ X, double2 *__restrict__ ACC, double2 *__restrict__ YN,
+			const int *__restrict__ mof, int L, int nlm, int nmem, double visc, int stage, double bs,
+			double anext, double dt)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= (long)nlm * nmem) return;
+	int mem = (int)(i / nlm);
+	int idx = (int)(i - (long)mem * nlm);
+	int m = mof[idx];
+	int l = idx - lm_index(L, 0, m);
+	const double lap = -(double)l * ((double)l + 1.0);
+	const double2 *y = Y + (long)mem * 3 * nlm;
+	double2 *x = X + (long)mem * 3 * nlm;
+	double2 *acc = ACC + (long)mem * 3 * nlm;
+	double2 *yn = YN + (long)mem * 3 * nlm;
+	const double2 *s4 = S4 + (long)mem * 4 * nlm;
+	double2 t[3];
+	t[0] = TH[(long)mem * nlm + idx];
+	t[1] = s4[2 * nlm + idx];
+	t[2] = s4[3 * nlm + idx];
+#pragma unroll
+	for (int f = 0; f < 3; f++) {
+		double2 k = t[f];
+		if (visc != 0.0) {
+			double2 yv = y[(long)f * nlm + idx];
+			k.x += (yv.x * lap) * visc;
+			k.y += (yv.y * lap) * visc;
+		}
+		long o = (long)f * nlm + idx;
+		double2 bk = make_double2(bs * k.x, bs * k.y);
+		if (stage == 0) {
+			acc[o] = bk;
+		} else if (stage < 3) {
+			acc[o] = cadd(acc[o], bk);
+		}
+		if (stage < 3) {
+			double2 xv = x[o];
+			double2 ak = make_double2(anext * k.x, anext * k.y);
+			yn[o] = make_double2(xv.x + dt * ak.x, xv.y + dt * ak.y);
+		} else {
+			double2 s = cadd(acc[o], bk);
+			double2 xv = x[o];
+			x[o] = make_double2(xv.x + dt * s.x, xv.y + dt * s.y);
+		}
+	}
+}
+
+inline int g1(long n) { return (int)((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmem, int nsteps)
+{
+	if (!ctx || !p || !d_state || nmem < 1 || nsteps < 0) return SPHSWE_ERR_ARG;
+	if (p->precision != SPHSWE_PREC_FP64)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: precision %d not available", p->precision);
+	const long nlm = ctx->nlm, nspat = ctx->nspat;
+	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
+	// workspace: Y (3 nlm), ACC (3), YN (3), S9 (9), S4 (4), S2 (2), TH (1) spectral;
+	// P9 (9), P4 (4), P2 (2), P1 (1) physical; Fourier for 9 fields
+	const size_t spec_e = (size_t)nlm * nmem * (3 + 3 + 3 + 9 + 4 + 2 + 1);
+	const size_t spat_e = (size_t)nspat * nmem * (9 + 4 + 2 + 1);
+	const size_t four_e = (size_t)nfour * nmem * 9;
+	const size_t bytes = spec_e * 16 + spat_e * 8 + four_e * 16 + 1024;
+	SweWs &ws = g_ws[ctx];
+	if (ws.bytes < bytes) {
+		if (ws.base) {
+			SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+			SPHSWE_HIP(ctx, hipFree(ws.base));
+			ws.base = nullptr;
+			ws.bytes = 0;
+		}
+		if (hipMalloc(&ws.base, bytes) != hipSuccess)
+			return sphswe_set_err(ctx, SPHSWE_ERR_NOMEM, "swe workspace of %zu bytes", bytes);
+		ws.bytes = bytes;
+	}
+	double2 *sp = (double2 *)ws.base;
+	double2 *Ya = sp; sp += 3 * nlm * nmem;
+	double2 *ACC = sp; sp += 3 * nlm * nmem;
+	double2 *Yb = sp; sp += 3 * nlm * nmem;
+	double2 *S9 = sp; sp += 9 * nlm * nmem;
+	double2 *S4 = sp; sp += 4 * nlm * nmem;
+	double2 *S2 = sp; sp += 2 * nlm * nmem;
+	double2 *TH = sp; sp += 1 * nlm * nmem;
+	double2 *F = sp; sp += 9 * nfour * nmem;
+	double *dp = (double *)sp;
+	double *P9 = dp; dp += 9 * nspat * nmem;
+	double *P4 = dp; dp += 4 * nspat * nmem;
+	double *P2 = dp; dp += 2 * nspat * nmem;
+	double *P1 = dp;
+
+	double2 *X = (double2 *)d_state;
+	const double a = p->earth_radius, g = p->gravitation;
+	const double ga = g / a, ia = 1.0 / a, two_omega = 2.0 * p->coriolis_omega;
+	const double visc = p->viscosity / (a * a);
+	const double dt = p->dt;
+	const double bcoef[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+	const double anext[3] = {0.5, 0.5, 1.0};
+	const int nm = nmem, L = ctx->L, nlat = ctx->nlat;
+	hipStream_t st = ctx->stream;
+	int rc;
+
+	for (int n = 0; n < nsteps; n++) {
+		const double2 *Y = X;
+		double2 *YN = Ya;
+		for (int s = 0; s < 4; s++) {
+			k_prep9<<<g1(nlm * nm), 256, 0, st>>>(Y, S9, ctx->d_mofidx, L, (int)nlm, nm);
+			if ((rc = sphswe_leg_synth(ctx, S9, F, 9 * nm, SPHSWE_PREC_FP64))) return rc;
+			if ((rc = sphswe_fft_c2r(ctx, F, P9, 9 * nm))) return rc;
+			k_phys1<<<g1(nspat * nm), 256, 0, st>>>(P9, P4, ctx->d_ct, ctx->d_st, ctx->d_sq, nlat, nspat, nm, ga,
+								 two_omega, ia, p->nonlinear, p->h0 / a);
+			if ((rc = sphswe_fft_r2c(ctx, P4, F, 4 * nm))) return rc;
+			if ((rc = sphswe_leg_anal(ctx, F, S4, 4 * nm, SPHSWE_PREC_FP64))) return rc;
+			k_prep2<<<g1(nlm * nm), 256, 0, st>>>(S4, S2, ctx->d_mofidx, L, (int)nlm, nm);
+			if ((rc = sphswe_leg_synth(ctx, S2, F, 2 * nm, SPHSWE_PREC_FP64))) return rc;
+			if ((rc = sphswe_fft_c2r(ctx, F, P2, 2 * nm))) return rc;
+			k_phys2<<<g1(nspat * nm), 256, 0, st>>>(P2, P1, ctx->d_st, ctx->d_sq, nlat, nspat, nm,
+								 p->nonlinear ? ia : p->h0 / a);
+			if ((rc = sphswe_fft_r2c(ctx, P1, F, nm))) return rc;
+			if ((rc = sphswe_leg_anal(ctx, F, TH, nm, SPHSWE_PREC_FP64))) return rc;
+			k_stage<<<g1(nlm * nm), 256, 0, st>>>(TH, S4, Y, X, ACC, YN, ctx->d_mofidx, L, (int)nlm, nm, visc, s,
+							       bcoef[s], s < 3 ? anext[s] : 0.0, dt);
+			SPHSWE_LAUNCH_CHECK(ctx);
+			if (s < 3) {
+				Y = YN;
+				YN = (YN == Ya) ? Yb : Ya;
+			}
+		}
+	}
+	return SPHSWE_OK;
+}

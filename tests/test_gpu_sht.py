@@ -1,0 +1,100 @@
+"""GPU SHT pair (libsphswe.so) against the CPU oracle and exact properties."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sph_swe_amd import capi
+from tests._util import random_cplx_spec, random_real_spec, rel_l2, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+# FP64 parity tolerance (relative L2 / Linf over the field)
+TOL = 1e-13
+
+
+@pytest.fixture(scope="module", params=[16, 31, 64, 127])
+def pair(request):
+    L = request.param
+    ctx = capi.Context(L)
+    orc = O.Oracle(L, ctx.nlat, ctx.nphi)
+    yield ctx, orc
+    ctx.close()
+
+
+def test_grid_matches_oracle(pair):
+    ctx, orc = pair
+    assert np.max(np.abs(ctx.ct - orc.ct)) < 1e-15
+    assert np.max(np.abs(ctx.wg - orc.w)) < 3e-14
+
+
+def test_real_synthesis_and_analysis(pair):
+    ctx, orc = pair
+    qs = np.stack([random_real_spec(ctx.L, seed=s) for s in range(5)])
+    f_gpu = ctx.synthesis(qs)
+    for k in range(5):
+        f_ref = orc.synthesis(qs[k])
+        assert rel_l2(f_gpu[k], f_ref) < TOL
+        assert rel_linf(f_gpu[k], f_ref) < TOL
+    q_gpu = ctx.analysis(f_gpu)
+    for k in range(5):
+        q_ref = orc.analysis(f_gpu[k])
+        assert rel_l2(q_gpu[k], q_ref) < TOL
+        assert rel_l2(q_gpu[k], qs[k]) < 10 * TOL   # round trip of a band-limited field
+
+
+def test_analysis_of_unresolved_field(pair):
+    # a field that is not band-limited: truncation must match the oracle exactly
+    ctx, orc = pair
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal((2, ctx.nspat))
+    q = ctx.analysis(f)
+    for k in range(2):
+        assert rel_l2(q[k], orc.analysis(f[k])) < TOL
+
+
+def test_batch_sizes(pair):
+    # every field-group split of the kernels (1..11 fields) gives the same result
+    ctx, orc = pair
+    qs = np.stack([random_real_spec(ctx.L, seed=100 + s) for s in range(11)])
+    full = ctx.synthesis(qs)
+    for n in (1, 2, 3, 4, 7, 9):
+        part = ctx.synthesis(qs[:n])
+        assert np.array_equal(part, full[:n])
+    fa = ctx.analysis(full)
+    for n in (1, 2, 4, 5):
+        assert np.array_equal(ctx.analysis(full[:n]), fa[:n])
+
+
+def test_complex_pair(pair):
+    ctx, orc = pair
+    qs = np.stack([random_cplx_spec(ctx.L, seed=s) for s in range(3)])
+    f = ctx.synthesis_cplx(qs)
+    for k in range(3):
+        assert rel_l2(f[k], orc.synthesis_cplx(qs[k])) < TOL
+    q = ctx.analysis_cplx(f)
+    for k in range(3):
+        assert rel_l2(q[k], orc.analysis_cplx(f[k])) < TOL
+        assert rel_l2(q[k], qs[k]) < 10 * TOL
+
+
+def test_m0_imaginary_ignored(pair):
+    ctx, _ = pair
+    q = random_real_spec(ctx.L, seed=9)
+    q2 = q.copy()
+    q2[: ctx.L + 1] += 1j
+    assert np.array_equal(ctx.synthesis(q), ctx.synthesis(q2))
+
+
+def test_t511_round_trip_three_fields():
+    # the SHT round-trip benchmark size: 768 x 1536, 3 fields, power ~ l^-3
+    ctx = capi.Context(511)
+    assert (ctx.nlat, ctx.nphi) == (768, 1536)
+    qs = np.stack([random_real_spec(511, seed=s, power=-1.5) for s in range(3)])
+    f = ctx.synthesis(qs)
+    q = ctx.analysis(f)
+    for k in range(3):
+        assert rel_l2(q[k], qs[k]) < 1e-12
+    # spot check one field against the oracle (a few seconds on the host)
+    orc = O.Oracle(511)
+    assert rel_l2(f[0], orc.synthesis(qs[0])) < 1e-12
+    ctx.close()
