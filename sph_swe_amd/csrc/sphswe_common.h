@@ -35,9 +35,14 @@ struct sphswe_ctx {
 	double *d_sq = nullptr;       // sqrt(1 - mu^2) [nlat]
 	double *d_coslat = nullptr;   // cos(lat[j]) [nlat]
 	double *d_wg = nullptr;       // Gauss weights [nlat]
-	double2 *d_coef = nullptr;    // recurrence (a_lm, b_lm) [nlm], index of the real spectrum
+	double2 *d_coef = nullptr;    // recurrence (a_lm, a_lm*b_lm) [nlm], index of the real spectrum
 	int *d_l0 = nullptr;          // first significant degree [(L+1) * NJ]
-	double2 *d_seed = nullptr;    // (Pbar_l0, Pbar_l0+1) [(L+1) * NJ]
+	double2 *d_seed = nullptr;    // (Pbar_s, Pbar_s+1) at each degree-segment start [nseg_total * NJ]
+	int *d_segoff = nullptr;      // first segment of each m [L+2]
+	int *d_unit_m = nullptr;      // m of each segment [nseg_total]
+	int seg = 16;                 // degrees per segment
+	int nseg_total = 0;
+	int max_lds = 65536;          // LDS bytes a workgroup may allocate
 	int *d_mofidx = nullptr;      // m of real spectral index [nlm]
 	int *d_lofcidx = nullptr;     // l of complex spectral index [(L+1)^2]
 	double2 *d_tw = nullptr;      // e^{-2 pi i k / nphi} [nphi]

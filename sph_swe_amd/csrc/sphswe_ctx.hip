@@ -93,20 +93,30 @@ static bool fft_plan(int n, FftPlan &p)
 	return r == 1;
 }
 
-// Seeds of the Pbar_l^m recurrence: for every (m, north ring j) the first degree
-// l0 with |Pbar_l0^m(mu_j)| >= 2^-930 and the values at l0, l0+1.  Values before
-// l0 are below 1e-280 and are treated as zero by the kernels.
-static void make_seeds(int L, int NJ, const std::vector<double> &ct, const std::vector<double> &st,
-		       std::vector<int> &l0, std::vector<double2> &seed)
+// Seeds of the Pbar_l^m recurrence.  For every (m, north ring j): the first
+// degree l0 with |Pbar_l0^m(mu_j)| >= 2^-930 (values before it are below 1e-280
+// and are treated as zero), and for every degree segment k of length seg
+// (degrees m + k*seg ...) the pair (Pbar_s, Pbar_s+1) at s = max(m + k*seg, l0).
+// The kernels run the recurrence segment by segment from these FP64 values, so
+// no chain is longer than seg degrees.
+static void make_seeds(int L, int NJ, int seg, const std::vector<double> &ct, const std::vector<double> &st,
+		       std::vector<int> &l0, std::vector<int> &segoff, std::vector<double2> &seed)
 {
+	segoff.assign(L + 2, 0);
+	for (int m = 0; m <= L; m++) segoff[m + 1] = segoff[m] + (L - m + seg) / seg;
 	l0.assign((size_t)(L + 1) * NJ, L + 1);
-	seed.assign((size_t)(L + 1) * NJ, make_double2(0, 0));
+	seed.assign((size_t)segoff[L + 1] * NJ, make_double2(0, 0));
 	const double thr_log2 = -930.0;
 	// running Pbar_m^m(mu_j) = xm[j] * 2^Em[j], built by the product
 	// Pbar_m^m = -sqrt((2m+1)/(2m)) st Pbar_{m-1}^{m-1} (error ~ sqrt(m) eps)
 	std::vector<double> xm(NJ, 1.0 / sqrt(4.0 * M_PI));
 	std::vector<int> Em(NJ, 0);
+	std::vector<double> av(L + 2), bv(L + 2);
 	for (int m = 0; m <= L; m++) {
+		for (int l = m + 2; l <= L; l++) {
+			av[l] = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+			bv[l] = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
+		}
 		for (int j = 0; j < NJ; j++) {
 			if (m > 0) {
 				xm[j] *= -sqrt((2.0 * m + 1.0) / (2.0 * m)) * st[j];
@@ -119,23 +129,13 @@ static void make_seeds(int L, int NJ, const std::vector<double> &ct, const std::
 			double x0 = xm[j];
 			double x1 = (m < L) ? sqrt(2.0 * m + 3.0) * ct[j] * x0 : 0.0;
 			int l = m;
-			// (x0, x1) = scaled (P_l, P_l+1)
+			// scaled search for the first significant degree; (x0, x1) = (P_l, P_l+1) * 2^-E
+			bool found = false;
 			while (true) {
-				bool sig = x0 != 0.0 && (log2(fabs(x0)) + E >= thr_log2);
-				if (sig) {
-					size_t k = (size_t)m * NJ + j;
-					l0[k] = l;
-					seed[k] = make_double2(ldexp(x0, E), (l + 1 <= L) ? ldexp(x1, E) : 0.0);
-					break;
-				}
+				if (x0 != 0.0 && (log2(fabs(x0)) + E >= thr_log2)) { found = true; break; }
 				if (l + 1 > L) break;
 				int ln = l + 2;
-				double x2 = 0.0;
-				if (ln <= L) {
-					double a = sqrt((4.0 * ln * ln - 1.0) / ((double)ln * ln - (double)m * m));
-					double b = sqrt(((ln - 1.0) * (ln - 1.0) - (double)m * m) / (4.0 * (ln - 1.0) * (ln - 1.0) - 1.0));
-					x2 = a * (ct[j] * x1 - b * x0);
-				}
+				double x2 = (ln <= L) ? av[ln] * (ct[j] * x1 - bv[ln] * x0) : 0.0;
 				x0 = x1;
 				x1 = x2;
 				l++;
@@ -144,6 +144,26 @@ static void make_seeds(int L, int NJ, const std::vector<double> &ct, const std::
 					x1 = ldexp(x1, -200);
 					E += 200;
 				}
+			}
+			if (!found) continue;
+			l0[(size_t)m * NJ + j] = l;
+			// from here on plain doubles: the values only grow out of 2^-930
+			double p0 = ldexp(x0, E), p1 = (l + 1 <= L) ? ldexp(x1, E) : 0.0;
+			const int nseg = segoff[m + 1] - segoff[m];
+			for (int k = 0; k < nseg; k++) {
+				const int s0 = m + k * seg;
+				const int se = std::min(s0 + seg - 1, L);
+				if (se < l) continue;
+				const int st_l = std::max(s0, l);
+				// advance (p0, p1) = (P_l, P_l+1) to l = st_l
+				while (l < st_l) {
+					int ln = l + 2;
+					double p2 = (ln <= L) ? av[ln] * (ct[j] * p1 - bv[ln] * p0) : 0.0;
+					p0 = p1;
+					p1 = p2;
+					l++;
+				}
+				seed[(size_t)(segoff[m] + k) * NJ + j] = make_double2(p0, (l + 1 <= L) ? p1 : 0.0);
 			}
 		}
 	}
@@ -209,7 +229,7 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 			if (l >= m + 2) {
 				double a = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
 				double b = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
-				coef[k] = make_double2(a, b);
+				coef[k] = make_double2(a, a * b);   // Pbar_l = (a mu) Pbar_{l-1} - (a b) Pbar_{l-2}
 			} else {
 				coef[k] = make_double2(0.0, 0.0);
 			}
@@ -217,9 +237,19 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	std::vector<int> lofc(ctx->ncplx);
 	for (int l = 0; l <= lmax; l++)
 		for (int m = -l; m <= l; m++) lofc[l * l + l + m] = l;
-	std::vector<int> l0;
+	std::vector<int> l0, segoff;
 	std::vector<double2> seed;
-	make_seeds(lmax, ctx->NJ, ctx->ct, ctx->st, l0, seed);
+	// 32-degree segments (one MFMA analysis wavefront each)
+	ctx->seg = 32;
+	make_seeds(lmax, ctx->NJ, ctx->seg, ctx->ct, ctx->st, l0, segoff, seed);
+	ctx->nseg_total = segoff[lmax + 1];
+	std::vector<int> unit_m(ctx->nseg_total);
+	for (int m = 0; m <= lmax; m++)
+		for (int k = segoff[m]; k < segoff[m + 1]; k++) unit_m[k] = m;
+	{
+		int v = 0;
+		if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess) ctx->max_lds = v;
+	}
 	std::vector<double2> tw(nphi);
 	for (int k = 0; k < nphi; k++) {
 		// long double argument reduction keeps every twiddle within 1 ulp
@@ -235,6 +265,8 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	if ((rc = upload(ctx, &ctx->d_coef, coef))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_l0, l0))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_seed, seed))) return fail(rc);
+	if ((rc = upload(ctx, &ctx->d_segoff, segoff))) return fail(rc);
+	if ((rc = upload(ctx, &ctx->d_unit_m, unit_m))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_mofidx, mof))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_lofcidx, lofc))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_tw, tw))) return fail(rc);
@@ -250,7 +282,7 @@ extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 	hipSetDevice(ctx->device);
 	if (ctx->stream) hipStreamSynchronize(ctx->stream);
 	void *ptrs[] = {ctx->d_ct, ctx->d_st, ctx->d_sq, ctx->d_coslat, ctx->d_wg, ctx->d_coef, ctx->d_l0,
-			ctx->d_seed, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_flag};
+			ctx->d_seed, ctx->d_segoff, ctx->d_unit_m, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_flag};
 	for (void *p : ptrs)
 		if (p) hipFree(p);
 	for (auto &kv : ctx->allocs) hipFree(kv.first);

@@ -11,6 +11,7 @@
 namespace {
 
 enum { MODE_R2C = 0, MODE_C2R = 1, MODE_C2C_FWD = 2, MODE_C2C_BWD = 3 };
+constexpr int KB = 8;   // loads in flight per thread in the staging loops
 
 struct FftArgs {
 	int N;
@@ -165,11 +166,22 @@ __global__ __launch_bounds__(256) void k_fft(const void *__restrict__ in, void *
 		const int j0 = lb * RB;
 		const double *src = (const double *)in + f * nspat;
 		double *lds = (double *)bufA;
-		for (int idx = tid; idx < N * RB; idx += nt) {
-			int i = idx / RB, r = idx - i * RB;
-			int j = j0 + r;
-			double v = (j < nlat) ? src[(size_t)i * nlat + j] : 0.0;
-			lds[2 * ((r >> 1) * N + i) + (r & 1)] = v;
+		for (int b = 0; b < N * RB; b += KB * nt) {
+			double v[KB];
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				int idx = min(b + u * nt + tid, N * RB - 1);
+				int i = idx / RB, r = idx - i * RB;
+				int j = min(j0 + r, nlat - 1);
+				v[u] = src[(size_t)i * nlat + j];
+			}
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				int idx = b + u * nt + tid;
+				if (idx >= N * RB) break;
+				int i = idx / RB, r = idx - i * RB;
+				lds[2 * ((r >> 1) * N + i) + (r & 1)] = (j0 + r < nlat) ? v[u] : 0.0;
+			}
 		}
 		__syncthreads();
 		double2 *res = fft_lds(bufA, bufB, plan, nseq, tw, -1);
@@ -192,11 +204,23 @@ __global__ __launch_bounds__(256) void k_fft(const void *__restrict__ in, void *
 		const int j0 = lb * RB;
 		const double2 *src = (const double2 *)in + (size_t)f * (L + 1) * nlat;
 		// stage G[s][m][part] in bufB
-		for (int idx = tid; idx < (L + 1) * RB; idx += nt) {
-			int m = idx / RB, r = idx - m * RB;
-			int j = j0 + r;
-			double2 g = (j < nlat) ? src[(size_t)m * nlat + j] : make_double2(0, 0);
-			bufB[((r >> 1) * (L + 1) + m) * 2 + (r & 1)] = g;
+		const int ng = (L + 1) * RB;
+		for (int b = 0; b < ng; b += KB * nt) {
+			double2 v[KB];
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				int idx = min(b + u * nt + tid, ng - 1);
+				int m = idx / RB, r = idx - m * RB;
+				int j = min(j0 + r, nlat - 1);
+				v[u] = src[(size_t)m * nlat + j];
+			}
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				int idx = b + u * nt + tid;
+				if (idx >= ng) break;
+				int m = idx / RB, r = idx - m * RB;
+				bufB[((r >> 1) * (L + 1) + m) * 2 + (r & 1)] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
+			}
 		}
 		__syncthreads();
 		for (int idx = tid; idx < nseq * N; idx += nt) {
@@ -227,10 +251,21 @@ __global__ __launch_bounds__(256) void k_fft(const void *__restrict__ in, void *
 	} else if (MODE == MODE_C2C_FWD) {
 		const int j0 = lb * nseq;
 		const double2 *src = (const double2 *)in + f * nspat;
-		for (int idx = tid; idx < N * nseq; idx += nt) {
-			int i = idx / nseq, r = idx - i * nseq;
-			int j = j0 + r;
-			bufA[r * N + i] = (j < nlat) ? src[(size_t)i * nlat + j] : make_double2(0, 0);
+		for (int b = 0; b < N * nseq; b += KB * nt) {
+			double2 v[KB];
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				int idx = min(b + u * nt + tid, N * nseq - 1);
+				int i = idx / nseq, r = idx - i * nseq;
+				v[u] = src[(size_t)i * nlat + min(j0 + r, nlat - 1)];
+			}
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				int idx = b + u * nt + tid;
+				if (idx >= N * nseq) break;
+				int i = idx / nseq, r = idx - i * nseq;
+				bufA[r * N + i] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
+			}
 		}
 		__syncthreads();
 		double2 *res = fft_lds(bufA, bufB, plan, nseq, tw, -1);
@@ -249,12 +284,24 @@ __global__ __launch_bounds__(256) void k_fft(const void *__restrict__ in, void *
 		const double2 *src = (const double2 *)in + (size_t)f * (2 * L + 1) * nlat;
 		for (int idx = tid; idx < nseq * N; idx += nt) bufA[idx] = make_double2(0, 0);
 		__syncthreads();
-		for (int idx = tid; idx < (2 * L + 1) * nseq; idx += nt) {
-			int mi = idx / nseq, r = idx - mi * nseq;
-			int j = j0 + r;
-			int k = mi - L;
-			if (k < 0) k += N;
-			bufA[r * N + k] = (j < nlat) ? src[(size_t)mi * nlat + j] : make_double2(0, 0);
+		const int ng = (2 * L + 1) * nseq;
+		for (int b = 0; b < ng; b += KB * nt) {
+			double2 v[KB];
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				int idx = min(b + u * nt + tid, ng - 1);
+				int mi = idx / nseq, r = idx - mi * nseq;
+				v[u] = src[(size_t)mi * nlat + min(j0 + r, nlat - 1)];
+			}
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				int idx = b + u * nt + tid;
+				if (idx >= ng) break;
+				int mi = idx / nseq, r = idx - mi * nseq;
+				int k = mi - L;
+				if (k < 0) k += N;
+				bufA[r * N + k] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
+			}
 		}
 		__syncthreads();
 		double2 *res = fft_lds(bufA, bufB, plan, nseq, tw, +1);

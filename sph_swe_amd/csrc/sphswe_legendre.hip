@@ -38,10 +38,11 @@ __device__ inline int wave_max(int v)
 struct Geo {
 	const double2 *coef;
 	const int *l0;
-	const double2 *seed;
+	const double2 *seed;    // per degree segment
+	const int *segoff;
 	const double *ct;
 	const double *wg;
-	int L, nlat, NJ;
+	int L, nlat, NJ, seg;
 };
 
 // slot addressing ------------------------------------------------------------
@@ -85,214 +86,372 @@ __device__ inline void make_slots(Slot<CPLX> (&sl)[NS], int am, int group, int n
 // synthesis
 // ----------------------------------------------------------------------------
 
-template <bool CPLX, int NS>
-__global__ __launch_bounds__(64) void k_leg_synth(const double2 *__restrict__ spec, double2 *__restrict__ four, Geo g,
-						  int nfields)
+// one recurrence step: Pbar_l = (a mu) Pbar_{l-1} - (a b) Pbar_{l-2}
+__device__ inline double rec_step(double2 c, double mu, double p1, double p0)
 {
+	return fma(c.x * mu, p1, -(c.y * p0));
+}
+
+// Synthesis: a workgroup handles one m and 128 rings (two per lane, i.e. two
+// independent recurrence chains per lane); its wavefronts take groups of G
+// degree segments of that m, each starting from the seeds of the segment that
+// holds its first significant degree.  The coefficient row and the spectral
+// rows of the slots are staged in LDS (broadcast reads); the partial sums of
+// the wavefronts are reduced through LDS in a fixed order (deterministic).
+constexpr int SY_RPL = 2;          // rings per lane
+constexpr int SY_WAVES = 4;        // wavefronts per workgroup
+
+template <int NS>
+__host__ __device__ inline size_t synth_lds_bytes(int L, int waves)
+{
+	size_t cq = sizeof(double2) * (size_t)(L + 1) * (NS + 1);
+	size_t red = sizeof(double) * (size_t)waves * NS * 4 * 64 * SY_RPL;
+	return cq > red ? cq : red;
+}
+
+template <bool CPLX, int NS>
+__global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__restrict__ spec,
+							     double2 *__restrict__ four, Geo g, int nfields, int G)
+{
+	extern __shared__ __align__(16) unsigned char lds_raw[];
+	double2 *cq = (double2 *)lds_raw;    // [(l - m)][NS + 1]: coef, q of each slot
+	double *red = (double *)lds_raw;     // after the main loop: [wave][NS*4][64*SY_RPL]
 	const int am = blockIdx.y;
-	const int L = g.L, NJ = g.NJ;
-	const int j = blockIdx.x * 64 + threadIdx.x;
-	const int jc = j < NJ ? j : NJ - 1;
-	const double mu = g.ct[jc];
-	const int ls = g.l0[(long)am * NJ + jc];
-	const double2 sd = g.seed[(long)am * NJ + jc];
-	const int cb = lm_index(L, 0, am);
+	const int L = g.L, NJ = g.NJ, seg = g.seg;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	int jr[SY_RPL], jcr[SY_RPL], l0r[SY_RPL];
+	double mu[SY_RPL];
+#pragma unroll
+	for (int r = 0; r < SY_RPL; r++) {
+		jr[r] = blockIdx.x * 64 * SY_RPL + r * 64 + lane;
+		jcr[r] = jr[r] < NJ ? jr[r] : NJ - 1;
+		mu[r] = g.ct[jcr[r]];
+		l0r[r] = g.l0[(long)am * NJ + jcr[r]];
+	}
+	const int so = g.segoff[am];
+	const int nseg = g.segoff[am + 1] - so;
 	Slot<CPLX> sl[NS];
 	make_slots<CPLX, NS>(sl, am, blockIdx.z, nfields, L, g.nlat);
 
-	double eR[NS], eI[NS], oR[NS], oI[NS];
+	// stage the row: every thread issues its loads back to back (clamped
+	// addresses), then writes them to LDS
+	const int nrow = L - am + 1;
+	const int ntot = nrow * (NS + 1);
+	constexpr int SK = 8;
+	for (int base = 0; base < ntot; base += SK * (int)blockDim.x) {
+		double2 v[SK];
 #pragma unroll
-	for (int s = 0; s < NS; s++) eR[s] = eI[s] = oR[s] = oI[s] = 0.0;
-
-	const int lmn = wave_min(ls), lmx = wave_max(ls);
-	double p0 = 0.0, p1 = 0.0;
-	int l = lmn;
-	// phase 1: degrees where some lane injects its seeds
-	const int l1 = min(lmx + 1, L);
-	for (; l <= l1; ++l) {
-		const double2 c = g.coef[cb + l];
-		const double rec = c.x * (mu * p1 - c.y * p0);
-		const double P = (l == ls) ? sd.x : ((l == ls + 1) ? sd.y : rec);
-		p0 = p1;
-		p1 = P;
-		const bool odd = (l - am) & 1;
+		for (int u = 0; u < SK; u++) {
+			const int idx = base + u * (int)blockDim.x + (int)threadIdx.x;
+			const int ic = idx < ntot ? idx : ntot - 1;
+			const int t = ic / (NS + 1), sidx = ic - t * (NS + 1);
+			const int l = am + t;
+			long qsel = sl[0].qb + Slot<CPLX>::off(l);
 #pragma unroll
-		for (int s = 0; s < NS; s++) {
-			const double2 q = spec[sl[s].qb + Slot<CPLX>::off(l)];
-			if (odd) { oR[s] = fma(q.x, P, oR[s]); oI[s] = fma(q.y, P, oI[s]); }
-			else { eR[s] = fma(q.x, P, eR[s]); eI[s] = fma(q.y, P, eI[s]); }
+			for (int ss = 1; ss < NS; ss++)
+				if (ss == sidx - 1) qsel = sl[ss].qb + Slot<CPLX>::off(l);
+			const double2 *src = (sidx == 0) ? g.coef + lm_index(L, l, am) : spec + qsel;
+			v[u] = *src;
+		}
+#pragma unroll
+		for (int u = 0; u < SK; u++) {
+			const int idx = base + u * (int)blockDim.x + (int)threadIdx.x;
+			if (idx < ntot) cq[idx] = v[u];
 		}
 	}
-	// phase 2: plain recurrence, unrolled by parity
-#define STEP(ACR, ACI)                                                           \
-	{                                                                        \
-		const double2 c = g.coef[cb + l];                                \
-		const double P = c.x * (mu * p1 - c.y * p0);                     \
-		p0 = p1;                                                         \
-		p1 = P;                                                          \
-		_Pragma("unroll") for (int s = 0; s < NS; s++)                   \
-		{                                                                \
-			const double2 q = spec[sl[s].qb + Slot<CPLX>::off(l)];   \
-			ACR[s] = fma(q.x, P, ACR[s]);                            \
-			ACI[s] = fma(q.y, P, ACI[s]);                            \
-		}                                                                \
-		++l;                                                             \
-	}
-	if (l <= L && ((l - am) & 1)) STEP(oR, oI)
-	for (; l + 1 <= L;) {
-		STEP(eR, eI)
-		STEP(oR, oI)
-	}
-	if (l <= L) STEP(eR, eI)
-#undef STEP
+	__syncthreads();
 
-	if (j < NJ) {
-		const int js = g.nlat - 1 - j;
+	double eR[SY_RPL][NS], eI[SY_RPL][NS], oR[SY_RPL][NS], oI[SY_RPL][NS];
 #pragma unroll
-		for (int s = 0; s < NS; s++) {
-			if (!sl[s].valid) continue;
-			double2 n = make_double2(sl[s].sgn * (eR[s] + oR[s]), sl[s].sgn * (eI[s] + oI[s]));
-			double2 so = make_double2(sl[s].sgn * (eR[s] - oR[s]), sl[s].sgn * (eI[s] - oI[s]));
-			if (!CPLX && am == 0) { n.y = 0.0; so.y = 0.0; }  // Im q_l0 is ignored
-			four[sl[s].fb + j] = n;
-			four[sl[s].fb + js] = so;
+	for (int r = 0; r < SY_RPL; r++)
+#pragma unroll
+		for (int s = 0; s < NS; s++) eR[r][s] = eI[r][s] = oR[r][s] = oI[r][s] = 0.0;
+
+	const int nwork = (nseg + G - 1) / G;
+	if (wave < nwork) {
+		const int s0 = am + wave * G * seg;
+		const int se = min(s0 + G * seg - 1, L);
+		int st[SY_RPL];
+		double2 sd[SY_RPL];
+		int lo = L + 1, hi = -1;
+#pragma unroll
+		for (int r = 0; r < SY_RPL; r++) {
+			st[r] = max(s0, l0r[r]);     // > se: nothing in this range
+			const int kst = (st[r] <= se) ? (st[r] - am) / seg : wave * G;
+			sd[r] = g.seed[(long)(so + kst) * NJ + jcr[r]];
+			lo = min(lo, st[r]);
+			if (st[r] <= se) hi = max(hi, st[r]);
+		}
+		const int lmn = __builtin_amdgcn_readfirstlane(wave_min(lo));
+		const int lmx = __builtin_amdgcn_readfirstlane(wave_max(hi));
+		double p0[SY_RPL], p1[SY_RPL];
+#pragma unroll
+		for (int r = 0; r < SY_RPL; r++) p0[r] = p1[r] = 0.0;
+		int l = lmn;
+		// degrees where some lane injects its seeds
+		const int l1 = min(lmx + 1, se);
+		for (; l <= l1; ++l) {
+			const double2 *row = cq + (size_t)(l - am) * (NS + 1);
+			const double2 c = row[0];
+			const bool odd = (l - am) & 1;
+#pragma unroll
+			for (int r = 0; r < SY_RPL; r++) {
+				const double rec = rec_step(c, mu[r], p1[r], p0[r]);
+				const double P = (l == st[r]) ? sd[r].x : ((l == st[r] + 1) ? sd[r].y : rec);
+				p0[r] = p1[r];
+				p1[r] = P;
+#pragma unroll
+				for (int s = 0; s < NS; s++) {
+					const double2 q = row[1 + s];
+					if (odd) { oR[r][s] = fma(q.x, P, oR[r][s]); oI[r][s] = fma(q.y, P, oI[r][s]); }
+					else { eR[r][s] = fma(q.x, P, eR[r][s]); eI[r][s] = fma(q.y, P, eI[r][s]); }
+				}
+			}
+		}
+		// plain recurrence: align to even (l - m), then pairs of degrees
+		if (l <= se && ((l - am) & 1)) {
+			const double2 *row = cq + (size_t)(l - am) * (NS + 1);
+#pragma unroll
+			for (int r = 0; r < SY_RPL; r++) {
+				const double P = rec_step(row[0], mu[r], p1[r], p0[r]);
+				p0[r] = p1[r];
+				p1[r] = P;
+#pragma unroll
+				for (int s = 0; s < NS; s++) {
+					oR[r][s] = fma(row[1 + s].x, P, oR[r][s]);
+					oI[r][s] = fma(row[1 + s].y, P, oI[r][s]);
+				}
+			}
+			++l;
+		}
+		for (; l + 1 <= se; l += 2) {
+			const double2 *row = cq + (size_t)(l - am) * (NS + 1);
+			const double2 ce = row[0], co = row[NS + 1];
+			double2 qe[NS], qo[NS];
+#pragma unroll
+			for (int s = 0; s < NS; s++) { qe[s] = row[1 + s]; qo[s] = row[NS + 2 + s]; }
+#pragma unroll
+			for (int r = 0; r < SY_RPL; r++) {
+				const double Pe = rec_step(ce, mu[r], p1[r], p0[r]);
+				const double Po = rec_step(co, mu[r], Pe, p1[r]);
+				p0[r] = Pe;
+				p1[r] = Po;
+#pragma unroll
+				for (int s = 0; s < NS; s++) {
+					eR[r][s] = fma(qe[s].x, Pe, eR[r][s]);
+					eI[r][s] = fma(qe[s].y, Pe, eI[r][s]);
+					oR[r][s] = fma(qo[s].x, Po, oR[r][s]);
+					oI[r][s] = fma(qo[s].y, Po, oI[r][s]);
+				}
+			}
+		}
+		if (l <= se) {
+			const double2 *row = cq + (size_t)(l - am) * (NS + 1);
+#pragma unroll
+			for (int r = 0; r < SY_RPL; r++) {
+				const double P = rec_step(row[0], mu[r], p1[r], p0[r]);
+#pragma unroll
+				for (int s = 0; s < NS; s++) {
+					eR[r][s] = fma(row[1 + s].x, P, eR[r][s]);
+					eI[r][s] = fma(row[1 + s].y, P, eI[r][s]);
+				}
+			}
+		}
+	}
+	__syncthreads();   // LDS now holds the partial sums of the wavefronts
+	constexpr int RW = 64 * SY_RPL;
+	if (wave > 0 && wave < nwork) {
+		double *rw = red + (size_t)wave * NS * 4 * RW;
+#pragma unroll
+		for (int r = 0; r < SY_RPL; r++)
+#pragma unroll
+			for (int s = 0; s < NS; s++) {
+				const int o = r * 64 + lane;
+				rw[(4 * s + 0) * RW + o] = eR[r][s];
+				rw[(4 * s + 1) * RW + o] = eI[r][s];
+				rw[(4 * s + 2) * RW + o] = oR[r][s];
+				rw[(4 * s + 3) * RW + o] = oI[r][s];
+			}
+	}
+	__syncthreads();
+	if (wave == 0) {
+		for (int w = 1; w < nwork; w++) {
+			const double *rw = red + (size_t)w * NS * 4 * RW;
+#pragma unroll
+			for (int r = 0; r < SY_RPL; r++)
+#pragma unroll
+				for (int s = 0; s < NS; s++) {
+					const int o = r * 64 + lane;
+					eR[r][s] += rw[(4 * s + 0) * RW + o];
+					eI[r][s] += rw[(4 * s + 1) * RW + o];
+					oR[r][s] += rw[(4 * s + 2) * RW + o];
+					oI[r][s] += rw[(4 * s + 3) * RW + o];
+				}
+		}
+#pragma unroll
+		for (int r = 0; r < SY_RPL; r++) {
+			const int j = jr[r];
+			if (j >= NJ) continue;
+			const int js = g.nlat - 1 - j;
+#pragma unroll
+			for (int s = 0; s < NS; s++) {
+				if (!sl[s].valid) continue;
+				double2 nn = make_double2(sl[s].sgn * (eR[r][s] + oR[r][s]), sl[s].sgn * (eI[r][s] + oI[r][s]));
+				double2 ss2 = make_double2(sl[s].sgn * (eR[r][s] - oR[r][s]), sl[s].sgn * (eI[r][s] - oI[r][s]));
+				if (!CPLX && am == 0) { nn.y = 0.0; ss2.y = 0.0; }  // Im q_l0 is ignored
+				four[sl[s].fb + j] = nn;
+				four[sl[s].fb + js] = ss2;
+			}
 		}
 	}
 }
 
 // ----------------------------------------------------------------------------
-// analysis
+// analysis (FP64 MFMA)
 // ----------------------------------------------------------------------------
 
-constexpr int LT = 16;      // degrees per tile
-constexpr int MAXW = 4;     // wavefronts per workgroup
-constexpr int MAXCH = 4;    // 64-ring chunks per wavefront
+// One wavefront per (m, 32-degree segment) covers every ring, so no reduction
+// crosses wavefronts.  For each 64-ring chunk the wavefront
+//   - runs the recurrence with lane = ring over the 32 degrees of its segment
+//     and writes the Pbar tile to its own LDS region, split by parity of l-m;
+//   - contracts over rings with v_mfma_f64_16x16x4f64:
+//       Q[16 rows of one parity][16 columns] += Pbar[rows][4 rings] * E[4 rings][cols]
+//     where the columns are (slot, re/im) and E is the even (F_N + F_S) or odd
+//     (F_N - F_S) hemisphere combination times the Gauss weight.
+// Operand layouts of v_mfma_f64_16x16x4f64 (wave64): A[i][k] in lane i + 16k,
+// B[k][n] in lane n + 16k, D[i][n] in lane n + 16 (i % 4), element i / 4.
+constexpr int AN_SEG = 32;        // degrees per wavefront (16 of each parity)
+constexpr int AN_WAVES = 4;       // independent wavefronts per workgroup
+constexpr int AN_PS = 65;         // LDS row stride (doubles) of the Pbar tile
+
+typedef double d4v __attribute__((ext_vector_type(4)));
 
 template <bool CPLX, int NS>
-__global__ __launch_bounds__(256) void k_leg_anal(const double2 *__restrict__ four, double2 *__restrict__ spec, Geo g,
-						  int nfields, int nw, int nch)
+__global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__restrict__ four, double2 *__restrict__ spec,
+							    Geo g, const int *__restrict__ unit_m, int nunits,
+							    int nfields)
 {
-	__shared__ double Ps[MAXW][LT][65];
-	__shared__ double2 Es[MAXW][2][64][NS];
-	__shared__ double2 Qp[MAXW][LT][NS];
-
-	const int am = blockIdx.x;
-	const int L = g.L, NJ = g.NJ, nlat = g.nlat;
+	__shared__ double Pt[AN_WAVES][2][16][AN_PS];
+	__shared__ double2 Cfs[AN_WAVES][AN_SEG];
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	const int cb = lm_index(L, 0, am);
+	const int u = blockIdx.x * AN_WAVES + wave;
+	if (u >= nunits) return;   // whole wavefront; no workgroup barrier below
+	const int am = unit_m[u];
+	const int L = g.L, NJ = g.NJ, nlat = g.nlat;
+	const int so = g.segoff[am];
+	const int kseg = u - so;
+	const int s0 = am + kseg * AN_SEG, se = min(s0 + AN_SEG - 1, L);
 	Slot<CPLX> sl[NS];
 	make_slots<CPLX, NS>(sl, am, blockIdx.y, nfields, L, nlat);
 
-	// per-chunk recurrence state
-	double p0[MAXCH], p1[MAXCH], mu[MAXCH], w[MAXCH];
-	double2 sd[MAXCH];
-	int ls[MAXCH], lmn[MAXCH], lmx[MAXCH];
-	bool val[MAXCH];
+	// recurrence coefficients of the segment, read back as LDS broadcasts
+	double2 *cf = Cfs[wave];
+	if (lane < AN_SEG) cf[lane] = g.coef[lm_index(L, min(s0 + lane, se), am)];
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+	const int n = lane & 15, kq = lane >> 4;       // MFMA column / k-row of this lane
+	const int slot = n >> 1, comp = n & 1;
+	long fbase = sl[0].fb;
+	bool colv = false;
 #pragma unroll
-	for (int k = 0; k < MAXCH; k++) {
-		int j = (wave + k * nw) * 64 + lane;
-		val[k] = (k < nch) && (j < NJ);
-		int jc = j < NJ ? j : NJ - 1;
-		mu[k] = g.ct[jc];
-		w[k] = g.wg[jc];
-		ls[k] = val[k] ? g.l0[(long)am * NJ + jc] : L + 1;
-		sd[k] = g.seed[(long)am * NJ + jc];
-		lmn[k] = wave_min(ls[k]);
-		lmx[k] = wave_max(val[k] ? ls[k] : -1);
-		p0[k] = p1[k] = 0.0;
+	for (int ss = 0; ss < NS; ss++)
+		if (ss == slot) { fbase = sl[ss].fb; colv = true; }
+	const double cmask = colv ? 1.0 : 0.0;
+	const double *fd = (const double *)four;
+
+	d4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+	double (*P)[16][AN_PS] = Pt[wave];
+	const int nch = (NJ + 63) / 64;
+	for (int c = 0; c < nch; c++) {
+		// B operands: E_even / E_odd of rings jb + 4s, column n.  Rings past NJ
+		// stay inside the Fourier rows (64 * nch <= NJ + 63 < nlat) and get
+		// weight 0, so the loads need no clamping.
+		const int jb = c * 64 + kq;
+		const double *pn = fd + 2 * (fbase + jb) + comp;
+		const double *psth = fd + 2 * (fbase + (nlat - 1 - jb)) + comp;
+		const double *pw = g.wg + jb;
+		double fn[16], fs[16], wv[16];
+#pragma unroll
+		for (int s = 0; s < 16; s++) {
+			fn[s] = pn[8 * s];
+			fs[s] = psth[-8 * s];
+			wv[s] = pw[4 * s];
+		}
+		// Pbar tile: lane = ring (rings past NJ give finite values that meet weight 0)
+		const int j = c * 64 + lane;
+		const int jc = j < NJ ? j : NJ - 1;
+		const double mu = g.ct[jc];
+		const int st = max(s0, g.l0[(long)am * NJ + jc]);
+		const double2 sd = g.seed[(long)u * NJ + jc];
+		const bool all_at_s0 = __builtin_amdgcn_readfirstlane(wave_max(j < NJ ? st : s0)) == s0;
+		if (all_at_s0) {
+			// every lane starts at s0: seeds, then the plain recurrence
+			// rows past se (short last segment) hold finite values that are never stored
+			double p0 = sd.x, p1 = sd.y;
+			P[0][0][lane] = p0;
+			P[1][0][lane] = p1;
+#pragma unroll
+			for (int t = 2; t < AN_SEG; t++) {
+				const double Pv = rec_step(cf[t], mu, p1, p0);
+				p0 = p1;
+				p1 = Pv;
+				P[t & 1][t >> 1][lane] = Pv;
+			}
+		} else {
+			double p0 = 0.0, p1 = 0.0;
+#pragma unroll
+			for (int t = 0; t < AN_SEG; t++) {
+				const int l = s0 + t;
+				const double rec = rec_step(cf[t], mu, p1, p0);
+				const double Pv = (l < st) ? 0.0 : ((l == st) ? sd.x : ((l == st + 1) ? sd.y : rec));
+				p0 = p1;
+				p1 = Pv;
+				P[t & 1][t >> 1][lane] = Pv;
+			}
+		}
+		double be[16], bo[16];
+#pragma unroll
+		for (int s = 0; s < 16; s++) {
+			const double w = (jb + 4 * s < NJ) ? wv[s] * cmask : 0.0;
+			be[s] = w * (fn[s] + fs[s]);
+			bo[s] = w * (fn[s] - fs[s]);
+		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		// contraction over the 64 rings of the chunk, 4 per MFMA
+#pragma unroll
+		for (int s = 0; s < 16; s++) {
+			const double a0 = P[0][n][4 * s + kq];
+			const double a1 = P[1][n][4 * s + kq];
+			acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, be[s], acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bo[s], acc1, 0, 0, 0);
+		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 	}
-
-	const int lq = lane & (LT - 1), part = lane >> 4;
-	for (int lt0 = am; lt0 <= L; lt0 += LT) {
-		double2 acc[NS];
+	// D[i][n] in lane n + 16 (i % 4), element i / 4; row i of parity p is degree s0 + 2i + p
+	if (colv) {
+		double *sd_ = (double *)spec;
 #pragma unroll
-		for (int s = 0; s < NS; s++) acc[s] = make_double2(0.0, 0.0);
-
+		for (int r = 0; r < 4; r++) {
+			const int i = kq + 4 * r;
 #pragma unroll
-		for (int k = 0; k < MAXCH; k++) {
-			if (k >= nch) break;
-			const int j = (wave + k * nw) * 64 + lane;
-			// even / odd combinations of the two hemispheres, Gauss weight folded in
+			for (int p = 0; p < 2; p++) {
+				const int l = s0 + 2 * i + p;
+				const double v = p ? acc1[r] : acc0[r];
+				if (l <= se) {
 #pragma unroll
-			for (int s = 0; s < NS; s++) {
-				double2 e = make_double2(0, 0), o = make_double2(0, 0);
-				if (val[k]) {
-					double2 fn = four[sl[s].fb + j];
-					double2 fs = four[sl[s].fb + (nlat - 1 - j)];
-					e = make_double2(w[k] * (fn.x + fs.x), w[k] * (fn.y + fs.y));
-					o = make_double2(w[k] * (fn.x - fs.x), w[k] * (fn.y - fs.y));
-				}
-				Es[wave][0][lane][s] = e;
-				Es[wave][1][lane][s] = o;
-			}
-			// Pbar tile for degrees lt0 .. lt0+LT-1
-#pragma unroll 4
-			for (int t = 0; t < LT; t++) {
-				const int l = lt0 + t;
-				double P = 0.0;
-				if (l <= L && l >= lmn[k]) {
-					const double2 c = g.coef[cb + l];
-					const double rec = c.x * (mu[k] * p1[k] - c.y * p0[k]);
-					if (l <= lmx[k] + 1) P = (l == ls[k]) ? sd[k].x : ((l == ls[k] + 1) ? sd[k].y : rec);
-					else P = rec;
-					p0[k] = p1[k];
-					p1[k] = P;
-				}
-				Ps[wave][t][lane] = val[k] ? P : 0.0;
-			}
-			__syncthreads();
-			const int par = (lt0 + lq - am) & 1;
-#pragma unroll 4
-			for (int t = 0; t < 16; t++) {
-				const int jl = part * 16 + t;
-				const double P = Ps[wave][lq][jl];
-#pragma unroll
-				for (int s = 0; s < NS; s++) {
-					const double2 e = Es[wave][par][jl][s];
-					acc[s].x = fma(P, e.x, acc[s].x);
-					acc[s].y = fma(P, e.y, acc[s].y);
+					for (int ss = 0; ss < NS; ss++)
+						if (ss == slot && sl[ss].valid)
+							sd_[2 * (sl[ss].qb + Slot<CPLX>::off(l)) + comp] = sl[ss].sgn * v;
 				}
 			}
-			__syncthreads();
 		}
-		// reduce the four ring quarters
-#pragma unroll
-		for (int s = 0; s < NS; s++) {
-			acc[s].x += __shfl_xor(acc[s].x, 16);
-			acc[s].y += __shfl_xor(acc[s].y, 16);
-			acc[s].x += __shfl_xor(acc[s].x, 32);
-			acc[s].y += __shfl_xor(acc[s].y, 32);
-		}
-		if (part == 0) {
-#pragma unroll
-			for (int s = 0; s < NS; s++) Qp[wave][lq][s] = acc[s];
-		}
-		__syncthreads();
-		if (threadIdx.x < LT * NS) {
-			const int q = threadIdx.x & (LT - 1), s = threadIdx.x / LT;
-			const int l = lt0 + q;
-			double2 v = make_double2(0, 0);
-			for (int ww = 0; ww < nw; ww++) {
-				double2 x = Qp[ww][q][s];
-				v.x += x.x;
-				v.y += x.y;
-			}
-			// the slot loop is static, select it here
-#pragma unroll
-			for (int ss = 0; ss < NS; ss++)
-				if (ss == s && l <= L && sl[ss].valid)
-					spec[sl[ss].qb + Slot<CPLX>::off(l)] = make_double2(sl[ss].sgn * v.x, sl[ss].sgn * v.y);
-		}
-		__syncthreads();
 	}
-}
-
-void wave_layout(int NJ, int &nw, int &nch)
-{
-	int nchunks = (NJ + 63) / 64;
-	nw = nchunks < MAXW ? nchunks : MAXW;
-	nch = (nchunks + nw - 1) / nw;
-	nw = (nchunks + nch - 1) / nch;
 }
 
 Geo geo_of(sphswe_ctx *ctx)
@@ -301,6 +460,8 @@ Geo geo_of(sphswe_ctx *ctx)
 	g.coef = ctx->d_coef;
 	g.l0 = ctx->d_l0;
 	g.seed = ctx->d_seed;
+	g.segoff = ctx->d_segoff;
+	g.seg = ctx->seg;
 	g.ct = ctx->d_ct;
 	g.wg = ctx->d_wg;
 	g.L = ctx->L;
@@ -313,8 +474,16 @@ template <bool CPLX, int NS>
 int launch_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf)
 {
 	const int per = CPLX ? NS / 2 : NS;
-	dim3 grid((ctx->NJ + 63) / 64, ctx->L + 1, (nf + per - 1) / per);
-	k_leg_synth<CPLX, NS><<<grid, 64, 0, ctx->stream>>>(spec, four, geo_of(ctx), nf);
+	const int nseg0 = (ctx->L + ctx->seg) / ctx->seg;   // segments of m = 0
+	const int waves = nseg0 < SY_WAVES ? nseg0 : SY_WAVES;
+	const int G = (nseg0 + waves - 1) / waves;
+	const size_t lds = synth_lds_bytes<NS>(ctx->L, waves);
+	if ((long)lds > ctx->max_lds) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "synthesis needs %zu bytes of LDS", lds);
+	if (lds > 65536)
+		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_leg_synth<CPLX, NS>,
+						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	dim3 grid((ctx->NJ + 64 * SY_RPL - 1) / (64 * SY_RPL), ctx->L + 1, (nf + per - 1) / per);
+	k_leg_synth<CPLX, NS><<<grid, 64 * waves, lds, ctx->stream>>>(spec, four, geo_of(ctx), nf, G);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }
@@ -323,11 +492,9 @@ template <bool CPLX, int NS>
 int launch_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf)
 {
 	const int per = CPLX ? NS / 2 : NS;
-	int nw, nch;
-	wave_layout(ctx->NJ, nw, nch);
-	if (nch > MAXCH) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "nlat too large for the analysis kernel");
-	dim3 grid(ctx->L + 1, (nf + per - 1) / per);
-	k_leg_anal<CPLX, NS><<<grid, 64 * nw, 0, ctx->stream>>>(four, spec, geo_of(ctx), nf, nw, nch);
+	const int nunits = ctx->nseg_total;
+	dim3 grid((nunits + AN_WAVES - 1) / AN_WAVES, (nf + per - 1) / per);
+	k_leg_anal<CPLX, NS><<<grid, 64 * AN_WAVES, 0, ctx->stream>>>(four, spec, geo_of(ctx), ctx->d_unit_m, nunits, nf);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }
@@ -348,14 +515,12 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
 	int rc = SPHSWE_OK;
 	int f = 0;
-	// groups of 8 slots amortise the recurrence best
-	if (nf - f >= 8) {
-		int g = (nf - f) / 8;
-		if ((rc = launch_synth<false, 8>(ctx, spec + f * nlm, four + f * nfour, g * 8))) return rc;
-		f += g * 8;
+	if (nf - f >= 4) {
+		int g = (nf - f) / 4;
+		if ((rc = launch_synth<false, 4>(ctx, spec + f * nlm, four + f * nfour, g * 4))) return rc;
+		f += g * 4;
 	}
 	int r = nf - f;
-	if (r >= 4) { if ((rc = launch_synth<false, 4>(ctx, spec + f * nlm, four + f * nfour, 4))) return rc; f += 4; r -= 4; }
 	if (r == 3) rc = launch_synth<false, 3>(ctx, spec + f * nlm, four + f * nfour, 3);
 	else if (r == 2) rc = launch_synth<false, 2>(ctx, spec + f * nlm, four + f * nfour, 2);
 	else if (r == 1) rc = launch_synth<false, 1>(ctx, spec + f * nlm, four + f * nfour, 1);
@@ -369,13 +534,15 @@ int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf,
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
 	int rc = SPHSWE_OK;
 	int f = 0;
-	if (nf - f >= 3) {
-		int g = (nf - f) / 3;
-		if ((rc = launch_anal<false, 3>(ctx, four, spec, g * 3))) return rc;
-		f += g * 3;
+	if (nf >= 8) {
+		int g = nf / 8;
+		if ((rc = launch_anal<false, 8>(ctx, four, spec, g * 8))) return rc;
+		f = g * 8;
 	}
 	int r = nf - f;
-	if (r == 2) rc = launch_anal<false, 2>(ctx, four + f * nfour, spec + f * nlm, 2);
+	if (r > 4) rc = launch_anal<false, 8>(ctx, four + f * nfour, spec + f * nlm, r);
+	else if (r > 2) rc = launch_anal<false, 4>(ctx, four + f * nfour, spec + f * nlm, r);
+	else if (r == 2) rc = launch_anal<false, 2>(ctx, four + f * nfour, spec + f * nlm, 2);
 	else if (r == 1) rc = launch_anal<false, 1>(ctx, four + f * nfour, spec + f * nlm, 1);
 	return rc;
 }
@@ -387,13 +554,12 @@ int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, i
 	const long nfour = (long)(2 * ctx->L + 1) * ctx->nlat;
 	int rc = SPHSWE_OK;
 	int f = 0;
-	if (nf >= 4) {
-		int g = nf / 4;
-		if ((rc = launch_synth<true, 8>(ctx, spec, four, g * 4))) return rc;
-		f = g * 4;
+	if (nf >= 2) {
+		int g = nf / 2;
+		if ((rc = launch_synth<true, 4>(ctx, spec, four, g * 2))) return rc;
+		f = g * 2;
 	}
 	int r = nf - f;
-	if (r >= 2) { if ((rc = launch_synth<true, 4>(ctx, spec + f * nc, four + f * nfour, 2))) return rc; f += 2; r -= 2; }
 	if (r == 1) rc = launch_synth<true, 2>(ctx, spec + f * nc, four + f * nfour, 1);
 	return rc;
 }
@@ -401,7 +567,7 @@ int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, i
 int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
-	return launch_anal<true, 2>(ctx, four, spec, nf);
+	return launch_anal<true, 8>(ctx, four, spec, nf);
 }
 
 extern "C" int sphswe_legendre_analysis(sphswe_ctx *ctx, const double *d_four, double *d_spec, int nfields, int precision)
