@@ -153,6 +153,31 @@ typedef struct {
  * must be spectral (the caller analyses a physical initial condition first). */
 int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmembers, int nsteps);
 
+/* ---- linear SWE by REXI (AppSWE.hpp:336-565; SWERexiSPH.hpp:68-244;
+ *      SPHSolverComplex.hpp:88-511; BandedMatrixSolver.hpp:235-434) */
+typedef struct {
+	double earth_radius;
+	double gravitation;
+	double coriolis_omega;   /* 0: no Coriolis, diagonal Helmholtz solves */
+	double h0;               /* average height; Phi = g h0 */
+	double viscosity;        /* implicit viscosity after each step (0: none) */
+	double dt;               /* must be > 0 */
+	double rexi_h;           /* REXI sampling width (default 0.2) */
+	int rexi_M;              /* REXI approximation area (default 128) */
+	int rexi_half;           /* 1: half the poles (default) */
+} sphswe_rexi_params;
+
+/* REXI::setup for (h, M, half) and SWERexiSPH::setup for every pole: pole
+ * coefficients, band assembly z1..z8 and the pivoted LU factorisation of every
+ * per-m system (factors cached on the device; SPHSWE_ERR_SINGULAR if a pivot
+ * is zero, the reference's zgbsv INFO > 0). */
+int sphswe_rexi_create(sphswe_ctx *ctx, const sphswe_rexi_params *p, sphswe_rexi **rexi);
+int sphswe_rexi_destroy(sphswe_rexi *rexi);
+/* number of poles and, if non-NULL, alpha / beta_re of REXI::setup (re, im pairs) */
+int sphswe_rexi_poles(const sphswe_rexi *rexi, int *npoles, double *alpha, double *beta);
+/* nsteps REXI steps of the state (h, u, v spectral, 3*nlm complex), in place */
+int sphswe_rexi_step(sphswe_ctx *ctx, sphswe_rexi *rexi, double *d_state, int nsteps);
+
 /* ---- timing on the context stream: ms between two sphswe_timer_mark calls */
 int sphswe_timer_mark(sphswe_ctx *ctx, int slot);
 int sphswe_timer_elapsed(sphswe_ctx *ctx, int slot0, int slot1, float *ms);

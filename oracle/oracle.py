@@ -60,6 +60,8 @@ def _load():
         "orc_mat_solve": (ctypes.c_int, [vp, _dp, _dp]),
         "orc_gbsv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _dp]),
         "orc_swe_rexi_run": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp]),
+        "orc_rexi_pole_solve": (None, [vp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
+                                       _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -206,6 +208,18 @@ class Oracle:
         lib().orc_swe_rexi_run(self.h, _d(p), *[_d(x.view(np.float64)) for x in ins], nsteps,
                                *[_d(o.view(np.float64)) for o in outs])
         return outs
+
+    def rexi_pole_solve(self, params, alpha, phi0, u0, v0):
+        """One SWERexiSPH pole (beta = 1, dt = 1, complex result before the real part).
+        params dict: a, g, omega, h0; phi0/u0/v0 real spectral.  Returns complex
+        physical phi, u, v of shape (nphi, nlat)."""
+        p = np.array([params["a"], params["g"], params["omega"], params["h0"]])
+        ins = [np.ascontiguousarray(x, dtype=np.complex128) for x in (phi0, u0, v0)]
+        outs = [np.empty(self.nspat, dtype=np.complex128) for _ in range(3)]
+        lib().orc_rexi_pole_solve(self.h, _d(p), float(np.real(alpha)), float(np.imag(alpha)),
+                                  *[_d(x.view(np.float64)) for x in ins],
+                                  *[_d(o.view(np.float64)) for o in outs])
+        return [o.reshape(self.nphi, self.nlat) for o in outs]
 
 
 class Matrix:

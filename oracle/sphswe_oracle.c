@@ -1297,8 +1297,10 @@ static ofield *helmholtz_c(ofield *x, cplx a, cplx b, double r)
 	return o;
 }
 
-static void pole_solve(orexi_pole *p, ofield *i_phi0, ofield *i_u0, ofield *i_v0,
-		       ofield *o_phi, ofield *o_u, ofield *o_v)
+/* one pole of SWERexiSPH::solve (SWERexiSPH.hpp:137-244) up to, not including,
+ * the beta scaling and the real part: complex phi, u, v */
+static void pole_solve_c(orexi_pole *p, ofield *i_phi0, ofield *i_u0, ofield *i_v0,
+			 ofield **o_phi, ofield **o_u, ofield **o_v)
 {
 	ocfg *c = i_phi0->c;
 	ofield *mu = TC(c);
@@ -1341,12 +1343,52 @@ static void pole_solve(orexi_pole *p, ofield *i_phi0, ofield *i_u0, ofield *i_v0
 		v = scale(add(v0, scale(grad_lat(phi), p->ir)), 1.0 / a);
 	}
 	(void)phi0;
+	*o_phi = phi; *o_u = u; *o_v = v;
+}
+
+static void pole_solve(orexi_pole *p, ofield *i_phi0, ofield *i_u0, ofield *i_v0,
+		       ofield *o_phi, ofield *o_u, ofield *o_v)
+{
+	ofield *phi, *u, *v;
+	pole_solve_c(p, i_phi0, i_u0, i_v0, &phi, &u, &v);
 	iscale(phi, p->beta);
 	iscale(u, p->beta);
 	iscale(v, p->beta);
 	real_part_to(phi, o_phi);
 	real_part_to(u, o_u);
 	real_part_to(v, o_v);
+}
+
+/*
+ * One pole solve for a given alpha (dt = 1, beta = 1), for residual checks.
+ * params: [a, g, omega, h0]; inputs spectral (nlm) real fields phi0 (geopotential),
+ * u0, v0; outputs the complex physical fields phi, u, v (nspat each).
+ */
+void orc_rexi_pole_solve(ocfg *c, const double *params, double alpha_re, double alpha_im,
+			 const cplx *phi_in, const cplx *u_in, const cplx *v_in,
+			 cplx *out_phi, cplx *out_u, cplx *out_v)
+{
+	double a = params[0], g = params[1], omega = params[2], h0 = params[3];
+	orexi_pole pole;
+	memset(&pole, 0, sizeof(pole));
+	pole_setup(&pole, c, alpha_re + I * alpha_im, 1.0, a, omega, h0 * g, 1.0, omega != 0);
+	ofield *f[3];
+	const cplx *in[3] = {phi_in, u_in, v_in};
+	for (int k = 0; k < 3; k++) {
+		f[k] = field_alloc(c, 0, 0);
+		memcpy(f[k]->spec, in[k], sizeof(cplx) * c->nlm);
+		f[k]->spec_valid = 1;
+	}
+	ofield *phi, *u, *v;
+	pole_solve_c(&pole, f[0], f[1], f[2], &phi, &u, &v);
+	req_spat(phi); req_spat(u); req_spat(v);
+	memcpy(out_phi, phi->cspat, sizeof(cplx) * c->nspat);
+	memcpy(out_u, u->cspat, sizeof(cplx) * c->nspat);
+	memcpy(out_v, v->cspat, sizeof(cplx) * c->nspat);
+	arena_reset();
+	for (int k = 0; k < 3; k++) field_free(f[k]);
+	orc_mat_destroy(pole.Mphi);
+	orc_mat_destroy(pole.Mvel);
 }
 
 /*

@@ -32,6 +32,11 @@ class SweParams(ctypes.Structure):
                 ("viscosity", _d), ("dt", _d), ("nonlinear", _i), ("precision", _i)]
 
 
+class RexiParams(ctypes.Structure):
+    _fields_ = [("earth_radius", _d), ("gravitation", _d), ("coriolis_omega", _d), ("h0", _d),
+                ("viscosity", _d), ("dt", _d), ("rexi_h", _d), ("rexi_M", _i), ("rexi_half", _i)]
+
+
 _SIGS = {
     "sphswe_grid_auto": [_i, _i, _ip, _ip],
     "sphswe_create": [_i, _i, _i, _i, _i, ctypes.POINTER(_vp)],
@@ -62,6 +67,10 @@ _SIGS = {
     "sphswe_any_nan": [_vp, _vp, _sz, _ip],
     "sphswe_reduce_minmax": [_vp, _vp, _sz, _dp, _dp, _dp],
     "sphswe_swe_rk4": [_vp, ctypes.POINTER(SweParams), _vp, _i, _i],
+    "sphswe_rexi_create": [_vp, ctypes.POINTER(RexiParams), ctypes.POINTER(_vp)],
+    "sphswe_rexi_destroy": [_vp],
+    "sphswe_rexi_poles": [_vp, _ip, _dp, _dp],
+    "sphswe_rexi_step": [_vp, _vp, _vp, _i],
     "sphswe_timer_mark": [_vp, _i],
     "sphswe_timer_elapsed": [_vp, _i, _i, _fp],
 }
@@ -251,6 +260,10 @@ class Context:
         """state: device buffer holding nmembers*(h,u,v) spectral fields."""
         self._check(self.lib.sphswe_swe_rk4(self.h, ctypes.byref(params), state.ptr, nmembers, nsteps))
 
+    # REXI -------------------------------------------------------------------
+    def rexi(self, params):
+        return Rexi(self, params)
+
     # timing ----------------------------------------------------------------
     def mark(self, slot):
         self._check(self.lib.sphswe_timer_mark(self.h, slot))
@@ -259,6 +272,45 @@ class Context:
         ms = ctypes.c_float()
         self._check(self.lib.sphswe_timer_elapsed(self.h, s0, s1, ctypes.byref(ms)))
         return ms.value
+
+
+class Rexi:
+    """Device REXI integrator (poles, cached band factors) for one context."""
+
+    def __init__(self, ctx, params):
+        self.ctx = ctx
+        h = _vp()
+        ctx._check(ctx.lib.sphswe_rexi_create(ctx.h, ctypes.byref(params), ctypes.byref(h)))
+        self.h = h.value
+        n = _i()
+        ctx._check(ctx.lib.sphswe_rexi_poles(self.h, ctypes.byref(n), None, None))
+        self.npoles = n.value
+
+    def poles(self):
+        a = np.empty(2 * self.npoles)
+        b = np.empty(2 * self.npoles)
+        self.ctx._check(self.ctx.lib.sphswe_rexi_poles(self.h, ctypes.byref(_i()), a.ctypes.data_as(_dp),
+                                                        b.ctypes.data_as(_dp)))
+        return a.view(np.complex128), b.view(np.complex128)
+
+    def step(self, state, nsteps=1):
+        self.ctx._check(self.ctx.lib.sphswe_rexi_step(self.ctx.h, self.h, state.ptr, nsteps))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.sphswe_rexi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rexi_params(a=6.37122e6, g=9.80616, omega=7.292e-5, h0=10000.0, visc=1e5, dt=3600.0, rexi_h=0.2, rexi_M=128,
+                half=1):
+    return RexiParams(a, g, omega, h0, visc, dt, rexi_h, rexi_M, half)
 
 
 def swe_params(a=6.37122e6, g=9.80616, omega=7.292e-5, h0=10000.0, visc=1e5, dt=120.0, nonlinear=1,

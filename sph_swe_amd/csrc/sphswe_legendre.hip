@@ -42,32 +42,43 @@ struct Geo {
 	const int *segoff;
 	const double *ct;
 	const double *wg;
+	const int *moff;        // pole layout: first row of each m block, index m + L
 	int L, nlat, NJ, seg;
+	int P;                  // pole layout: number of poles (row stride)
 };
 
 // slot addressing ------------------------------------------------------------
-// real:    spectral q(l) = spec[f*nlm + m(2L-m+1)/2 + l], Fourier row f*(L+1)*nlat + m*nlat
-// complex: spectral q(l) = spec[f*ncplx + l*l + l + m],   Fourier row f*(2L+1)*nlat + (m+L)*nlat
-template <bool CPLX>
+// LAY 0 real:            q(l) = spec[f*nlm + m(2L-m+1)/2 + l], Fourier row f*(L+1)*nlat + m*nlat
+// LAY 1 complex:         q(l) = spec[f*ncplx + l*l + l + m],   Fourier row f*(2L+1)*nlat + (m+L)*nlat
+// LAY 2 complex, poles:  q(l) = spec[(moff[m+L] + l - |m|)*P + p] (pole fastest), Fourier row as LAY 1
+enum { LAY_REAL = 0, LAY_CPLX = 1, LAY_POLE = 2 };
+
+template <int LAY>
 struct Slot {
 	long qb;      // spectral base
+	long qs;      // spectral stride per degree (LAY_POLE)
 	long fb;      // Fourier row base
 	double sgn;   // (-1)^m for negative complex m
 	bool valid;
-	__device__ static inline long off(int l) { return CPLX ? (long)l * l + l : (long)l; }
+	__device__ inline long q(int l, int am) const
+	{
+		return LAY == LAY_REAL ? qb + l : (LAY == LAY_CPLX ? qb + (long)l * l + l : qb + (long)(l - am) * qs);
+	}
 };
 
-template <bool CPLX, int NS>
-__device__ inline void make_slots(Slot<CPLX> (&sl)[NS], int am, int group, int nfields, int L, int nlat)
+template <int LAY, int NS>
+__device__ inline void make_slots(Slot<LAY> (&sl)[NS], int am, int group, int nfields, const Geo &g)
 {
+	const int L = g.L, nlat = g.nlat;
 	const long nlm = (long)(L + 1) * (L + 2) / 2, ncplx = (long)(L + 1) * (L + 1);
 #pragma unroll
 	for (int s = 0; s < NS; s++) {
-		if (!CPLX) {
+		if (LAY == LAY_REAL) {
 			int f = group * NS + s;
 			sl[s].valid = f < nfields;
 			if (f >= nfields) f = nfields - 1;
 			sl[s].qb = f * nlm + lm_index(L, 0, am);
+			sl[s].qs = 1;
 			sl[s].fb = (long)f * (L + 1) * nlat + (long)am * nlat;
 			sl[s].sgn = 1.0;
 		} else {
@@ -75,7 +86,13 @@ __device__ inline void make_slots(Slot<CPLX> (&sl)[NS], int am, int group, int n
 			int m = (s & 1) ? -am : am;
 			sl[s].valid = f < nfields && !((s & 1) && am == 0);
 			if (f >= nfields) f = nfields - 1;
-			sl[s].qb = f * ncplx + m;
+			if (LAY == LAY_CPLX) {
+				sl[s].qb = f * ncplx + m;
+				sl[s].qs = 1;
+			} else {
+				sl[s].qb = (long)g.moff[m + L] * g.P + f;
+				sl[s].qs = g.P;
+			}
 			sl[s].fb = (long)f * (2 * L + 1) * nlat + (long)(m + L) * nlat;
 			sl[s].sgn = ((s & 1) && (am & 1)) ? -1.0 : 1.0;
 		}
@@ -109,9 +126,10 @@ __host__ __device__ inline size_t synth_lds_bytes(int L, int waves)
 	return cq > red ? cq : red;
 }
 
-template <bool CPLX, int NS>
+template <int LAY, int NS>
 __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__restrict__ spec,
-							     double2 *__restrict__ four, Geo g, int nfields, int G)
+							     double2 *__restrict__ four, Geo g, int nfields, int G,
+							     const double *__restrict__ oscale)
 {
 	extern __shared__ __align__(16) unsigned char lds_raw[];
 	double2 *cq = (double2 *)lds_raw;    // [(l - m)][NS + 1]: coef, q of each slot
@@ -130,8 +148,8 @@ __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__re
 	}
 	const int so = g.segoff[am];
 	const int nseg = g.segoff[am + 1] - so;
-	Slot<CPLX> sl[NS];
-	make_slots<CPLX, NS>(sl, am, blockIdx.z, nfields, L, g.nlat);
+	Slot<LAY> sl[NS];
+	make_slots<LAY, NS>(sl, am, blockIdx.z, nfields, g);
 
 	// stage the row: every thread issues its loads back to back (clamped
 	// addresses), then writes them to LDS
@@ -146,10 +164,10 @@ __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__re
 			const int ic = idx < ntot ? idx : ntot - 1;
 			const int t = ic / (NS + 1), sidx = ic - t * (NS + 1);
 			const int l = am + t;
-			long qsel = sl[0].qb + Slot<CPLX>::off(l);
+			long qsel = sl[0].q(l, am);
 #pragma unroll
 			for (int ss = 1; ss < NS; ss++)
-				if (ss == sidx - 1) qsel = sl[ss].qb + Slot<CPLX>::off(l);
+				if (ss == sidx - 1) qsel = sl[ss].q(l, am);
 			const double2 *src = (sidx == 0) ? g.coef + lm_index(L, l, am) : spec + qsel;
 			v[u] = *src;
 		}
@@ -296,9 +314,10 @@ __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__re
 #pragma unroll
 			for (int s = 0; s < NS; s++) {
 				if (!sl[s].valid) continue;
-				double2 nn = make_double2(sl[s].sgn * (eR[r][s] + oR[r][s]), sl[s].sgn * (eI[r][s] + oI[r][s]));
-				double2 ss2 = make_double2(sl[s].sgn * (eR[r][s] - oR[r][s]), sl[s].sgn * (eI[r][s] - oI[r][s]));
-				if (!CPLX && am == 0) { nn.y = 0.0; ss2.y = 0.0; }  // Im q_l0 is ignored
+				const double sc = oscale ? sl[s].sgn * oscale[j] : sl[s].sgn;   // oscale symmetric in j
+				double2 nn = make_double2(sc * (eR[r][s] + oR[r][s]), sc * (eI[r][s] + oI[r][s]));
+				double2 ss2 = make_double2(sc * (eR[r][s] - oR[r][s]), sc * (eI[r][s] - oI[r][s]));
+				if (!(LAY != LAY_REAL) && am == 0) { nn.y = 0.0; ss2.y = 0.0; }  // Im q_l0 is ignored
 				four[sl[s].fb + j] = nn;
 				four[sl[s].fb + js] = ss2;
 			}
@@ -326,7 +345,7 @@ constexpr int AN_PS = 65;         // LDS row stride (doubles) of the Pbar tile
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-template <bool CPLX, int NS>
+template <int LAY, int NS>
 __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__restrict__ four, double2 *__restrict__ spec,
 							    Geo g, const int *__restrict__ unit_m, int nunits,
 							    int nfields)
@@ -341,8 +360,8 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__res
 	const int so = g.segoff[am];
 	const int kseg = u - so;
 	const int s0 = am + kseg * AN_SEG, se = min(s0 + AN_SEG - 1, L);
-	Slot<CPLX> sl[NS];
-	make_slots<CPLX, NS>(sl, am, blockIdx.y, nfields, L, nlat);
+	Slot<LAY> sl[NS];
+	make_slots<LAY, NS>(sl, am, blockIdx.y, nfields, g);
 
 	// recurrence coefficients of the segment, read back as LDS broadcasts
 	double2 *cf = Cfs[wave];
@@ -447,7 +466,7 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__res
 #pragma unroll
 					for (int ss = 0; ss < NS; ss++)
 						if (ss == slot && sl[ss].valid)
-							sd_[2 * (sl[ss].qb + Slot<CPLX>::off(l)) + comp] = sl[ss].sgn * v;
+							sd_[2 * sl[ss].q(l, am) + comp] = sl[ss].sgn * v;
 				}
 			}
 		}
@@ -464,37 +483,39 @@ Geo geo_of(sphswe_ctx *ctx)
 	g.seg = ctx->seg;
 	g.ct = ctx->d_ct;
 	g.wg = ctx->d_wg;
+	g.moff = nullptr;
 	g.L = ctx->L;
 	g.nlat = ctx->nlat;
 	g.NJ = ctx->NJ;
+	g.P = 0;
 	return g;
 }
 
-template <bool CPLX, int NS>
-int launch_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf)
+template <int LAY, int NS>
+int launch_synth(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 *four, int nf, const double *oscale)
 {
-	const int per = CPLX ? NS / 2 : NS;
+	const int per = (LAY != LAY_REAL) ? NS / 2 : NS;
 	const int nseg0 = (ctx->L + ctx->seg) / ctx->seg;   // segments of m = 0
 	const int waves = nseg0 < SY_WAVES ? nseg0 : SY_WAVES;
 	const int G = (nseg0 + waves - 1) / waves;
 	const size_t lds = synth_lds_bytes<NS>(ctx->L, waves);
 	if ((long)lds > ctx->max_lds) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "synthesis needs %zu bytes of LDS", lds);
 	if (lds > 65536)
-		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_leg_synth<CPLX, NS>,
+		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_leg_synth<LAY, NS>,
 						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 	dim3 grid((ctx->NJ + 64 * SY_RPL - 1) / (64 * SY_RPL), ctx->L + 1, (nf + per - 1) / per);
-	k_leg_synth<CPLX, NS><<<grid, 64 * waves, lds, ctx->stream>>>(spec, four, geo_of(ctx), nf, G);
+	k_leg_synth<LAY, NS><<<grid, 64 * waves, lds, ctx->stream>>>(spec, four, g, nf, G, oscale);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }
 
-template <bool CPLX, int NS>
-int launch_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf)
+template <int LAY, int NS>
+int launch_anal(sphswe_ctx *ctx, const Geo &g, const double2 *four, double2 *spec, int nf)
 {
-	const int per = CPLX ? NS / 2 : NS;
+	const int per = (LAY != LAY_REAL) ? NS / 2 : NS;
 	const int nunits = ctx->nseg_total;
 	dim3 grid((nunits + AN_WAVES - 1) / AN_WAVES, (nf + per - 1) / per);
-	k_leg_anal<CPLX, NS><<<grid, 64 * AN_WAVES, 0, ctx->stream>>>(four, spec, geo_of(ctx), ctx->d_unit_m, nunits, nf);
+	k_leg_anal<LAY, NS><<<grid, 64 * AN_WAVES, 0, ctx->stream>>>(four, spec, g, ctx->d_unit_m, nunits, nf);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }
@@ -517,13 +538,13 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 	int f = 0;
 	if (nf - f >= 4) {
 		int g = (nf - f) / 4;
-		if ((rc = launch_synth<false, 4>(ctx, spec + f * nlm, four + f * nfour, g * 4))) return rc;
+		if ((rc = launch_synth<LAY_REAL, 4>(ctx, geo_of(ctx), spec + f * nlm, four + f * nfour, g * 4, nullptr))) return rc;
 		f += g * 4;
 	}
 	int r = nf - f;
-	if (r == 3) rc = launch_synth<false, 3>(ctx, spec + f * nlm, four + f * nfour, 3);
-	else if (r == 2) rc = launch_synth<false, 2>(ctx, spec + f * nlm, four + f * nfour, 2);
-	else if (r == 1) rc = launch_synth<false, 1>(ctx, spec + f * nlm, four + f * nfour, 1);
+	if (r == 3) rc = launch_synth<LAY_REAL, 3>(ctx, geo_of(ctx), spec + f * nlm, four + f * nfour, 3, nullptr);
+	else if (r == 2) rc = launch_synth<LAY_REAL, 2>(ctx, geo_of(ctx), spec + f * nlm, four + f * nfour, 2, nullptr);
+	else if (r == 1) rc = launch_synth<LAY_REAL, 1>(ctx, geo_of(ctx), spec + f * nlm, four + f * nfour, 1, nullptr);
 	return rc;
 }
 
@@ -536,14 +557,14 @@ int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf,
 	int f = 0;
 	if (nf >= 8) {
 		int g = nf / 8;
-		if ((rc = launch_anal<false, 8>(ctx, four, spec, g * 8))) return rc;
+		if ((rc = launch_anal<LAY_REAL, 8>(ctx, geo_of(ctx), four, spec, g * 8))) return rc;
 		f = g * 8;
 	}
 	int r = nf - f;
-	if (r > 4) rc = launch_anal<false, 8>(ctx, four + f * nfour, spec + f * nlm, r);
-	else if (r > 2) rc = launch_anal<false, 4>(ctx, four + f * nfour, spec + f * nlm, r);
-	else if (r == 2) rc = launch_anal<false, 2>(ctx, four + f * nfour, spec + f * nlm, 2);
-	else if (r == 1) rc = launch_anal<false, 1>(ctx, four + f * nfour, spec + f * nlm, 1);
+	if (r > 4) rc = launch_anal<LAY_REAL, 8>(ctx, geo_of(ctx), four + f * nfour, spec + f * nlm, r);
+	else if (r > 2) rc = launch_anal<LAY_REAL, 4>(ctx, geo_of(ctx), four + f * nfour, spec + f * nlm, r);
+	else if (r == 2) rc = launch_anal<LAY_REAL, 2>(ctx, geo_of(ctx), four + f * nfour, spec + f * nlm, 2);
+	else if (r == 1) rc = launch_anal<LAY_REAL, 1>(ctx, geo_of(ctx), four + f * nfour, spec + f * nlm, 1);
 	return rc;
 }
 
@@ -556,18 +577,37 @@ int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, i
 	int f = 0;
 	if (nf >= 2) {
 		int g = nf / 2;
-		if ((rc = launch_synth<true, 4>(ctx, spec, four, g * 2))) return rc;
+		if ((rc = launch_synth<LAY_CPLX, 4>(ctx, geo_of(ctx), spec, four, g * 2, nullptr))) return rc;
 		f = g * 2;
 	}
 	int r = nf - f;
-	if (r == 1) rc = launch_synth<true, 2>(ctx, spec + f * nc, four + f * nfour, 1);
+	if (r == 1) rc = launch_synth<LAY_CPLX, 2>(ctx, geo_of(ctx), spec + f * nc, four + f * nfour, 1, nullptr);
 	return rc;
 }
 
 int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
-	return launch_anal<true, 8>(ctx, four, spec, nf);
+	return launch_anal<LAY_CPLX, 8>(ctx, geo_of(ctx), four, spec, nf);
+}
+
+// pole layout (REXI): nfields columns, row stride P, m-block offsets moff;
+// synthesis output multiplied by oscale[j] when given
+int sphswe_leg_synth_pole(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
+			  const double *oscale)
+{
+	Geo g = geo_of(ctx);
+	g.moff = d_moff;
+	g.P = P;
+	return launch_synth<LAY_POLE, 4>(ctx, g, spec, four, nf, oscale);
+}
+
+int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff)
+{
+	Geo g = geo_of(ctx);
+	g.moff = d_moff;
+	g.P = P;
+	return launch_anal<LAY_POLE, 8>(ctx, g, four, spec, nf);
 }
 
 extern "C" int sphswe_legendre_analysis(sphswe_ctx *ctx, const double *d_four, double *d_spec, int nfields, int precision)

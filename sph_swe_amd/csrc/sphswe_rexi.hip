@@ -1,0 +1,721 @@
+// Linear SWE time step by REXI (AppSWE.hpp:336-565 with SWERexiSPH::solve,
+// src/SWERexiSPH.hpp:137-244), batched over all poles.
+//
+// Per time step:
+//  1. pole-independent fields, following the reference's projection graph with
+//     the GPU complex SHT: div0, eta0 (div/vort of u0, v0 with their metric
+//     divisions), phi0 in complex form, the Coriolis forcing pieces
+//     W1..W3 = A(g * S(u0 | mu2 u0 | mu v0)) with g = S((2 Omega / a) A(grad_lat mu)),
+//     and their mu / mu2 images;
+//  2. one kernel over (m, pole): assemble the phi right-hand side
+//        rhs = a^2 foo + (2W)^2 mu2(foo) + (Phi/a) Fc     (SWERexiSPH.hpp:176-195)
+//     from those fields and the pole scalars, solve the 9-diagonal system with
+//     the cached pivoted LU factors, emit i m phi and D' phi for the gradients,
+//     and reduce beta*phi over the poles of the wavefront;
+//  3. grad_lon / grad_lat of phi = A(S(.) / sqrt(1-mu^2)) per m, done without
+//     FFT: Legendre synthesis to (m, ring) space, scaling by 1/sqrt(1-mu^2),
+//     Legendre analysis (the FFT pair would be an identity on one m);
+//  4. one kernel over (m, pole): a = u0 + G_lon/r, b = v0 + G_lat/r,
+//     rhs_a = alpha a - 2W mu(b), rhs_b = 2W mu(a) + alpha b, the two
+//     5-diagonal solves, reduction of beta*u, beta*v;
+//  5. sum of the pole partials (FP64, fixed order), real part through the
+//     complex/real SHT pair, h = phi/g, implicit viscosity.
+//
+// Layout of per-pole data ("pole layout"): row = moff[m+L] + (n - |m|) in the
+// reference's NCompact order (SPHConfig.hpp:136-156), pole fastest, so the
+// threads of a wavefront (64 poles of one m) read consecutive addresses.
+#include "sphswe_band.h"
+#include "sphswe_common.h"
+
+#include <algorithm>
+#include <cstring>
+#include <thread>
+
+using sphswe_host::cplx;
+
+struct sphswe_rexi {
+	sphswe_ctx *ctx = nullptr;
+	sphswe_rexi_params prm;
+	int P = 0;
+	int coriolis = 0;
+	std::vector<cplx> alpha0, beta0;   // as returned by REXI::setup
+	double2 *d_alpha = nullptr, *d_beta = nullptr;   // divided by dt (SWERexiSPH.hpp:82-83)
+	int *d_moff = nullptr;
+	double2 *d_fphi = nullptr, *d_fvel = nullptr;
+	unsigned char *d_pphi = nullptr, *d_pvel = nullptr;
+	double *d_oscale = nullptr;        // 1 / sqrt(1 - mu^2)
+	// work
+	double2 *d_X = nullptr, *d_G = nullptr, *d_F = nullptr, *d_Y = nullptr;
+	double2 *d_part = nullptr, *d_basis = nullptr, *d_acc = nullptr;
+	double *d_phys = nullptr;
+	double2 *d_cphys = nullptr, *d_cspec = nullptr;
+	int ngroups = 0;
+	size_t rows = 0;
+};
+
+namespace {
+
+constexpr int NBASIS = 12;   // B1..B10, u0c, v0c
+enum { B_DIV = 0, B_MUETA, B_PHI, B_MU2PHI, B_MU2DIV, B_MU2MUETA, B_MU2MU2PHI, B_W1, B_W2, B_W3, B_U0, B_V0 };
+
+__device__ inline double2 cm(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cs(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ inline double2 ca(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 cb(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ inline double2 cinv(double2 a)
+{
+	const double d = a.x * a.x + a.y * a.y;
+	return make_double2(a.x / d, -a.y / d);
+}
+
+__device__ inline double2 wave_sum(double2 v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		v.x += __shfl_xor(v.x, o);
+		v.y += __shfl_xor(v.y, o);
+	}
+	return v;
+}
+
+// m of workgroup row y: 0, -1, 1, -2, 2, ... (longest systems first)
+__device__ inline int m_of(int y) { return (y & 1) ? -((y + 1) >> 1) : (y >> 1); }
+
+// swap window[0] <-> window[k] for a runtime k in [0, K]
+template <int K>
+__device__ inline void pivot_swap(double2 (&w)[K + 1], int k)
+{
+	double2 t = w[0];
+#pragma unroll
+	for (int i = 1; i <= K; i++) t = (k == i) ? w[i] : t;
+#pragma unroll
+	for (int i = 1; i <= K; i++)
+		if (k == i) w[i] = w[0];
+	w[0] = t;
+}
+
+struct PhiArgs {
+	const double2 *alpha, *beta;
+	const int *moff;
+	const double2 *fac;
+	const unsigned char *piv;
+	const double2 *basis;     // NBASIS complex spectral fields, standard layout
+	double2 *Y;               // forward-substitution scratch, pole layout
+	double2 *X;               // [rows][2P]: i m phi | D' phi
+	double2 *part;            // [group][3][rows]: pole partial sums
+	int L, P, ncplx;
+	double two_omega, avg_geo;
+	int coriolis;
+	size_t rows;
+};
+
+// phi: assemble the right-hand side, solve, emit gradient inputs, reduce beta*phi
+template <int KL>
+__global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs a)
+{
+	constexpr int KV = 2 * KL, NE = 1 + KV + KL;
+	const int m = m_of(blockIdx.y);
+	const int am = abs(m);
+	const int L = a.L, P = a.P;
+	const int N = L + 1 - am;
+	const int p = blockIdx.x * 64 + threadIdx.x;
+	const bool pv = p < P;
+	const int pc = pv ? p : P - 1;
+	const size_t row0 = a.moff[m + L];
+	const double2 al = a.alpha[pc], be = a.beta[pc];
+	const double2 ial = cinv(al);
+	const double G = a.avg_geo, W = a.two_omega, W2 = W * W;
+	const double2 al2 = cm(al, al);
+	const double2 *B = a.basis;
+
+	auto rhs_at = [&](int j) -> double2 {
+		const int n = am + j;
+		const long k = (long)n * n + n + m;
+		const double2 b1 = B[B_DIV * (long)a.ncplx + k], b3 = B[B_PHI * (long)a.ncplx + k];
+		if (!a.coriolis) {
+			// rhs = Phi div0 + alpha phi0 (SWERexiSPH.hpp:230)
+			return ca(cs(b1, G), cm(al, b3));
+		}
+		const double2 b2 = B[B_MUETA * (long)a.ncplx + k], b4 = B[B_MU2PHI * (long)a.ncplx + k];
+		const double2 b5 = B[B_MU2DIV * (long)a.ncplx + k], b6 = B[B_MU2MUETA * (long)a.ncplx + k];
+		const double2 b7 = B[B_MU2MU2PHI * (long)a.ncplx + k];
+		const double2 w1 = B[B_W1 * (long)a.ncplx + k], w2 = B[B_W2 * (long)a.ncplx + k];
+		const double2 w3 = B[B_W3 * (long)a.ncplx + k];
+		const double2 wa = cs(ial, W), w2a = cs(ial, W2);
+		// foo = G (div0 - (W/alpha) mu(eta0)) + (alpha phi0 + (W^2/alpha) mu2(phi0))
+		const double2 foo = ca(cs(cb(b1, cm(wa, b2)), G), ca(cm(al, b3), cm(w2a, b4)));
+		const double2 mfoo = ca(cs(cb(b5, cm(wa, b6)), G), ca(cm(al, b4), cm(w2a, b7)));
+		// Fc = -alpha^2 W1 + W^2 W2 + 2 alpha W W3 (the W's carry the factor W/a)
+		const double2 fc = ca(cb(cs(w2, W2), cm(al2, w1)), cm(cs(al, 2.0 * W), w3));
+		return ca(ca(cm(al2, foo), cs(mfoo, W2)), cm(cs(ial, G), fc));
+	};
+
+	double2 phi[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};   // phi(j), phi(j+1), phi(j+2)
+	auto emit = [&](int j, double2 x_j, double2 x_jm1) {
+		// D' phi at row j needs phi(j-1) and phi(j+1)
+		const int n = am + j;
+		const double dn = n, dm = m;
+		const double c1 = (-dn + 1.0) * idR(dn - 1, dm), c2 = (dn + 2.0) * idS(dn + 1, dm);
+		const double2 d = ca(cs(x_jm1, c1), cs(phi[1], c2));
+		(void)x_j;
+		if (pv) a.X[(row0 + j) * 2 * P + P + p] = d;
+	};
+
+	if (!a.coriolis) {
+		// diagonal Helmholtz: phi = rhs / (alpha^2 + (Phi / r^2) n (n+1)) is done by
+		// the caller through factors holding 1 / diag; rows independent
+		for (int j = 0; j < N; j++) {
+			const double2 x = cm(rhs_at(j), a.fac[(row0 + j) * (size_t)NE * P + pc]);
+			if (pv) a.X[(row0 + j) * 2 * P + p] = make_double2(-m * x.y, m * x.x);
+			a.Y[(row0 + j) * P + pc] = x;
+			double2 bx = wave_sum(pv ? cm(be, x) : make_double2(0, 0));
+			if (threadIdx.x == 0) a.part[((size_t)blockIdx.x * 3 + 0) * a.rows + row0 + j] = bx;
+		}
+		// D' phi from the stored phi
+		for (int j = 0; j < N; j++) {
+			const int n = am + j;
+			const double dn = n, dm = m;
+			const double c1 = (-dn + 1.0) * idR(dn - 1, dm), c2 = (dn + 2.0) * idS(dn + 1, dm);
+			const double2 xm = (j > 0) ? a.Y[(row0 + j - 1) * P + pc] : make_double2(0, 0);
+			const double2 xp = (j + 1 < N) ? a.Y[(row0 + j + 1) * P + pc] : make_double2(0, 0);
+			if (pv) a.X[(row0 + j) * 2 * P + P + p] = ca(cs(xm, c1), cs(xp, c2));
+		}
+		return;
+	}
+
+	// forward substitution with row interchanges (L of the pivoted LU)
+	double2 w[KL + 1];
+#pragma unroll
+	for (int i = 0; i <= KL; i++) w[i] = (i < N) ? rhs_at(i) : make_double2(0, 0);
+	for (int j = 0; j < N; j++) {
+		const double2 *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+		const int k = a.piv[(row0 + j) * P + pc];
+		pivot_swap<KL>(w, k);
+#pragma unroll
+		for (int i = 1; i <= KL; i++) w[i] = cb(w[i], cm(f[(size_t)(KV + i) * P], w[0]));
+		a.Y[(row0 + j) * P + pc] = w[0];
+#pragma unroll
+		for (int i = 0; i < KL; i++) w[i] = w[i + 1];
+		w[KL] = (j + KL + 1 < N) ? rhs_at(j + KL + 1) : make_double2(0, 0);
+	}
+	// backward substitution with U (KV super-diagonals)
+	double2 v[KV + 1];
+#pragma unroll
+	for (int i = 0; i <= KV; i++) v[i] = (N - 1 - i >= 0) ? a.Y[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
+	for (int j = N - 1; j >= 0; j--) {
+		const double2 *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+		const double2 x = cm(v[0], f[0]);
+#pragma unroll
+		for (int i = 1; i <= KV; i++) v[i] = cb(v[i], cm(f[(size_t)i * P], x));
+		// outputs: i m phi at j, D' phi at j+1 (needs phi(j) and phi(j+2))
+		if (pv) a.X[(row0 + j) * 2 * P + p] = make_double2(-m * x.y, m * x.x);
+		phi[2] = phi[1];
+		phi[1] = phi[0];
+		phi[0] = x;
+		if (j + 1 < N) {
+			// row j+1: phi(j) = x, phi(j+2) = phi[2]
+			const int n = am + j + 1;
+			const double dn = n, dm = m;
+			const double c1 = (-dn + 1.0) * idR(dn - 1, dm), c2 = (dn + 2.0) * idS(dn + 1, dm);
+			if (pv) a.X[(row0 + j + 1) * 2 * P + P + p] = ca(cs(x, c1), cs(phi[2], c2));
+		}
+		double2 bx = wave_sum(pv ? cm(be, x) : make_double2(0, 0));
+		if (threadIdx.x == 0) a.part[((size_t)blockIdx.x * 3 + 0) * a.rows + row0 + j] = bx;
+#pragma unroll
+		for (int i = 0; i < KV; i++) v[i] = v[i + 1];
+		v[KV] = (j - KV - 1 >= 0) ? a.Y[(row0 + j - KV - 1) * P + pc] : make_double2(0, 0);
+	}
+	// row 0: phi(-1) = 0, phi(1) = phi[1]
+	{
+		const int n = am;
+		const double dn = n, dm = m;
+		const double c2 = (dn + 2.0) * idS(dn + 1, dm);
+		if (pv) a.X[row0 * 2 * P + P + p] = cs(phi[1], c2);
+	}
+	(void)emit;
+}
+
+struct VelArgs {
+	const double2 *alpha, *beta;
+	const int *moff;
+	const double2 *fac;
+	const unsigned char *piv;
+	const double2 *basis;
+	const double2 *G;         // [rows][2P]: G_lon | G_lat
+	double2 *Y;               // scratch [2][rows][P]
+	double2 *part;
+	int L, P, ncplx;
+	double two_omega, ir;
+	int coriolis;
+	size_t rows;
+};
+
+template <int KL>
+__global__ __launch_bounds__(64) void k_rexi_vel(VelArgs a)
+{
+	constexpr int KV = 2 * KL, NE = 1 + KV + KL;
+	const int m = m_of(blockIdx.y);
+	const int am = abs(m);
+	const int L = a.L, P = a.P;
+	const int N = L + 1 - am;
+	const int p = blockIdx.x * 64 + threadIdx.x;
+	const bool pv = p < P;
+	const int pc = pv ? p : P - 1;
+	const size_t row0 = a.moff[m + L];
+	const double2 al = a.alpha[pc], be = a.beta[pc];
+	const double W = a.two_omega;
+	const double2 *B = a.basis;
+
+	// a(n) = u0 + G_lon / r, b(n) = v0 + G_lat / r
+	auto ab_at = [&](int j, double2 &av, double2 &bv) {
+		if (j < 0 || j >= N) { av = bv = make_double2(0, 0); return; }
+		const int n = am + j;
+		const long k = (long)n * n + n + m;
+		const double2 gl = a.G[(row0 + j) * 2 * P + pc], gt = a.G[(row0 + j) * 2 * P + P + pc];
+		av = ca(B[B_U0 * (long)a.ncplx + k], cs(gl, a.ir));
+		bv = ca(B[B_V0 * (long)a.ncplx + k], cs(gt, a.ir));
+	};
+	const double2 ial = cinv(al);
+	if (!a.coriolis) {
+		// u = (1/alpha) (u0 + ir grad_lon phi), v likewise (SWERexiSPH.hpp:232-233)
+		for (int j = 0; j < N; j++) {
+			double2 av, bv;
+			ab_at(j, av, bv);
+			const double2 u = cm(ial, av), v = cm(ial, bv);
+			double2 su = wave_sum(pv ? cm(be, u) : make_double2(0, 0));
+			double2 sv = wave_sum(pv ? cm(be, v) : make_double2(0, 0));
+			if (threadIdx.x == 0) {
+				a.part[((size_t)blockIdx.x * 3 + 1) * a.rows + row0 + j] = su;
+				a.part[((size_t)blockIdx.x * 3 + 2) * a.rows + row0 + j] = sv;
+			}
+		}
+		return;
+	}
+	// rhs_a = alpha a - W mu(b), rhs_b = W mu(a) + alpha b, mu = R(n-1) x_{n-1} + S(n+1) x_{n+1}
+	double2 aw[3], bw[3];   // (j-1, j, j+1)
+	aw[0] = bw[0] = make_double2(0, 0);
+	ab_at(0, aw[1], bw[1]);
+	ab_at(1, aw[2], bw[2]);
+	auto rhs_next = [&](int j, double2 &ra, double2 &rb) {
+		// uses aw/bw = values at (j-1, j, j+1), then shifts in j+2
+		const int n = am + j;
+		const double dn = n, dm = m;
+		const double r = idR(dn - 1, dm), s = idS(dn + 1, dm);
+		const double2 mub = ca(cs(bw[0], r), cs(bw[2], s));
+		const double2 mua = ca(cs(aw[0], r), cs(aw[2], s));
+		ra = cb(cm(al, aw[1]), cs(mub, W));
+		rb = ca(cs(mua, W), cm(al, bw[1]));
+		aw[0] = aw[1]; aw[1] = aw[2];
+		bw[0] = bw[1]; bw[1] = bw[2];
+		ab_at(j + 2, aw[2], bw[2]);
+	};
+	double2 wa[KL + 1], wb[KL + 1];
+#pragma unroll
+	for (int i = 0; i <= KL; i++) {
+		if (i < N) rhs_next(i, wa[i], wb[i]);
+		else wa[i] = wb[i] = make_double2(0, 0);
+	}
+	int jn = KL + 1;   // next rhs row to produce
+	double2 *Ya = a.Y, *Yb = a.Y + a.rows * P;
+	for (int j = 0; j < N; j++) {
+		const double2 *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+		const int k = a.piv[(row0 + j) * P + pc];
+		pivot_swap<KL>(wa, k);
+		pivot_swap<KL>(wb, k);
+#pragma unroll
+		for (int i = 1; i <= KL; i++) {
+			const double2 lf = f[(size_t)(KV + i) * P];
+			wa[i] = cb(wa[i], cm(lf, wa[0]));
+			wb[i] = cb(wb[i], cm(lf, wb[0]));
+		}
+		Ya[(row0 + j) * P + pc] = wa[0];
+		Yb[(row0 + j) * P + pc] = wb[0];
+#pragma unroll
+		for (int i = 0; i < KL; i++) { wa[i] = wa[i + 1]; wb[i] = wb[i + 1]; }
+		if (jn < N) { rhs_next(jn, wa[KL], wb[KL]); jn++; }
+		else wa[KL] = wb[KL] = make_double2(0, 0);
+	}
+	double2 va[KV + 1], vb[KV + 1];
+#pragma unroll
+	for (int i = 0; i <= KV; i++) {
+		const bool in = N - 1 - i >= 0;
+		va[i] = in ? Ya[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
+		vb[i] = in ? Yb[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
+	}
+	for (int j = N - 1; j >= 0; j--) {
+		const double2 *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+		const double2 xa = cm(va[0], f[0]), xb = cm(vb[0], f[0]);
+#pragma unroll
+		for (int i = 1; i <= KV; i++) {
+			const double2 uf = f[(size_t)i * P];
+			va[i] = cb(va[i], cm(uf, xa));
+			vb[i] = cb(vb[i], cm(uf, xb));
+		}
+		double2 su = wave_sum(pv ? cm(be, xa) : make_double2(0, 0));
+		double2 sv = wave_sum(pv ? cm(be, xb) : make_double2(0, 0));
+		if (threadIdx.x == 0) {
+			a.part[((size_t)blockIdx.x * 3 + 1) * a.rows + row0 + j] = su;
+			a.part[((size_t)blockIdx.x * 3 + 2) * a.rows + row0 + j] = sv;
+		}
+#pragma unroll
+		for (int i = 0; i < KV; i++) { va[i] = va[i + 1]; vb[i] = vb[i + 1]; }
+		const bool in = j - KV - 1 >= 0;
+		va[KV] = in ? Ya[(row0 + j - KV - 1) * P + pc] : make_double2(0, 0);
+		vb[KV] = in ? Yb[(row0 + j - KV - 1) * P + pc] : make_double2(0, 0);
+	}
+}
+
+// sum the pole-group partials (fixed order) into 3 complex spectra, standard layout
+__global__ void k_rexi_sum(const double2 *__restrict__ part, double2 *__restrict__ out, const int *__restrict__ lofc,
+			   const int *__restrict__ moff, int L, int ngroups, size_t rows, int ncplx)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= ncplx) return;
+	const int l = lofc[i], m = i - l * l - l;
+	const size_t row = moff[m + L] + (l - abs(m));
+#pragma unroll
+	for (int f = 0; f < 3; f++) {
+		double2 s = make_double2(0, 0);
+		for (int g = 0; g < ngroups; g++) {
+			const double2 v = part[((size_t)g * 3 + f) * rows + row];
+			s.x += v.x;
+			s.y += v.y;
+		}
+		out[(size_t)f * ncplx + i] = s;
+	}
+}
+
+__global__ void k_real_to_cplx(const double *__restrict__ a, double2 *__restrict__ o, long n)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i < n) o[i] = make_double2(a[i], 0.0);
+}
+__global__ void k_cplx_real_part(const double2 *__restrict__ a, double *__restrict__ o, long n)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i < n) o[i] = a[i].x;
+}
+__global__ void k_fill_mu(double2 *__restrict__ o, const double *__restrict__ ct, int nlat, long n)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i < n) o[i] = make_double2(ct[i % nlat], 0.0);
+}
+// sum / difference of spectral pairs with a scale: o = s*(x + sgn*y)
+__global__ void k_comb(const double2 *__restrict__ x, const double2 *__restrict__ y, double2 *__restrict__ o, double sgn,
+		       double s, long n)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i < n) o[i] = make_double2(s * (x[i].x + sgn * y[i].x), s * (x[i].y + sgn * y[i].y));
+}
+// o = g * x (complex pointwise, g one field, x nf fields)
+__global__ void k_cmul_bcast(const double2 *__restrict__ g, const double2 *__restrict__ x, double2 *__restrict__ o,
+			     long nspat, int nf)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= nspat * nf) return;
+	const double2 a = g[i % nspat], b = x[i];
+	o[i] = make_double2(b.x * a.x - b.y * a.y, b.x * a.y + b.y * a.x);
+}
+
+inline int g256(long n) { return (int)((n + 255) / 256); }
+
+template <typename T>
+int dev_alloc(sphswe_ctx *ctx, T **p, size_t n)
+{
+	if (hipMalloc((void **)p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess)
+		return sphswe_set_err(ctx, SPHSWE_ERR_NOMEM, "rexi: allocation of %zu bytes failed", sizeof(T) * n);
+	return SPHSWE_OK;
+}
+
+}  // namespace
+
+extern "C" int sphswe_rexi_destroy(sphswe_rexi *r)
+{
+	if (!r) return SPHSWE_OK;
+	if (r->ctx) hipStreamSynchronize(r->ctx->stream);
+	void *ptrs[] = {r->d_alpha, r->d_beta, r->d_moff, r->d_fphi, r->d_fvel, r->d_pphi, r->d_pvel, r->d_oscale,
+			r->d_X, r->d_G, r->d_F, r->d_Y, r->d_part, r->d_basis, r->d_acc, r->d_phys, r->d_cphys, r->d_cspec};
+	for (void *p : ptrs)
+		if (p) hipFree(p);
+	delete r;
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_rexi_create(sphswe_ctx *ctx, const sphswe_rexi_params *prm, sphswe_rexi **out)
+{
+	using namespace sphswe_host;
+	if (!ctx || !prm || !out) return SPHSWE_ERR_ARG;
+	*out = nullptr;
+	if (!(prm->dt > 0)) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI needs a positive time step (AppSWE.hpp:185-190)");
+	if (prm->rexi_M < 1 || !(prm->rexi_h > 0)) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: invalid h or M");
+	sphswe_rexi *r = new sphswe_rexi();
+	r->ctx = ctx;
+	r->prm = *prm;
+	r->coriolis = prm->coriolis_omega != 0.0;
+	rexi_coefficients(prm->rexi_h, prm->rexi_M, prm->rexi_half != 0, r->alpha0, r->beta0);
+	const int P = r->P = (int)r->alpha0.size();
+	const int L = ctx->L;
+	r->rows = (size_t)(L + 1) * (L + 1);
+	r->ngroups = (P + 63) / 64;
+	std::vector<double2> al(P), be(P);
+	std::vector<cplx> alc(P);
+	for (int i = 0; i < P; i++) {
+		const cplx a = r->alpha0[i] / prm->dt, b = r->beta0[i] / prm->dt;
+		alc[i] = a;
+		al[i] = make_double2(a.real(), a.imag());
+		be[i] = make_double2(b.real(), b.imag());
+	}
+	std::vector<int> moff(2 * L + 1);
+	for (int m = -L; m <= L; m++) moff[m + L] = ncompact(L, std::abs(m), m);
+
+	// assemble and factor the band matrices of every pole (SWERexiSPH::setup,
+	// SWERexiSPH.hpp:104-129), in parallel over poles
+	const double rr = prm->earth_radius, W = 2.0 * prm->coriolis_omega, G = prm->h0 * prm->gravitation;
+	const int NEp = 1 + 8 + 4, NEv = 1 + 4 + 2;
+	std::vector<double2> hfphi, hfvel;
+	std::vector<unsigned char> hpphi, hpvel;
+	int info_all = 0;
+	if (r->coriolis) {
+		hfphi.assign(r->rows * NEp * P, make_double2(0, 0));
+		hfvel.assign(r->rows * NEv * P, make_double2(0, 0));
+		hpphi.assign(r->rows * P, 0);
+		hpvel.assign(r->rows * P, 0);
+		unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+		std::vector<std::thread> th;
+		std::vector<int> infos(nth, 0);
+		for (unsigned t = 0; t < nth; t++)
+			th.emplace_back([&, t]() {
+				std::vector<cplx> fac;
+				std::vector<unsigned char> piv;
+				for (int pp = (int)t; pp < P; pp += (int)nth) {
+					const cplx a = alc[pp];
+					BandRows Bp(L, 4);
+					band_add(Bp, 11, (a * a) * (a * a), rr);
+					band_add(Bp, 12, 2.0 * W * W * a * a, rr);
+					band_add(Bp, 13, (W * W) * (W * W), rr);
+					band_add(Bp, 14, -G * a * W, rr);
+					band_add(Bp, 15, G / a * W * W * W, rr);
+					band_add(Bp, 16, G * 2.0 * W * W, rr);
+					band_add(Bp, 17, -G * a * a, rr);
+					band_add(Bp, 18, -G * W * W, rr);
+					BandRows Bv(L, 2);
+					band_add(Bv, 11, a * a, rr);
+					band_add(Bv, 12, W * W, rr);
+					for (int m = -L; m <= L; m++) {
+						const size_t r0 = moff[m + L];
+						int info = band_factor_block(Bp, m, fac, piv);
+						if (info && !infos[t]) infos[t] = info;
+						for (size_t j = 0; j < piv.size(); j++) {
+							for (int e = 0; e < NEp; e++) {
+								const cplx v = fac[j * NEp + e];
+								hfphi[((r0 + j) * NEp + e) * P + pp] = make_double2(v.real(), v.imag());
+							}
+							hpphi[(r0 + j) * P + pp] = piv[j];
+						}
+						info = band_factor_block(Bv, m, fac, piv);
+						if (info && !infos[t]) infos[t] = info;
+						for (size_t j = 0; j < piv.size(); j++) {
+							for (int e = 0; e < NEv; e++) {
+								const cplx v = fac[j * NEv + e];
+								hfvel[((r0 + j) * NEv + e) * P + pp] = make_double2(v.real(), v.imag());
+							}
+							hpvel[(r0 + j) * P + pp] = piv[j];
+						}
+					}
+				}
+			});
+		for (auto &x : th) x.join();
+		for (int v : infos)
+			if (v && !info_all) info_all = v;
+	} else {
+		// no Coriolis: diagonal Helmholtz 1 / (alpha^2 + (Phi/r^2) n(n+1)) (SWERexiSPH.hpp:230-231)
+		hfphi.assign(r->rows * NEp * P, make_double2(0, 0));
+		for (int m = -L; m <= L; m++)
+			for (int n = std::abs(m); n <= L; n++)
+				for (int pp = 0; pp < P; pp++) {
+					const cplx a2 = alc[pp] * alc[pp];
+					const cplx bb = cplx(-G, 0) / (rr * rr);
+					const cplx d = a2 + (-bb * (double)n * ((double)n + 1.0));
+					const cplx inv = 1.0 / d;
+					hfphi[((moff[m + L] + n - std::abs(m)) * NEp) * P + pp] = make_double2(inv.real(), inv.imag());
+				}
+	}
+	if (info_all) {
+		delete r;
+		return sphswe_set_err(ctx, SPHSWE_ERR_SINGULAR, "zgbsv-equivalent factorisation: INFO = %d", info_all);
+	}
+	std::vector<double> osc(ctx->nlat);
+	for (int j = 0; j < ctx->nlat; j++) osc[j] = 1.0 / sqrt(1.0 - ctx->ct[j] * ctx->ct[j]);
+
+	int rc = SPHSWE_OK;
+	auto up = [&](void *d, const void *h, size_t b) -> int {
+		if (hipMemcpy(d, h, b, hipMemcpyHostToDevice) != hipSuccess)
+			return sphswe_set_err(ctx, SPHSWE_ERR_HIP, "rexi: upload failed");
+		return SPHSWE_OK;
+	};
+	const size_t nsp = (size_t)ctx->nspat;
+	if ((rc = dev_alloc(ctx, &r->d_alpha, P)) || (rc = dev_alloc(ctx, &r->d_beta, P)) ||
+	    (rc = dev_alloc(ctx, &r->d_moff, 2 * L + 1)) || (rc = dev_alloc(ctx, &r->d_fphi, hfphi.size())) ||
+	    (rc = dev_alloc(ctx, &r->d_oscale, ctx->nlat)) ||
+	    (rc = dev_alloc(ctx, &r->d_X, r->rows * 2 * P)) || (rc = dev_alloc(ctx, &r->d_G, r->rows * 2 * P)) ||
+	    (rc = dev_alloc(ctx, &r->d_F, (size_t)2 * P * (2 * L + 1) * ctx->nlat)) ||
+	    (rc = dev_alloc(ctx, &r->d_Y, r->rows * 2 * P)) ||
+	    (rc = dev_alloc(ctx, &r->d_part, (size_t)r->ngroups * 3 * r->rows)) ||
+	    (rc = dev_alloc(ctx, &r->d_basis, (size_t)NBASIS * ctx->ncplx)) ||
+	    (rc = dev_alloc(ctx, &r->d_acc, (size_t)3 * ctx->ncplx)) || (rc = dev_alloc(ctx, &r->d_phys, 3 * nsp)) ||
+	    (rc = dev_alloc(ctx, &r->d_cphys, 8 * nsp)) || (rc = dev_alloc(ctx, &r->d_cspec, (size_t)16 * ctx->ncplx))) {
+		sphswe_rexi_destroy(r);
+		return rc;
+	}
+	if ((rc = up(r->d_alpha, al.data(), sizeof(double2) * P)) || (rc = up(r->d_beta, be.data(), sizeof(double2) * P)) ||
+	    (rc = up(r->d_moff, moff.data(), sizeof(int) * moff.size())) ||
+	    (rc = up(r->d_fphi, hfphi.data(), sizeof(double2) * hfphi.size())) ||
+	    (rc = up(r->d_oscale, osc.data(), sizeof(double) * osc.size()))) {
+		sphswe_rexi_destroy(r);
+		return rc;
+	}
+	if (r->coriolis) {
+		if ((rc = dev_alloc(ctx, &r->d_fvel, hfvel.size())) || (rc = dev_alloc(ctx, &r->d_pphi, hpphi.size())) ||
+		    (rc = dev_alloc(ctx, &r->d_pvel, hpvel.size())) ||
+		    (rc = up(r->d_fvel, hfvel.data(), sizeof(double2) * hfvel.size())) ||
+		    (rc = up(r->d_pphi, hpphi.data(), hpphi.size())) || (rc = up(r->d_pvel, hpvel.data(), hpvel.size()))) {
+			sphswe_rexi_destroy(r);
+			return rc;
+		}
+	}
+	*out = r;
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_rexi_poles(const sphswe_rexi *r, int *npoles, double *alpha, double *beta)
+{
+	if (!r || !npoles) return SPHSWE_ERR_ARG;
+	*npoles = r->P;
+	for (int i = 0; i < r->P; i++) {
+		if (alpha) { alpha[2 * i] = r->alpha0[i].real(); alpha[2 * i + 1] = r->alpha0[i].imag(); }
+		if (beta) { beta[2 * i] = r->beta0[i].real(); beta[2 * i + 1] = r->beta0[i].imag(); }
+	}
+	return SPHSWE_OK;
+}
+
+// pole-independent fields of one step from the complex physical phi0, u0, v0
+static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
+{
+	const long nsp = ctx->nspat, nc = ctx->ncplx;
+	double2 *cp = r->d_cphys;      // complex physical scratch, 8 fields
+	double2 *cs_ = r->d_cspec;     // complex spectral scratch, 16 fields
+	double2 *B = r->d_basis;
+	const double ir = 1.0 / r->prm.earth_radius, W = 2.0 * r->prm.coriolis_omega;
+	hipStream_t st = ctx->stream;
+	int rc;
+	auto SPEC = [&](int k) { return (double *)(cs_ + (size_t)k * nc); };
+	auto PHYS = [&](int k) { return (double *)(cp + (size_t)k * nsp); };
+	auto BF = [&](int k) { return (double *)(B + (size_t)k * nc); };
+	// cp[0..2] = phi0, u0, v0 (complex physical); cp[3] = v0 * st; cp[4] = mu
+	if ((rc = sphswe_spat_op_cplx(ctx, SPHSWE_SPAT_MUL_ST, PHYS(2), nullptr, PHYS(3), 1.0, 1))) return rc;
+	k_fill_mu<<<g256(nsp), 256, 0, st>>>(cp + 4 * nsp, ctx->d_ct, ctx->nlat, nsp);
+	// level A: spec[0..4] = A(phi0), A(u0), A(v0), A(v0 st), A(mu)
+	if ((rc = sphswe_sht_analysis_cplx(ctx, PHYS(0), SPEC(0), 5, SPHSWE_PREC_FP64))) return rc;
+	// phi0c, u0c, v0c
+	SPHSWE_HIP(ctx, hipMemcpyAsync(BF(B_PHI), SPEC(0), sizeof(double2) * nc, hipMemcpyDeviceToDevice, st));
+	SPHSWE_HIP(ctx, hipMemcpyAsync(BF(B_U0), SPEC(1), sizeof(double2) * nc, hipMemcpyDeviceToDevice, st));
+	SPHSWE_HIP(ctx, hipMemcpyAsync(BF(B_V0), SPEC(2), sizeof(double2) * nc, hipMemcpyDeviceToDevice, st));
+	// level B (spectral): spec[5..9] = im u0c, D' A(v0 st), im v0c, D' u0c, D' A(mu)
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_DIFF_LON, SPEC(1), SPEC(5), 1, 1, 0, 0, 0, 0))) return rc;
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_ONE_MINUS_MU2_DMU, SPEC(3), SPEC(6), 1, 1, 0, 0, 0, 0))) return rc;
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_DIFF_LON, SPEC(2), SPEC(7), 1, 1, 0, 0, 0, 0))) return rc;
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_ONE_MINUS_MU2_DMU, SPEC(1), SPEC(8), 1, 1, 0, 0, 0, 0))) return rc;
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_ONE_MINUS_MU2_DMU, SPEC(4), SPEC(9), 1, 1, 0, 0, 0, 0))) return rc;
+	// synthesis, metric divisions: grad_lon / grad_lat divide by sqrt(1-mu^2); the
+	// complex div_lat further divides by cos(lat) (SPHOperatorsComplex.hpp:326-333)
+	if ((rc = sphswe_sht_synthesis_cplx(ctx, SPEC(5), PHYS(0), 5, SPHSWE_PREC_FP64))) return rc;
+	if ((rc = sphswe_spat_op_cplx(ctx, SPHSWE_SPAT_DIV_COSPHI, PHYS(0), nullptr, PHYS(0), 1.0, 5))) return rc;
+	if ((rc = sphswe_spat_op_cplx(ctx, SPHSWE_SPAT_DIV_COSLAT, PHYS(1), nullptr, PHYS(1), 1.0, 1))) return rc;
+	// level C: spec[10..14] = analyses
+	if ((rc = sphswe_sht_analysis_cplx(ctx, PHYS(0), SPEC(10), 5, SPHSWE_PREC_FP64))) return rc;
+	// div0 = ir (A1 + A2), eta0 = ir (A3 - A4), gms = (W ir) A5
+	k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 10 * nc, cs_ + 11 * nc, B + B_DIV * nc, 1.0, ir, nc);
+	k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 12 * nc, cs_ + 13 * nc, cs_ + 15 * nc, -1.0, ir, nc);   // eta0 in spec[15]
+	SPHSWE_LAUNCH_CHECK(ctx);
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_IDENTITY, SPEC(14), SPEC(5), 1, W * ir, 0, 0, 0, 0))) return rc;   // gms -> spec[5]
+	// mu / mu2 images
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU, SPEC(15), BF(B_MUETA), 1, 1, 0, 0, 0, 0))) return rc;
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU2, BF(B_PHI), BF(B_MU2PHI), 1, 1, 0, 0, 0, 0))) return rc;
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU2, BF(B_DIV), BF(B_MU2DIV), 1, 1, 0, 0, 0, 0))) return rc;
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU2, BF(B_MUETA), BF(B_MU2MUETA), 1, 1, 0, 0, 0, 0))) return rc;
+	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU2, BF(B_MU2PHI), BF(B_MU2MU2PHI), 1, 1, 0, 0, 0, 0))) return rc;
+	if (r->coriolis) {
+		// Coriolis forcing pieces: S of gms, u0c, mu2 u0c, mu v0c; products with g; analyses
+		SPHSWE_HIP(ctx, hipMemcpyAsync(SPEC(6), SPEC(1), sizeof(double2) * nc, hipMemcpyDeviceToDevice, st));
+		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU2, SPEC(1), SPEC(7), 1, 1, 0, 0, 0, 0))) return rc;
+		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU, SPEC(2), SPEC(8), 1, 1, 0, 0, 0, 0))) return rc;
+		if ((rc = sphswe_sht_synthesis_cplx(ctx, SPEC(5), PHYS(0), 4, SPHSWE_PREC_FP64))) return rc;
+		k_cmul_bcast<<<g256(3 * nsp), 256, 0, st>>>(cp, cp + nsp, cp + 4 * nsp, nsp, 3);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if ((rc = sphswe_sht_analysis_cplx(ctx, PHYS(4), BF(B_W1), 3, SPHSWE_PREC_FP64))) return rc;
+	} else {
+		SPHSWE_HIP(ctx, hipMemsetAsync(BF(B_W1), 0, sizeof(double2) * 3 * nc, st));
+	}
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_rexi_step(sphswe_ctx *ctx, sphswe_rexi *r, double *d_state, int nsteps)
+{
+	if (!ctx || !r || !d_state || nsteps < 0 || r->ctx != ctx) return SPHSWE_ERR_ARG;
+	const long nsp = ctx->nspat, nlm = ctx->nlm, nc = ctx->ncplx;
+	const int L = ctx->L, P = r->P;
+	const double g = r->prm.gravitation, a = r->prm.earth_radius;
+	hipStream_t st = ctx->stream;
+	double2 *X = (double2 *)d_state;
+	int rc;
+	for (int step = 0; step < nsteps; step++) {
+		// phi = g h; real synthesis of (phi, u, v); complex physical copies
+		double2 *sp = r->d_cspec;   // use as real-spectral scratch first (3 * nlm <= 16 * ncplx)
+		if ((rc = sphswe_spec_axpby(ctx, nlm, g, 0, (double *)X, 0, 0, nullptr, (double *)sp))) return rc;
+		SPHSWE_HIP(ctx, hipMemcpyAsync(sp + nlm, X + nlm, sizeof(double2) * 2 * nlm, hipMemcpyDeviceToDevice, st));
+		if ((rc = sphswe_sht_synthesis(ctx, (double *)sp, r->d_phys, 3, SPHSWE_PREC_FP64))) return rc;
+		k_real_to_cplx<<<g256(3 * nsp), 256, 0, st>>>(r->d_phys, r->d_cphys, 3 * nsp);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if ((rc = rexi_basis(ctx, r))) return rc;
+		// phi solves over (m, pole)
+		PhiArgs pa;
+		pa.alpha = r->d_alpha; pa.beta = r->d_beta; pa.moff = r->d_moff;
+		pa.fac = r->d_fphi; pa.piv = r->d_pphi; pa.basis = r->d_basis;
+		pa.Y = r->d_Y; pa.X = r->d_X; pa.part = r->d_part;
+		pa.L = L; pa.P = P; pa.ncplx = (int)nc;
+		pa.two_omega = 2.0 * r->prm.coriolis_omega; pa.avg_geo = r->prm.h0 * g;
+		pa.coriolis = r->coriolis; pa.rows = r->rows;
+		dim3 grid(r->ngroups, 2 * L + 1);
+		k_rexi_phi<4><<<grid, 64, 0, st>>>(pa);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		// gradients of phi: A(S(x) / sqrt(1-mu^2)) per m, without FFT
+		if ((rc = sphswe_leg_synth_pole(ctx, r->d_X, r->d_F, 2 * P, 2 * P, r->d_moff, r->d_oscale))) return rc;
+		if ((rc = sphswe_leg_anal_pole(ctx, r->d_F, r->d_G, 2 * P, 2 * P, r->d_moff))) return rc;
+		// velocity solves
+		VelArgs va;
+		va.alpha = r->d_alpha; va.beta = r->d_beta; va.moff = r->d_moff;
+		va.fac = r->d_fvel; va.piv = r->d_pvel; va.basis = r->d_basis; va.G = r->d_G;
+		va.Y = r->d_Y; va.part = r->d_part;
+		va.L = L; va.P = P; va.ncplx = (int)nc;
+		va.two_omega = 2.0 * r->prm.coriolis_omega; va.ir = 1.0 / a;
+		va.coriolis = r->coriolis; va.rows = r->rows;
+		k_rexi_vel<2><<<grid, 64, 0, st>>>(va);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		// pole sum, real part (spat_RealToSPHData + accumulation, AppSWE.hpp:519-521)
+		k_rexi_sum<<<g256(nc), 256, 0, st>>>(r->d_part, r->d_acc, ctx->d_lofcidx, r->d_moff, L, r->ngroups, r->rows,
+						      (int)nc);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if ((rc = sphswe_sht_synthesis_cplx(ctx, (double *)r->d_acc, (double *)r->d_cphys, 3, SPHSWE_PREC_FP64))) return rc;
+		k_cplx_real_part<<<g256(3 * nsp), 256, 0, st>>>(r->d_cphys, r->d_phys, 3 * nsp);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if ((rc = sphswe_sht_analysis(ctx, r->d_phys, (double *)X, 3, SPHSWE_PREC_FP64))) return rc;
+		// h = phi / g
+		if ((rc = sphswe_spec_axpby(ctx, nlm, 1.0 / g, 0, (double *)X, 0, 0, nullptr, (double *)X))) return rc;
+		// implicit viscosity (1 - dt nu D2) x_new = x (AppSWE.hpp:544-555)
+		if (r->prm.viscosity != 0.0) {
+			const double sc = r->prm.viscosity * r->prm.dt;
+			if ((rc = sphswe_spec_op(ctx, SPHSWE_OP_HELMHOLTZ, (double *)X, (double *)X, 3, 1.0, 0.0, -sc, a))) return rc;
+		}
+	}
+	return SPHSWE_OK;
+}
