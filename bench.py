@@ -6,8 +6,10 @@ A step is one SHT round trip (synthesis + analysis) of 3 fields (phi, u, v) at
 T511 on the 768 x 1536 Gaussian grid, FP64, spectral inputs already resident
 in HBM, seeded coefficients with power ~ l^-3.  The JSON line also carries the
 nonlinear SWE RK4 rate at T511 (Gaussian dam, dt = 0.001 a / nlat = 8.296 s),
-the roofline of the dominant kernel (the Legendre synthesis) measured with HIP
-events on the library's stream, and the CPU oracle timed on a bounded sample.
+the linear REXI rate at T256 (200 poles, dt = 3600 s, seeded (phi, vort, div)
+start state), the roofline of the dominant kernel (the Legendre synthesis)
+measured with HIP events on the library's stream, and the CPU oracle timed on a
+bounded sample.
 """
 import argparse
 import json
@@ -20,7 +22,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from sph_swe_amd import capi  # noqa: E402
+from sph_swe_amd import capi, workloads  # noqa: E402
 
 L_BENCH = 511
 NFIELDS = 3
@@ -122,6 +124,31 @@ def bench_swe(ctx, steps, warmup):
     return steps / (ms / 1000.0), dt
 
 
+def bench_rexi(steps, L=256, M=188):
+    """Linear SWE by REXI: seeded start state, 200 poles, dt = 3600 s."""
+    ctx = capi.Context(L)
+    state = ctx.to_device(workloads.rexi_state(ctx))
+    prm = capi.rexi_params(dt=3600.0, rexi_h=0.2, rexi_M=M, half=1, visc=1e5)
+    t0 = time.perf_counter()
+    rx = ctx.rexi(prm)
+    ctx.sync()
+    setup_s = time.perf_counter() - t0
+    rx.step(state, 1)
+    ctx.sync()
+    ctx.mark(8)
+    rx.step(state, steps)
+    ctx.mark(9)
+    ms = ctx.elapsed_ms(8, 9)
+    out = ctx.to_host(state, np.complex128, (3, ctx.nlm))
+    if not np.all(np.isfinite(out)):
+        raise RuntimeError("REXI state became non-finite")
+    cfg = {"lmax": L, "nlat": ctx.nlat, "nphi": ctx.nphi, "rexi_M": M, "rexi_h": 0.2, "poles": rx.npoles,
+           "dt": 3600.0, "steps": steps, "viscosity": 1e5, "setup_s": setup_s}
+    rx.close()
+    ctx.close()
+    return steps / (ms / 1000.0), cfg
+
+
 def cpu_baseline(L, ctx_nlat, ctx_nphi):
     """The oracle (single-threaded C restatement) on a bounded sample: one
     round trip of 3 fields at T511."""
@@ -143,6 +170,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--swe-steps", type=int, default=50)
+    ap.add_argument("--rexi-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -157,6 +185,7 @@ def main():
     ms_syn, ms_ana, flops = bench_legendre_kernel(ctx)
     swe_rate, swe_dt = bench_swe(ctx, args.swe_steps, 2)
     achieved = flops / (ms_syn * 1e-3) / 1e12
+    rexi_rate, rexi_cfg = bench_rexi(args.rexi_steps) if args.rexi_steps > 0 else (None, None)
     cpu = None if args.no_cpu else cpu_baseline(L_BENCH, ctx.nlat, ctx.nphi)
     line = {
         "metric": "SHT round-trips/sec at T511 (3 fields, FP64); SWE steps/sec at T511 in swe_steps_per_s",
@@ -178,6 +207,8 @@ def main():
         "swe_steps_per_s": swe_rate,
         "swe_config": {"lmax": L_BENCH, "dt": swe_dt, "scheme": "RK4 nonlinear", "steps": args.swe_steps,
                        "viscosity": 1e5},
+        "rexi_steps_per_s": rexi_rate,
+        "rexi_config": rexi_cfg,
         "roofline": {"bound": "mfma", "engine": "fp64 VALU (fp64 vector peak = fp64 matrix peak on MI355X)",
                      "kernel": "k_leg_synth (Legendre synthesis, 3 fields)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -15,6 +15,10 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "sphswe.h")
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SINGULAR, ERR_NAN = range(6)
 PREC_FP64, PREC_FP32 = 0, 1
+# spectral operators (sphswe_spec_op / _cplx)
+OP_IDENTITY, OP_DIFF_LON, OP_ONE_MINUS_MU2_DMU, OP_MU, OP_MU2, OP_LAPLACE, OP_HELMHOLTZ = range(7)
+# physical operators (sphswe_spat_op / _cplx)
+SPAT_MUL, SPAT_DIV_COSPHI, SPAT_MUL_ST, SPAT_DIV_ST, SPAT_MUL_MU, SPAT_SCALE, SPAT_DIV_COSLAT = range(7)
 (OP_IDENTITY, OP_DIFF_LON, OP_ONE_MINUS_MU2_DMU, OP_MU, OP_MU2, OP_LAPLACE, OP_HELMHOLTZ) = range(7)
 (SPAT_MUL, SPAT_DIV_COSPHI, SPAT_MUL_ST, SPAT_DIV_ST, SPAT_MUL_MU, SPAT_SCALE, SPAT_DIV_COSLAT) = range(7)
 

@@ -384,19 +384,18 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__res
 	double (*P)[16][AN_PS] = Pt[wave];
 	const int nch = (NJ + 63) / 64;
 	for (int c = 0; c < nch; c++) {
-		// B operands: E_even / E_odd of rings jb + 4s, column n.  Rings past NJ
-		// stay inside the Fourier rows (64 * nch <= NJ + 63 < nlat) and get
-		// weight 0, so the loads need no clamping.
+		// B operands: E_even / E_odd of rings jb + 4s, column n.  The ring of a
+		// load is clamped to NJ - 1 (the chunk can reach 63 rings past NJ, which
+		// on small grids lies outside the Fourier row); those rings get weight 0.
 		const int jb = c * 64 + kq;
-		const double *pn = fd + 2 * (fbase + jb) + comp;
-		const double *psth = fd + 2 * (fbase + (nlat - 1 - jb)) + comp;
-		const double *pw = g.wg + jb;
+		const double *prow = fd + 2 * fbase + comp;
 		double fn[16], fs[16], wv[16];
 #pragma unroll
 		for (int s = 0; s < 16; s++) {
-			fn[s] = pn[8 * s];
-			fs[s] = psth[-8 * s];
-			wv[s] = pw[4 * s];
+			const int rj = min(jb + 4 * s, NJ - 1);
+			fn[s] = prow[2 * rj];
+			fs[s] = prow[2 * (nlat - 1 - rj)];
+			wv[s] = g.wg[rj];
 		}
 		// Pbar tile: lane = ring (rings past NJ give finite values that meet weight 0)
 		const int j = c * 64 + lane;

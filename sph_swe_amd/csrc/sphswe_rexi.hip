@@ -13,8 +13,9 @@
 //     the cached pivoted LU factors, emit i m phi and D' phi for the gradients,
 //     and reduce beta*phi over the poles of the wavefront;
 //  3. grad_lon / grad_lat of phi = A(S(.) / sqrt(1-mu^2)) per m, done without
-//     FFT: Legendre synthesis to (m, ring) space, scaling by 1/sqrt(1-mu^2),
-//     Legendre analysis (the FFT pair would be an identity on one m);
+//     FFT: Legendre synthesis to (m, ring) space, scaling by 2 pi / sqrt(1-mu^2),
+//     Legendre analysis (the FFT pair would be an identity on one m up to the
+//     2 pi of its normalisation);
 //  4. one kernel over (m, pole): a = u0 + G_lon/r, b = v0 + G_lat/r,
 //     rhs_a = alpha a - 2W mu(b), rhs_b = 2W mu(a) + alpha b, the two
 //     5-diagonal solves, reduction of beta*u, beta*v;
@@ -43,7 +44,7 @@ struct sphswe_rexi {
 	int *d_moff = nullptr;
 	double2 *d_fphi = nullptr, *d_fvel = nullptr;
 	unsigned char *d_pphi = nullptr, *d_pvel = nullptr;
-	double *d_oscale = nullptr;        // 1 / sqrt(1 - mu^2)
+	double *d_oscale = nullptr;        // 2 pi / sqrt(1 - mu^2)
 	// work
 	double2 *d_X = nullptr, *d_G = nullptr, *d_F = nullptr, *d_Y = nullptr;
 	double2 *d_part = nullptr, *d_basis = nullptr, *d_acc = nullptr;
@@ -545,7 +546,11 @@ extern "C" int sphswe_rexi_create(sphswe_ctx *ctx, const sphswe_rexi_params *prm
 		return sphswe_set_err(ctx, SPHSWE_ERR_SINGULAR, "zgbsv-equivalent factorisation: INFO = %d", info_all);
 	}
 	std::vector<double> osc(ctx->nlat);
-	for (int j = 0; j < ctx->nlat; j++) osc[j] = 1.0 / sqrt(1.0 - ctx->ct[j] * ctx->ct[j]);
+	// the metric division of grad_lon / grad_lat, times the 2 pi that the forward
+	// FFT (normalised by 2 pi / nphi) would give the Fourier coefficients the
+	// Legendre analysis expects; the pole path goes from synthesis to analysis
+	// without the FFT pair
+	for (int j = 0; j < ctx->nlat; j++) osc[j] = 2.0 * M_PI / sqrt(1.0 - ctx->ct[j] * ctx->ct[j]);
 
 	int rc = SPHSWE_OK;
 	auto up = [&](void *d, const void *h, size_t b) -> int {
