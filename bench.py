@@ -164,6 +164,38 @@ def cpu_baseline(L, ctx_nlat, ctx_nphi):
             "sample": "1 round trip of 3 fields at T511 (768x1536), oracle/sphswe_oracle.c, %.1f s" % dt}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leg_synth_T511_3fields_hbm.json")
+
+
+def pmc_kernels():
+    """Counter-collection workload: 5 Legendre syntheses (3 fields, T511) and 5
+    launches of a copy-shaped kernel of known size (k_axpby with y = 0: 64 MiB
+    read, 64 MiB written) that calibrates FETCH_SIZE / WRITE_SIZE."""
+    ctx = capi.Context(L_BENCH)
+    spec = ctx.to_device(seeded_spec(ctx.L, NFIELDS, seed=7))
+    four = ctx.empty(NFIELDS * (ctx.L + 1) * ctx.nlat * 16)
+    ncopy = 1 << 22
+    src = ctx.to_device(np.ones(ncopy, np.complex128))
+    dst = ctx.empty(ncopy * 16)
+    for _ in range(5):
+        ctx._check(ctx.lib.sphswe_legendre_synthesis(ctx.h, spec.ptr, four.ptr, NFIELDS, capi.PREC_FP64))
+    for _ in range(5):
+        ctx._check(ctx.lib.sphswe_spec_axpby(ctx.h, ncopy, 1.0, 0.0, src.ptr, 0.0, 0.0, None, dst.ptr))
+    ctx.sync()
+    ctx.close()
+    print("pmc kernels done", flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the Legendre synthesis from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,11 +204,16 @@ def main():
     ap.add_argument("--swe-steps", type=int, default=50)
     ap.add_argument("--rexi-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pmc-kernels", action="store_true",
+                    help="only launch the kernels used for counter collection")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
               file=sys.stderr)
         sys.exit(2)
+    if args.pmc_kernels:
+        pmc_kernels()
+        return
 
     ctx = capi.Context(L_BENCH)
     wall, dev_ms, rt_err = bench_sht(ctx, args.steps, args.warmup)
@@ -212,7 +249,7 @@ def main():
         "roofline": {"bound": "mfma", "engine": "fp64 VALU (fp64 vector peak = fp64 matrix peak on MI355X)",
                      "kernel": "k_leg_synth (Legendre synthesis, 3 fields)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "avg_ms": ms_syn, "analysis_avg_ms": ms_ana,
                      "analysis_achieved": flops / (ms_ana * 1e-3) / 1e12},
         "cpu_baseline": cpu,

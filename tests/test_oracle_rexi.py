@@ -11,6 +11,8 @@
   rotation, the inertial oscillation, and first-order agreement with the linear
   tendency with rotation.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.linalg.lapack as lapack
@@ -27,6 +29,33 @@ def test_rexi_coefficients_vs_reference_build(h, M, half):
     assert a.shape == a_ref.shape
     assert np.max(np.abs(a - a_ref)) <= 1e-15 * np.max(np.abs(a_ref))
     assert np.max(np.abs(b - b_ref)) <= 1e-14 * np.max(np.abs(b_ref))
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("h,M,half", [(0.2, 64, True), (0.2, 128, True), (0.2, 188, True), (0.15, 32, False)])
+def test_rexi_coefficients_vs_golden(h, M, half):
+    # vectors written by tests/golden/make_golden.py from the reference's REXI.hpp build
+    arr = np.loadtxt(os.path.join(GOLDEN, "rexi_h%s_M%d_%s.txt" % (repr(h), M, "half" if half else "full")))
+    a_ref, b_ref = arr[:, 0] + 1j * arr[:, 1], arr[:, 2] + 1j * arr[:, 3]
+    a, b = O.rexi_coeffs(h, M, half)
+    assert a.shape == a_ref.shape
+    assert np.max(np.abs(a - a_ref)) <= 1e-15 * np.max(np.abs(a_ref))
+    assert np.max(np.abs(b - b_ref)) <= 1e-14 * np.max(np.abs(b_ref))
+
+
+def test_identities_vs_golden():
+    ref = np.loadtxt(os.path.join(GOLDEN, "identities_L20.txt"))
+    for row in ref:
+        n, m = row[0], row[1]
+        d = O.identities(n, m)
+        for k, name in enumerate("RSABCDE"):
+            want = row[2 + k]
+            if np.isnan(want):
+                assert np.isnan(d[name]), (n, m, name)
+            else:
+                assert abs(d[name] - want) <= 1e-15 * max(1.0, abs(want)), (n, m, name)
 
 
 def test_pole_counts():
