@@ -115,6 +115,13 @@ int sphswe_spec_axpby(sphswe_ctx *ctx, size_t n, double a_re, double a_im, const
 /* spectral (0,0) += v*sqrt(4 pi) (SPHData.hpp:427-440) */
 int sphswe_spec_add_scalar(sphswe_ctx *ctx, double *d_spec, double v_re, double v_im);
 
+/* real <-> complex physical fields (SPHDataComplex::spat_fromSPHData and
+ * spat_RealToSPHData / spat_ImagToSPHData, SPHDataComplex.hpp:129-175):
+ * nfields fields of nspat points; part 0 = real part, 1 = imaginary part */
+int sphswe_spat_real_to_cplx(sphswe_ctx *ctx, const double *d_real, double *d_cplx, int nfields);
+int sphswe_spat_cplx_part(sphswe_ctx *ctx, const double *d_cplx, double *d_real, int part, int nfields);
+
+
 /* ---- physical pointwise operators on real fields (spat_update_lambda*,
  *      operator* , SPHData.hpp:334-353,645-746) */
 enum {
@@ -177,6 +184,45 @@ int sphswe_rexi_destroy(sphswe_rexi *rexi);
 int sphswe_rexi_poles(const sphswe_rexi *rexi, int *npoles, double *alpha, double *beta);
 /* nsteps REXI steps of the state (h, u, v spectral, 3*nlm complex), in place */
 int sphswe_rexi_step(sphswe_ctx *ctx, sphswe_rexi *rexi, double *d_state, int nsteps);
+
+/* REXI::setup(h, M, 0, half) on the host (REXI.hpp:52-131), no context needed:
+ * *npoles = 2(M+11)+1 (full) or M+12 (half); alpha, beta_re and, if non-NULL,
+ * beta_im receive npoles (re, im) pairs each.  With all three arrays NULL only
+ * the count is returned. */
+int sphswe_rexi_coefficients(double h, int M, int half, int *npoles, double *alpha, double *beta_re,
+			     double *beta_im);
+
+/* As sphswe_rexi_create, for a caller-given pole set: alpha / beta as REXI::setup
+ * returns them (before the division by dt that SWERexiSPH::setup applies,
+ * SWERexiSPH.hpp:82-83).  One pole is one SWERexiSPH object. */
+int sphswe_rexi_create_poles(sphswe_ctx *ctx, const sphswe_rexi_params *p, int npoles, const double *alpha,
+			     const double *beta, sphswe_rexi **rexi);
+
+/* The pole sum of one step: out = sum_n Re(beta_n x_n), x_n the solution of
+ * SWERexiSPH::solve for pole n (SWERexiSPH.hpp:137-244) and the accumulation of
+ * AppSWE.hpp:519-521, without the phi = g h conversion and the viscosity.
+ * d_in: real spectra (phi0, u0, v0), 3*nlm complex; d_out: 3*nlm complex; they
+ * may be the same buffer. */
+int sphswe_rexi_apply(sphswe_ctx *ctx, sphswe_rexi *rexi, const double *d_in, double *d_out);
+
+/* ---- banded complex solver (SPHSolverComplex<T> with SPHMatrixComplex storage,
+ *      SPHSolverComplex.hpp:50-511; BandedMatrixSolver::solve_diagBandedInverse_Carray
+ *      + zgbsv, BandedMatrixSolver.hpp:235-434).  Rows in NCompact order, halo 1..4. */
+typedef struct sphswe_band sphswe_band;
+enum {
+	SPHSWE_BAND_SCALAR = 0,           /* solver_component_scalar_phi */
+	SPHSWE_BAND_MU = 1,               /* solver_component_mu_phi */
+	SPHSWE_BAND_ONE_MINUS_MU2_DMU = 2, /* solver_component_one_minus_mu_mu_diff_mu_phi */
+	SPHSWE_BAND_REXI_Z1 = 11          /* solver_component_rexi_z1 .. z8 = 11 .. 18 */
+};
+int sphswe_band_create(sphswe_ctx *ctx, int halo, sphswe_band **band);
+int sphswe_band_destroy(sphswe_band *band);
+/* add scalar * component (radius r for z4..z8) to the matrix (host side) */
+int sphswe_band_add(sphswe_band *band, int component, double s_re, double s_im, double r);
+/* SPHSolverComplex::solve: x = A^-1 rhs for every m, complex spectra ((L+1)^2
+ * complex, index l^2+l+m).  The matrix is factorised (partial pivoting, as
+ * zgbsv) on the first solve after a change; SPHSWE_ERR_SINGULAR on a zero pivot. */
+int sphswe_band_solve(sphswe_ctx *ctx, sphswe_band *band, const double *d_rhs, double *d_x);
 
 /* ---- timing on the context stream: ms between two sphswe_timer_mark calls */
 int sphswe_timer_mark(sphswe_ctx *ctx, int slot);

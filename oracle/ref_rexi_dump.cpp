@@ -3,7 +3,8 @@
 // GaussianApproximation.hpp, ExponentialApproximation.hpp (src/include/rexi/)
 // and SPHIdentities.hpp (src/include/sph/), compiled from /root/reference.
 //
-// usage: ref_rexi_dump rexi <h> <M> <half>     -> "re_alpha im_alpha re_beta im_beta" per pole
+// usage: ref_rexi_dump rexi <h> <M> <half>     -> "re_alpha im_alpha re_beta_re im_beta_re re_beta_im im_beta_im"
+//                                                  per pole
 //        ref_rexi_dump ident <L>               -> "n m R S A B C D E" for 0<=m<=n<=L (and m<0)
 #include <cmath>
 #include <complex>
@@ -19,8 +20,8 @@ int main(int argc, char **argv)
 		REXI r;
 		r.setup(std::atof(argv[2]), std::atoi(argv[3]), 0, std::atoi(argv[4]) != 0);
 		for (std::size_t i = 0; i < r.alpha.size(); i++)
-			std::printf("%.17g %.17g %.17g %.17g\n", r.alpha[i].real(), r.alpha[i].imag(),
-				    r.beta_re[i].real(), r.beta_re[i].imag());
+			std::printf("%.17g %.17g %.17g %.17g %.17g %.17g\n", r.alpha[i].real(), r.alpha[i].imag(),
+				    r.beta_re[i].real(), r.beta_re[i].imag(), r.beta_im[i].real(), r.beta_im[i].imag());
 		return 0;
 	}
 	if (argc >= 3 && std::strcmp(argv[1], "ident") == 0) {

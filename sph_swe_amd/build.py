@@ -1,9 +1,12 @@
-"""Build recipe for libsphswe.so (gfx950) and the host driver.
+"""Build recipe for libsphswe.so (gfx950) and the host program.
 
-    python -m sph_swe_amd.build          # compile every HIP source, link the library
+    python -m sph_swe_amd.build          # compile every HIP source, link the library,
+                                         # build the host program
 
 Every HIP translation unit is compiled with hipcc for gfx950 only; objects go to
-sph_swe_amd/lib/obj, the shared library to sph_swe_amd/lib/libsphswe.so.
+sph_swe_amd/lib/obj, the shared library to sph_swe_amd/lib/libsphswe.so.  The
+host program (host/main.cpp with the C++ classes of host/include) is plain C++
+built with g++ against the C ABI into sph_swe_amd/lib/sphswe_app.
 """
 import os
 import subprocess
@@ -16,6 +19,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "libsphswe.so")
+HOST = os.path.join(HERE, "host")
+APP = os.path.join(LIBDIR, "sphswe_app")
+CXX = os.environ.get("CXX", "g++")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = [
@@ -50,9 +56,25 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_app()
     if verbose:
         print("built", LIB)
+        print("built", APP)
     return LIB
+
+
+def build_app():
+    """The host program: C++17, the C ABI header, linked to libsphswe.so with an
+    rpath to its directory."""
+    tmp = APP + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wno-unused-function",
+           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HOST, "include"),
+           os.path.join(HOST, "main.cpp"), "-L" + LIBDIR, "-lsphswe", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host program build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(tmp, APP)
+    return APP
 
 
 if __name__ == "__main__":

@@ -75,6 +75,15 @@ _SIGS = {
     "sphswe_rexi_destroy": [_vp],
     "sphswe_rexi_poles": [_vp, _ip, _dp, _dp],
     "sphswe_rexi_step": [_vp, _vp, _vp, _i],
+    "sphswe_rexi_coefficients": [_d, _i, _i, _ip, _dp, _dp, _dp],
+    "sphswe_rexi_create_poles": [_vp, ctypes.POINTER(RexiParams), _i, _dp, _dp, ctypes.POINTER(_vp)],
+    "sphswe_rexi_apply": [_vp, _vp, _vp, _vp],
+    "sphswe_spat_real_to_cplx": [_vp, _vp, _vp, _i],
+    "sphswe_spat_cplx_part": [_vp, _vp, _vp, _i, _i],
+    "sphswe_band_create": [_vp, _i, ctypes.POINTER(_vp)],
+    "sphswe_band_destroy": [_vp],
+    "sphswe_band_add": [_vp, _i, _d, _d, _d],
+    "sphswe_band_solve": [_vp, _vp, _vp, _vp],
     "sphswe_timer_mark": [_vp, _i],
     "sphswe_timer_elapsed": [_vp, _i, _i, _fp],
 }
@@ -265,8 +274,11 @@ class Context:
         self._check(self.lib.sphswe_swe_rk4(self.h, ctypes.byref(params), state.ptr, nmembers, nsteps))
 
     # REXI -------------------------------------------------------------------
-    def rexi(self, params):
-        return Rexi(self, params)
+    def rexi(self, params, alpha=None, beta=None):
+        return Rexi(self, params, alpha, beta)
+
+    def band(self, halo):
+        return Band(self, halo)
 
     # timing ----------------------------------------------------------------
     def mark(self, slot):
@@ -278,13 +290,39 @@ class Context:
         return ms.value
 
 
-class Rexi:
-    """Device REXI integrator (poles, cached band factors) for one context."""
+def rexi_coefficients(h, M, half=True, with_beta_im=False):
+    """REXI::setup on the host (no GPU): alpha, beta_re[, beta_im] as complex arrays."""
+    lib = load()
+    n = _i()
+    _check_rc(lib.sphswe_rexi_coefficients(h, M, int(half), ctypes.byref(n), None, None, None))
+    a, br, bi = (np.empty(2 * n.value) for _ in range(3))
+    _check_rc(lib.sphswe_rexi_coefficients(h, M, int(half), ctypes.byref(n), a.ctypes.data_as(_dp),
+                                           br.ctypes.data_as(_dp),
+                                           bi.ctypes.data_as(_dp) if with_beta_im else None))
+    out = (a.view(np.complex128), br.view(np.complex128))
+    return out + (bi.view(np.complex128),) if with_beta_im else out
 
-    def __init__(self, ctx, params):
+
+def _check_rc(rc):
+    if rc:
+        raise SphSweError(rc, "sphswe call failed")
+
+
+class Rexi:
+    """Device REXI integrator (poles, cached band factors) for one context.
+    With `alpha`/`beta` given, the integrator uses that pole set."""
+
+    def __init__(self, ctx, params, alpha=None, beta=None):
         self.ctx = ctx
         h = _vp()
-        ctx._check(ctx.lib.sphswe_rexi_create(ctx.h, ctypes.byref(params), ctypes.byref(h)))
+        if alpha is None:
+            ctx._check(ctx.lib.sphswe_rexi_create(ctx.h, ctypes.byref(params), ctypes.byref(h)))
+        else:
+            a = np.ascontiguousarray(alpha, dtype=np.complex128).view(np.float64)
+            b = np.ascontiguousarray(beta, dtype=np.complex128).view(np.float64)
+            ctx._check(ctx.lib.sphswe_rexi_create_poles(ctx.h, ctypes.byref(params), a.size // 2,
+                                                        a.ctypes.data_as(_dp), b.ctypes.data_as(_dp),
+                                                        ctypes.byref(h)))
         self.h = h.value
         n = _i()
         ctx._check(ctx.lib.sphswe_rexi_poles(self.h, ctypes.byref(n), None, None))
@@ -299,6 +337,10 @@ class Rexi:
 
     def step(self, state, nsteps=1):
         self.ctx._check(self.ctx.lib.sphswe_rexi_step(self.ctx.h, self.h, state.ptr, nsteps))
+
+    def apply(self, d_in, d_out):
+        """sum_n Re(beta_n x_n) for real spectra (phi0, u0, v0) -> (phi, u, v)."""
+        self.ctx._check(self.ctx.lib.sphswe_rexi_apply(self.ctx.h, self.h, d_in.ptr, d_out.ptr))
 
     def close(self):
         if getattr(self, "h", None):
@@ -320,3 +362,36 @@ def rexi_params(a=6.37122e6, g=9.80616, omega=7.292e-5, h0=10000.0, visc=1e5, dt
 def swe_params(a=6.37122e6, g=9.80616, omega=7.292e-5, h0=10000.0, visc=1e5, dt=120.0, nonlinear=1,
                precision=PREC_FP64):
     return SweParams(a, g, omega, h0, visc, dt, nonlinear, precision)
+
+
+class Band:
+    """SPHSolverComplex on the device: components added on the host, per-m
+    pivoted LU, solves on complex spectra."""
+
+    def __init__(self, ctx, halo):
+        self.ctx = ctx
+        h = _vp()
+        ctx._check(ctx.lib.sphswe_band_create(ctx.h, halo, ctypes.byref(h)))
+        self.h = h.value
+
+    def add(self, component, scalar=1.0, r=1.0):
+        s = complex(scalar)
+        self.ctx._check(self.ctx.lib.sphswe_band_add(self.h, component, s.real, s.imag, r))
+
+    def solve(self, rhs):
+        """rhs: complex spectrum ((L+1)^2) on the host; returns the solution on the host."""
+        d = self.ctx.to_device(np.ascontiguousarray(rhs, dtype=np.complex128))
+        x = self.ctx.empty(d.nbytes)
+        self.ctx._check(self.ctx.lib.sphswe_band_solve(self.ctx.h, self.h, d.ptr, x.ptr))
+        return self.ctx.to_host(x, np.complex128, rhs.shape)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.sphswe_band_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -10,9 +10,11 @@ namespace sphswe_host {
 
 typedef std::complex<double> cplx;
 
-// REXI poles alpha_n and weights beta_n (REXI::setup, src/include/rexi/REXI.hpp:52-131):
-// e^{ix} ~ sum Re(beta_n / (ix + alpha_n)); beta returned is the reference's beta_re.
-void rexi_coefficients(double h, int M, bool half, std::vector<cplx> &alpha, std::vector<cplx> &beta);
+// REXI poles alpha_n and weights (REXI::setup, src/include/rexi/REXI.hpp:52-131):
+// cos x ~ sum Re(beta_re_n / (ix + alpha_n)), sin x ~ sum Re(beta_im_n / (ix + alpha_n)).
+// beta_im is filled when non-null.
+void rexi_coefficients(double h, int M, bool half, std::vector<cplx> &alpha, std::vector<cplx> &beta_re,
+		       std::vector<cplx> *beta_im = nullptr);
 
 // Row-band storage of one SPHMatrixComplex: rows in the NCompact order
 // (SPHConfig.hpp:136-156), num_diagonals = 2*halo+1 entries per row.
@@ -26,6 +28,9 @@ struct BandRows {
 // component ids follow SPHSolverComplex: 0 scalar, 1 mu, 2 (1-mu^2) d/dmu,
 // 11..18 rexi z1..z8 (SPHSolverComplex.hpp:88-420)
 void band_add(BandRows &B, int comp, cplx scalar, double r);
+
+// number of off-diagonals a component touches (0 for an unknown id)
+int band_component_width(int comp);
 
 // NCompact offset of (n = |m|, m): first row of the m block
 int ncompact(int L, int n, int m);

@@ -36,19 +36,24 @@ static const double kGaussA[23][2] = {
 	{-1.0816457995911385e-7, -2.954309729192276e-8},
 };
 
-void rexi_coefficients(double h, int M, bool half, std::vector<cplx> &alpha, std::vector<cplx> &beta)
+void rexi_coefficients(double h, int M, bool half, std::vector<cplx> &alpha, std::vector<cplx> &beta,
+		       std::vector<cplx> *beta_im)
 {
 	const int L = 11;
 	const int N = M + L;
 	alpha.assign(2 * N + 1, cplx(0, 0));
 	beta.assign(2 * N + 1, cplx(0, 0));
-	// b_m = e^{h^2} e^{-i m h} (ExponentialApproximation.hpp:59-123); beta_n = h sum_{l+m=n} Re(b_m) a_l
+	if (beta_im) beta_im->assign(2 * N + 1, cplx(0, 0));
+	// b_m = e^{h^2} e^{-i m h} (ExponentialApproximation.hpp:59-123); the weights of pole
+	// n = l + m collect h Re(b_m) a_l and h Im(b_m) a_l
 	for (int l = -L; l <= L; l++)
 		for (int m = -M; m <= M; m++) {
 			const int n = l + m;
 			alpha[n + N] = h * (kGaussMu + cplx(0, n));
 			const cplx bm = std::exp(h * h) * std::exp(-cplx(0, 1) * ((double)m * h));
-			beta[n + N] += bm.real() * h * cplx(kGaussA[l + L][0], kGaussA[l + L][1]);
+			const cplx al(kGaussA[l + L][0], kGaussA[l + L][1]);
+			beta[n + N] += bm.real() * h * al;
+			if (beta_im) (*beta_im)[n + N] += bm.imag() * h * al;
 		}
 	if (half) {
 		// the conjugate half of the poles contributes the complex conjugate:
@@ -56,6 +61,10 @@ void rexi_coefficients(double h, int M, bool half, std::vector<cplx> &alpha, std
 		alpha.resize(N + 1);
 		beta.resize(N + 1);
 		for (int i = 0; i < N; i++) beta[i] *= 2.0;
+		if (beta_im) {
+			beta_im->resize(N + 1);
+			for (int i = 0; i < N; i++) (*beta_im)[i] *= 2.0;
+		}
 	}
 }
 
@@ -137,6 +146,17 @@ void band_add(BandRows &B, int comp, cplx s, double r)
 			default: break;
 			}
 		}
+}
+
+int band_component_width(int comp)
+{
+	switch (comp) {
+	case 0: case 11: case 14: case 17: return 0;
+	case 1: case 2: return 1;
+	case 12: case 15: case 16: case 18: return 2;
+	case 13: return 4;
+	default: return -1;
+	}
 }
 
 int band_factor_block(const BandRows &B, int m, std::vector<cplx> &fac, std::vector<unsigned char> &piv)

@@ -268,6 +268,42 @@ extern "C" int sphswe_spat_op_cplx(sphswe_ctx *ctx, int op, const double *d_a, c
 	return SPHSWE_OK;
 }
 
+namespace {
+
+__global__ void k_real_to_cplx_f(const double *__restrict__ a, double2 *__restrict__ o, long n)
+{
+	const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i < n) o[i] = make_double2(a[i], 0.0);
+}
+
+__global__ void k_cplx_part(const double2 *__restrict__ a, double *__restrict__ o, int part, long n)
+{
+	const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i < n) o[i] = part ? a[i].y : a[i].x;
+}
+
+}  // namespace
+
+extern "C" int sphswe_spat_real_to_cplx(sphswe_ctx *ctx, const double *d_real, double *d_cplx, int nfields)
+{
+	if (!ctx || !d_real || !d_cplx || nfields < 0) return SPHSWE_ERR_ARG;
+	const long total = (long)ctx->nspat * nfields;
+	if (total == 0) return SPHSWE_OK;
+	k_real_to_cplx_f<<<grid1(total), 256, 0, ctx->stream>>>(d_real, (double2 *)d_cplx, total);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_spat_cplx_part(sphswe_ctx *ctx, const double *d_cplx, double *d_real, int part, int nfields)
+{
+	if (!ctx || !d_real || !d_cplx || nfields < 0 || (part != 0 && part != 1)) return SPHSWE_ERR_ARG;
+	const long total = (long)ctx->nspat * nfields;
+	if (total == 0) return SPHSWE_OK;
+	k_cplx_part<<<grid1(total), 256, 0, ctx->stream>>>((const double2 *)d_cplx, d_real, part, total);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return SPHSWE_OK;
+}
+
 // ---- reductions --------------------------------------------------------------
 
 namespace {

@@ -50,6 +50,7 @@ struct sphswe_rexi {
 	double2 *d_part = nullptr, *d_basis = nullptr, *d_acc = nullptr;
 	double *d_phys = nullptr;
 	double2 *d_cphys = nullptr, *d_cspec = nullptr;
+	double2 *d_in = nullptr;           // phi = g h, u, v of a step (3 real spectra)
 	int ngroups = 0;
 	size_t rows = 0;
 };
@@ -435,25 +436,29 @@ extern "C" int sphswe_rexi_destroy(sphswe_rexi *r)
 	if (!r) return SPHSWE_OK;
 	if (r->ctx) hipStreamSynchronize(r->ctx->stream);
 	void *ptrs[] = {r->d_alpha, r->d_beta, r->d_moff, r->d_fphi, r->d_fvel, r->d_pphi, r->d_pvel, r->d_oscale,
-			r->d_X, r->d_G, r->d_F, r->d_Y, r->d_part, r->d_basis, r->d_acc, r->d_phys, r->d_cphys, r->d_cspec};
+			r->d_X, r->d_G, r->d_F, r->d_Y, r->d_part, r->d_basis, r->d_acc, r->d_phys, r->d_cphys, r->d_cspec,
+			r->d_in};
 	for (void *p : ptrs)
 		if (p) hipFree(p);
 	delete r;
 	return SPHSWE_OK;
 }
 
-extern "C" int sphswe_rexi_create(sphswe_ctx *ctx, const sphswe_rexi_params *prm, sphswe_rexi **out)
+// SWERexiSPH::setup for every pole of (alpha0, beta0): band assembly, pivoted LU
+// factors, device buffers
+static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std::vector<cplx> &alpha0,
+		      const std::vector<cplx> &beta0, sphswe_rexi **out)
 {
 	using namespace sphswe_host;
-	if (!ctx || !prm || !out) return SPHSWE_ERR_ARG;
 	*out = nullptr;
 	if (!(prm->dt > 0)) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI needs a positive time step (AppSWE.hpp:185-190)");
-	if (prm->rexi_M < 1 || !(prm->rexi_h > 0)) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: invalid h or M");
+	if (alpha0.empty() || alpha0.size() != beta0.size()) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: empty pole set");
 	sphswe_rexi *r = new sphswe_rexi();
 	r->ctx = ctx;
 	r->prm = *prm;
 	r->coriolis = prm->coriolis_omega != 0.0;
-	rexi_coefficients(prm->rexi_h, prm->rexi_M, prm->rexi_half != 0, r->alpha0, r->beta0);
+	r->alpha0 = alpha0;
+	r->beta0 = beta0;
 	const int P = r->P = (int)r->alpha0.size();
 	const int L = ctx->L;
 	r->rows = (size_t)(L + 1) * (L + 1);
@@ -568,7 +573,8 @@ extern "C" int sphswe_rexi_create(sphswe_ctx *ctx, const sphswe_rexi_params *prm
 	    (rc = dev_alloc(ctx, &r->d_part, (size_t)r->ngroups * 3 * r->rows)) ||
 	    (rc = dev_alloc(ctx, &r->d_basis, (size_t)NBASIS * ctx->ncplx)) ||
 	    (rc = dev_alloc(ctx, &r->d_acc, (size_t)3 * ctx->ncplx)) || (rc = dev_alloc(ctx, &r->d_phys, 3 * nsp)) ||
-	    (rc = dev_alloc(ctx, &r->d_cphys, 8 * nsp)) || (rc = dev_alloc(ctx, &r->d_cspec, (size_t)16 * ctx->ncplx))) {
+	    (rc = dev_alloc(ctx, &r->d_cphys, 8 * nsp)) || (rc = dev_alloc(ctx, &r->d_cspec, (size_t)16 * ctx->ncplx)) ||
+	    (rc = dev_alloc(ctx, &r->d_in, (size_t)3 * ctx->nlm))) {
 		sphswe_rexi_destroy(r);
 		return rc;
 	}
@@ -589,6 +595,45 @@ extern "C" int sphswe_rexi_create(sphswe_ctx *ctx, const sphswe_rexi_params *prm
 		}
 	}
 	*out = r;
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_rexi_create(sphswe_ctx *ctx, const sphswe_rexi_params *prm, sphswe_rexi **out)
+{
+	if (!ctx || !prm || !out) return SPHSWE_ERR_ARG;
+	*out = nullptr;
+	if (prm->rexi_M < 1 || !(prm->rexi_h > 0)) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: invalid h or M");
+	std::vector<cplx> al, be;
+	sphswe_host::rexi_coefficients(prm->rexi_h, prm->rexi_M, prm->rexi_half != 0, al, be);
+	return rexi_build(ctx, prm, al, be, out);
+}
+
+extern "C" int sphswe_rexi_create_poles(sphswe_ctx *ctx, const sphswe_rexi_params *prm, int npoles, const double *alpha,
+					const double *beta, sphswe_rexi **out)
+{
+	if (!ctx || !prm || !out || npoles < 1 || !alpha || !beta) return SPHSWE_ERR_ARG;
+	*out = nullptr;
+	std::vector<cplx> al(npoles), be(npoles);
+	for (int i = 0; i < npoles; i++) {
+		al[i] = cplx(alpha[2 * i], alpha[2 * i + 1]);
+		be[i] = cplx(beta[2 * i], beta[2 * i + 1]);
+		if (al[i] == cplx(0, 0)) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: pole %d is zero", i);
+	}
+	return rexi_build(ctx, prm, al, be, out);
+}
+
+extern "C" int sphswe_rexi_coefficients(double h, int M, int half, int *npoles, double *alpha, double *beta_re,
+					double *beta_im)
+{
+	if (!npoles || M < 1 || !(h > 0)) return SPHSWE_ERR_ARG;
+	std::vector<cplx> al, br, bi;
+	sphswe_host::rexi_coefficients(h, M, half != 0, al, br, beta_im ? &bi : nullptr);
+	*npoles = (int)al.size();
+	for (size_t i = 0; i < al.size(); i++) {
+		if (alpha) { alpha[2 * i] = al[i].real(); alpha[2 * i + 1] = al[i].imag(); }
+		if (beta_re) { beta_re[2 * i] = br[i].real(); beta_re[2 * i + 1] = br[i].imag(); }
+		if (beta_im) { beta_im[2 * i] = bi[i].real(); beta_im[2 * i + 1] = bi[i].imag(); }
+	}
 	return SPHSWE_OK;
 }
 
@@ -664,56 +709,74 @@ static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 	return SPHSWE_OK;
 }
 
+// sum over the poles of Re(beta_n x_n) for real spectra (phi0, u0, v0) in `in`;
+// result as real spectra in `out` (may alias `in`)
+static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, double2 *out)
+{
+	const long nsp = ctx->nspat, nc = ctx->ncplx;
+	const int L = ctx->L, P = r->P;
+	const double a = r->prm.earth_radius, g = r->prm.gravitation;
+	hipStream_t st = ctx->stream;
+	int rc;
+	// real synthesis of (phi, u, v); complex physical copies
+	if ((rc = sphswe_sht_synthesis(ctx, (const double *)in, r->d_phys, 3, SPHSWE_PREC_FP64))) return rc;
+	k_real_to_cplx<<<g256(3 * nsp), 256, 0, st>>>(r->d_phys, r->d_cphys, 3 * nsp);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	if ((rc = rexi_basis(ctx, r))) return rc;
+	// phi solves over (m, pole)
+	PhiArgs pa;
+	pa.alpha = r->d_alpha; pa.beta = r->d_beta; pa.moff = r->d_moff;
+	pa.fac = r->d_fphi; pa.piv = r->d_pphi; pa.basis = r->d_basis;
+	pa.Y = r->d_Y; pa.X = r->d_X; pa.part = r->d_part;
+	pa.L = L; pa.P = P; pa.ncplx = (int)nc;
+	pa.two_omega = 2.0 * r->prm.coriolis_omega; pa.avg_geo = r->prm.h0 * g;
+	pa.coriolis = r->coriolis; pa.rows = r->rows;
+	dim3 grid(r->ngroups, 2 * L + 1);
+	k_rexi_phi<4><<<grid, 64, 0, st>>>(pa);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	// gradients of phi: A(S(x) / sqrt(1-mu^2)) per m, without FFT
+	if ((rc = sphswe_leg_synth_pole(ctx, r->d_X, r->d_F, 2 * P, 2 * P, r->d_moff, r->d_oscale))) return rc;
+	if ((rc = sphswe_leg_anal_pole(ctx, r->d_F, r->d_G, 2 * P, 2 * P, r->d_moff))) return rc;
+	// velocity solves
+	VelArgs va;
+	va.alpha = r->d_alpha; va.beta = r->d_beta; va.moff = r->d_moff;
+	va.fac = r->d_fvel; va.piv = r->d_pvel; va.basis = r->d_basis; va.G = r->d_G;
+	va.Y = r->d_Y; va.part = r->d_part;
+	va.L = L; va.P = P; va.ncplx = (int)nc;
+	va.two_omega = 2.0 * r->prm.coriolis_omega; va.ir = 1.0 / a;
+	va.coriolis = r->coriolis; va.rows = r->rows;
+	k_rexi_vel<2><<<grid, 64, 0, st>>>(va);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	// pole sum, real part (spat_RealToSPHData + accumulation, AppSWE.hpp:519-521)
+	k_rexi_sum<<<g256(nc), 256, 0, st>>>(r->d_part, r->d_acc, ctx->d_lofcidx, r->d_moff, L, r->ngroups, r->rows,
+					      (int)nc);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	if ((rc = sphswe_sht_synthesis_cplx(ctx, (double *)r->d_acc, (double *)r->d_cphys, 3, SPHSWE_PREC_FP64))) return rc;
+	k_cplx_real_part<<<g256(3 * nsp), 256, 0, st>>>(r->d_cphys, r->d_phys, 3 * nsp);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return sphswe_sht_analysis(ctx, r->d_phys, (double *)out, 3, SPHSWE_PREC_FP64);
+}
+
+extern "C" int sphswe_rexi_apply(sphswe_ctx *ctx, sphswe_rexi *r, const double *d_in, double *d_out)
+{
+	if (!ctx || !r || !d_in || !d_out || r->ctx != ctx) return SPHSWE_ERR_ARG;
+	return rexi_pole_sum(ctx, r, (const double2 *)d_in, (double2 *)d_out);
+}
+
 extern "C" int sphswe_rexi_step(sphswe_ctx *ctx, sphswe_rexi *r, double *d_state, int nsteps)
 {
 	if (!ctx || !r || !d_state || nsteps < 0 || r->ctx != ctx) return SPHSWE_ERR_ARG;
-	const long nsp = ctx->nspat, nlm = ctx->nlm, nc = ctx->ncplx;
-	const int L = ctx->L, P = r->P;
+	const long nlm = ctx->nlm;
 	const double g = r->prm.gravitation, a = r->prm.earth_radius;
 	hipStream_t st = ctx->stream;
 	double2 *X = (double2 *)d_state;
 	int rc;
 	for (int step = 0; step < nsteps; step++) {
-		// phi = g h; real synthesis of (phi, u, v); complex physical copies
-		double2 *sp = r->d_cspec;   // use as real-spectral scratch first (3 * nlm <= 16 * ncplx)
+		// phi = g h; (u, v) as they are (AppSWE.hpp:436-440)
+		double2 *sp = r->d_in;
 		if ((rc = sphswe_spec_axpby(ctx, nlm, g, 0, (double *)X, 0, 0, nullptr, (double *)sp))) return rc;
 		SPHSWE_HIP(ctx, hipMemcpyAsync(sp + nlm, X + nlm, sizeof(double2) * 2 * nlm, hipMemcpyDeviceToDevice, st));
-		if ((rc = sphswe_sht_synthesis(ctx, (double *)sp, r->d_phys, 3, SPHSWE_PREC_FP64))) return rc;
-		k_real_to_cplx<<<g256(3 * nsp), 256, 0, st>>>(r->d_phys, r->d_cphys, 3 * nsp);
-		SPHSWE_LAUNCH_CHECK(ctx);
-		if ((rc = rexi_basis(ctx, r))) return rc;
-		// phi solves over (m, pole)
-		PhiArgs pa;
-		pa.alpha = r->d_alpha; pa.beta = r->d_beta; pa.moff = r->d_moff;
-		pa.fac = r->d_fphi; pa.piv = r->d_pphi; pa.basis = r->d_basis;
-		pa.Y = r->d_Y; pa.X = r->d_X; pa.part = r->d_part;
-		pa.L = L; pa.P = P; pa.ncplx = (int)nc;
-		pa.two_omega = 2.0 * r->prm.coriolis_omega; pa.avg_geo = r->prm.h0 * g;
-		pa.coriolis = r->coriolis; pa.rows = r->rows;
-		dim3 grid(r->ngroups, 2 * L + 1);
-		k_rexi_phi<4><<<grid, 64, 0, st>>>(pa);
-		SPHSWE_LAUNCH_CHECK(ctx);
-		// gradients of phi: A(S(x) / sqrt(1-mu^2)) per m, without FFT
-		if ((rc = sphswe_leg_synth_pole(ctx, r->d_X, r->d_F, 2 * P, 2 * P, r->d_moff, r->d_oscale))) return rc;
-		if ((rc = sphswe_leg_anal_pole(ctx, r->d_F, r->d_G, 2 * P, 2 * P, r->d_moff))) return rc;
-		// velocity solves
-		VelArgs va;
-		va.alpha = r->d_alpha; va.beta = r->d_beta; va.moff = r->d_moff;
-		va.fac = r->d_fvel; va.piv = r->d_pvel; va.basis = r->d_basis; va.G = r->d_G;
-		va.Y = r->d_Y; va.part = r->d_part;
-		va.L = L; va.P = P; va.ncplx = (int)nc;
-		va.two_omega = 2.0 * r->prm.coriolis_omega; va.ir = 1.0 / a;
-		va.coriolis = r->coriolis; va.rows = r->rows;
-		k_rexi_vel<2><<<grid, 64, 0, st>>>(va);
-		SPHSWE_LAUNCH_CHECK(ctx);
-		// pole sum, real part (spat_RealToSPHData + accumulation, AppSWE.hpp:519-521)
-		k_rexi_sum<<<g256(nc), 256, 0, st>>>(r->d_part, r->d_acc, ctx->d_lofcidx, r->d_moff, L, r->ngroups, r->rows,
-						      (int)nc);
-		SPHSWE_LAUNCH_CHECK(ctx);
-		if ((rc = sphswe_sht_synthesis_cplx(ctx, (double *)r->d_acc, (double *)r->d_cphys, 3, SPHSWE_PREC_FP64))) return rc;
-		k_cplx_real_part<<<g256(3 * nsp), 256, 0, st>>>(r->d_cphys, r->d_phys, 3 * nsp);
-		SPHSWE_LAUNCH_CHECK(ctx);
-		if ((rc = sphswe_sht_analysis(ctx, r->d_phys, (double *)X, 3, SPHSWE_PREC_FP64))) return rc;
+		if ((rc = rexi_pole_sum(ctx, r, sp, X))) return rc;
 		// h = phi / g
 		if ((rc = sphswe_spec_axpby(ctx, nlm, 1.0 / g, 0, (double *)X, 0, 0, nullptr, (double *)X))) return rc;
 		// implicit viscosity (1 - dt nu D2) x_new = x (AppSWE.hpp:544-555)
