@@ -168,9 +168,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leg_synth_T511_3fields_hbm.json")
 
 
 def pmc_kernels():
-    """Counter-collection workload: 5 Legendre syntheses (3 fields, T511) and 5
-    launches of a copy-shaped kernel of known size (k_axpby with y = 0: 64 MiB
-    read, 64 MiB written) that calibrates FETCH_SIZE / WRITE_SIZE."""
+    """Counter-collection workload: 5 Legendre syntheses and 5 analyses (3 fields,
+    T511) and 5 launches of a copy-shaped kernel of known size (k_axpby with y = 0:
+    64 MiB read, 64 MiB written) that calibrates FETCH_SIZE / WRITE_SIZE."""
     ctx = capi.Context(L_BENCH)
     spec = ctx.to_device(seeded_spec(ctx.L, NFIELDS, seed=7))
     four = ctx.empty(NFIELDS * (ctx.L + 1) * ctx.nlat * 16)
@@ -179,6 +179,8 @@ def pmc_kernels():
     dst = ctx.empty(ncopy * 16)
     for _ in range(5):
         ctx._check(ctx.lib.sphswe_legendre_synthesis(ctx.h, spec.ptr, four.ptr, NFIELDS, capi.PREC_FP64))
+    for _ in range(5):
+        ctx._check(ctx.lib.sphswe_legendre_analysis(ctx.h, four.ptr, spec.ptr, NFIELDS, capi.PREC_FP64))
     for _ in range(5):
         ctx._check(ctx.lib.sphswe_spec_axpby(ctx.h, ncopy, 1.0, 0.0, src.ptr, 0.0, 0.0, None, dst.ptr))
     ctx.sync()

@@ -36,8 +36,7 @@ struct sphswe_ctx {
 	double *d_coslat = nullptr;   // cos(lat[j]) [nlat]
 	double *d_wg = nullptr;       // Gauss weights [nlat]
 	double2 *d_coef = nullptr;    // recurrence (a_lm, a_lm*b_lm) [nlm], index of the real spectrum
-	int *d_l0 = nullptr;          // first significant degree [(L+1) * NJ]
-	double2 *d_seed = nullptr;    // (Pbar_s, Pbar_s+1) at each degree-segment start [nseg_total * NJ]
+	double2 *d_seed = nullptr;    // (Pbar_s, Pbar_s+1) at each degree-segment start s [nseg_total * NJ]
 	int *d_segoff = nullptr;      // first segment of each m [L+2]
 	int *d_unit_m = nullptr;      // m of each segment [nseg_total]
 	int seg = 16;                 // degrees per segment

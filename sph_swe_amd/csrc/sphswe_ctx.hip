@@ -93,20 +93,21 @@ static bool fft_plan(int n, FftPlan &p)
 	return r == 1;
 }
 
-// Seeds of the Pbar_l^m recurrence.  For every (m, north ring j): the first
-// degree l0 with |Pbar_l0^m(mu_j)| >= 2^-930 (values before it are below 1e-280
-// and are treated as zero), and for every degree segment k of length seg
-// (degrees m + k*seg ...) the pair (Pbar_s, Pbar_s+1) at s = max(m + k*seg, l0).
-// The kernels run the recurrence segment by segment from these FP64 values, so
-// no chain is longer than seg degrees.
+// Seeds of the Legendre recurrence at the start s = m + seg*k of every degree
+// segment and every northern ring: (Pbar_s, Pbar_{s+1}).  The recurrence runs
+// from Pbar_m^m on values scaled by 2^-E (E an integer, renormalised by powers
+// of two), so values far below the double range keep their scale on the way;
+// the stored seeds are the true values, which below the double range round to
+// subnormals or 0.  The kernels restart the recurrence from these values at
+// every segment, so no chain runs further than seg degrees from a seed, and a
+// seed that underflows only drops terms below 2^-981 (a segment of the
+// recurrence grows by less than 2^41).
 static void make_seeds(int L, int NJ, int seg, const std::vector<double> &ct, const std::vector<double> &st,
-		       std::vector<int> &l0, std::vector<int> &segoff, std::vector<double2> &seed)
+		       std::vector<int> &segoff, std::vector<double2> &seed)
 {
 	segoff.assign(L + 2, 0);
 	for (int m = 0; m <= L; m++) segoff[m + 1] = segoff[m] + (L - m + seg) / seg;
-	l0.assign((size_t)(L + 1) * NJ, L + 1);
 	seed.assign((size_t)segoff[L + 1] * NJ, make_double2(0, 0));
-	const double thr_log2 = -930.0;
 	// running Pbar_m^m(mu_j) = xm[j] * 2^Em[j], built by the product
 	// Pbar_m^m = -sqrt((2m+1)/(2m)) st Pbar_{m-1}^{m-1} (error ~ sqrt(m) eps)
 	std::vector<double> xm(NJ, 1.0 / sqrt(4.0 * M_PI));
@@ -117,6 +118,7 @@ static void make_seeds(int L, int NJ, int seg, const std::vector<double> &ct, co
 			av[l] = sqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
 			bv[l] = sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0));
 		}
+		const int nseg = segoff[m + 1] - segoff[m];
 		for (int j = 0; j < NJ; j++) {
 			if (m > 0) {
 				xm[j] *= -sqrt((2.0 * m + 1.0) / (2.0 * m)) * st[j];
@@ -125,45 +127,32 @@ static void make_seeds(int L, int NJ, int seg, const std::vector<double> &ct, co
 					Em[j] -= 200;
 				}
 			}
+			// (x0, x1) = (P_l, P_l+1) * 2^-E
 			int E = Em[j];
 			double x0 = xm[j];
 			double x1 = (m < L) ? sqrt(2.0 * m + 3.0) * ct[j] * x0 : 0.0;
 			int l = m;
-			// scaled search for the first significant degree; (x0, x1) = (P_l, P_l+1) * 2^-E
-			bool found = false;
-			while (true) {
-				if (x0 != 0.0 && (log2(fabs(x0)) + E >= thr_log2)) { found = true; break; }
-				if (l + 1 > L) break;
-				int ln = l + 2;
-				double x2 = (ln <= L) ? av[ln] * (ct[j] * x1 - bv[ln] * x0) : 0.0;
-				x0 = x1;
-				x1 = x2;
-				l++;
-				if (fabs(x0) > 0x1p200 || fabs(x1) > 0x1p200) {
-					x0 = ldexp(x0, -200);
-					x1 = ldexp(x1, -200);
-					E += 200;
-				}
-			}
-			if (!found) continue;
-			l0[(size_t)m * NJ + j] = l;
-			// from here on plain doubles: the values only grow out of 2^-930
-			double p0 = ldexp(x0, E), p1 = (l + 1 <= L) ? ldexp(x1, E) : 0.0;
-			const int nseg = segoff[m + 1] - segoff[m];
 			for (int k = 0; k < nseg; k++) {
 				const int s0 = m + k * seg;
-				const int se = std::min(s0 + seg - 1, L);
-				if (se < l) continue;
-				const int st_l = std::max(s0, l);
-				// advance (p0, p1) = (P_l, P_l+1) to l = st_l
-				while (l < st_l) {
-					int ln = l + 2;
-					double p2 = (ln <= L) ? av[ln] * (ct[j] * p1 - bv[ln] * p0) : 0.0;
-					p0 = p1;
-					p1 = p2;
+				while (l < s0) {
+					const int ln = l + 2;
+					const double x2 = (ln <= L) ? av[ln] * (ct[j] * x1 - bv[ln] * x0) : 0.0;
+					x0 = x1;
+					x1 = x2;
 					l++;
+					if (fabs(x0) > 0x1p200 || fabs(x1) > 0x1p200) {
+						x0 = ldexp(x0, -200);
+						x1 = ldexp(x1, -200);
+						E += 200;
+					} else if (fabs(x0) < 0x1p-200 && fabs(x1) < 0x1p-200 && (x0 != 0.0 || x1 != 0.0)) {
+						x0 = ldexp(x0, 200);
+						x1 = ldexp(x1, 200);
+						E -= 200;
+					}
 				}
-				seed[(size_t)(segoff[m] + k) * NJ + j] = make_double2(p0, (l + 1 <= L) ? p1 : 0.0);
+				// true values (below the double range they round to subnormals or 0)
+				const double y1 = (l + 1 <= L) ? x1 : 0.0;
+				seed[(size_t)(segoff[m] + k) * NJ + j] = make_double2(ldexp(x0, E), ldexp(y1, E));
 			}
 		}
 	}
@@ -237,11 +226,11 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	std::vector<int> lofc(ctx->ncplx);
 	for (int l = 0; l <= lmax; l++)
 		for (int m = -l; m <= l; m++) lofc[l * l + l + m] = l;
-	std::vector<int> l0, segoff;
+	std::vector<int> segoff;
 	std::vector<double2> seed;
 	// 32-degree segments (one MFMA analysis wavefront each)
 	ctx->seg = 32;
-	make_seeds(lmax, ctx->NJ, ctx->seg, ctx->ct, ctx->st, l0, segoff, seed);
+	make_seeds(lmax, ctx->NJ, ctx->seg, ctx->ct, ctx->st, segoff, seed);
 	ctx->nseg_total = segoff[lmax + 1];
 	std::vector<int> unit_m(ctx->nseg_total);
 	for (int m = 0; m <= lmax; m++)
@@ -263,7 +252,6 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	if ((rc = upload(ctx, &ctx->d_coslat, coslat))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_wg, w))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_coef, coef))) return fail(rc);
-	if ((rc = upload(ctx, &ctx->d_l0, l0))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_seed, seed))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_segoff, segoff))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_unit_m, unit_m))) return fail(rc);
@@ -281,7 +269,7 @@ extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 	if (!ctx) return SPHSWE_OK;
 	hipSetDevice(ctx->device);
 	if (ctx->stream) hipStreamSynchronize(ctx->stream);
-	void *ptrs[] = {ctx->d_ct, ctx->d_st, ctx->d_sq, ctx->d_coslat, ctx->d_wg, ctx->d_coef, ctx->d_l0,
+	void *ptrs[] = {ctx->d_ct, ctx->d_st, ctx->d_sq, ctx->d_coslat, ctx->d_wg, ctx->d_coef,
 			ctx->d_seed, ctx->d_segoff, ctx->d_unit_m, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_flag};
 	for (void *p : ptrs)
 		if (p) hipFree(p);

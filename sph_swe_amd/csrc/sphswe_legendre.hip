@@ -3,18 +3,18 @@
 //
 // Pbar_l^m(mu_j) is never stored: every lane runs the FP64 three-term
 // recurrence Pbar_l = a_lm (mu Pbar_{l-1} - b_lm Pbar_{l-2}) for its own
-// latitude, started from per-(m, j) seeds computed at setup (so tiny values
-// near the poles never underflow).  Only the northern rings are visited; the
-// southern ones follow from Pbar_l^m(-mu) = (-1)^(l-m) Pbar_l^m(mu), so every
-// pass splits into an even and an odd partial sum.
+// latitude, restarted every 32 degrees from seeds computed at setup (values
+// that underflow there stay below 2^-981 over the segment).  Only the
+// northern rings are visited; the southern ones follow from
+// Pbar_l^m(-mu) = (-1)^(l-m) Pbar_l^m(mu), so every pass splits into an even
+// and an odd partial sum.
 //
-// Synthesis: one lane per (m, j); the spectral coefficient of degree l is
-// uniform over the wavefront (scalar loads), so a step costs 3 FP64 ops of
-// recurrence plus 2 FMAs per (field, re/im).
-// Analysis: the sum over j is a reduction, so a wavefront writes a 16-degree
-// tile of Pbar (64 rings) to LDS, then re-reads it with lanes mapped to
-// (degree, quarter of the rings), accumulates in registers, and reduces the
-// quarters with cross-lane shuffles and the wavefronts through LDS.
+// Synthesis: lanes are rings; the spectral coefficient of degree l is uniform
+// over the wavefront (scalar loads), so a step costs 3 FP64 ops of recurrence
+// plus 2 FMAs per (field, re/im).
+// Analysis: the sum over rings is a contraction, done on the FP64 matrix
+// cores: a wavefront writes a tile of Pbar (32 degrees x 64 rings) to LDS and
+// multiplies it into the hemisphere-combined, weighted Fourier rows.
 //
 // "Slots": a real field is one slot; a complex field is two (+m and -m, which
 // share Pbar up to the factor (-1)^m).
@@ -22,23 +22,9 @@
 
 namespace {
 
-__device__ inline int wave_min(int v)
-{
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-	return v;
-}
-__device__ inline int wave_max(int v)
-{
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
-	return v;
-}
-
 struct Geo {
 	const double2 *coef;
-	const int *l0;
-	const double2 *seed;    // per degree segment
+	const double2 *seed;    // (Pbar_s0, Pbar_s0+1) at each degree-segment start s0
 	const int *segoff;
 	const double *ct;
 	const double *wg;
@@ -99,6 +85,24 @@ __device__ inline void make_slots(Slot<LAY> (&sl)[NS], int am, int group, int nf
 	}
 }
 
+// address of q(l) of slot ss of order am (what make_slots + Slot::q give),
+// computed from ss without indexing a slot array
+template <int LAY, int NS>
+__device__ inline long slot_qaddr(int ss, int am, int l, int group, int nfields, const Geo &g)
+{
+	const int L = g.L;
+	if (LAY == LAY_REAL) {
+		int f = group * NS + ss;
+		if (f >= nfields) f = nfields - 1;
+		return (long)f * ((long)(L + 1) * (L + 2) / 2) + lm_index(L, l, am);
+	}
+	int f = group * (NS / 2) + (ss >> 1);
+	if (f >= nfields) f = nfields - 1;
+	const int m = (ss & 1) ? -am : am;
+	if (LAY == LAY_CPLX) return (long)f * ((long)(L + 1) * (L + 1)) + (long)l * l + l + m;
+	return ((long)g.moff[m + L] + (l - am)) * g.P + f;
+}
+
 // ----------------------------------------------------------------------------
 // synthesis
 // ----------------------------------------------------------------------------
@@ -109,145 +113,86 @@ __device__ inline double rec_step(double2 c, double mu, double p1, double p0)
 	return fma(c.x * mu, p1, -(c.y * p0));
 }
 
-// Synthesis: a workgroup handles one m and 128 rings (two per lane, i.e. two
-// independent recurrence chains per lane); its wavefronts take groups of G
-// degree segments of that m, each starting from the seeds of the segment that
-// holds its first significant degree.  The coefficient row and the spectral
-// rows of the slots are staged in LDS (broadcast reads); the partial sums of
-// the wavefronts are reduced through LDS in a fixed order (deterministic).
+// Synthesis: a workgroup handles the pair of orders (m, L - m) (together L + 2
+// degrees, so every workgroup has the same work) for 128 rings, two per lane
+// (two independent recurrence chains per lane).  Its wavefronts take
+// contiguous groups of 32-degree segments of one order each and restart the
+// recurrence from the stored seeds at every segment.  Recurrence coefficients
+// and spectral values are uniform over a wavefront (scalar loads); the partial
+// sums of the wavefronts of an order are reduced through LDS in a fixed order
+// (deterministic).
 constexpr int SY_RPL = 2;          // rings per lane
 constexpr int SY_WAVES = 4;        // wavefronts per workgroup
+constexpr int SY_RW = 64 * SY_RPL; // rings per workgroup
 
 template <int NS>
-__host__ __device__ inline size_t synth_lds_bytes(int L, int waves)
+__host__ __device__ inline size_t synth_lds_bytes()
 {
-	size_t cq = sizeof(double2) * (size_t)(L + 1) * (NS + 1);
-	size_t red = sizeof(double) * (size_t)waves * NS * 4 * 64 * SY_RPL;
-	return cq > red ? cq : red;
+	return sizeof(double) * (size_t)(SY_WAVES - 1) * NS * 4 * SY_RW;
 }
 
-template <int LAY, int NS>
-__global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__restrict__ spec,
-							     double2 *__restrict__ four, Geo g, int nfields, int G,
-							     const double *__restrict__ oscale)
+// wavefronts given to the first order of a pair with nseg0 and nseg1 segments:
+// the split that minimises the largest share
+__host__ __device__ inline int synth_split(int nseg0, int nseg1)
 {
-	extern __shared__ __align__(16) unsigned char lds_raw[];
-	double2 *cq = (double2 *)lds_raw;    // [(l - m)][NS + 1]: coef, q of each slot
-	double *red = (double *)lds_raw;     // after the main loop: [wave][NS*4][64*SY_RPL]
-	const int am = blockIdx.y;
+	if (nseg1 == 0) return SY_WAVES;
+	int best = 1, cost = 1 << 30;
+	for (int k = 1; k < SY_WAVES; k++) {
+		const int a = (nseg0 + k - 1) / k, b = (nseg1 + SY_WAVES - k - 1) / (SY_WAVES - k);
+		const int c = a > b ? a : b;
+		if (c < cost) { cost = c; best = k; }
+	}
+	return best;
+}
+
+// partial sums of degrees [s0, se] (s0 at a segment start) of order am over
+// this lane's rings.  Every segment restarts the recurrence from its stored
+// seeds; a seed that underflows only drops terms below 2^-981 (the recurrence
+// grows by less than 2^41 over a segment).
+template <int LAY, int NS>
+__device__ __forceinline__ void synth_range(const double2 *__restrict__ spec, const Slot<LAY> (&sl)[NS], int am, int s0,
+					    int se, int so, const Geo &g, const int (&jcr)[SY_RPL], const double (&mu)[SY_RPL],
+					    double (&eR)[SY_RPL][NS], double (&eI)[SY_RPL][NS], double (&oR)[SY_RPL][NS],
+					    double (&oI)[SY_RPL][NS])
+{
 	const int L = g.L, NJ = g.NJ, seg = g.seg;
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-	int jr[SY_RPL], jcr[SY_RPL], l0r[SY_RPL];
-	double mu[SY_RPL];
-#pragma unroll
-	for (int r = 0; r < SY_RPL; r++) {
-		jr[r] = blockIdx.x * 64 * SY_RPL + r * 64 + lane;
-		jcr[r] = jr[r] < NJ ? jr[r] : NJ - 1;
-		mu[r] = g.ct[jcr[r]];
-		l0r[r] = g.l0[(long)am * NJ + jcr[r]];
-	}
-	const int so = g.segoff[am];
-	const int nseg = g.segoff[am + 1] - so;
-	Slot<LAY> sl[NS];
-	make_slots<LAY, NS>(sl, am, blockIdx.z, nfields, g);
-
-	// stage the row: every thread issues its loads back to back (clamped
-	// addresses), then writes them to LDS
-	const int nrow = L - am + 1;
-	const int ntot = nrow * (NS + 1);
-	constexpr int SK = 8;
-	for (int base = 0; base < ntot; base += SK * (int)blockDim.x) {
-		double2 v[SK];
-#pragma unroll
-		for (int u = 0; u < SK; u++) {
-			const int idx = base + u * (int)blockDim.x + (int)threadIdx.x;
-			const int ic = idx < ntot ? idx : ntot - 1;
-			const int t = ic / (NS + 1), sidx = ic - t * (NS + 1);
-			const int l = am + t;
-			long qsel = sl[0].q(l, am);
-#pragma unroll
-			for (int ss = 1; ss < NS; ss++)
-				if (ss == sidx - 1) qsel = sl[ss].q(l, am);
-			const double2 *src = (sidx == 0) ? g.coef + lm_index(L, l, am) : spec + qsel;
-			v[u] = *src;
-		}
-#pragma unroll
-		for (int u = 0; u < SK; u++) {
-			const int idx = base + u * (int)blockDim.x + (int)threadIdx.x;
-			if (idx < ntot) cq[idx] = v[u];
-		}
-	}
-	__syncthreads();
-
-	double eR[SY_RPL][NS], eI[SY_RPL][NS], oR[SY_RPL][NS], oI[SY_RPL][NS];
-#pragma unroll
-	for (int r = 0; r < SY_RPL; r++)
-#pragma unroll
-		for (int s = 0; s < NS; s++) eR[r][s] = eI[r][s] = oR[r][s] = oI[r][s] = 0.0;
-
-	const int nwork = (nseg + G - 1) / G;
-	if (wave < nwork) {
-		const int s0 = am + wave * G * seg;
-		const int se = min(s0 + G * seg - 1, L);
-		int st[SY_RPL];
-		double2 sd[SY_RPL];
-		int lo = L + 1, hi = -1;
-#pragma unroll
-		for (int r = 0; r < SY_RPL; r++) {
-			st[r] = max(s0, l0r[r]);     // > se: nothing in this range
-			const int kst = (st[r] <= se) ? (st[r] - am) / seg : wave * G;
-			sd[r] = g.seed[(long)(so + kst) * NJ + jcr[r]];
-			lo = min(lo, st[r]);
-			if (st[r] <= se) hi = max(hi, st[r]);
-		}
-		const int lmn = __builtin_amdgcn_readfirstlane(wave_min(lo));
-		const int lmx = __builtin_amdgcn_readfirstlane(wave_max(hi));
+	for (int sb = s0; sb <= se; sb += seg) {
+		const int segend = min(sb + seg - 1, se);
+		const long k = (long)so + (sb - am) / seg;
 		double p0[SY_RPL], p1[SY_RPL];
 #pragma unroll
-		for (int r = 0; r < SY_RPL; r++) p0[r] = p1[r] = 0.0;
-		int l = lmn;
-		// degrees where some lane injects its seeds
-		const int l1 = min(lmx + 1, se);
-		for (; l <= l1; ++l) {
-			const double2 *row = cq + (size_t)(l - am) * (NS + 1);
-			const double2 c = row[0];
-			const bool odd = (l - am) & 1;
-#pragma unroll
-			for (int r = 0; r < SY_RPL; r++) {
-				const double rec = rec_step(c, mu[r], p1[r], p0[r]);
-				const double P = (l == st[r]) ? sd[r].x : ((l == st[r] + 1) ? sd[r].y : rec);
-				p0[r] = p1[r];
-				p1[r] = P;
-#pragma unroll
-				for (int s = 0; s < NS; s++) {
-					const double2 q = row[1 + s];
-					if (odd) { oR[r][s] = fma(q.x, P, oR[r][s]); oI[r][s] = fma(q.y, P, oI[r][s]); }
-					else { eR[r][s] = fma(q.x, P, eR[r][s]); eI[r][s] = fma(q.y, P, eI[r][s]); }
-				}
-			}
+		for (int r = 0; r < SY_RPL; r++) {
+			const double2 sd = g.seed[k * NJ + jcr[r]];
+			p0[r] = sd.x;
+			p1[r] = sd.y;
 		}
-		// plain recurrence: align to even (l - m), then pairs of degrees
-		if (l <= se && ((l - am) & 1)) {
-			const double2 *row = cq + (size_t)(l - am) * (NS + 1);
-#pragma unroll
-			for (int r = 0; r < SY_RPL; r++) {
-				const double P = rec_step(row[0], mu[r], p1[r], p0[r]);
-				p0[r] = p1[r];
-				p1[r] = P;
-#pragma unroll
-				for (int s = 0; s < NS; s++) {
-					oR[r][s] = fma(row[1 + s].x, P, oR[r][s]);
-					oI[r][s] = fma(row[1 + s].y, P, oI[r][s]);
-				}
-			}
-			++l;
-		}
-		for (; l + 1 <= se; l += 2) {
-			const double2 *row = cq + (size_t)(l - am) * (NS + 1);
-			const double2 ce = row[0], co = row[NS + 1];
+		// degrees sb and sb + 1 are the seeds (sb - am is even; the seed of a
+		// degree past L is 0)
+		{
 			double2 qe[NS], qo[NS];
 #pragma unroll
-			for (int s = 0; s < NS; s++) { qe[s] = row[1 + s]; qo[s] = row[NS + 2 + s]; }
+			for (int s = 0; s < NS; s++) {
+				qe[s] = spec[sl[s].q(sb, am)];
+				qo[s] = sb + 1 <= segend ? spec[sl[s].q(sb + 1, am)] : make_double2(0.0, 0.0);
+			}
+#pragma unroll
+			for (int r = 0; r < SY_RPL; r++)
+#pragma unroll
+				for (int s = 0; s < NS; s++) {
+					eR[r][s] = fma(qe[s].x, p0[r], eR[r][s]);
+					eI[r][s] = fma(qe[s].y, p0[r], eI[r][s]);
+					oR[r][s] = fma(qo[s].x, p1[r], oR[r][s]);
+					oI[r][s] = fma(qo[s].y, p1[r], oI[r][s]);
+				}
+		}
+		// pairs of degrees; coefficients and spectral values are uniform over
+		// the wavefront and come through scalar loads
+		int l = sb + 2;
+		for (; l + 1 <= segend; l += 2) {
+			const double2 ce = g.coef[lm_index(L, l, am)], co = g.coef[lm_index(L, l + 1, am)];
+			double2 qe[NS], qo[NS];
+#pragma unroll
+			for (int s = 0; s < NS; s++) { qe[s] = spec[sl[s].q(l, am)]; qo[s] = spec[sl[s].q(l + 1, am)]; }
 #pragma unroll
 			for (int r = 0; r < SY_RPL; r++) {
 				const double Pe = rec_step(ce, mu[r], p1[r], p0[r]);
@@ -263,64 +208,109 @@ __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__re
 				}
 			}
 		}
-		if (l <= se) {
-			const double2 *row = cq + (size_t)(l - am) * (NS + 1);
+		if (l == segend) {
+			// a last single degree (even parity)
+			const double2 c = g.coef[lm_index(L, l, am)];
+			double2 q[NS];
+#pragma unroll
+			for (int s = 0; s < NS; s++) q[s] = spec[sl[s].q(l, am)];
 #pragma unroll
 			for (int r = 0; r < SY_RPL; r++) {
-				const double P = rec_step(row[0], mu[r], p1[r], p0[r]);
+				const double P = rec_step(c, mu[r], p1[r], p0[r]);
 #pragma unroll
 				for (int s = 0; s < NS; s++) {
-					eR[r][s] = fma(row[1 + s].x, P, eR[r][s]);
-					eI[r][s] = fma(row[1 + s].y, P, eI[r][s]);
+					eR[r][s] = fma(q[s].x, P, eR[r][s]);
+					eI[r][s] = fma(q[s].y, P, eI[r][s]);
 				}
 			}
 		}
 	}
-	__syncthreads();   // LDS now holds the partial sums of the wavefronts
-	constexpr int RW = 64 * SY_RPL;
-	if (wave > 0 && wave < nwork) {
-		double *rw = red + (size_t)wave * NS * 4 * RW;
+}
+
+template <int LAY, int NS>
+__global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__restrict__ spec,
+							     double2 *__restrict__ four, Geo g, int nfields,
+							     const double *__restrict__ oscale)
+{
+	extern __shared__ __align__(16) unsigned char lds_raw[];
+	double *red = (double *)lds_raw;     // [wave - 1][NS*4][SY_RW]
+	const int L = g.L, NJ = g.NJ, seg = g.seg;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int ma = blockIdx.y, mb = L - (int)blockIdx.y;
+	const bool two = mb > ma;            // the middle order of an even L stands alone
+	const int nrow0 = L - ma + 1, nrow1 = two ? L - mb + 1 : 0;
+	const int nseg0 = (nrow0 + seg - 1) / seg, nseg1 = (nrow1 + seg - 1) / seg;
+	const int kA = synth_split(nseg0, nseg1);
+	// this wavefront's order and segment range [sa, sb)
+	const int part = __builtin_amdgcn_readfirstlane(wave >= kA);
+	const int am = __builtin_amdgcn_readfirstlane(part ? mb : ma);
+	const int nw = part ? SY_WAVES - kA : kA, wi = part ? wave - kA : wave, ns = part ? nseg1 : nseg0;
+	const int sa = wi * ns / nw, sb = (wi + 1) * ns / nw;
+
+	int jr[SY_RPL], jcr[SY_RPL];
+	double mu[SY_RPL];
+#pragma unroll
+	for (int r = 0; r < SY_RPL; r++) {
+		jr[r] = blockIdx.x * SY_RW + r * 64 + lane;
+		jcr[r] = jr[r] < NJ ? jr[r] : NJ - 1;
+		mu[r] = g.ct[jcr[r]];
+	}
+	Slot<LAY> sl[NS];
+	make_slots<LAY, NS>(sl, am, blockIdx.z, nfields, g);
+
+	double eR[SY_RPL][NS], eI[SY_RPL][NS], oR[SY_RPL][NS], oI[SY_RPL][NS];
+#pragma unroll
+	for (int r = 0; r < SY_RPL; r++)
+#pragma unroll
+		for (int s = 0; s < NS; s++) eR[r][s] = eI[r][s] = oR[r][s] = oI[r][s] = 0.0;
+	if (sb > sa)
+		synth_range<LAY, NS>(spec, sl, am, __builtin_amdgcn_readfirstlane(am + sa * seg),
+				     __builtin_amdgcn_readfirstlane(min(am + sb * seg - 1, L)), g.segoff[am], g, jcr, mu, eR, eI,
+				     oR, oI);
+	// the first wavefront of each order reduces; the others write their partials
+	const bool leader = wi == 0;
+	if (!leader) {
+		double *rw = red + (size_t)(wave - 1) * NS * 4 * SY_RW;
 #pragma unroll
 		for (int r = 0; r < SY_RPL; r++)
 #pragma unroll
 			for (int s = 0; s < NS; s++) {
 				const int o = r * 64 + lane;
-				rw[(4 * s + 0) * RW + o] = eR[r][s];
-				rw[(4 * s + 1) * RW + o] = eI[r][s];
-				rw[(4 * s + 2) * RW + o] = oR[r][s];
-				rw[(4 * s + 3) * RW + o] = oI[r][s];
+				rw[(4 * s + 0) * SY_RW + o] = eR[r][s];
+				rw[(4 * s + 1) * SY_RW + o] = eI[r][s];
+				rw[(4 * s + 2) * SY_RW + o] = oR[r][s];
+				rw[(4 * s + 3) * SY_RW + o] = oI[r][s];
 			}
 	}
 	__syncthreads();
-	if (wave == 0) {
-		for (int w = 1; w < nwork; w++) {
-			const double *rw = red + (size_t)w * NS * 4 * RW;
+	if (!leader) return;
+	for (int w = wave + 1; w < wave + nw; w++) {
+		const double *rw = red + (size_t)(w - 1) * NS * 4 * SY_RW;
 #pragma unroll
-			for (int r = 0; r < SY_RPL; r++)
-#pragma unroll
-				for (int s = 0; s < NS; s++) {
-					const int o = r * 64 + lane;
-					eR[r][s] += rw[(4 * s + 0) * RW + o];
-					eI[r][s] += rw[(4 * s + 1) * RW + o];
-					oR[r][s] += rw[(4 * s + 2) * RW + o];
-					oI[r][s] += rw[(4 * s + 3) * RW + o];
-				}
-		}
-#pragma unroll
-		for (int r = 0; r < SY_RPL; r++) {
-			const int j = jr[r];
-			if (j >= NJ) continue;
-			const int js = g.nlat - 1 - j;
+		for (int r = 0; r < SY_RPL; r++)
 #pragma unroll
 			for (int s = 0; s < NS; s++) {
-				if (!sl[s].valid) continue;
-				const double sc = oscale ? sl[s].sgn * oscale[j] : sl[s].sgn;   // oscale symmetric in j
-				double2 nn = make_double2(sc * (eR[r][s] + oR[r][s]), sc * (eI[r][s] + oI[r][s]));
-				double2 ss2 = make_double2(sc * (eR[r][s] - oR[r][s]), sc * (eI[r][s] - oI[r][s]));
-				if (!(LAY != LAY_REAL) && am == 0) { nn.y = 0.0; ss2.y = 0.0; }  // Im q_l0 is ignored
-				four[sl[s].fb + j] = nn;
-				four[sl[s].fb + js] = ss2;
+				const int o = r * 64 + lane;
+				eR[r][s] += rw[(4 * s + 0) * SY_RW + o];
+				eI[r][s] += rw[(4 * s + 1) * SY_RW + o];
+				oR[r][s] += rw[(4 * s + 2) * SY_RW + o];
+				oI[r][s] += rw[(4 * s + 3) * SY_RW + o];
 			}
+	}
+#pragma unroll
+	for (int r = 0; r < SY_RPL; r++) {
+		const int j = jr[r];
+		if (j >= NJ) continue;
+		const int js = g.nlat - 1 - j;
+#pragma unroll
+		for (int s = 0; s < NS; s++) {
+			if (!sl[s].valid) continue;
+			const double sc = oscale ? sl[s].sgn * oscale[j] : sl[s].sgn;   // oscale symmetric in j
+			double2 nn = make_double2(sc * (eR[r][s] + oR[r][s]), sc * (eI[r][s] + oI[r][s]));
+			double2 ss2 = make_double2(sc * (eR[r][s] - oR[r][s]), sc * (eI[r][s] - oI[r][s]));
+			if (!(LAY != LAY_REAL) && am == 0) { nn.y = 0.0; ss2.y = 0.0; }  // Im q_l0 is ignored
+			four[sl[s].fb + j] = nn;
+			four[sl[s].fb + js] = ss2;
 		}
 	}
 }
@@ -401,33 +391,18 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__res
 		const int j = c * 64 + lane;
 		const int jc = j < NJ ? j : NJ - 1;
 		const double mu = g.ct[jc];
-		const int st = max(s0, g.l0[(long)am * NJ + jc]);
 		const double2 sd = g.seed[(long)u * NJ + jc];
-		const bool all_at_s0 = __builtin_amdgcn_readfirstlane(wave_max(j < NJ ? st : s0)) == s0;
-		if (all_at_s0) {
-			// every lane starts at s0: seeds, then the plain recurrence
-			// rows past se (short last segment) hold finite values that are never stored
-			double p0 = sd.x, p1 = sd.y;
-			P[0][0][lane] = p0;
-			P[1][0][lane] = p1;
+		// seeds, then the plain recurrence; rows past se (short last segment)
+		// hold finite values that are never stored
+		double p0 = sd.x, p1 = sd.y;
+		P[0][0][lane] = p0;
+		P[1][0][lane] = p1;
 #pragma unroll
-			for (int t = 2; t < AN_SEG; t++) {
-				const double Pv = rec_step(cf[t], mu, p1, p0);
-				p0 = p1;
-				p1 = Pv;
-				P[t & 1][t >> 1][lane] = Pv;
-			}
-		} else {
-			double p0 = 0.0, p1 = 0.0;
-#pragma unroll
-			for (int t = 0; t < AN_SEG; t++) {
-				const int l = s0 + t;
-				const double rec = rec_step(cf[t], mu, p1, p0);
-				const double Pv = (l < st) ? 0.0 : ((l == st) ? sd.x : ((l == st + 1) ? sd.y : rec));
-				p0 = p1;
-				p1 = Pv;
-				P[t & 1][t >> 1][lane] = Pv;
-			}
+		for (int t = 2; t < AN_SEG; t++) {
+			const double Pv = rec_step(cf[t], mu, p1, p0);
+			p0 = p1;
+			p1 = Pv;
+			P[t & 1][t >> 1][lane] = Pv;
 		}
 		double be[16], bo[16];
 #pragma unroll
@@ -476,7 +451,6 @@ Geo geo_of(sphswe_ctx *ctx)
 {
 	Geo g;
 	g.coef = ctx->d_coef;
-	g.l0 = ctx->d_l0;
 	g.seed = ctx->d_seed;
 	g.segoff = ctx->d_segoff;
 	g.seg = ctx->seg;
@@ -494,16 +468,14 @@ template <int LAY, int NS>
 int launch_synth(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 *four, int nf, const double *oscale)
 {
 	const int per = (LAY != LAY_REAL) ? NS / 2 : NS;
-	const int nseg0 = (ctx->L + ctx->seg) / ctx->seg;   // segments of m = 0
-	const int waves = nseg0 < SY_WAVES ? nseg0 : SY_WAVES;
-	const int G = (nseg0 + waves - 1) / waves;
-	const size_t lds = synth_lds_bytes<NS>(ctx->L, waves);
+	const size_t lds = synth_lds_bytes<NS>();
 	if ((long)lds > ctx->max_lds) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "synthesis needs %zu bytes of LDS", lds);
 	if (lds > 65536)
 		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_leg_synth<LAY, NS>,
 						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-	dim3 grid((ctx->NJ + 64 * SY_RPL - 1) / (64 * SY_RPL), ctx->L + 1, (nf + per - 1) / per);
-	k_leg_synth<LAY, NS><<<grid, 64 * waves, lds, ctx->stream>>>(spec, four, g, nf, G, oscale);
+	// one workgroup per (128 rings, pair of orders (m, L - m), group of slots)
+	dim3 grid((ctx->NJ + SY_RW - 1) / SY_RW, ctx->L / 2 + 1, (nf + per - 1) / per);
+	k_leg_synth<LAY, NS><<<grid, 64 * SY_WAVES, lds, ctx->stream>>>(spec, four, g, nf, oscale);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }

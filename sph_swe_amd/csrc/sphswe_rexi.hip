@@ -83,17 +83,19 @@ __device__ inline double2 wave_sum(double2 v)
 // m of workgroup row y: 0, -1, 1, -2, 2, ... (longest systems first)
 __device__ inline int m_of(int y) { return (y & 1) ? -((y + 1) >> 1) : (y >> 1); }
 
-// swap window[0] <-> window[k] for a runtime k in [0, K]
+// swap window[0] <-> window[k] for a runtime k in [0, K], as K pairwise
+// conditional swaps on registers (a select chain reading w[k] is turned into an
+// indexed access by the compiler, which moves the window to scratch memory)
 template <int K>
 __device__ inline void pivot_swap(double2 (&w)[K + 1], int k)
 {
-	double2 t = w[0];
 #pragma unroll
-	for (int i = 1; i <= K; i++) t = (k == i) ? w[i] : t;
-#pragma unroll
-	for (int i = 1; i <= K; i++)
-		if (k == i) w[i] = w[0];
-	w[0] = t;
+	for (int i = 1; i <= K; i++) {
+		const bool s = (k == i);
+		const double2 a = w[0], b = w[i];
+		w[0] = make_double2(s ? b.x : a.x, s ? b.y : a.y);
+		w[i] = make_double2(s ? a.x : b.x, s ? a.y : b.y);
+	}
 }
 
 struct PhiArgs {
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs a)
 	const double2 al2 = cm(al, al);
 	const double2 *B = a.basis;
 
-	auto rhs_at = [&](int j) -> double2 {
+	auto rhs_at = [&](int j) __attribute__((always_inline)) -> double2 {
 		const int n = am + j;
 		const long k = (long)n * n + n + m;
 		const double2 b1 = B[B_DIV * (long)a.ncplx + k], b3 = B[B_PHI * (long)a.ncplx + k];
@@ -153,15 +155,6 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs a)
 	};
 
 	double2 phi[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};   // phi(j), phi(j+1), phi(j+2)
-	auto emit = [&](int j, double2 x_j, double2 x_jm1) {
-		// D' phi at row j needs phi(j-1) and phi(j+1)
-		const int n = am + j;
-		const double dn = n, dm = m;
-		const double c1 = (-dn + 1.0) * idR(dn - 1, dm), c2 = (dn + 2.0) * idS(dn + 1, dm);
-		const double2 d = ca(cs(x_jm1, c1), cs(phi[1], c2));
-		(void)x_j;
-		if (pv) a.X[(row0 + j) * 2 * P + P + p] = d;
-	};
 
 	if (!a.coriolis) {
 		// diagonal Helmholtz: phi = rhs / (alpha^2 + (Phi / r^2) n (n+1)) is done by
@@ -234,7 +227,6 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs a)
 		const double c2 = (dn + 2.0) * idS(dn + 1, dm);
 		if (pv) a.X[row0 * 2 * P + P + p] = cs(phi[1], c2);
 	}
-	(void)emit;
 }
 
 struct VelArgs {
@@ -269,7 +261,7 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs a)
 	const double2 *B = a.basis;
 
 	// a(n) = u0 + G_lon / r, b(n) = v0 + G_lat / r
-	auto ab_at = [&](int j, double2 &av, double2 &bv) {
+	auto ab_at = [&](int j, double2 &av, double2 &bv) __attribute__((always_inline)) {
 		if (j < 0 || j >= N) { av = bv = make_double2(0, 0); return; }
 		const int n = am + j;
 		const long k = (long)n * n + n + m;
@@ -298,7 +290,7 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs a)
 	aw[0] = bw[0] = make_double2(0, 0);
 	ab_at(0, aw[1], bw[1]);
 	ab_at(1, aw[2], bw[2]);
-	auto rhs_next = [&](int j, double2 &ra, double2 &rb) {
+	auto rhs_next = [&](int j, double2 &ra, double2 &rb) __attribute__((always_inline)) {
 		// uses aw/bw = values at (j-1, j, j+1), then shifts in j+2
 		const int n = am + j;
 		const double dn = n, dm = m;

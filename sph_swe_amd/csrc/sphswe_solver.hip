@@ -51,14 +51,16 @@ __global__ __launch_bounds__(64) void k_band_solve(const double2 *__restrict__ f
 	for (int j = 0; j < N; j++) {
 		const double2 *f = fac + (row0 + j) * NE;
 		const int k = piv[row0 + j];
-		// interchange rows j and j + k of the right-hand side
-		double2 tmp = w[0];
+		// interchange rows j and j + k of the right-hand side, as pairwise
+		// conditional swaps (a select chain reading w[k] becomes an indexed
+		// access and moves the window to scratch memory)
 #pragma unroll
-		for (int i = 1; i <= KL; i++) tmp = (k == i) ? w[i] : tmp;
-#pragma unroll
-		for (int i = 1; i <= KL; i++)
-			if (k == i) w[i] = w[0];
-		w[0] = tmp;
+		for (int i = 1; i <= KL; i++) {
+			const bool s = (k == i);
+			const double2 a = w[0], b = w[i];
+			w[0] = make_double2(s ? b.x : a.x, s ? b.y : a.y);
+			w[i] = make_double2(s ? a.x : b.x, s ? a.y : b.y);
+		}
 #pragma unroll
 		for (int i = 1; i <= KL; i++) w[i] = bsub(w[i], bmul(f[KV + i], w[0]));
 		y[row0 + j] = w[0];
