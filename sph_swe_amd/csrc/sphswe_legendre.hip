@@ -186,25 +186,45 @@ __device__ __forceinline__ void synth_range(const double2 *__restrict__ spec, co
 				}
 		}
 		// pairs of degrees; coefficients and spectral values are uniform over
-		// the wavefront and come through scalar loads
+		// the wavefront and come through scalar loads, issued one pair ahead
+		// so that their latency overlaps the arithmetic of the current pair
+		// (the last pair of the segment reloads itself: every load stays
+		// within [sb, segend])
 		int l = sb + 2;
-		for (; l + 1 <= segend; l += 2) {
-			const double2 ce = g.coef[lm_index(L, l, am)], co = g.coef[lm_index(L, l + 1, am)];
+		if (l + 1 <= segend) {
+			double2 ce = g.coef[lm_index(L, l, am)], co = g.coef[lm_index(L, l + 1, am)];
 			double2 qe[NS], qo[NS];
 #pragma unroll
 			for (int s = 0; s < NS; s++) { qe[s] = spec[sl[s].q(l, am)]; qo[s] = spec[sl[s].q(l + 1, am)]; }
+			for (;;) {
+				const bool more = l + 3 <= segend;
+				const int ln = more ? l + 2 : l;
+				const double2 ce_n = g.coef[lm_index(L, ln, am)], co_n = g.coef[lm_index(L, ln + 1, am)];
+				double2 qe_n[NS], qo_n[NS];
 #pragma unroll
-			for (int r = 0; r < SY_RPL; r++) {
-				const double Pe = rec_step(ce, mu[r], p1[r], p0[r]);
-				const double Po = rec_step(co, mu[r], Pe, p1[r]);
-				p0[r] = Pe;
-				p1[r] = Po;
+				for (int s = 0; s < NS; s++) { qe_n[s] = spec[sl[s].q(ln, am)]; qo_n[s] = spec[sl[s].q(ln + 1, am)]; }
+#pragma unroll
+				for (int r = 0; r < SY_RPL; r++) {
+					const double Pe = rec_step(ce, mu[r], p1[r], p0[r]);
+					const double Po = rec_step(co, mu[r], Pe, p1[r]);
+					p0[r] = Pe;
+					p1[r] = Po;
+#pragma unroll
+					for (int s = 0; s < NS; s++) {
+						eR[r][s] = fma(qe[s].x, Pe, eR[r][s]);
+						eI[r][s] = fma(qe[s].y, Pe, eI[r][s]);
+						oR[r][s] = fma(qo[s].x, Po, oR[r][s]);
+						oI[r][s] = fma(qo[s].y, Po, oI[r][s]);
+					}
+				}
+				l += 2;
+				if (!more) break;
+				ce = ce_n;
+				co = co_n;
 #pragma unroll
 				for (int s = 0; s < NS; s++) {
-					eR[r][s] = fma(qe[s].x, Pe, eR[r][s]);
-					eI[r][s] = fma(qe[s].y, Pe, eI[r][s]);
-					oR[r][s] = fma(qo[s].x, Po, oR[r][s]);
-					oI[r][s] = fma(qo[s].y, Po, oI[r][s]);
+					qe[s] = qe_n[s];
+					qo[s] = qo_n[s];
 				}
 			}
 		}
