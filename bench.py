@@ -5,7 +5,10 @@
 A step is one SHT round trip (synthesis + analysis) of 3 fields (phi, u, v) at
 T511 on the 768 x 1536 Gaussian grid, FP64, spectral inputs already resident
 in HBM, seeded coefficients with power ~ l^-3.  The JSON line also carries the
-nonlinear SWE RK4 rate at T511 (Gaussian dam, dt = 0.001 a / nlat = 8.296 s),
+same round trip with the Legendre contractions in FP32, BF16 (hi/lo) and BF16
+(hi/mid/lo) with per-field errors, the nonlinear SWE RK4 rate at T511
+(Gaussian dam, dt = 0.001 a / nlat = 8.296 s) in FP64 and in each reduced mode
+with the geopotential error after 100 steps,
 the linear REXI rate at T256 (200 poles, dt = 3600 s, seeded (phi, vort, div)
 start state), the roofline of the dominant kernel (the Legendre synthesis)
 measured with HIP events on the library's stream, and the CPU oracle timed on a
@@ -149,6 +152,87 @@ def bench_rexi(steps, L=256, M=188):
     return steps / (ms / 1000.0), cfg
 
 
+MODES = ("fp32", "bf16", "bf16x3")
+
+
+def _rel(x, ref):
+    return float(np.linalg.norm(x - ref) / np.linalg.norm(ref)), float(np.abs(x - ref).max() / np.abs(ref).max())
+
+
+def bench_sht_modes(ctx, steps, warmup):
+    """SHT round trip of 3 fields at T511 with the Legendre contractions in each
+    reduced mode: round trips/s (HIP events) and, per field, the relative L2 and
+    Linf errors of the synthesis and of the analysis against the FP64 GPU path
+    (itself within 1e-13 of the FP64 oracle)."""
+    lib, h = ctx.lib, ctx.h
+    q = seeded_spec(ctx.L, NFIELDS)
+    spec = ctx.to_device(q)
+    spat = ctx.empty(NFIELDS * ctx.nspat * 8)
+    spec2 = ctx.empty(NFIELDS * ctx.nlm * 16)
+    ref_f = ctx.synthesis(q)
+    ref_q = ctx.analysis(ref_f)
+    out = {}
+    for name in MODES:
+        prec = capi.PRECISIONS[name]
+
+        def round_trip():
+            ctx._check(lib.sphswe_sht_synthesis(h, spec.ptr, spat.ptr, NFIELDS, prec))
+            ctx._check(lib.sphswe_sht_analysis(h, spat.ptr, spec2.ptr, NFIELDS, prec))
+
+        for _ in range(warmup):
+            round_trip()
+        ctx.sync()
+        ctx.mark(10)
+        for _ in range(steps):
+            round_trip()
+        ctx.mark(11)
+        ms = ctx.elapsed_ms(10, 11)
+        f = ctx.synthesis(q, prec)
+        qa = ctx.analysis(ref_f, prec)
+        es = [_rel(f[k], ref_f[k]) for k in range(NFIELDS)]
+        ea = [_rel(qa[k], ref_q[k]) for k in range(NFIELDS)]
+        out[name] = {"round_trips_per_s": steps / (ms / 1000.0),
+                     "synthesis_rel_l2": [e[0] for e in es], "synthesis_rel_linf": [e[1] for e in es],
+                     "analysis_rel_l2": [e[0] for e in ea], "analysis_rel_linf": [e[1] for e in ea]}
+    return out
+
+
+def bench_swe_modes(ctx, nsteps=100):
+    """Nonlinear SWE at T511 (Gaussian dam, 100 steps) in FP64 and with the
+    Legendre contractions in each reduced mode: steps/s and the relative L2
+    error of the geopotential spectrum (phi = g h) after the run against the
+    FP64 GPU run.  dt = 0.0005 a / nlat = 4.148 s, half the reference's rule:
+    at 8.296 s the m = 1 polar mode of the scalar-wind formulation grows by
+    about 1.6 per step and the FP64 run itself overflows before step 100
+    (stable up to about 7.5 s), so errors there would measure that growth."""
+    lon = np.arange(ctx.nphi) / ctx.nphi * 2 * np.pi
+    hgt = gaussian_dam(lon, ctx.ct)
+    z = np.zeros_like(hgt)
+    x0 = ctx.analysis(np.stack([hgt, z, z]))
+    dt = 0.0005 * 6.37122e6 / ctx.nlat
+    res, finals = {}, {}
+    for name in ("fp64",) + MODES:
+        prm = capi.swe_params(dt=dt, precision=capi.PRECISIONS[name])
+        warm = ctx.to_device(x0)
+        ctx.swe_rk4(prm, warm, 1, 2)
+        state = ctx.to_device(x0)
+        ctx.sync()
+        ctx.mark(12)
+        ctx.swe_rk4(prm, state, 1, nsteps)
+        ctx.mark(13)
+        ms = ctx.elapsed_ms(12, 13)
+        finals[name] = ctx.to_host(state, np.complex128, (3, ctx.nlm))
+        if not np.all(np.isfinite(finals[name])):
+            raise RuntimeError("SWE state became non-finite (%s)" % name)
+        res[name] = {"steps_per_s": nsteps / (ms / 1000.0)}
+    for name in MODES:
+        res[name]["phi_rel_l2"] = _rel(finals[name][0], finals["fp64"][0])[0]
+        res[name]["u_rel_l2"] = _rel(finals[name][1], finals["fp64"][1])[0]
+        res[name]["v_rel_l2"] = _rel(finals[name][2], finals["fp64"][2])[0]
+        res[name]["speedup_vs_fp64"] = res[name]["steps_per_s"] / res["fp64"]["steps_per_s"]
+    return {"steps": nsteps, "dt": dt, "modes": res}
+
+
 def cpu_baseline(L, ctx_nlat, ctx_nphi):
     """The oracle (single-threaded C restatement) on a bounded sample: one
     round trip of 3 fields at T511."""
@@ -224,6 +308,8 @@ def main():
     ms_syn, ms_ana, flops = bench_legendre_kernel(ctx)
     swe_rate, swe_dt = bench_swe(ctx, args.swe_steps, 2)
     achieved = flops / (ms_syn * 1e-3) / 1e12
+    mixed_sht = bench_sht_modes(ctx, max(args.steps // 4, 10), 3)
+    mixed_swe = bench_swe_modes(ctx) if args.swe_steps > 0 else None
     rexi_rate, rexi_cfg = bench_rexi(args.rexi_steps) if args.rexi_steps > 0 else (None, None)
     cpu = None if args.no_cpu else cpu_baseline(L_BENCH, ctx.nlat, ctx.nphi)
     line = {
@@ -246,6 +332,8 @@ def main():
         "swe_steps_per_s": swe_rate,
         "swe_config": {"lmax": L_BENCH, "dt": swe_dt, "scheme": "RK4 nonlinear", "steps": args.swe_steps,
                        "viscosity": 1e5},
+        "mixed_sht_T511_3fields": mixed_sht,
+        "mixed_swe_T511_dam_100steps": mixed_swe,
         "rexi_steps_per_s": rexi_rate,
         "rexi_config": rexi_cfg,
         "roofline": {"bound": "mfma", "engine": "fp64 VALU (fp64 vector peak = fp64 matrix peak on MI355X)",

@@ -38,10 +38,20 @@ enum {
 	SPHSWE_ERR_NAN = 5        /* NaN/Inf detected */
 };
 
-/* precision of the Legendre contraction; the recurrence is always FP64 */
+/* precision of the Legendre contraction; the associated-Legendre recurrence is
+ * always FP64 and the Fourier stage always FP64.  The reduced modes run the
+ * contraction on the matrix cores with FP32 accumulation:
+ *   FP32    operands rounded to FP32;
+ *   BF16    operands split into bf16 hi + lo, products hi*hi + hi*lo + lo*hi;
+ *   BF16X3  operands split into bf16 hi + mid + lo (exact for FP32 values),
+ *           the six products of total order <= 2.
+ * The real and complex transforms take every mode; the pole-layout transforms
+ * inside the REXI step are FP64. */
 enum {
 	SPHSWE_PREC_FP64 = 0,
-	SPHSWE_PREC_FP32 = 1
+	SPHSWE_PREC_FP32 = 1,
+	SPHSWE_PREC_BF16 = 2,
+	SPHSWE_PREC_BF16X3 = 3
 };
 
 /* ---- configuration (replaces shtns_create / shtns_set_grid{,_auto} and

@@ -18,100 +18,13 @@
 //
 // "Slots": a real field is one slot; a complex field is two (+m and -m, which
 // share Pbar up to the factor (-1)^m).
-#include "sphswe_common.h"
+#include "sphswe_leg_common.h"
 
 namespace {
-
-struct Geo {
-	const double2 *coef;
-	const double2 *seed;    // (Pbar_s0, Pbar_s0+1) at each degree-segment start s0
-	const int *segoff;
-	const double *ct;
-	const double *wg;
-	const int *moff;        // pole layout: first row of each m block, index m + L
-	int L, nlat, NJ, seg;
-	int P;                  // pole layout: number of poles (row stride)
-};
-
-// slot addressing ------------------------------------------------------------
-// LAY 0 real:            q(l) = spec[f*nlm + m(2L-m+1)/2 + l], Fourier row f*(L+1)*nlat + m*nlat
-// LAY 1 complex:         q(l) = spec[f*ncplx + l*l + l + m],   Fourier row f*(2L+1)*nlat + (m+L)*nlat
-// LAY 2 complex, poles:  q(l) = spec[(moff[m+L] + l - |m|)*P + p] (pole fastest), Fourier row as LAY 1
-enum { LAY_REAL = 0, LAY_CPLX = 1, LAY_POLE = 2 };
-
-template <int LAY>
-struct Slot {
-	long qb;      // spectral base
-	long qs;      // spectral stride per degree (LAY_POLE)
-	long fb;      // Fourier row base
-	double sgn;   // (-1)^m for negative complex m
-	bool valid;
-	__device__ inline long q(int l, int am) const
-	{
-		return LAY == LAY_REAL ? qb + l : (LAY == LAY_CPLX ? qb + (long)l * l + l : qb + (long)(l - am) * qs);
-	}
-};
-
-template <int LAY, int NS>
-__device__ inline void make_slots(Slot<LAY> (&sl)[NS], int am, int group, int nfields, const Geo &g)
-{
-	const int L = g.L, nlat = g.nlat;
-	const long nlm = (long)(L + 1) * (L + 2) / 2, ncplx = (long)(L + 1) * (L + 1);
-#pragma unroll
-	for (int s = 0; s < NS; s++) {
-		if (LAY == LAY_REAL) {
-			int f = group * NS + s;
-			sl[s].valid = f < nfields;
-			if (f >= nfields) f = nfields - 1;
-			sl[s].qb = f * nlm + lm_index(L, 0, am);
-			sl[s].qs = 1;
-			sl[s].fb = (long)f * (L + 1) * nlat + (long)am * nlat;
-			sl[s].sgn = 1.0;
-		} else {
-			int f = group * (NS / 2) + (s >> 1);
-			int m = (s & 1) ? -am : am;
-			sl[s].valid = f < nfields && !((s & 1) && am == 0);
-			if (f >= nfields) f = nfields - 1;
-			if (LAY == LAY_CPLX) {
-				sl[s].qb = f * ncplx + m;
-				sl[s].qs = 1;
-			} else {
-				sl[s].qb = (long)g.moff[m + L] * g.P + f;
-				sl[s].qs = g.P;
-			}
-			sl[s].fb = (long)f * (2 * L + 1) * nlat + (long)(m + L) * nlat;
-			sl[s].sgn = ((s & 1) && (am & 1)) ? -1.0 : 1.0;
-		}
-	}
-}
-
-// address of q(l) of slot ss of order am (what make_slots + Slot::q give),
-// computed from ss without indexing a slot array
-template <int LAY, int NS>
-__device__ inline long slot_qaddr(int ss, int am, int l, int group, int nfields, const Geo &g)
-{
-	const int L = g.L;
-	if (LAY == LAY_REAL) {
-		int f = group * NS + ss;
-		if (f >= nfields) f = nfields - 1;
-		return (long)f * ((long)(L + 1) * (L + 2) / 2) + lm_index(L, l, am);
-	}
-	int f = group * (NS / 2) + (ss >> 1);
-	if (f >= nfields) f = nfields - 1;
-	const int m = (ss & 1) ? -am : am;
-	if (LAY == LAY_CPLX) return (long)f * ((long)(L + 1) * (L + 1)) + (long)l * l + l + m;
-	return ((long)g.moff[m + L] + (l - am)) * g.P + f;
-}
 
 // ----------------------------------------------------------------------------
 // synthesis
 // ----------------------------------------------------------------------------
-
-// one recurrence step: Pbar_l = (a mu) Pbar_{l-1} - (a b) Pbar_{l-2}
-__device__ inline double rec_step(double2 c, double mu, double p1, double p0)
-{
-	return fma(c.x * mu, p1, -(c.y * p0));
-}
 
 // Synthesis: a workgroup handles the pair of orders (m, L - m) (together L + 2
 // degrees, so every workgroup has the same work) for 128 rings, two per lane
@@ -131,19 +44,6 @@ __host__ __device__ inline size_t synth_lds_bytes()
 	return sizeof(double) * (size_t)(SY_WAVES - 1) * NS * 4 * SY_RW;
 }
 
-// wavefronts given to the first order of a pair with nseg0 and nseg1 segments:
-// the split that minimises the largest share
-__host__ __device__ inline int synth_split(int nseg0, int nseg1)
-{
-	if (nseg1 == 0) return SY_WAVES;
-	int best = 1, cost = 1 << 30;
-	for (int k = 1; k < SY_WAVES; k++) {
-		const int a = (nseg0 + k - 1) / k, b = (nseg1 + SY_WAVES - k - 1) / (SY_WAVES - k);
-		const int c = a > b ? a : b;
-		if (c < cost) { cost = c; best = k; }
-	}
-	return best;
-}
 
 // partial sums of degrees [s0, se] (s0 at a segment start) of order am over
 // this lane's rings.  Every segment restarts the recurrence from its stored
@@ -260,7 +160,7 @@ __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__re
 	const bool two = mb > ma;            // the middle order of an even L stands alone
 	const int nrow0 = L - ma + 1, nrow1 = two ? L - mb + 1 : 0;
 	const int nseg0 = (nrow0 + seg - 1) / seg, nseg1 = (nrow1 + seg - 1) / seg;
-	const int kA = synth_split(nseg0, nseg1);
+	const int kA = synth_split<SY_WAVES>(nseg0, nseg1);
 	// this wavefront's order and segment range [sa, sb)
 	const int part = __builtin_amdgcn_readfirstlane(wave >= kA);
 	const int am = __builtin_amdgcn_readfirstlane(part ? mb : ma);
@@ -467,22 +367,6 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__res
 	}
 }
 
-Geo geo_of(sphswe_ctx *ctx)
-{
-	Geo g;
-	g.coef = ctx->d_coef;
-	g.seed = ctx->d_seed;
-	g.segoff = ctx->d_segoff;
-	g.seg = ctx->seg;
-	g.ct = ctx->d_ct;
-	g.wg = ctx->d_wg;
-	g.moff = nullptr;
-	g.L = ctx->L;
-	g.nlat = ctx->nlat;
-	g.NJ = ctx->NJ;
-	g.P = 0;
-	return g;
-}
 
 template <int LAY, int NS>
 int launch_synth(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 *four, int nf, const double *oscale)
@@ -513,7 +397,7 @@ int launch_anal(sphswe_ctx *ctx, const Geo &g, const double2 *four, double2 *spe
 
 int check_prec(sphswe_ctx *ctx, int prec)
 {
-	if (prec != SPHSWE_PREC_FP64)
+	if (prec < SPHSWE_PREC_FP64 || prec > SPHSWE_PREC_BF16X3)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "precision %d is not available for this transform", prec);
 	return SPHSWE_OK;
 }
@@ -523,6 +407,7 @@ int check_prec(sphswe_ctx *ctx, int prec)
 int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
+	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_synth_mx(ctx, spec, four, nf, prec);
 	const long nlm = ctx->nlm;
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
 	int rc = SPHSWE_OK;
@@ -542,6 +427,7 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
+	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_mx(ctx, four, spec, nf, prec);
 	const long nlm = ctx->nlm;
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
 	int rc = SPHSWE_OK;
@@ -562,6 +448,7 @@ int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf,
 int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
+	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_synth_cplx_mx(ctx, spec, four, nf, prec);
 	const long nc = ctx->ncplx;
 	const long nfour = (long)(2 * ctx->L + 1) * ctx->nlat;
 	int rc = SPHSWE_OK;
@@ -579,6 +466,7 @@ int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, i
 int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
+	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_cplx_mx(ctx, four, spec, nf, prec);
 	return launch_anal<LAY_CPLX, 8>(ctx, geo_of(ctx), four, spec, nf);
 }
 

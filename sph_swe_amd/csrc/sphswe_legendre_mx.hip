@@ -1,0 +1,468 @@
+// Mixed-precision Legendre transforms.  The associated-Legendre values come
+// from the same FP64 recurrence as in the FP64 path (seeds every 32 degrees);
+// only the contraction runs in lower precision, on the matrix cores, with FP32
+// accumulation:
+//   PM_F32     v_mfma_f32_16x16x4f32, operands rounded to FP32;
+//   PM_BF16    v_mfma_f32_16x16x16bf16_1k, each operand split into
+//              hi + lo bf16 (16 bits), products hi*hi + hi*lo + lo*hi;
+//   PM_BF16X3  the same with hi + mid + lo (all 24 bits of the FP32 value),
+//              products of total order <= 2 (six).
+//
+// Synthesis: a workgroup takes 64 rings and the pair of orders (m, L - m); its
+// four wavefronts split the 32-degree segments of the two orders.  Per segment
+// a wavefront runs the recurrence with lane = ring, writes the Pbar tile
+// (degrees x rings, split by parity of l - m) to its LDS region, and adds
+//   D[ring][column] += Pbar^T[ring][degree] * Q[degree][column]
+// for four 16-ring tiles and both parities; the columns are (slot, re/im), 16
+// of them.  The partial sums of the wavefronts of an order are reduced through
+// LDS in a fixed order.
+// Analysis: one wavefront per (m, 32-degree segment), looping over 64-ring
+// chunks, as in the FP64 analysis:
+//   D[degree][column] += Pbar[degree][ring] * E[ring][column]
+// with E the hemisphere-combined Fourier rows times the Gauss weight.
+//
+// Operand layouts (wave64), checked on the device: v_mfma_f32_16x16x4f32
+// A[i][k] in lane i + 16k, B[k][n] in lane n + 16k; v_mfma_f32_16x16x16bf16_1k
+// A[i][k] in lane i + 16 (k / 4) element k % 4, B[k][n] in lane n + 16 (k / 4)
+// element k % 4; both D[i][n] in lane n + 16 (i / 4) element i % 4.
+#include "sphswe_leg_common.h"
+
+namespace {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+enum { PM_F32 = SPHSWE_PREC_FP32, PM_BF16 = SPHSWE_PREC_BF16, PM_BF16X3 = SPHSWE_PREC_BF16X3 };
+
+template <int PM>
+struct Terms {
+	static constexpr int n = PM == PM_BF16 ? 2 : 3;
+};
+
+// bf16 of a finite x, rounded to nearest even
+__device__ inline short bf16_rn(float x)
+{
+	const unsigned u = __float_as_uint(x);
+	return (short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
+
+// split four values into NT bf16 terms: term k is the bf16 of what the
+// previous terms leave (each difference is exact in FP32)
+template <int NT>
+__device__ inline void split4(const float (&v)[4], s4v (&t)[NT])
+{
+#pragma unroll
+	for (int e = 0; e < 4; e++) {
+		float r = v[e];
+#pragma unroll
+		for (int k = 0; k < NT; k++) {
+			const short h = bf16_rn(r);
+			t[k][e] = h;
+			r -= bf16_f(h);
+		}
+	}
+}
+
+// acc += A * B over the split terms of total order below NT
+template <int NT>
+__device__ inline f4v mfma_split(const s4v (&a)[NT], const s4v (&b)[NT], f4v acc)
+{
+	if (NT == 3) {
+		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1], b[1], acc, 0, 0, 0);
+		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b[2], acc, 0, 0, 0);
+		acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[2], b[0], acc, 0, 0, 0);
+	}
+	acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b[1], acc, 0, 0, 0);
+	acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1], b[0], acc, 0, 0, 0);
+	acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b[0], acc, 0, 0, 0);
+	return acc;
+}
+
+// the column of an MFMA lane: slot = n / 2, re/im = n % 2; its validity, the
+// sign (-1)^m of a negative-m complex slot, spectral address and Fourier row
+// base, all from arithmetic on the slot index
+template <int LAY, int NS>
+struct Column {
+	bool valid;
+	double sgn;
+	int f, m;
+};
+
+template <int LAY, int NS>
+__device__ inline Column<LAY, NS> column_of(int slot, int am, int group, int nfields)
+{
+	Column<LAY, NS> c;
+	if (LAY == LAY_REAL) {
+		c.f = group * NS + slot;
+		c.m = am;
+		c.valid = slot < NS && c.f < nfields;
+		c.sgn = 1.0;
+	} else {
+		c.f = group * (NS / 2) + (slot >> 1);
+		c.m = (slot & 1) ? -am : am;
+		c.valid = slot < NS && c.f < nfields && !((slot & 1) && am == 0);
+		c.sgn = ((slot & 1) && (am & 1)) ? -1.0 : 1.0;
+	}
+	if (c.f >= nfields) c.f = nfields - 1;
+	return c;
+}
+
+template <int LAY, int NS>
+__device__ inline long column_q(const Column<LAY, NS> &c, int am, int l, const Geo &g)
+{
+	const int L = g.L;
+	if (LAY == LAY_REAL) return (long)c.f * ((long)(L + 1) * (L + 2) / 2) + lm_index(L, l, am);
+	if (LAY == LAY_CPLX) return (long)c.f * ((long)(L + 1) * (L + 1)) + (long)l * l + l + c.m;
+	return ((long)g.moff[c.m + L] + (l - am)) * g.P + c.f;
+}
+
+template <int LAY, int NS>
+__device__ inline long column_fb(const Column<LAY, NS> &c, const Geo &g)
+{
+	const int L = g.L, nlat = g.nlat;
+	if (LAY == LAY_REAL) return (long)c.f * (L + 1) * nlat + (long)c.m * nlat;
+	return (long)c.f * (2 * L + 1) * nlat + (long)(c.m + L) * nlat;
+}
+
+constexpr int MX_SEG = 32;      // degrees per segment (16 of each parity)
+constexpr int MX_WAVES = 4;     // wavefronts per workgroup
+constexpr int MX_NS = 8;        // slots per group: 16 MFMA columns
+
+// ----------------------------------------------------------------------------
+// synthesis
+// ----------------------------------------------------------------------------
+
+// row stride (floats) of the Pbar tile: conflict-free A-operand reads
+// (16 mod 32 for one value per lane, 12 mod 32 for four degrees per lane)
+template <int PM>
+struct SxStride {
+	static constexpr int v = PM == PM_F32 ? 80 : 76;
+};
+
+template <int LAY, int NS, int PM>
+__global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *__restrict__ spec, double2 *__restrict__ four,
+								Geo g, int nfields, const double *__restrict__ oscale)
+{
+	constexpr int PS = SxStride<PM>::v;
+	__shared__ float Pt[MX_WAVES][2][16][PS];
+	__shared__ double2 Cfs[MX_WAVES][MX_SEG];
+	const int L = g.L, NJ = g.NJ;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int ma = blockIdx.y, mb = L - (int)blockIdx.y;
+	const bool two = mb > ma;
+	const int nrow0 = L - ma + 1, nrow1 = two ? L - mb + 1 : 0;
+	const int nseg0 = (nrow0 + MX_SEG - 1) / MX_SEG, nseg1 = (nrow1 + MX_SEG - 1) / MX_SEG;
+	const int kA = synth_split<MX_WAVES>(nseg0, nseg1);
+	const int part = __builtin_amdgcn_readfirstlane(wave >= kA);
+	const int am = __builtin_amdgcn_readfirstlane(part ? mb : ma);
+	const int nw = part ? MX_WAVES - kA : kA, wi = part ? wave - kA : wave, ns = part ? nseg1 : nseg0;
+	const int sa = wi * ns / nw, sb = (wi + 1) * ns / nw;
+	const int so = g.segoff[am];
+
+	const int j = blockIdx.x * 64 + lane;
+	const int jc = j < NJ ? j : NJ - 1;
+	const double mu = g.ct[jc];
+	const int n = lane & 15, kq = lane >> 4;
+	const Column<LAY, NS> col = column_of<LAY, NS>(n >> 1, am, blockIdx.z, nfields);
+	const int comp = n & 1;
+	const double *sd_ = (const double *)spec;
+
+	f4v acc[2][4];
+#pragma unroll
+	for (int p = 0; p < 2; p++)
+#pragma unroll
+		for (int t = 0; t < 4; t++) acc[p][t] = f4v{0, 0, 0, 0};
+	float (*P)[16][PS] = Pt[wave];
+	double2 *cf = Cfs[wave];
+
+	for (int k = sa; k < sb; k++) {
+		const int s0 = am + k * MX_SEG, se = min(s0 + MX_SEG - 1, L);
+		if (lane < MX_SEG) cf[lane] = g.coef[lm_index(L, min(s0 + lane, se), am)];
+		// B operands: Q[degree][column]; degrees past se are 0
+		float bq[2][4];
+		s4v bs[2][Terms<PM>::n];
+		if (PM == PM_F32) {
+#pragma unroll
+			for (int p = 0; p < 2; p++)
+#pragma unroll
+				for (int s = 0; s < 4; s++) {
+					const int l = s0 + 2 * (4 * s + kq) + p;
+					const int lc = min(l, se);
+					const double v = sd_[2 * column_q<LAY, NS>(col, am, lc, g) + comp];
+					bq[p][s] = (col.valid && l <= se) ? (float)(col.sgn * v) : 0.0f;
+				}
+		} else {
+#pragma unroll
+			for (int p = 0; p < 2; p++) {
+				float v4[4];
+#pragma unroll
+				for (int e = 0; e < 4; e++) {
+					const int l = s0 + 2 * (4 * kq + e) + p;
+					const int lc = min(l, se);
+					const double v = sd_[2 * column_q<LAY, NS>(col, am, lc, g) + comp];
+					v4[e] = (col.valid && l <= se) ? (float)(col.sgn * v) : 0.0f;
+				}
+				split4<Terms<PM>::n>(v4, bs[p]);
+			}
+		}
+		wave_sync();
+		// Pbar tile, lane = ring; rows past se hold finite values that meet Q = 0
+		{
+			const double2 sdv = g.seed[((long)so + k) * NJ + jc];
+			double p0 = sdv.x, p1 = sdv.y;
+			P[0][0][lane] = (float)p0;
+			P[1][0][lane] = (float)p1;
+#pragma unroll
+			for (int t = 2; t < MX_SEG; t++) {
+				const double Pv = rec_step(cf[t], mu, p1, p0);
+				p0 = p1;
+				p1 = Pv;
+				P[t & 1][t >> 1][lane] = (float)Pv;
+			}
+		}
+		wave_sync();
+		if (PM == PM_F32) {
+#pragma unroll
+			for (int p = 0; p < 2; p++)
+#pragma unroll
+				for (int s = 0; s < 4; s++)
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+						acc[p][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(P[p][4 * s + kq][16 * t + n], bq[p][s],
+												 acc[p][t], 0, 0, 0);
+		} else {
+#pragma unroll
+			for (int p = 0; p < 2; p++)
+#pragma unroll
+				for (int t = 0; t < 4; t++) {
+					float a4[4];
+#pragma unroll
+					for (int e = 0; e < 4; e++) a4[e] = P[p][4 * kq + e][16 * t + n];
+					s4v as[Terms<PM>::n];
+					split4<Terms<PM>::n>(a4, as);
+					acc[p][t] = mfma_split<Terms<PM>::n>(as, bs[p], acc[p][t]);
+				}
+		}
+		wave_sync();
+	}
+
+	// reduce the wavefronts of each order (LDS reused), leaders write
+	__syncthreads();
+	float *red = &Pt[0][0][0][0];
+	const bool leader = wi == 0;
+	if (!leader) {
+		float *rw = red + (size_t)(wave - 1) * 32 * 64;
+#pragma unroll
+		for (int p = 0; p < 2; p++)
+#pragma unroll
+			for (int t = 0; t < 4; t++)
+#pragma unroll
+				for (int r = 0; r < 4; r++) rw[((p * 4 + t) * 4 + r) * 64 + lane] = acc[p][t][r];
+	}
+	__syncthreads();
+	if (!leader) return;
+	for (int w = wave + 1; w < wave + nw; w++) {
+		const float *rw = red + (size_t)(w - 1) * 32 * 64;
+#pragma unroll
+		for (int p = 0; p < 2; p++)
+#pragma unroll
+			for (int t = 0; t < 4; t++)
+#pragma unroll
+				for (int r = 0; r < 4; r++) acc[p][t][r] += rw[((p * 4 + t) * 4 + r) * 64 + lane];
+	}
+	if (!col.valid) return;
+	const bool zero_im = LAY == LAY_REAL && am == 0 && comp == 1;   // Im q_l0 is ignored
+	double *fo = (double *)four;
+	const long fb = column_fb<LAY, NS>(col, g);
+#pragma unroll
+	for (int t = 0; t < 4; t++)
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int jj = blockIdx.x * 64 + 16 * t + 4 * kq + r;
+			if (jj >= NJ) continue;
+			const double sc = zero_im ? 0.0 : (oscale ? oscale[jj] : 1.0);
+			const double E = acc[0][t][r], O = acc[1][t][r];
+			fo[2 * (fb + jj) + comp] = sc * (E + O);
+			fo[2 * (fb + g.nlat - 1 - jj) + comp] = sc * (E - O);
+		}
+}
+
+// ----------------------------------------------------------------------------
+// analysis
+// ----------------------------------------------------------------------------
+
+// row stride (floats) of the Pbar tile: 2 mod 32 (one value per lane), a
+// multiple of 4 for the four-ring reads of the bf16 layout
+template <int PM>
+struct AxStride {
+	static constexpr int v = PM == PM_F32 ? 66 : 68;
+};
+
+template <int LAY, int NS, int PM>
+__global__ __launch_bounds__(64 * MX_WAVES) void k_leg_anal_mx(const double2 *__restrict__ four, double2 *__restrict__ spec,
+							       Geo g, const int *__restrict__ unit_m, int nunits, int nfields)
+{
+	constexpr int PS = AxStride<PM>::v;
+	__shared__ __align__(16) float Pt[MX_WAVES][2][16][PS];
+	__shared__ double2 Cfs[MX_WAVES][MX_SEG];
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int u = blockIdx.x * MX_WAVES + wave;
+	if (u >= nunits) return;   // whole wavefront; no workgroup barrier below
+	const int am = unit_m[u];
+	const int L = g.L, NJ = g.NJ, nlat = g.nlat;
+	const int so = g.segoff[am];
+	const int s0 = am + (u - so) * MX_SEG, se = min(s0 + MX_SEG - 1, L);
+
+	double2 *cf = Cfs[wave];
+	if (lane < MX_SEG) cf[lane] = g.coef[lm_index(L, min(s0 + lane, se), am)];
+	wave_sync();
+
+	const int n = lane & 15, kq = lane >> 4;
+	const Column<LAY, NS> col = column_of<LAY, NS>(n >> 1, am, blockIdx.y, nfields);
+	const int comp = n & 1;
+	const double cmask = col.valid ? 1.0 : 0.0;
+	const double *prow = (const double *)four + 2 * column_fb<LAY, NS>(col, g) + comp;
+
+	f4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+	float (*P)[16][PS] = Pt[wave];
+	const int nch = (NJ + 63) / 64;
+	for (int c = 0; c < nch; c++) {
+		// B operands E_even / E_odd of the rings of this lane (ring loads are
+		// clamped to NJ - 1; rings past NJ get weight 0)
+		float be[16], bo[16];
+#pragma unroll
+		for (int s = 0; s < 16; s++) {
+			// F32: ring 4s + kq (16 K-steps of 4); bf16: ring 16 (s/4) + 4 kq + s%4 (4 K-steps of 16)
+			const int rr = PM == PM_F32 ? 4 * s + kq : 16 * (s >> 2) + 4 * kq + (s & 3);
+			const int rj = c * 64 + rr;
+			const int rc = min(rj, NJ - 1);
+			const double fn = prow[2 * rc], fs = prow[2 * (nlat - 1 - rc)];
+			const double w = rj < NJ ? g.wg[rc] * cmask : 0.0;
+			be[s] = (float)(w * (fn + fs));
+			bo[s] = (float)(w * (fn - fs));
+		}
+		// Pbar tile: lane = ring
+		{
+			const int j = c * 64 + lane;
+			const int jc = j < NJ ? j : NJ - 1;
+			const double mu = g.ct[jc];
+			const double2 sdv = g.seed[(long)u * NJ + jc];
+			double p0 = sdv.x, p1 = sdv.y;
+			P[0][0][lane] = (float)p0;
+			P[1][0][lane] = (float)p1;
+#pragma unroll
+			for (int t = 2; t < MX_SEG; t++) {
+				const double Pv = rec_step(cf[t], mu, p1, p0);
+				p0 = p1;
+				p1 = Pv;
+				P[t & 1][t >> 1][lane] = (float)Pv;
+			}
+		}
+		wave_sync();
+		if (PM == PM_F32) {
+#pragma unroll
+			for (int s = 0; s < 16; s++) {
+				acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(P[0][n][4 * s + kq], be[s], acc0, 0, 0, 0);
+				acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(P[1][n][4 * s + kq], bo[s], acc1, 0, 0, 0);
+			}
+		} else {
+			constexpr int NT = Terms<PM>::n;
+#pragma unroll
+			for (int s = 0; s < 4; s++) {
+				float a4[4], b4[4];
+				s4v as[NT], bs[NT];
+#pragma unroll
+				for (int e = 0; e < 4; e++) { a4[e] = P[0][n][16 * s + 4 * kq + e]; b4[e] = be[4 * s + e]; }
+				split4<NT>(a4, as);
+				split4<NT>(b4, bs);
+				acc0 = mfma_split<NT>(as, bs, acc0);
+#pragma unroll
+				for (int e = 0; e < 4; e++) { a4[e] = P[1][n][16 * s + 4 * kq + e]; b4[e] = bo[4 * s + e]; }
+				split4<NT>(a4, as);
+				split4<NT>(b4, bs);
+				acc1 = mfma_split<NT>(as, bs, acc1);
+			}
+		}
+		wave_sync();
+	}
+	// D[i][n] in lane n + 16 (i / 4), element i % 4; row i of parity p is degree s0 + 2i + p
+	if (!col.valid) return;
+	double *sdo = (double *)spec;
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		const int i = 4 * kq + r;
+#pragma unroll
+		for (int p = 0; p < 2; p++) {
+			const int l = s0 + 2 * i + p;
+			if (l <= se) sdo[2 * column_q<LAY, NS>(col, am, l, g) + comp] = col.sgn * (double)(p ? acc1[r] : acc0[r]);
+		}
+	}
+}
+
+template <int LAY, int PM>
+int launch_synth_mx(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 *four, int nf, const double *oscale)
+{
+	const int per = LAY == LAY_REAL ? MX_NS : MX_NS / 2;
+	dim3 grid((ctx->NJ + 63) / 64, ctx->L / 2 + 1, (nf + per - 1) / per);
+	k_leg_synth_mx<LAY, MX_NS, PM><<<grid, 64 * MX_WAVES, 0, ctx->stream>>>(spec, four, g, nf, oscale);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return SPHSWE_OK;
+}
+
+template <int LAY, int PM>
+int launch_anal_mx(sphswe_ctx *ctx, const Geo &g, const double2 *four, double2 *spec, int nf)
+{
+	const int per = LAY == LAY_REAL ? MX_NS : MX_NS / 2;
+	const int nunits = ctx->nseg_total;
+	dim3 grid((nunits + MX_WAVES - 1) / MX_WAVES, (nf + per - 1) / per);
+	k_leg_anal_mx<LAY, MX_NS, PM><<<grid, 64 * MX_WAVES, 0, ctx->stream>>>(four, spec, g, ctx->d_unit_m, nunits, nf);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return SPHSWE_OK;
+}
+
+template <int LAY>
+int synth_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
+{
+	const Geo g = geo_of(ctx);
+	switch (prec) {
+	case PM_F32: return launch_synth_mx<LAY, PM_F32>(ctx, g, spec, four, nf, nullptr);
+	case PM_BF16: return launch_synth_mx<LAY, PM_BF16>(ctx, g, spec, four, nf, nullptr);
+	case PM_BF16X3: return launch_synth_mx<LAY, PM_BF16X3>(ctx, g, spec, four, nf, nullptr);
+	}
+	return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "unknown precision %d", prec);
+}
+
+template <int LAY>
+int anal_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
+{
+	const Geo g = geo_of(ctx);
+	switch (prec) {
+	case PM_F32: return launch_anal_mx<LAY, PM_F32>(ctx, g, four, spec, nf);
+	case PM_BF16: return launch_anal_mx<LAY, PM_BF16>(ctx, g, four, spec, nf);
+	case PM_BF16X3: return launch_anal_mx<LAY, PM_BF16X3>(ctx, g, four, spec, nf);
+	}
+	return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "unknown precision %d", prec);
+}
+
+}  // namespace
+
+int sphswe_leg_synth_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
+{
+	return synth_mx<LAY_REAL>(ctx, spec, four, nf, prec);
+}
+
+int sphswe_leg_anal_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
+{
+	return anal_mx<LAY_REAL>(ctx, four, spec, nf, prec);
+}
+
+int sphswe_leg_synth_cplx_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
+{
+	return synth_mx<LAY_CPLX>(ctx, spec, four, nf, prec);
+}
+
+int sphswe_leg_anal_cplx_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
+{
+	return anal_mx<LAY_CPLX>(ctx, four, spec, nf, prec);
+}

@@ -186,8 +186,9 @@ inline int g1(long n) { return (int)((n + 255) / 256); }
 extern "C" int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmem, int nsteps)
 {
 	if (!ctx || !p || !d_state || nmem < 1 || nsteps < 0) return SPHSWE_ERR_ARG;
-	if (p->precision != SPHSWE_PREC_FP64)
+	if (p->precision < SPHSWE_PREC_FP64 || p->precision > SPHSWE_PREC_BF16X3)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: precision %d not available", p->precision);
+	const int prec = p->precision;   // of the Legendre contractions; everything else is FP64
 	const long nlm = ctx->nlm, nspat = ctx->nspat;
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
 	// workspace: Y (3 nlm), ACC (3), YN (3), S9 (9), S4 (4), S2 (2), TH (1) spectral;
@@ -239,19 +240,19 @@ extern "C" int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, doubl
 		double2 *YN = Ya;
 		for (int s = 0; s < 4; s++) {
 			k_prep9<<<g1(nlm * nm), 256, 0, st>>>(Y, S9, ctx->d_mofidx, L, (int)nlm, nm);
-			if ((rc = sphswe_leg_synth(ctx, S9, F, 9 * nm, SPHSWE_PREC_FP64))) return rc;
+			if ((rc = sphswe_leg_synth(ctx, S9, F, 9 * nm, prec))) return rc;
 			if ((rc = sphswe_fft_c2r(ctx, F, P9, 9 * nm))) return rc;
 			k_phys1<<<g1(nspat * nm), 256, 0, st>>>(P9, P4, ctx->d_ct, ctx->d_st, ctx->d_sq, nlat, nspat, nm, ga,
 								 two_omega, ia, p->nonlinear, p->h0 / a);
 			if ((rc = sphswe_fft_r2c(ctx, P4, F, 4 * nm))) return rc;
-			if ((rc = sphswe_leg_anal(ctx, F, S4, 4 * nm, SPHSWE_PREC_FP64))) return rc;
+			if ((rc = sphswe_leg_anal(ctx, F, S4, 4 * nm, prec))) return rc;
 			k_prep2<<<g1(nlm * nm), 256, 0, st>>>(S4, S2, ctx->d_mofidx, L, (int)nlm, nm);
-			if ((rc = sphswe_leg_synth(ctx, S2, F, 2 * nm, SPHSWE_PREC_FP64))) return rc;
+			if ((rc = sphswe_leg_synth(ctx, S2, F, 2 * nm, prec))) return rc;
 			if ((rc = sphswe_fft_c2r(ctx, F, P2, 2 * nm))) return rc;
 			k_phys2<<<g1(nspat * nm), 256, 0, st>>>(P2, P1, ctx->d_st, ctx->d_sq, nlat, nspat, nm,
 								 p->nonlinear ? ia : p->h0 / a);
 			if ((rc = sphswe_fft_r2c(ctx, P1, F, nm))) return rc;
-			if ((rc = sphswe_leg_anal(ctx, F, TH, nm, SPHSWE_PREC_FP64))) return rc;
+			if ((rc = sphswe_leg_anal(ctx, F, TH, nm, prec))) return rc;
 			k_stage<<<g1(nlm * nm), 256, 0, st>>>(TH, S4, Y, X, ACC, YN, ctx->d_mofidx, L, (int)nlm, nm, visc, s,
 							       bcoef[s], s < 3 ? anext[s] : 0.0, dt);
 			SPHSWE_LAUNCH_CHECK(ctx);
