@@ -50,6 +50,8 @@ struct sphswe_ctx {
 	// scratch (grown on demand, never shrunk)
 	void *d_scratch = nullptr;
 	size_t scratch_bytes = 0;
+	void *d_escratch = nullptr;   // analysis operands (weighted Fourier rows, operand order)
+	size_t escratch_bytes = 0;
 	double *d_flag = nullptr;     // small reduction scratch (64 doubles)
 
 	std::unordered_map<void *, size_t> allocs;
@@ -79,6 +81,7 @@ int sphswe_set_err(sphswe_ctx *ctx, int code, const char *fmt, ...);
 
 // scratch of at least `bytes`, valid until the next call that asks for scratch
 int sphswe_scratch(sphswe_ctx *ctx, size_t bytes, void **p);
+int sphswe_escratch(sphswe_ctx *ctx, size_t bytes, void **p);
 
 // internal launchers ----------------------------------------------------------
 

@@ -270,7 +270,7 @@ extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 	hipSetDevice(ctx->device);
 	if (ctx->stream) hipStreamSynchronize(ctx->stream);
 	void *ptrs[] = {ctx->d_ct, ctx->d_st, ctx->d_sq, ctx->d_coslat, ctx->d_wg, ctx->d_coef,
-			ctx->d_seed, ctx->d_segoff, ctx->d_unit_m, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_flag};
+			ctx->d_seed, ctx->d_segoff, ctx->d_unit_m, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_escratch, ctx->d_flag};
 	for (void *p : ptrs)
 		if (p) hipFree(p);
 	for (auto &kv : ctx->allocs) hipFree(kv.first);
@@ -315,23 +315,33 @@ extern "C" int sphswe_sync(sphswe_ctx *ctx)
 
 extern "C" void *sphswe_stream(sphswe_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-int sphswe_scratch(sphswe_ctx *ctx, size_t bytes, void **p)
+static int grow(sphswe_ctx *ctx, void **buf, size_t *have, size_t bytes, void **p)
 {
-	if (bytes > ctx->scratch_bytes) {
-		if (ctx->d_scratch) {
+	if (bytes > *have) {
+		if (*buf) {
 			SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
-			SPHSWE_HIP(ctx, hipFree(ctx->d_scratch));
-			ctx->d_scratch = nullptr;
+			SPHSWE_HIP(ctx, hipFree(*buf));
+			*buf = nullptr;
 		}
-		size_t nb = std::max(bytes, ctx->scratch_bytes * 3 / 2);
-		if (hipMalloc(&ctx->d_scratch, nb) != hipSuccess) {
-			ctx->scratch_bytes = 0;
+		size_t nb = std::max(bytes, *have * 3 / 2);
+		if (hipMalloc(buf, nb) != hipSuccess) {
+			*have = 0;
 			return sphswe_set_err(ctx, SPHSWE_ERR_NOMEM, "scratch allocation of %zu bytes failed", nb);
 		}
-		ctx->scratch_bytes = nb;
+		*have = nb;
 	}
-	*p = ctx->d_scratch;
+	*p = *buf;
 	return SPHSWE_OK;
+}
+
+int sphswe_scratch(sphswe_ctx *ctx, size_t bytes, void **p)
+{
+	return grow(ctx, &ctx->d_scratch, &ctx->scratch_bytes, bytes, p);
+}
+
+int sphswe_escratch(sphswe_ctx *ctx, size_t bytes, void **p)
+{
+	return grow(ctx, &ctx->d_escratch, &ctx->escratch_bytes, bytes, p);
 }
 
 // ---- device memory pool: freed blocks are cached by size and reused (the

@@ -6,6 +6,8 @@
 
 #include "sphswe_common.h"
 
+#include <algorithm>
+
 namespace {
 
 struct Geo {
@@ -89,6 +91,88 @@ __device__ inline long slot_qaddr(int ss, int am, int l, int group, int nfields,
 	return ((long)g.moff[m + L] + (l - am)) * g.P + f;
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+enum { PM_F64 = SPHSWE_PREC_FP64, PM_F32 = SPHSWE_PREC_FP32, PM_BF16 = SPHSWE_PREC_BF16, PM_BF16X3 = SPHSWE_PREC_BF16X3 };
+
+template <int PM>
+struct Terms {
+	static constexpr int n = PM == PM_BF16 ? 2 : 3;
+};
+
+// bf16 of a finite x, rounded to nearest even
+__device__ inline short bf16_rn(float x)
+{
+	const unsigned u = __float_as_uint(x);
+	return (short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
+
+// split four values into NT bf16 terms: term k is the bf16 of what the
+// previous terms leave (each difference is exact in FP32)
+template <int NT>
+__device__ inline void split4(const float (&v)[4], s4v (&t)[NT])
+{
+#pragma unroll
+	for (int e = 0; e < 4; e++) {
+		float r = v[e];
+#pragma unroll
+		for (int k = 0; k < NT; k++) {
+			const short h = bf16_rn(r);
+			t[k][e] = h;
+			r -= bf16_f(h);
+		}
+	}
+}
+
+
+// the column of an MFMA lane: slot = n / 2, re/im = n % 2; its validity, the
+// sign (-1)^m of a negative-m complex slot, spectral address and Fourier row
+// base, all from arithmetic on the slot index
+template <int LAY, int NS>
+struct Column {
+	bool valid;
+	double sgn;
+	int f, m;
+};
+
+template <int LAY, int NS>
+__device__ inline Column<LAY, NS> column_of(int slot, int am, int group, int nfields)
+{
+	Column<LAY, NS> c;
+	if (LAY == LAY_REAL) {
+		c.f = group * NS + slot;
+		c.m = am;
+		c.valid = slot < NS && c.f < nfields;
+		c.sgn = 1.0;
+	} else {
+		c.f = group * (NS / 2) + (slot >> 1);
+		c.m = (slot & 1) ? -am : am;
+		c.valid = slot < NS && c.f < nfields && !((slot & 1) && am == 0);
+		c.sgn = ((slot & 1) && (am & 1)) ? -1.0 : 1.0;
+	}
+	if (c.f >= nfields) c.f = nfields - 1;
+	return c;
+}
+
+template <int LAY, int NS>
+__device__ inline long column_q(const Column<LAY, NS> &c, int am, int l, const Geo &g)
+{
+	const int L = g.L;
+	if (LAY == LAY_REAL) return (long)c.f * ((long)(L + 1) * (L + 2) / 2) + lm_index(L, l, am);
+	if (LAY == LAY_CPLX) return (long)c.f * ((long)(L + 1) * (L + 1)) + (long)l * l + l + c.m;
+	return ((long)g.moff[c.m + L] + (l - am)) * g.P + c.f;
+}
+
+template <int LAY, int NS>
+__device__ inline long column_fb(const Column<LAY, NS> &c, const Geo &g)
+{
+	const int L = g.L, nlat = g.nlat;
+	if (LAY == LAY_REAL) return (long)c.f * (L + 1) * nlat + (long)c.m * nlat;
+	return (long)c.f * (2 * L + 1) * nlat + (long)(c.m + L) * nlat;
+}
+
 // one recurrence step: Pbar_l = (a mu) Pbar_{l-1} - (a b) Pbar_{l-2}
 __device__ inline double rec_step(double2 c, double mu, double p1, double p0)
 {
@@ -135,6 +219,109 @@ Geo geo_of(sphswe_ctx *ctx)
 	g.NJ = ctx->NJ;
 	g.P = 0;
 	return g;
+}
+
+// ---- analysis operands ------------------------------------------------------
+// E[ring][column] = w_j (F_N + F_S) (even parity of l - m) and w_j (F_N - F_S)
+// (odd), for the 16 MFMA columns of a slot group, written once per (order,
+// 64-ring chunk) in the order in which the analysis lanes consume them, so
+// that the analysis reads them with coalesced loads and every 32-degree
+// segment of the order reuses them.  Rings past NJ get weight 0.
+//   FP64 / FP32: per (m, chunk, parity) 16 steps x 64 lanes of one value;
+//                step s, lane n + 16 kq holds ring 4 s + kq of column n.
+//   bf16 (NT terms): per (m, chunk, parity) 4 steps x NT terms x 64 lanes of
+//                four values; step s, lane n + 16 kq holds rings
+//                16 s + 4 kq + e (e = 0..3) of column n.
+template <int PM>
+__host__ __device__ inline size_t eop_bytes_per_chunk()
+{
+	// both parities
+	return PM == PM_F64 ? 2 * 16 * 64 * 8 : (PM == PM_F32 ? 2 * 16 * 64 * 4 : (size_t)2 * 4 * Terms<PM>::n * 64 * 8);
+}
+
+template <int LAY, int NS, int PM>
+__global__ __launch_bounds__(256) void k_leg_eprep(const double2 *__restrict__ four, void *__restrict__ E, Geo g,
+						   int group0, int nfields)
+{
+	const int nch = (g.NJ + 63) / 64;
+	const long tid = blockIdx.x * 256L + threadIdx.x;
+	if (tid >= (long)(g.L + 1) * nch * 64) return;
+	const int lane = (int)(tid & 63);
+	const long mc = tid >> 6;                 // am * nch + c
+	const int c = (int)(mc % nch), am = (int)(mc / nch);
+	const int group = group0 + (int)blockIdx.y;
+	const int n = lane & 15, kq = lane >> 4;
+	const Column<LAY, NS> col = column_of<LAY, NS>(n >> 1, am, group, nfields);
+	const double *prow = (const double *)four + 2 * column_fb<LAY, NS>(col, g) + (n & 1);
+	const double cmask = col.valid ? 1.0 : 0.0;
+	const int NJ = g.NJ, nlat = g.nlat;
+	double be[16], bo[16];
+#pragma unroll
+	for (int s = 0; s < 16; s++) {
+		const int rr = (PM == PM_F64 || PM == PM_F32) ? 4 * s + kq : 16 * (s >> 2) + 4 * kq + (s & 3);
+		const int rj = c * 64 + rr;
+		const int rc = min(rj, NJ - 1);
+		const double fn = prow[2 * rc], fs = prow[2 * (nlat - 1 - rc)];
+		const double w = rj < NJ ? g.wg[rc] * cmask : 0.0;
+		be[s] = w * (fn + fs);
+		bo[s] = w * (fn - fs);
+	}
+	const size_t base = (size_t)blockIdx.y * (g.L + 1) * nch * eop_bytes_per_chunk<PM>() + mc * eop_bytes_per_chunk<PM>();
+	char *eb = (char *)E + base;
+	if (PM == PM_F64) {
+		double *e = (double *)eb;
+#pragma unroll
+		for (int s = 0; s < 16; s++) {
+			e[s * 64 + lane] = be[s];
+			e[(16 + s) * 64 + lane] = bo[s];
+		}
+	} else if (PM == PM_F32) {
+		float *e = (float *)eb;
+#pragma unroll
+		for (int s = 0; s < 16; s++) {
+			e[s * 64 + lane] = (float)be[s];
+			e[(16 + s) * 64 + lane] = (float)bo[s];
+		}
+	} else {
+		constexpr int NT = Terms<PM>::n;
+		s4v *e = (s4v *)eb;
+#pragma unroll
+		for (int p = 0; p < 2; p++)
+#pragma unroll
+			for (int s = 0; s < 4; s++) {
+				float v4[4];
+#pragma unroll
+				for (int k = 0; k < 4; k++) v4[k] = (float)(p ? bo[4 * s + k] : be[4 * s + k]);
+				s4v t[NT];
+				split4<NT>(v4, t);
+#pragma unroll
+				for (int k = 0; k < NT; k++) e[((p * 4 + s) * NT + k) * 64 + lane] = t[k];
+			}
+	}
+}
+
+// the operand buffer for as many slot groups as fit in max_bytes (at least
+// one), then for each batch: operands, analysis.  anal(E, group0, ngroups).
+template <int LAY, int NS, int PM, typename F>
+int analysis_batches(sphswe_ctx *ctx, const Geo &g, const double2 *four, int nf, F anal)
+{
+	const int per = LAY == LAY_REAL ? NS : NS / 2;
+	const int groups = (nf + per - 1) / per;
+	const int nch = (g.NJ + 63) / 64;
+	const size_t gbytes = (size_t)(g.L + 1) * nch * eop_bytes_per_chunk<PM>();
+	const size_t max_bytes = (size_t)256 << 20;
+	int gb = (int)std::max<size_t>(1, std::min<size_t>(groups, max_bytes / gbytes));
+	void *E = nullptr;
+	if (int rc = sphswe_escratch(ctx, gbytes * gb, &E)) return rc;
+	for (int g0 = 0; g0 < groups; g0 += gb) {
+		const int ng = std::min(gb, groups - g0);
+		const long nthr = (long)(g.L + 1) * nch * 64;
+		dim3 grid((unsigned)((nthr + 255) / 256), ng);
+		k_leg_eprep<LAY, NS, PM><<<grid, 256, 0, ctx->stream>>>(four, E, g, g0, nf);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if (int rc = anal(E, g0, ng)) return rc;
+	}
+	return SPHSWE_OK;
 }
 
 }  // namespace

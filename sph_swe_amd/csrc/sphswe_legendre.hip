@@ -241,23 +241,26 @@ __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__re
 
 // One wavefront per (m, 32-degree segment) covers every ring, so no reduction
 // crosses wavefronts.  For each 64-ring chunk the wavefront
+//   - loads its B operands, the weighted hemisphere combinations E of the
+//     chunk's rings for its column (written once per (m, chunk) by the
+//     operand pass, coalesced);
 //   - runs the recurrence with lane = ring over the 32 degrees of its segment
 //     and writes the Pbar tile to its own LDS region, split by parity of l-m;
 //   - contracts over rings with v_mfma_f64_16x16x4f64:
 //       Q[16 rows of one parity][16 columns] += Pbar[rows][4 rings] * E[4 rings][cols]
-//     where the columns are (slot, re/im) and E is the even (F_N + F_S) or odd
-//     (F_N - F_S) hemisphere combination times the Gauss weight.
+//     where the columns are (slot, re/im).
 // Operand layouts of v_mfma_f64_16x16x4f64 (wave64): A[i][k] in lane i + 16k,
 // B[k][n] in lane n + 16k, D[i][n] in lane n + 16 (i % 4), element i / 4.
 constexpr int AN_SEG = 32;        // degrees per wavefront (16 of each parity)
 constexpr int AN_WAVES = 4;       // independent wavefronts per workgroup
 constexpr int AN_PS = 65;         // LDS row stride (doubles) of the Pbar tile
+constexpr int AN_NS = 8;          // slots per group: 16 MFMA columns
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 template <int LAY, int NS>
-__global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__restrict__ four, double2 *__restrict__ spec,
-							    Geo g, const int *__restrict__ unit_m, int nunits,
+__global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double *__restrict__ E, double2 *__restrict__ spec, Geo g,
+							    const int *__restrict__ unit_m, int nunits, int group0,
 							    int nfields)
 {
 	__shared__ double Pt[AN_WAVES][2][16][AN_PS];
@@ -266,54 +269,36 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__res
 	const int u = blockIdx.x * AN_WAVES + wave;
 	if (u >= nunits) return;   // whole wavefront; no workgroup barrier below
 	const int am = unit_m[u];
-	const int L = g.L, NJ = g.NJ, nlat = g.nlat;
+	const int L = g.L, NJ = g.NJ;
 	const int so = g.segoff[am];
-	const int kseg = u - so;
-	const int s0 = am + kseg * AN_SEG, se = min(s0 + AN_SEG - 1, L);
-	Slot<LAY> sl[NS];
-	make_slots<LAY, NS>(sl, am, blockIdx.y, nfields, g);
+	const int s0 = am + (u - so) * AN_SEG, se = min(s0 + AN_SEG - 1, L);
+	const int nch = (NJ + 63) / 64;
+	const int n = lane & 15, kq = lane >> 4;
+	const Column<LAY, NS> col = column_of<LAY, NS>(n >> 1, am, group0 + (int)blockIdx.y, nfields);
+	const int comp = n & 1;
 
 	// recurrence coefficients of the segment, read back as LDS broadcasts
 	double2 *cf = Cfs[wave];
 	if (lane < AN_SEG) cf[lane] = g.coef[lm_index(L, min(s0 + lane, se), am)];
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+	wave_sync();
 
-	const int n = lane & 15, kq = lane >> 4;       // MFMA column / k-row of this lane
-	const int slot = n >> 1, comp = n & 1;
-	long fbase = sl[0].fb;
-	bool colv = false;
-#pragma unroll
-	for (int ss = 0; ss < NS; ss++)
-		if (ss == slot) { fbase = sl[ss].fb; colv = true; }
-	const double cmask = colv ? 1.0 : 0.0;
-	const double *fd = (const double *)four;
-
+	const double *Em = E + ((size_t)blockIdx.y * (L + 1) + am) * nch * (2 * 16 * 64);
 	d4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
 	double (*P)[16][AN_PS] = Pt[wave];
-	const int nch = (NJ + 63) / 64;
 	for (int c = 0; c < nch; c++) {
-		// B operands: E_even / E_odd of rings jb + 4s, column n.  The ring of a
-		// load is clamped to NJ - 1 (the chunk can reach 63 rings past NJ, which
-		// on small grids lies outside the Fourier row); those rings get weight 0.
-		const int jb = c * 64 + kq;
-		const double *prow = fd + 2 * fbase + comp;
-		double fn[16], fs[16], wv[16];
+		const double *Ec = Em + (size_t)c * (2 * 16 * 64);
+		double be[16], bo[16];
 #pragma unroll
 		for (int s = 0; s < 16; s++) {
-			const int rj = min(jb + 4 * s, NJ - 1);
-			fn[s] = prow[2 * rj];
-			fs[s] = prow[2 * (nlat - 1 - rj)];
-			wv[s] = g.wg[rj];
+			be[s] = Ec[s * 64 + lane];
+			bo[s] = Ec[(16 + s) * 64 + lane];
 		}
-		// Pbar tile: lane = ring (rings past NJ give finite values that meet weight 0)
+		// Pbar tile: lane = ring (rings past NJ give finite values that meet
+		// weight 0; rows past se, of a short last segment, are never stored)
 		const int j = c * 64 + lane;
 		const int jc = j < NJ ? j : NJ - 1;
 		const double mu = g.ct[jc];
 		const double2 sd = g.seed[(long)u * NJ + jc];
-		// seeds, then the plain recurrence; rows past se (short last segment)
-		// hold finite values that are never stored
 		double p0 = sd.x, p1 = sd.y;
 		P[0][0][lane] = p0;
 		P[1][0][lane] = p1;
@@ -324,49 +309,28 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double2 *__res
 			p1 = Pv;
 			P[t & 1][t >> 1][lane] = Pv;
 		}
-		double be[16], bo[16];
-#pragma unroll
-		for (int s = 0; s < 16; s++) {
-			const double w = (jb + 4 * s < NJ) ? wv[s] * cmask : 0.0;
-			be[s] = w * (fn[s] + fs[s]);
-			bo[s] = w * (fn[s] - fs[s]);
-		}
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-		__builtin_amdgcn_wave_barrier();
-		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		wave_sync();
 		// contraction over the 64 rings of the chunk, 4 per MFMA
 #pragma unroll
 		for (int s = 0; s < 16; s++) {
-			const double a0 = P[0][n][4 * s + kq];
-			const double a1 = P[1][n][4 * s + kq];
-			acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, be[s], acc0, 0, 0, 0);
-			acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bo[s], acc1, 0, 0, 0);
+			acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(P[0][n][4 * s + kq], be[s], acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(P[1][n][4 * s + kq], bo[s], acc1, 0, 0, 0);
 		}
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-		__builtin_amdgcn_wave_barrier();
-		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		wave_sync();
 	}
 	// D[i][n] in lane n + 16 (i % 4), element i / 4; row i of parity p is degree s0 + 2i + p
-	if (colv) {
-		double *sd_ = (double *)spec;
+	if (!col.valid) return;
+	double *sdo = (double *)spec;
 #pragma unroll
-		for (int r = 0; r < 4; r++) {
-			const int i = kq + 4 * r;
+	for (int r = 0; r < 4; r++) {
+		const int i = kq + 4 * r;
 #pragma unroll
-			for (int p = 0; p < 2; p++) {
-				const int l = s0 + 2 * i + p;
-				const double v = p ? acc1[r] : acc0[r];
-				if (l <= se) {
-#pragma unroll
-					for (int ss = 0; ss < NS; ss++)
-						if (ss == slot && sl[ss].valid)
-							sd_[2 * sl[ss].q(l, am) + comp] = sl[ss].sgn * v;
-				}
-			}
+		for (int p = 0; p < 2; p++) {
+			const int l = s0 + 2 * i + p;
+			if (l <= se) sdo[2 * column_q<LAY, NS>(col, am, l, g) + comp] = col.sgn * (p ? acc1[r] : acc0[r]);
 		}
 	}
 }
-
 
 template <int LAY, int NS>
 int launch_synth(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 *four, int nf, const double *oscale)
@@ -384,15 +348,17 @@ int launch_synth(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 *fo
 	return SPHSWE_OK;
 }
 
-template <int LAY, int NS>
+template <int LAY>
 int launch_anal(sphswe_ctx *ctx, const Geo &g, const double2 *four, double2 *spec, int nf)
 {
-	const int per = (LAY != LAY_REAL) ? NS / 2 : NS;
 	const int nunits = ctx->nseg_total;
-	dim3 grid((nunits + AN_WAVES - 1) / AN_WAVES, (nf + per - 1) / per);
-	k_leg_anal<LAY, NS><<<grid, 64 * AN_WAVES, 0, ctx->stream>>>(four, spec, g, ctx->d_unit_m, nunits, nf);
-	SPHSWE_LAUNCH_CHECK(ctx);
-	return SPHSWE_OK;
+	return analysis_batches<LAY, AN_NS, PM_F64>(ctx, g, four, nf, [&](void *E, int g0, int ng) -> int {
+		dim3 grid((nunits + AN_WAVES - 1) / AN_WAVES, ng);
+		k_leg_anal<LAY, AN_NS><<<grid, 64 * AN_WAVES, 0, ctx->stream>>>((const double *)E, spec, g, ctx->d_unit_m,
+										     nunits, g0, nf);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		return SPHSWE_OK;
+	});
 }
 
 int check_prec(sphswe_ctx *ctx, int prec)
@@ -428,21 +394,7 @@ int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf,
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_mx(ctx, four, spec, nf, prec);
-	const long nlm = ctx->nlm;
-	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
-	int rc = SPHSWE_OK;
-	int f = 0;
-	if (nf >= 8) {
-		int g = nf / 8;
-		if ((rc = launch_anal<LAY_REAL, 8>(ctx, geo_of(ctx), four, spec, g * 8))) return rc;
-		f = g * 8;
-	}
-	int r = nf - f;
-	if (r > 4) rc = launch_anal<LAY_REAL, 8>(ctx, geo_of(ctx), four + f * nfour, spec + f * nlm, r);
-	else if (r > 2) rc = launch_anal<LAY_REAL, 4>(ctx, geo_of(ctx), four + f * nfour, spec + f * nlm, r);
-	else if (r == 2) rc = launch_anal<LAY_REAL, 2>(ctx, geo_of(ctx), four + f * nfour, spec + f * nlm, 2);
-	else if (r == 1) rc = launch_anal<LAY_REAL, 1>(ctx, geo_of(ctx), four + f * nfour, spec + f * nlm, 1);
-	return rc;
+	return launch_anal<LAY_REAL>(ctx, geo_of(ctx), four, spec, nf);
 }
 
 int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
@@ -467,7 +419,7 @@ int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, in
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_cplx_mx(ctx, four, spec, nf, prec);
-	return launch_anal<LAY_CPLX, 8>(ctx, geo_of(ctx), four, spec, nf);
+	return launch_anal<LAY_CPLX>(ctx, geo_of(ctx), four, spec, nf);
 }
 
 // pole layout (REXI): nfields columns, row stride P, m-block offsets moff;
@@ -486,7 +438,7 @@ int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, in
 	Geo g = geo_of(ctx);
 	g.moff = d_moff;
 	g.P = P;
-	return launch_anal<LAY_POLE, 8>(ctx, g, four, spec, nf);
+	return launch_anal<LAY_POLE>(ctx, g, four, spec, nf);
 }
 
 extern "C" int sphswe_legendre_analysis(sphswe_ctx *ctx, const double *d_four, double *d_spec, int nfields, int precision)

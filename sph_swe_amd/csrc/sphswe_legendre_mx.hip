@@ -19,7 +19,8 @@
 // Analysis: one wavefront per (m, 32-degree segment), looping over 64-ring
 // chunks, as in the FP64 analysis:
 //   D[degree][column] += Pbar[degree][ring] * E[ring][column]
-// with E the hemisphere-combined Fourier rows times the Gauss weight.
+// with E the hemisphere-combined Fourier rows times the Gauss weight, rounded
+// (FP32) or split (bf16) once per (m, chunk) by the operand pass.
 //
 // Operand layouts (wave64), checked on the device: v_mfma_f32_16x16x4f32
 // A[i][k] in lane i + 16k, B[k][n] in lane n + 16k; v_mfma_f32_16x16x16bf16_1k
@@ -28,41 +29,6 @@
 #include "sphswe_leg_common.h"
 
 namespace {
-
-typedef float f4v __attribute__((ext_vector_type(4)));
-typedef short s4v __attribute__((ext_vector_type(4)));
-
-enum { PM_F32 = SPHSWE_PREC_FP32, PM_BF16 = SPHSWE_PREC_BF16, PM_BF16X3 = SPHSWE_PREC_BF16X3 };
-
-template <int PM>
-struct Terms {
-	static constexpr int n = PM == PM_BF16 ? 2 : 3;
-};
-
-// bf16 of a finite x, rounded to nearest even
-__device__ inline short bf16_rn(float x)
-{
-	const unsigned u = __float_as_uint(x);
-	return (short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ inline float bf16_f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
-
-// split four values into NT bf16 terms: term k is the bf16 of what the
-// previous terms leave (each difference is exact in FP32)
-template <int NT>
-__device__ inline void split4(const float (&v)[4], s4v (&t)[NT])
-{
-#pragma unroll
-	for (int e = 0; e < 4; e++) {
-		float r = v[e];
-#pragma unroll
-		for (int k = 0; k < NT; k++) {
-			const short h = bf16_rn(r);
-			t[k][e] = h;
-			r -= bf16_f(h);
-		}
-	}
-}
 
 // acc += A * B over the split terms of total order below NT
 template <int NT>
@@ -77,52 +43,6 @@ __device__ inline f4v mfma_split(const s4v (&a)[NT], const s4v (&b)[NT], f4v acc
 	acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1], b[0], acc, 0, 0, 0);
 	acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b[0], acc, 0, 0, 0);
 	return acc;
-}
-
-// the column of an MFMA lane: slot = n / 2, re/im = n % 2; its validity, the
-// sign (-1)^m of a negative-m complex slot, spectral address and Fourier row
-// base, all from arithmetic on the slot index
-template <int LAY, int NS>
-struct Column {
-	bool valid;
-	double sgn;
-	int f, m;
-};
-
-template <int LAY, int NS>
-__device__ inline Column<LAY, NS> column_of(int slot, int am, int group, int nfields)
-{
-	Column<LAY, NS> c;
-	if (LAY == LAY_REAL) {
-		c.f = group * NS + slot;
-		c.m = am;
-		c.valid = slot < NS && c.f < nfields;
-		c.sgn = 1.0;
-	} else {
-		c.f = group * (NS / 2) + (slot >> 1);
-		c.m = (slot & 1) ? -am : am;
-		c.valid = slot < NS && c.f < nfields && !((slot & 1) && am == 0);
-		c.sgn = ((slot & 1) && (am & 1)) ? -1.0 : 1.0;
-	}
-	if (c.f >= nfields) c.f = nfields - 1;
-	return c;
-}
-
-template <int LAY, int NS>
-__device__ inline long column_q(const Column<LAY, NS> &c, int am, int l, const Geo &g)
-{
-	const int L = g.L;
-	if (LAY == LAY_REAL) return (long)c.f * ((long)(L + 1) * (L + 2) / 2) + lm_index(L, l, am);
-	if (LAY == LAY_CPLX) return (long)c.f * ((long)(L + 1) * (L + 1)) + (long)l * l + l + c.m;
-	return ((long)g.moff[c.m + L] + (l - am)) * g.P + c.f;
-}
-
-template <int LAY, int NS>
-__device__ inline long column_fb(const Column<LAY, NS> &c, const Geo &g)
-{
-	const int L = g.L, nlat = g.nlat;
-	if (LAY == LAY_REAL) return (long)c.f * (L + 1) * nlat + (long)c.m * nlat;
-	return (long)c.f * (2 * L + 1) * nlat + (long)(c.m + L) * nlat;
 }
 
 constexpr int MX_SEG = 32;      // degrees per segment (16 of each parity)
@@ -300,8 +220,9 @@ struct AxStride {
 };
 
 template <int LAY, int NS, int PM>
-__global__ __launch_bounds__(64 * MX_WAVES) void k_leg_anal_mx(const double2 *__restrict__ four, double2 *__restrict__ spec,
-							       Geo g, const int *__restrict__ unit_m, int nunits, int nfields)
+__global__ __launch_bounds__(64 * MX_WAVES) void k_leg_anal_mx(const void *__restrict__ E, double2 *__restrict__ spec,
+							       Geo g, const int *__restrict__ unit_m, int nunits, int group0,
+							       int nfields)
 {
 	constexpr int PS = AxStride<PM>::v;
 	__shared__ __align__(16) float Pt[MX_WAVES][2][16][PS];
@@ -310,37 +231,43 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_anal_mx(const double2 *__
 	const int u = blockIdx.x * MX_WAVES + wave;
 	if (u >= nunits) return;   // whole wavefront; no workgroup barrier below
 	const int am = unit_m[u];
-	const int L = g.L, NJ = g.NJ, nlat = g.nlat;
+	const int L = g.L, NJ = g.NJ;
 	const int so = g.segoff[am];
 	const int s0 = am + (u - so) * MX_SEG, se = min(s0 + MX_SEG - 1, L);
+	const int nch = (NJ + 63) / 64;
 
 	double2 *cf = Cfs[wave];
 	if (lane < MX_SEG) cf[lane] = g.coef[lm_index(L, min(s0 + lane, se), am)];
 	wave_sync();
 
 	const int n = lane & 15, kq = lane >> 4;
-	const Column<LAY, NS> col = column_of<LAY, NS>(n >> 1, am, blockIdx.y, nfields);
+	const Column<LAY, NS> col = column_of<LAY, NS>(n >> 1, am, group0 + (int)blockIdx.y, nfields);
 	const int comp = n & 1;
-	const double cmask = col.valid ? 1.0 : 0.0;
-	const double *prow = (const double *)four + 2 * column_fb<LAY, NS>(col, g) + comp;
+	const char *Em = (const char *)E + ((size_t)blockIdx.y * (L + 1) + am) * nch * eop_bytes_per_chunk<PM>();
 
 	f4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
 	float (*P)[16][PS] = Pt[wave];
-	const int nch = (NJ + 63) / 64;
 	for (int c = 0; c < nch; c++) {
-		// B operands E_even / E_odd of the rings of this lane (ring loads are
-		// clamped to NJ - 1; rings past NJ get weight 0)
+		const char *Ec = Em + (size_t)c * eop_bytes_per_chunk<PM>();
+		// B operands, in operand order (see k_leg_eprep)
 		float be[16], bo[16];
+		constexpr int NT = Terms<PM>::n;
+		s4v bs[2][4][NT];
+		if (PM == PM_F32) {
+			const float *e = (const float *)Ec;
 #pragma unroll
-		for (int s = 0; s < 16; s++) {
-			// F32: ring 4s + kq (16 K-steps of 4); bf16: ring 16 (s/4) + 4 kq + s%4 (4 K-steps of 16)
-			const int rr = PM == PM_F32 ? 4 * s + kq : 16 * (s >> 2) + 4 * kq + (s & 3);
-			const int rj = c * 64 + rr;
-			const int rc = min(rj, NJ - 1);
-			const double fn = prow[2 * rc], fs = prow[2 * (nlat - 1 - rc)];
-			const double w = rj < NJ ? g.wg[rc] * cmask : 0.0;
-			be[s] = (float)(w * (fn + fs));
-			bo[s] = (float)(w * (fn - fs));
+			for (int s = 0; s < 16; s++) {
+				be[s] = e[s * 64 + lane];
+				bo[s] = e[(16 + s) * 64 + lane];
+			}
+		} else {
+			const s4v *e = (const s4v *)Ec;
+#pragma unroll
+			for (int p = 0; p < 2; p++)
+#pragma unroll
+				for (int s = 0; s < 4; s++)
+#pragma unroll
+					for (int k = 0; k < NT; k++) bs[p][s][k] = e[((p * 4 + s) * NT + k) * 64 + lane];
 		}
 		// Pbar tile: lane = ring
 		{
@@ -367,21 +294,18 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_anal_mx(const double2 *__
 				acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(P[1][n][4 * s + kq], bo[s], acc1, 0, 0, 0);
 			}
 		} else {
-			constexpr int NT = Terms<PM>::n;
 #pragma unroll
 			for (int s = 0; s < 4; s++) {
-				float a4[4], b4[4];
-				s4v as[NT], bs[NT];
+				float a4[4];
+				s4v as[NT];
 #pragma unroll
-				for (int e = 0; e < 4; e++) { a4[e] = P[0][n][16 * s + 4 * kq + e]; b4[e] = be[4 * s + e]; }
+				for (int e = 0; e < 4; e++) a4[e] = P[0][n][16 * s + 4 * kq + e];
 				split4<NT>(a4, as);
-				split4<NT>(b4, bs);
-				acc0 = mfma_split<NT>(as, bs, acc0);
+				acc0 = mfma_split<NT>(as, bs[0][s], acc0);
 #pragma unroll
-				for (int e = 0; e < 4; e++) { a4[e] = P[1][n][16 * s + 4 * kq + e]; b4[e] = bo[4 * s + e]; }
+				for (int e = 0; e < 4; e++) a4[e] = P[1][n][16 * s + 4 * kq + e];
 				split4<NT>(a4, as);
-				split4<NT>(b4, bs);
-				acc1 = mfma_split<NT>(as, bs, acc1);
+				acc1 = mfma_split<NT>(as, bs[1][s], acc1);
 			}
 		}
 		wave_sync();
@@ -413,12 +337,13 @@ int launch_synth_mx(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 
 template <int LAY, int PM>
 int launch_anal_mx(sphswe_ctx *ctx, const Geo &g, const double2 *four, double2 *spec, int nf)
 {
-	const int per = LAY == LAY_REAL ? MX_NS : MX_NS / 2;
 	const int nunits = ctx->nseg_total;
-	dim3 grid((nunits + MX_WAVES - 1) / MX_WAVES, (nf + per - 1) / per);
-	k_leg_anal_mx<LAY, MX_NS, PM><<<grid, 64 * MX_WAVES, 0, ctx->stream>>>(four, spec, g, ctx->d_unit_m, nunits, nf);
-	SPHSWE_LAUNCH_CHECK(ctx);
-	return SPHSWE_OK;
+	return analysis_batches<LAY, MX_NS, PM>(ctx, g, four, nf, [&](void *E, int g0, int ng) -> int {
+		dim3 grid((nunits + MX_WAVES - 1) / MX_WAVES, ng);
+		k_leg_anal_mx<LAY, MX_NS, PM><<<grid, 64 * MX_WAVES, 0, ctx->stream>>>(E, spec, g, ctx->d_unit_m, nunits, g0, nf);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		return SPHSWE_OK;
+	});
 }
 
 template <int LAY>
