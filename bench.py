@@ -233,6 +233,62 @@ def bench_swe_modes(ctx, nsteps=100):
     return {"steps": nsteps, "dt": dt, "modes": res}
 
 
+def bench_ensemble(nsteps=200, nmem=64, L=127):
+    """Config 3: 64 nonlinear SWE members at T127 (192 x 384), 200 steps, each
+    a Gaussian dam with centre offsets U(-0.2, 0.2) rad and amplitude
+    U(300, 700) m (seeded), all members in the same launches.  FP64 stepped
+    eagerly and with the captured step; the reduced modes with the captured
+    step, with the largest relative L2 error of a member's geopotential
+    spectrum against its FP64 run.  dt is the reference's own rule at this
+    grid, 0.002 a / nlat = 66.4 s: with the 600 s of the configuration the
+    fastest gravity wave has omega dt = 3.8, outside the RK4 stability interval
+    (2.83), and every dt from 120 s up overflows within 200 steps (the m = 1
+    polar mode of the scalar-wind formulation)."""
+    ctx = capi.Context(L)
+    dt = 0.002 * 6.37122e6 / ctx.nlat
+    rng = np.random.default_rng(2024)
+    lon = np.arange(ctx.nphi) / ctx.nphi * 2 * np.pi
+    x0 = []
+    for _ in range(nmem):
+        h = gaussian_dam(lon, ctx.ct, lat_c=np.pi / 4 + rng.uniform(-0.2, 0.2), lon_c=np.pi + rng.uniform(-0.2, 0.2),
+                         amp=rng.uniform(300, 700))
+        z = np.zeros_like(h)
+        x0.append(ctx.analysis(np.stack([h, z, z])))
+    x0 = np.concatenate(x0)
+    res, finals = {}, {}
+    runs = [("fp64_eager", "fp64", False), ("fp64", "fp64", True)] + [(m, m, True) for m in MODES]
+    for key, name, graph in runs:
+        prm = capi.swe_params(dt=dt, precision=capi.PRECISIONS[name])
+        step = ctx.swe_rk4_graph if graph else ctx.swe_rk4
+        warm = ctx.to_device(x0)
+        step(prm, warm, nmem, 2)
+        state = ctx.to_device(x0) if not graph else warm
+        if graph:
+            # the captured step is bound to its buffer: restart it from x0
+            ctx._check(ctx.lib.sphswe_memcpy_h2d(ctx.h, state.ptr, np.ascontiguousarray(x0).ctypes.data, x0.nbytes))
+        ctx.sync()
+        ctx.mark(14)
+        step(prm, state, nmem, nsteps)
+        ctx.mark(15)
+        ms = ctx.elapsed_ms(14, 15)
+        finals[key] = ctx.to_host(state, np.complex128, (nmem * 3, ctx.nlm))
+        if not np.all(np.isfinite(finals[key])):
+            raise RuntimeError("ensemble state became non-finite (%s)" % key)
+        res[key] = {"member_steps_per_s": nmem * nsteps / (ms / 1000.0), "ms_per_step": ms / nsteps,
+                    "hip_graph": graph}
+    ref = finals["fp64_eager"]
+    for key in res:
+        if key == "fp64_eager":
+            continue
+        res[key]["phi_rel_l2_max_member"] = max(
+            float(np.linalg.norm(finals[key][3 * k] - ref[3 * k]) / np.linalg.norm(ref[3 * k])) for k in range(nmem))
+        res[key]["speedup_vs_fp64_eager"] = res[key]["member_steps_per_s"] / res["fp64_eager"]["member_steps_per_s"]
+    cfg = {"lmax": L, "nlat": ctx.nlat, "nphi": ctx.nphi, "members": nmem, "dt": dt, "steps": nsteps,
+           "viscosity": 1e5}
+    ctx.close()
+    return {"config": cfg, "runs": res}
+
+
 def cpu_baseline(L, ctx_nlat, ctx_nphi):
     """The oracle (single-threaded C restatement) on a bounded sample: one
     round trip of 3 fields at T511."""
@@ -289,6 +345,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--swe-steps", type=int, default=50)
     ap.add_argument("--rexi-steps", type=int, default=10)
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-kernels", action="store_true",
                     help="only launch the kernels used for counter collection")
@@ -310,6 +367,7 @@ def main():
     achieved = flops / (ms_syn * 1e-3) / 1e12
     mixed_sht = bench_sht_modes(ctx, max(args.steps // 4, 10), 3)
     mixed_swe = bench_swe_modes(ctx) if args.swe_steps > 0 else None
+    ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
     rexi_rate, rexi_cfg = bench_rexi(args.rexi_steps) if args.rexi_steps > 0 else (None, None)
     cpu = None if args.no_cpu else cpu_baseline(L_BENCH, ctx.nlat, ctx.nphi)
     line = {
@@ -334,6 +392,7 @@ def main():
                        "viscosity": 1e5},
         "mixed_sht_T511_3fields": mixed_sht,
         "mixed_swe_T511_dam_100steps": mixed_swe,
+        "ensemble_T127_64members": ensemble,
         "rexi_steps_per_s": rexi_rate,
         "rexi_config": rexi_cfg,
         "roofline": {"bound": "mfma", "engine": "fp64 VALU (fp64 vector peak = fp64 matrix peak on MI355X)",

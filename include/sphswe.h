@@ -169,6 +169,12 @@ typedef struct {
  * member, the spectral fields h, u, v back to back (3*nlm complex).  The state
  * must be spectral (the caller analyses a physical initial condition first). */
 int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmembers, int nsteps);
+/* The same steps with one RK4 step captured into a HIP graph and replayed
+ * nsteps times (the small launches of low truncations and of ensembles stop
+ * being launch-bound).  The graph is kept per context and rebuilt when p,
+ * d_state or nmembers change; the first call of a new shape synchronises the
+ * context stream.  Results are identical to sphswe_swe_rk4. */
+int sphswe_swe_rk4_graph(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmembers, int nsteps);
 
 /* ---- linear SWE by REXI (AppSWE.hpp:336-565; SWERexiSPH.hpp:68-244;
  *      SPHSolverComplex.hpp:88-511; BandedMatrixSolver.hpp:235-434) */

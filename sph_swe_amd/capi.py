@@ -72,6 +72,7 @@ _SIGS = {
     "sphswe_any_nan": [_vp, _vp, _sz, _ip],
     "sphswe_reduce_minmax": [_vp, _vp, _sz, _dp, _dp, _dp],
     "sphswe_swe_rk4": [_vp, ctypes.POINTER(SweParams), _vp, _i, _i],
+    "sphswe_swe_rk4_graph": [_vp, ctypes.POINTER(SweParams), _vp, _i, _i],
     "sphswe_rexi_create": [_vp, ctypes.POINTER(RexiParams), ctypes.POINTER(_vp)],
     "sphswe_rexi_destroy": [_vp],
     "sphswe_rexi_poles": [_vp, _ip, _dp, _dp],
@@ -273,6 +274,10 @@ class Context:
     def swe_rk4(self, params, state, nmembers, nsteps):
         """state: device buffer holding nmembers*(h,u,v) spectral fields."""
         self._check(self.lib.sphswe_swe_rk4(self.h, ctypes.byref(params), state.ptr, nmembers, nsteps))
+
+    def swe_rk4_graph(self, params, state, nmembers, nsteps):
+        """As swe_rk4, one captured step replayed nsteps times."""
+        self._check(self.lib.sphswe_swe_rk4_graph(self.h, ctypes.byref(params), state.ptr, nmembers, nsteps))
 
     # REXI -------------------------------------------------------------------
     def rexi(self, params, alpha=None, beta=None):

@@ -93,6 +93,8 @@ int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf,
 int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
 int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
 int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
+int sphswe_leg_anal_reserve(sphswe_ctx *ctx, int nf, int prec);
+void sphswe_swe_release(sphswe_ctx *ctx);
 int sphswe_leg_anal_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
 int sphswe_leg_synth_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
 int sphswe_leg_anal_cplx_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);

@@ -300,8 +300,9 @@ __global__ __launch_bounds__(256) void k_leg_eprep(const double2 *__restrict__ f
 	}
 }
 
-// the operand buffer for as many slot groups as fit in max_bytes (at least
-// one), then for each batch: operands, analysis.  anal(E, group0, ngroups).
+// the operand buffer for as many slot groups as fit in ANALYSIS_MAX_EBYTES (at
+// least one), then for each batch: operands, analysis.  anal(E, group0, ngroups).
+constexpr size_t ANALYSIS_MAX_EBYTES = (size_t)256 << 20;
 template <int LAY, int NS, int PM, typename F>
 int analysis_batches(sphswe_ctx *ctx, const Geo &g, const double2 *four, int nf, F anal)
 {
@@ -309,8 +310,7 @@ int analysis_batches(sphswe_ctx *ctx, const Geo &g, const double2 *four, int nf,
 	const int groups = (nf + per - 1) / per;
 	const int nch = (g.NJ + 63) / 64;
 	const size_t gbytes = (size_t)(g.L + 1) * nch * eop_bytes_per_chunk<PM>();
-	const size_t max_bytes = (size_t)256 << 20;
-	int gb = (int)std::max<size_t>(1, std::min<size_t>(groups, max_bytes / gbytes));
+	int gb = (int)std::max<size_t>(1, std::min<size_t>(groups, ANALYSIS_MAX_EBYTES / gbytes));
 	void *E = nullptr;
 	if (int rc = sphswe_escratch(ctx, gbytes * gb, &E)) return rc;
 	for (int g0 = 0; g0 < groups; g0 += gb) {

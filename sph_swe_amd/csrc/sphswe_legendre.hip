@@ -441,6 +441,26 @@ int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, in
 	return launch_anal<LAY_POLE>(ctx, g, four, spec, nf);
 }
 
+// sizes the analysis operand buffer for nf real fields in a precision, so that
+// later analyses allocate nothing (needed before a stream capture)
+int sphswe_leg_anal_reserve(sphswe_ctx *ctx, int nf, int prec)
+{
+	if (int rc = check_prec(ctx, prec)) return rc;
+	const int nch = (ctx->NJ + 63) / 64;
+	size_t per_chunk = 0;
+	switch (prec) {
+	case SPHSWE_PREC_FP64: per_chunk = eop_bytes_per_chunk<PM_F64>(); break;
+	case SPHSWE_PREC_FP32: per_chunk = eop_bytes_per_chunk<PM_F32>(); break;
+	case SPHSWE_PREC_BF16: per_chunk = eop_bytes_per_chunk<PM_BF16>(); break;
+	default: per_chunk = eop_bytes_per_chunk<PM_BF16X3>(); break;
+	}
+	const size_t gbytes = (size_t)(ctx->L + 1) * nch * per_chunk;
+	const int groups = (nf + 7) / 8;
+	const size_t gb = std::max<size_t>(1, std::min<size_t>(groups, ANALYSIS_MAX_EBYTES / gbytes));
+	void *E;
+	return sphswe_escratch(ctx, gb * gbytes, &E);
+}
+
 extern "C" int sphswe_legendre_analysis(sphswe_ctx *ctx, const double *d_four, double *d_spec, int nfields, int precision)
 {
 	if (!ctx || !d_four || !d_spec || nfields < 0) return SPHSWE_ERR_ARG;

@@ -22,6 +22,8 @@
 // one analysis; the analysis is linear, so the two agree to round-off.
 #include "sphswe_common.h"
 
+#include <cstring>
+
 namespace {
 
 struct SweWs {
@@ -181,14 +183,15 @@ __global__ void k_stage(const double2 *__restrict__ TH, const double2 *__restric
 
 inline int g1(long n) { return (int)((n + 255) / 256); }
 
-}  // namespace
 
-extern "C" int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmem, int nsteps)
+// device buffers of one RK4 step over nmem members
+struct SweBufs {
+	double2 *Ya, *ACC, *Yb, *S9, *S4, *S2, *TH, *F;
+	double *P9, *P4, *P2, *P1;
+};
+
+int swe_buffers(sphswe_ctx *ctx, int nmem, SweBufs &b)
 {
-	if (!ctx || !p || !d_state || nmem < 1 || nsteps < 0) return SPHSWE_ERR_ARG;
-	if (p->precision < SPHSWE_PREC_FP64 || p->precision > SPHSWE_PREC_BF16X3)
-		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: precision %d not available", p->precision);
-	const int prec = p->precision;   // of the Legendre contractions; everything else is FP64
 	const long nlm = ctx->nlm, nspat = ctx->nspat;
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
 	// workspace: Y (3 nlm), ACC (3), YN (3), S9 (9), S4 (4), S2 (2), TH (1) spectral;
@@ -210,57 +213,144 @@ extern "C" int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, doubl
 		ws.bytes = bytes;
 	}
 	double2 *sp = (double2 *)ws.base;
-	double2 *Ya = sp; sp += 3 * nlm * nmem;
-	double2 *ACC = sp; sp += 3 * nlm * nmem;
-	double2 *Yb = sp; sp += 3 * nlm * nmem;
-	double2 *S9 = sp; sp += 9 * nlm * nmem;
-	double2 *S4 = sp; sp += 4 * nlm * nmem;
-	double2 *S2 = sp; sp += 2 * nlm * nmem;
-	double2 *TH = sp; sp += 1 * nlm * nmem;
-	double2 *F = sp; sp += 9 * nfour * nmem;
+	b.Ya = sp; sp += 3 * nlm * nmem;
+	b.ACC = sp; sp += 3 * nlm * nmem;
+	b.Yb = sp; sp += 3 * nlm * nmem;
+	b.S9 = sp; sp += 9 * nlm * nmem;
+	b.S4 = sp; sp += 4 * nlm * nmem;
+	b.S2 = sp; sp += 2 * nlm * nmem;
+	b.TH = sp; sp += 1 * nlm * nmem;
+	b.F = sp; sp += 9 * nfour * nmem;
 	double *dp = (double *)sp;
-	double *P9 = dp; dp += 9 * nspat * nmem;
-	double *P4 = dp; dp += 4 * nspat * nmem;
-	double *P2 = dp; dp += 2 * nspat * nmem;
-	double *P1 = dp;
+	b.P9 = dp; dp += 9 * nspat * nmem;
+	b.P4 = dp; dp += 4 * nspat * nmem;
+	b.P2 = dp; dp += 2 * nspat * nmem;
+	b.P1 = dp;
+	return SPHSWE_OK;
+}
 
-	double2 *X = (double2 *)d_state;
+// one classic RK4 step of nmem members, X updated in place; launches only
+// (no allocation, no synchronisation), so that it can be captured
+int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, const SweBufs &b)
+{
+	const long nlm = ctx->nlm, nspat = ctx->nspat;
 	const double a = p->earth_radius, g = p->gravitation;
 	const double ga = g / a, ia = 1.0 / a, two_omega = 2.0 * p->coriolis_omega;
 	const double visc = p->viscosity / (a * a);
 	const double dt = p->dt;
 	const double bcoef[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 	const double anext[3] = {0.5, 0.5, 1.0};
-	const int nm = nmem, L = ctx->L, nlat = ctx->nlat;
+	const int prec = p->precision;   // of the Legendre contractions; everything else is FP64
+	const int L = ctx->L, nlat = ctx->nlat;
 	hipStream_t st = ctx->stream;
 	int rc;
-
-	for (int n = 0; n < nsteps; n++) {
-		const double2 *Y = X;
-		double2 *YN = Ya;
-		for (int s = 0; s < 4; s++) {
-			k_prep9<<<g1(nlm * nm), 256, 0, st>>>(Y, S9, ctx->d_mofidx, L, (int)nlm, nm);
-			if ((rc = sphswe_leg_synth(ctx, S9, F, 9 * nm, prec))) return rc;
-			if ((rc = sphswe_fft_c2r(ctx, F, P9, 9 * nm))) return rc;
-			k_phys1<<<g1(nspat * nm), 256, 0, st>>>(P9, P4, ctx->d_ct, ctx->d_st, ctx->d_sq, nlat, nspat, nm, ga,
-								 two_omega, ia, p->nonlinear, p->h0 / a);
-			if ((rc = sphswe_fft_r2c(ctx, P4, F, 4 * nm))) return rc;
-			if ((rc = sphswe_leg_anal(ctx, F, S4, 4 * nm, prec))) return rc;
-			k_prep2<<<g1(nlm * nm), 256, 0, st>>>(S4, S2, ctx->d_mofidx, L, (int)nlm, nm);
-			if ((rc = sphswe_leg_synth(ctx, S2, F, 2 * nm, prec))) return rc;
-			if ((rc = sphswe_fft_c2r(ctx, F, P2, 2 * nm))) return rc;
-			k_phys2<<<g1(nspat * nm), 256, 0, st>>>(P2, P1, ctx->d_st, ctx->d_sq, nlat, nspat, nm,
-								 p->nonlinear ? ia : p->h0 / a);
-			if ((rc = sphswe_fft_r2c(ctx, P1, F, nm))) return rc;
-			if ((rc = sphswe_leg_anal(ctx, F, TH, nm, prec))) return rc;
-			k_stage<<<g1(nlm * nm), 256, 0, st>>>(TH, S4, Y, X, ACC, YN, ctx->d_mofidx, L, (int)nlm, nm, visc, s,
-							       bcoef[s], s < 3 ? anext[s] : 0.0, dt);
-			SPHSWE_LAUNCH_CHECK(ctx);
-			if (s < 3) {
-				Y = YN;
-				YN = (YN == Ya) ? Yb : Ya;
-			}
+	const double2 *Y = X;
+	double2 *YN = b.Ya;
+	for (int s = 0; s < 4; s++) {
+		k_prep9<<<g1(nlm * nm), 256, 0, st>>>(Y, b.S9, ctx->d_mofidx, L, (int)nlm, nm);
+		if ((rc = sphswe_leg_synth(ctx, b.S9, b.F, 9 * nm, prec))) return rc;
+		if ((rc = sphswe_fft_c2r(ctx, b.F, b.P9, 9 * nm))) return rc;
+		k_phys1<<<g1(nspat * nm), 256, 0, st>>>(b.P9, b.P4, ctx->d_ct, ctx->d_st, ctx->d_sq, nlat, nspat, nm, ga,
+							 two_omega, ia, p->nonlinear, p->h0 / a);
+		if ((rc = sphswe_fft_r2c(ctx, b.P4, b.F, 4 * nm))) return rc;
+		if ((rc = sphswe_leg_anal(ctx, b.F, b.S4, 4 * nm, prec))) return rc;
+		k_prep2<<<g1(nlm * nm), 256, 0, st>>>(b.S4, b.S2, ctx->d_mofidx, L, (int)nlm, nm);
+		if ((rc = sphswe_leg_synth(ctx, b.S2, b.F, 2 * nm, prec))) return rc;
+		if ((rc = sphswe_fft_c2r(ctx, b.F, b.P2, 2 * nm))) return rc;
+		k_phys2<<<g1(nspat * nm), 256, 0, st>>>(b.P2, b.P1, ctx->d_st, ctx->d_sq, nlat, nspat, nm,
+							 p->nonlinear ? ia : p->h0 / a);
+		if ((rc = sphswe_fft_r2c(ctx, b.P1, b.F, nm))) return rc;
+		if ((rc = sphswe_leg_anal(ctx, b.F, b.TH, nm, prec))) return rc;
+		k_stage<<<g1(nlm * nm), 256, 0, st>>>(b.TH, b.S4, Y, X, b.ACC, YN, ctx->d_mofidx, L, (int)nlm, nm, visc, s,
+						       bcoef[s], s < 3 ? anext[s] : 0.0, dt);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if (s < 3) {
+			Y = YN;
+			YN = (YN == b.Ya) ? b.Yb : b.Ya;
 		}
 	}
+	return SPHSWE_OK;
+}
+
+int check_params(sphswe_ctx *ctx, const sphswe_swe_params *p)
+{
+	if (p->precision < SPHSWE_PREC_FP64 || p->precision > SPHSWE_PREC_BF16X3)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: precision %d not available", p->precision);
+	return SPHSWE_OK;
+}
+
+// one captured step per context, rebuilt when its arguments change
+struct SweGraph {
+	hipGraphExec_t exec = nullptr;
+	sphswe_swe_params prm;
+	double *state = nullptr;
+	int nmem = 0;
+};
+std::unordered_map<const sphswe_ctx *, SweGraph> g_graph;
+
+}  // namespace
+
+// frees the workspace and the captured step of a context (called by
+// sphswe_destroy after the stream has drained)
+void sphswe_swe_release(sphswe_ctx *ctx)
+{
+	auto g = g_graph.find(ctx);
+	if (g != g_graph.end()) {
+		if (g->second.exec) (void)hipGraphExecDestroy(g->second.exec);
+		g_graph.erase(g);
+	}
+	auto w = g_ws.find(ctx);
+	if (w != g_ws.end()) {
+		if (w->second.base) (void)hipFree(w->second.base);
+		g_ws.erase(w);
+	}
+}
+
+extern "C" int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmem, int nsteps)
+{
+	if (!ctx || !p || !d_state || nmem < 1 || nsteps < 0) return SPHSWE_ERR_ARG;
+	if (int rc = check_params(ctx, p)) return rc;
+	SweBufs b;
+	if (int rc = swe_buffers(ctx, nmem, b)) return rc;
+	for (int n = 0; n < nsteps; n++)
+		if (int rc = swe_step(ctx, p, (double2 *)d_state, nmem, b)) return rc;
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_swe_rk4_graph(sphswe_ctx *ctx, const sphswe_swe_params *p, double *d_state, int nmem, int nsteps)
+{
+	if (!ctx || !p || !d_state || nmem < 1 || nsteps < 0) return SPHSWE_ERR_ARG;
+	if (int rc = check_params(ctx, p)) return rc;
+	SweGraph &G = g_graph[ctx];
+	if (!G.exec || G.state != d_state || G.nmem != nmem || memcmp(&G.prm, p, sizeof *p) != 0) {
+		if (G.exec) {
+			SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+			SPHSWE_HIP(ctx, hipGraphExecDestroy(G.exec));
+			G.exec = nullptr;
+		}
+		// every buffer the step uses exists before the capture
+		SweBufs b;
+		if (int rc = swe_buffers(ctx, nmem, b)) return rc;
+		if (int rc = sphswe_leg_anal_reserve(ctx, 4 * nmem, p->precision)) return rc;
+		SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+		hipGraph_t graph;
+		SPHSWE_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+		const int rc = swe_step(ctx, p, (double2 *)d_state, nmem, b);
+		hipGraph_t captured = nullptr;
+		const hipError_t ec = hipStreamEndCapture(ctx->stream, &captured);
+		if (rc) {
+			if (captured) hipGraphDestroy(captured);
+			return rc;
+		}
+		SPHSWE_HIP(ctx, ec);
+		graph = captured;
+		const hipError_t ei = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+		hipGraphDestroy(graph);
+		SPHSWE_HIP(ctx, ei);
+		G.prm = *p;
+		G.state = d_state;
+		G.nmem = nmem;
+	}
+	for (int n = 0; n < nsteps; n++) SPHSWE_HIP(ctx, hipGraphLaunch(G.exec, ctx->stream));
 	return SPHSWE_OK;
 }

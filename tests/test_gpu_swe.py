@@ -80,3 +80,70 @@ def test_ensemble_members_are_independent():
         outs = ctx.to_host(single, np.complex128, (3, ctx.nlm))
         assert np.array_equal(outs, outb[3 * k:3 * k + 3])
     ctx.close()
+
+
+def _members(ctx, n, seed=3):
+    rng = np.random.default_rng(seed)
+    lon = np.arange(ctx.nphi) / ctx.nphi * 2 * np.pi
+    out = []
+    for _ in range(n):
+        h = gaussian_dam(lon, ctx.ct, lat_c=np.pi / 4 + rng.uniform(-0.2, 0.2),
+                         lon_c=np.pi + rng.uniform(-0.2, 0.2), amp=rng.uniform(300, 700))
+        z = np.zeros_like(h)
+        out.append(ctx.analysis(np.stack([h, z, z])))
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("prec", [capi.PREC_FP64, capi.PREC_BF16X3])
+def test_graph_replays_match_eager(prec):
+    # one captured RK4 step replayed: every replay equals the eager steps bit
+    # for bit, with eager work and synchronisations between the replays, across
+    # a change of dt (the graph is rebuilt) and of the member count
+    L = 31
+    ctx = capi.Context(L)
+    try:
+        x0 = _members(ctx, 3)
+        eager = ctx.to_device(x0)
+        graph = ctx.to_device(x0)
+        for k, (dt, nsteps) in enumerate([(300.0, 1), (300.0, 4), (150.0, 2), (300.0, 3)]):
+            prm = capi.swe_params(dt=dt, precision=prec)
+            ctx.swe_rk4(prm, eager, 3, nsteps)
+            ctx.swe_rk4_graph(prm, graph, 3, nsteps)
+            a = ctx.to_host(eager, np.complex128, (9, ctx.nlm))
+            b = ctx.to_host(graph, np.complex128, (9, ctx.nlm))
+            assert np.array_equal(a, b), k
+        # a different member count on another buffer: rebuilt, still identical
+        one_e = ctx.to_device(x0[:3])
+        one_g = ctx.to_device(x0[:3])
+        prm = capi.swe_params(dt=300.0, precision=prec)
+        ctx.swe_rk4(prm, one_e, 1, 2)
+        ctx.swe_rk4_graph(prm, one_g, 1, 2)
+        assert np.array_equal(ctx.to_host(one_e, np.complex128, (3, ctx.nlm)),
+                              ctx.to_host(one_g, np.complex128, (3, ctx.nlm)))
+    finally:
+        ctx.close()
+
+
+def test_ensemble_config3_members_match_single_runs():
+    # config 3 at reduced length: 64 members at T127 (192 x 384), perturbed dam
+    # centres and amplitudes, advanced together (graph path), dt the reference's
+    # rule 0.002 a / nlat (the configuration's 600 s is outside the RK4
+    # stability interval of the fastest gravity wave); each member equals its
+    # own single-member run to round-off
+    L = 127
+    ctx = capi.Context(L)
+    try:
+        x0 = _members(ctx, 64, seed=11)
+        prm = capi.swe_params(dt=0.002 * 6.37122e6 / ctx.nlat)
+        batch = ctx.to_device(x0)
+        ctx.swe_rk4_graph(prm, batch, 64, 5)
+        out = ctx.to_host(batch, np.complex128, (64 * 3, ctx.nlm))
+        assert np.all(np.isfinite(out))
+        for k in (0, 17, 63):
+            single = ctx.to_device(x0[3 * k:3 * k + 3])
+            ctx.swe_rk4(prm, single, 1, 5)
+            ref = ctx.to_host(single, np.complex128, (3, ctx.nlm))
+            for f in range(3):
+                assert rel_l2(out[3 * k + f], ref[f]) < 1e-13, (k, f)
+    finally:
+        ctx.close()
