@@ -13,11 +13,40 @@ namespace {
 enum { MODE_R2C = 0, MODE_C2R = 1, MODE_C2C_FWD = 2, MODE_C2C_BWD = 3 };
 constexpr int KB = 8;   // loads in flight per thread in the staging loops
 
+// n / d for 0 <= n < 2^20 without an integer division: the float quotient is
+// within 1/8 of n / d, one correction step makes it exact
+struct QDiv {
+	int d;
+	float inv;
+};
+__host__ inline QDiv qdiv_of(int d) { return QDiv{d, 1.0f / (float)d}; }
+__device__ __forceinline__ int qdiv(int n, const QDiv &q, int &r)
+{
+	int t = (int)((float)n * q.inv);
+	r = n - t * q.d;
+	if (r < 0) {
+		t--;
+		r += q.d;
+	} else if (r >= q.d) {
+		t++;
+		r -= q.d;
+	}
+	return t;
+}
+
 struct FftArgs {
 	int N;
 	int npass;
 	int radix[SPHSWE_MAX_RADIX];
+	QDiv nbq[SPHSWE_MAX_RADIX];   // N / radix of each pass
+	QDiv nsq[SPHSWE_MAX_RADIX];   // product of the radices before each pass
+	QDiv Nq, RBq, Lq;             // N, rings (or sequences) per workgroup, L + 1
 };
+
+// LDS element index of entry i of sequence s: one padding element every 8
+// complex values spreads the strided accesses of the early passes over the banks
+__device__ __forceinline__ int lidx(int s, int i, int NP) { return s * NP + i + (i >> 3); }
+__host__ __device__ inline int padded_len(int N) { return N + (N >> 3) + 1; }
 
 __device__ inline double2 mul_i(double2 a, int sign)  // sign * i * a
 {
@@ -94,21 +123,21 @@ template <>
 struct Dft<7> : DftOdd<7> {};
 
 template <int R>
-__device__ void stockham_pass(const double2 *__restrict__ src, double2 *__restrict__ dst, int N, int Ns, int nseq,
-			      const double2 *__restrict__ tw, int sign)
+__device__ void stockham_pass(const double2 *__restrict__ src, double2 *__restrict__ dst, int N, int Ns, const QDiv &nbq,
+			      const QDiv &nsq, int nseq, const double2 *__restrict__ tw, int sign)
 {
-	const int nb = N / R;
+	const int nb = nbq.d;
 	const int total = nb * nseq;
 	const int twstride = N / (Ns * R);
+	const int NP = padded_len(N);
 	for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-		int s = idx / nb;
-		int b = idx - s * nb;
-		const double2 *in = src + s * N;
-		double2 *out = dst + s * N;
-		int k = b % Ns;
+		int b;
+		const int s = qdiv(idx, nbq, b);
+		int k;
+		const int jb = qdiv(b, nsq, k);
 		double2 v[R];
 #pragma unroll
-		for (int t = 0; t < R; t++) v[t] = in[b + t * nb];
+		for (int t = 0; t < R; t++) v[t] = src[lidx(s, b + t * nb, NP)];
 		if (Ns > 1) {
 #pragma unroll
 			for (int t = 1; t < R; t++) {
@@ -118,9 +147,9 @@ __device__ void stockham_pass(const double2 *__restrict__ src, double2 *__restri
 			}
 		}
 		Dft<R>::run(v, sign);
-		int o = (b / Ns) * Ns * R + k;
+		const int o = jb * Ns * R + k;
 #pragma unroll
-		for (int t = 0; t < R; t++) out[o + t * Ns] = v[t];
+		for (int t = 0; t < R; t++) dst[lidx(s, o + t * Ns, NP)] = v[t];
 	}
 }
 
@@ -131,11 +160,11 @@ __device__ double2 *fft_lds(double2 *a, double2 *b, const FftArgs &plan, int nse
 	for (int p = 0; p < plan.npass; p++) {
 		int R = plan.radix[p];
 		switch (R) {
-		case 2: stockham_pass<2>(a, b, plan.N, Ns, nseq, tw, sign); break;
-		case 3: stockham_pass<3>(a, b, plan.N, Ns, nseq, tw, sign); break;
-		case 4: stockham_pass<4>(a, b, plan.N, Ns, nseq, tw, sign); break;
-		case 5: stockham_pass<5>(a, b, plan.N, Ns, nseq, tw, sign); break;
-		default: stockham_pass<7>(a, b, plan.N, Ns, nseq, tw, sign); break;
+		case 2: stockham_pass<2>(a, b, plan.N, Ns, plan.nbq[p], plan.nsq[p], nseq, tw, sign); break;
+		case 3: stockham_pass<3>(a, b, plan.N, Ns, plan.nbq[p], plan.nsq[p], nseq, tw, sign); break;
+		case 4: stockham_pass<4>(a, b, plan.N, Ns, plan.nbq[p], plan.nsq[p], nseq, tw, sign); break;
+		case 5: stockham_pass<5>(a, b, plan.N, Ns, plan.nbq[p], plan.nsq[p], nseq, tw, sign); break;
+		default: stockham_pass<7>(a, b, plan.N, Ns, plan.nbq[p], plan.nsq[p], nseq, tw, sign); break;
 		}
 		__syncthreads();
 		double2 *t = a;
@@ -154,15 +183,15 @@ __global__ __launch_bounds__(256) void k_fft(const void *__restrict__ in, void *
 	extern __shared__ double2 sm[];
 	const int lb = xcd_remap(blockIdx.x, nblk_pad);
 	if (lb >= nblk) return;
-	const int N = plan.N;
+	const int N = plan.N, NP = padded_len(N);
 	const int f = blockIdx.y;
 	const size_t nspat = (size_t)nlat * N;
 	double2 *bufA = sm;
-	double2 *bufB = sm + (size_t)nseq * N;
+	double2 *bufB = sm + (size_t)nseq * NP;
 	const int tid = threadIdx.x, nt = blockDim.x;
+	const int RB = plan.RBq.d;   // rings (real modes) or sequences (complex modes) of this workgroup
 
 	if (MODE == MODE_R2C) {
-		const int RB = 2 * nseq;
 		const int j0 = lb * RB;
 		const double *src = (const double *)in + f * nspat;
 		double *lds = (double *)bufA;
@@ -170,29 +199,30 @@ __global__ __launch_bounds__(256) void k_fft(const void *__restrict__ in, void *
 			double v[KB];
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
-				int idx = min(b + u * nt + tid, N * RB - 1);
-				int i = idx / RB, r = idx - i * RB;
-				int j = min(j0 + r, nlat - 1);
-				v[u] = src[(size_t)i * nlat + j];
+				int r;
+				const int i = qdiv(min(b + u * nt + tid, N * RB - 1), plan.RBq, r);
+				v[u] = src[(size_t)i * nlat + min(j0 + r, nlat - 1)];
 			}
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
-				int idx = b + u * nt + tid;
+				const int idx = b + u * nt + tid;
 				if (idx >= N * RB) break;
-				int i = idx / RB, r = idx - i * RB;
-				lds[2 * ((r >> 1) * N + i) + (r & 1)] = (j0 + r < nlat) ? v[u] : 0.0;
+				int r;
+				const int i = qdiv(idx, plan.RBq, r);
+				lds[2 * lidx(r >> 1, i, NP) + (r & 1)] = (j0 + r < nlat) ? v[u] : 0.0;
 			}
 		}
 		__syncthreads();
 		double2 *res = fft_lds(bufA, bufB, plan, nseq, tw, -1);
 		double2 *dst = (double2 *)out + (size_t)f * (L + 1) * nlat;
 		for (int idx = tid; idx < (L + 1) * RB; idx += nt) {
-			int m = idx / RB, r = idx - m * RB;
-			int j = j0 + r;
+			int r;
+			const int m = qdiv(idx, plan.RBq, r);
+			const int j = j0 + r;
 			if (j >= nlat) continue;
-			int s = r >> 1;
-			double2 Z = res[s * N + m];
-			double2 Zc = cconj(res[s * N + (m == 0 ? 0 : N - m)]);
+			const int s = r >> 1;
+			const double2 Z = res[lidx(s, m, NP)];
+			const double2 Zc = cconj(res[lidx(s, m == 0 ? 0 : N - m, NP)]);
 			double2 v;
 			if ((r & 1) == 0) v = make_double2(0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y));
 			else v = make_double2(0.5 * (Z.y - Zc.y), -0.5 * (Z.x - Zc.x));
@@ -200,118 +230,408 @@ __global__ __launch_bounds__(256) void k_fft(const void *__restrict__ in, void *
 			dst[(size_t)m * nlat + j] = make_double2(v.x * norm, v.y * norm);
 		}
 	} else if (MODE == MODE_C2R) {
-		const int RB = 2 * nseq;
 		const int j0 = lb * RB;
 		const double2 *src = (const double2 *)in + (size_t)f * (L + 1) * nlat;
-		// stage G[s][m][part] in bufB
+		// stage G[s][m][part] in bufB (2 (L + 1) <= N + 1 < NP entries per sequence)
 		const int ng = (L + 1) * RB;
 		for (int b = 0; b < ng; b += KB * nt) {
 			double2 v[KB];
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
-				int idx = min(b + u * nt + tid, ng - 1);
-				int m = idx / RB, r = idx - m * RB;
-				int j = min(j0 + r, nlat - 1);
-				v[u] = src[(size_t)m * nlat + j];
+				int r;
+				const int m = qdiv(min(b + u * nt + tid, ng - 1), plan.RBq, r);
+				v[u] = src[(size_t)m * nlat + min(j0 + r, nlat - 1)];
 			}
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
-				int idx = b + u * nt + tid;
+				const int idx = b + u * nt + tid;
 				if (idx >= ng) break;
-				int m = idx / RB, r = idx - m * RB;
+				int r;
+				const int m = qdiv(idx, plan.RBq, r);
 				bufB[((r >> 1) * (L + 1) + m) * 2 + (r & 1)] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
 			}
 		}
 		__syncthreads();
 		for (int idx = tid; idx < nseq * N; idx += nt) {
-			int s = idx / N, k = idx - s * N;
+			int k;
+			const int s = qdiv(idx, plan.Nq, k);
 			double2 z = make_double2(0, 0);
 			if (k == 0) {
-				double2 A = bufB[(s * (L + 1)) * 2], B = bufB[(s * (L + 1)) * 2 + 1];
+				const double2 A = bufB[(s * (L + 1)) * 2], B = bufB[(s * (L + 1)) * 2 + 1];
 				z = make_double2(A.x, B.x);
 			} else if (k <= L) {
-				double2 A = bufB[(s * (L + 1) + k) * 2], B = bufB[(s * (L + 1) + k) * 2 + 1];
+				const double2 A = bufB[(s * (L + 1) + k) * 2], B = bufB[(s * (L + 1) + k) * 2 + 1];
 				z = make_double2(A.x - B.y, A.y + B.x);
 			} else if (k >= N - L) {
-				int m = N - k;
-				double2 A = bufB[(s * (L + 1) + m) * 2], B = bufB[(s * (L + 1) + m) * 2 + 1];
+				const int m = N - k;
+				const double2 A = bufB[(s * (L + 1) + m) * 2], B = bufB[(s * (L + 1) + m) * 2 + 1];
 				z = make_double2(A.x + B.y, -A.y + B.x);
 			}
-			bufA[idx] = z;
+			bufA[lidx(s, k, NP)] = z;
 		}
 		__syncthreads();
 		double2 *res = fft_lds(bufA, bufB, plan, nseq, tw, +1);
 		double *dst = (double *)out + f * nspat;
 		const double *rl = (const double *)res;
 		for (int idx = tid; idx < N * RB; idx += nt) {
-			int i = idx / RB, r = idx - i * RB;
-			int j = j0 + r;
-			if (j < nlat) dst[(size_t)i * nlat + j] = rl[2 * ((r >> 1) * N + i) + (r & 1)];
+			int r;
+			const int i = qdiv(idx, plan.RBq, r);
+			const int j = j0 + r;
+			if (j < nlat) dst[(size_t)i * nlat + j] = rl[2 * lidx(r >> 1, i, NP) + (r & 1)];
 		}
 	} else if (MODE == MODE_C2C_FWD) {
-		const int j0 = lb * nseq;
+		const int j0 = lb * RB;
 		const double2 *src = (const double2 *)in + f * nspat;
-		for (int b = 0; b < N * nseq; b += KB * nt) {
+		for (int b = 0; b < N * RB; b += KB * nt) {
 			double2 v[KB];
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
-				int idx = min(b + u * nt + tid, N * nseq - 1);
-				int i = idx / nseq, r = idx - i * nseq;
+				int r;
+				const int i = qdiv(min(b + u * nt + tid, N * RB - 1), plan.RBq, r);
 				v[u] = src[(size_t)i * nlat + min(j0 + r, nlat - 1)];
 			}
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
-				int idx = b + u * nt + tid;
-				if (idx >= N * nseq) break;
-				int i = idx / nseq, r = idx - i * nseq;
-				bufA[r * N + i] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
+				const int idx = b + u * nt + tid;
+				if (idx >= N * RB) break;
+				int r;
+				const int i = qdiv(idx, plan.RBq, r);
+				bufA[lidx(r, i, NP)] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
 			}
 		}
 		__syncthreads();
 		double2 *res = fft_lds(bufA, bufB, plan, nseq, tw, -1);
 		double2 *dst = (double2 *)out + (size_t)f * (2 * L + 1) * nlat;
-		for (int idx = tid; idx < (2 * L + 1) * nseq; idx += nt) {
-			int mi = idx / nseq, r = idx - mi * nseq;
-			int j = j0 + r;
+		for (int idx = tid; idx < (2 * L + 1) * RB; idx += nt) {
+			int r;
+			const int mi = qdiv(idx, plan.RBq, r);
+			const int j = j0 + r;
 			if (j >= nlat) continue;
 			int k = mi - L;
 			if (k < 0) k += N;
-			double2 z = res[r * N + k];
+			const double2 z = res[lidx(r, k, NP)];
 			dst[(size_t)mi * nlat + j] = make_double2(z.x * norm, z.y * norm);
 		}
 	} else {  // MODE_C2C_BWD
-		const int j0 = lb * nseq;
+		const int j0 = lb * RB;
 		const double2 *src = (const double2 *)in + (size_t)f * (2 * L + 1) * nlat;
-		for (int idx = tid; idx < nseq * N; idx += nt) bufA[idx] = make_double2(0, 0);
+		for (int idx = tid; idx < nseq * NP; idx += nt) bufA[idx] = make_double2(0, 0);
 		__syncthreads();
-		const int ng = (2 * L + 1) * nseq;
+		const int ng = (2 * L + 1) * RB;
 		for (int b = 0; b < ng; b += KB * nt) {
 			double2 v[KB];
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
-				int idx = min(b + u * nt + tid, ng - 1);
-				int mi = idx / nseq, r = idx - mi * nseq;
+				int r;
+				const int mi = qdiv(min(b + u * nt + tid, ng - 1), plan.RBq, r);
 				v[u] = src[(size_t)mi * nlat + min(j0 + r, nlat - 1)];
 			}
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
-				int idx = b + u * nt + tid;
+				const int idx = b + u * nt + tid;
 				if (idx >= ng) break;
-				int mi = idx / nseq, r = idx - mi * nseq;
+				int r;
+				const int mi = qdiv(idx, plan.RBq, r);
 				int k = mi - L;
 				if (k < 0) k += N;
-				bufA[r * N + k] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
+				bufA[lidx(r, k, NP)] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
 			}
 		}
 		__syncthreads();
 		double2 *res = fft_lds(bufA, bufB, plan, nseq, tw, +1);
 		double2 *dst = (double2 *)out + f * nspat;
-		for (int idx = tid; idx < N * nseq; idx += nt) {
-			int i = idx / nseq, r = idx - i * nseq;
-			int j = j0 + r;
-			if (j < nlat) dst[(size_t)i * nlat + j] = res[r * N + i];
+		for (int idx = tid; idx < N * RB; idx += nt) {
+			int r;
+			const int i = qdiv(idx, plan.RBq, r);
+			const int j = j0 + r;
+			if (j < nlat) dst[(size_t)i * nlat + j] = res[lidx(r, i, NP)];
 		}
 	}
+}
+
+// ----------------------------------------------------------------------------
+// real transforms at compile-time sizes: radix-16/8 passes, constant strides
+// ----------------------------------------------------------------------------
+
+// DFT of 8: radix 2 then two DFTs of 4, w8^n = exp(sign i 2 pi n / 8)
+template <>
+struct Dft<8> {
+	__device__ static void run(double2 *v, int sign)
+	{
+		constexpr double h = 0.70710678118654752440;
+		double2 a[4], b[4];
+#pragma unroll
+		for (int n = 0; n < 4; n++) {
+			a[n] = cadd(v[n], v[n + 4]);
+			b[n] = csub(v[n], v[n + 4]);
+		}
+		b[1] = cmul(b[1], make_double2(h, sign * h));
+		b[2] = mul_i(b[2], sign);
+		b[3] = cmul(b[3], make_double2(-h, sign * h));
+		Dft<4>::run(a, sign);
+		Dft<4>::run(b, sign);
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			v[2 * k] = a[k];
+			v[2 * k + 1] = b[k];
+		}
+	}
+};
+
+// DFT of 16 as 4 x 4: DFTs of 4 over n1 of x[4 n1 + n2], twiddles
+// w16^(n2 k1), DFTs of 4 over n2; X[k1 + 4 k2]
+template <>
+struct Dft<16> {
+	__device__ static void run(double2 *v, int sign)
+	{
+		constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+		// w16^e for e = n2 k1 in {0, 1, 2, 3, 4, 6, 9}: (cos, sin) of 2 pi e / 16
+		double2 y[4][4];
+#pragma unroll
+		for (int n2 = 0; n2 < 4; n2++) {
+			double2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+			Dft<4>::run(t, sign);
+#pragma unroll
+			for (int k1 = 0; k1 < 4; k1++) y[n2][k1] = t[k1];
+		}
+		const double2 w1 = make_double2(c1, sign * s1), w2 = make_double2(h, sign * h), w3 = make_double2(s1, sign * c1);
+		const double2 w6 = make_double2(-h, sign * h), w9 = make_double2(-c1, -sign * s1);
+		y[1][1] = cmul(y[1][1], w1);
+		y[1][2] = cmul(y[1][2], w2);
+		y[1][3] = cmul(y[1][3], w3);
+		y[2][1] = cmul(y[2][1], w2);
+		y[2][2] = mul_i(y[2][2], sign);
+		y[2][3] = cmul(y[2][3], w6);
+		y[3][1] = cmul(y[3][1], w3);
+		y[3][2] = cmul(y[3][2], w6);
+		y[3][3] = cmul(y[3][3], w9);
+#pragma unroll
+		for (int k1 = 0; k1 < 4; k1++) {
+			double2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+			Dft<4>::run(t, sign);
+#pragma unroll
+			for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = t[k2];
+		}
+	}
+};
+
+constexpr int first_radix(int n)
+{
+	return n % 16 == 0 ? 16 : n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : n % 5 == 0 ? 5 : 7;
+}
+template <int N>
+struct CPlan {
+	static constexpr int npass()
+	{
+		int n = N, c = 0;
+		while (n > 1) {
+			n /= first_radix(n);
+			c++;
+		}
+		return c;
+	}
+	static constexpr int radix(int p)
+	{
+		int n = N;
+		for (int i = 0; i < p; i++) n /= first_radix(n);
+		return first_radix(n);
+	}
+	static constexpr int ns(int p)
+	{
+		int s = 1, n = N;
+		for (int i = 0; i < p; i++) {
+			const int r = first_radix(n);
+			s *= r;
+			n /= r;
+		}
+		return s;
+	}
+};
+
+// padded LDS index: one spare element every 16 complex values
+template <int N>
+struct CPad {
+	static constexpr int len = N + N / 16 + 1;
+	__device__ static __forceinline__ int at(int s, int i) { return s * len + i + (i >> 4); }
+};
+
+template <int N, int NSEQ, int R, int Ns, int SIGN>
+__device__ __forceinline__ void pass_ct(const double2 *__restrict__ src, double2 *__restrict__ dst,
+					const double2 *__restrict__ tw)
+{
+	constexpr int NB = N / R, TOT = NSEQ * NB, TWS = N / (Ns * R);
+	constexpr int IT = (TOT + 255) / 256;
+#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int idx = (int)threadIdx.x + it * 256;
+		if (TOT % 256 != 0 && idx >= TOT) break;
+		const int sq = idx / NB, b = idx % NB;
+		const int k = b % Ns, jb = b / Ns;
+		double2 v[R];
+#pragma unroll
+		for (int t = 0; t < R; t++) v[t] = src[CPad<N>::at(sq, b + t * NB)];
+		if (Ns > 1) {
+#pragma unroll
+			for (int t = 1; t < R; t++) {
+				double2 w = tw[t * k * TWS];
+				if (SIGN > 0) w.y = -w.y;
+				v[t] = cmul(v[t], w);
+			}
+		}
+		Dft<R>::run(v, SIGN);
+		const int o = jb * Ns * R + k;
+#pragma unroll
+		for (int t = 0; t < R; t++) dst[CPad<N>::at(sq, o + t * Ns)] = v[t];
+	}
+}
+
+template <int N, int NSEQ, int SIGN, int P>
+struct PassesCT {
+	__device__ static __forceinline__ double2 *run(double2 *a, double2 *b, const double2 *__restrict__ tw)
+	{
+		if constexpr (P == CPlan<N>::npass()) {
+			return a;
+		} else {
+			pass_ct<N, NSEQ, CPlan<N>::radix(P), CPlan<N>::ns(P), SIGN>(a, b, tw);
+			__syncthreads();
+			return PassesCT<N, NSEQ, SIGN, P + 1>::run(b, a, tw);
+		}
+	}
+};
+
+// MODE_R2C: nf real fields -> Fourier rows (L + 1, nlat); MODE_C2R: back.
+// Workgroup = 2 NSEQ adjacent rings packed into NSEQ complex sequences.
+template <int MODE, int N, int NSEQ>
+__global__ __launch_bounds__(256) void k_fft_real(const void *__restrict__ in, void *__restrict__ out,
+						  const double2 *__restrict__ tw, int nlat, int L, int nblk, int nblk_pad,
+						  double norm)
+{
+	constexpr int RB = 2 * NSEQ;
+	extern __shared__ double2 sm[];
+	const int lb = xcd_remap(blockIdx.x, nblk_pad);
+	if (lb >= nblk) return;
+	const int f = blockIdx.y;
+	const size_t nspat = (size_t)nlat * N;
+	double2 *bufA = sm;
+	double2 *bufB = sm + (size_t)NSEQ * CPad<N>::len;
+	const int tid = threadIdx.x;
+	const int j0 = lb * RB;
+	if (MODE == MODE_R2C) {
+		const double *src = (const double *)in + f * nspat;
+		double *lds = (double *)bufA;
+		constexpr int TOT = N * RB;
+		for (int b = 0; b < TOT; b += KB * 256) {
+			double v[KB];
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				const int idx = min(b + u * 256 + tid, TOT - 1);
+				const int i = idx / RB, r = idx % RB;
+				v[u] = src[(size_t)i * nlat + min(j0 + r, nlat - 1)];
+			}
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				const int idx = b + u * 256 + tid;
+				if (TOT % (KB * 256) != 0 && idx >= TOT) break;
+				const int i = idx / RB, r = idx % RB;
+				lds[2 * CPad<N>::at(r >> 1, i) + (r & 1)] = (j0 + r < nlat) ? v[u] : 0.0;
+			}
+		}
+		__syncthreads();
+		const double2 *res = PassesCT<N, NSEQ, -1, 0>::run(bufA, bufB, tw);
+		double2 *dst = (double2 *)out + (size_t)f * (L + 1) * nlat;
+		for (int idx = tid; idx < (L + 1) * RB; idx += 256) {
+			const int m = idx / RB, r = idx % RB;
+			const int j = j0 + r;
+			if (j >= nlat) continue;
+			const int sq = r >> 1;
+			const double2 Z = res[CPad<N>::at(sq, m)];
+			const double2 Zc = cconj(res[CPad<N>::at(sq, m == 0 ? 0 : N - m)]);
+			double2 v;
+			if ((r & 1) == 0) v = make_double2(0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y));
+			else v = make_double2(0.5 * (Z.y - Zc.y), -0.5 * (Z.x - Zc.x));
+			if (m == 0) v.y = 0.0;
+			dst[(size_t)m * nlat + j] = make_double2(v.x * norm, v.y * norm);
+		}
+	} else {
+		const double2 *src = (const double2 *)in + (size_t)f * (L + 1) * nlat;
+		// stage G[s][m][part] in bufB (2 (L + 1) <= N + 1 entries per sequence)
+		const int ng = (L + 1) * RB;
+		for (int b = 0; b < ng; b += KB * 256) {
+			double2 v[KB];
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				const int idx = min(b + u * 256 + tid, ng - 1);
+				const int m = idx / RB, r = idx % RB;
+				v[u] = src[(size_t)m * nlat + min(j0 + r, nlat - 1)];
+			}
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				const int idx = b + u * 256 + tid;
+				if (idx >= ng) break;
+				const int m = idx / RB, r = idx % RB;
+				bufB[((r >> 1) * (L + 1) + m) * 2 + (r & 1)] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
+			}
+		}
+		__syncthreads();
+		for (int idx = tid; idx < NSEQ * N; idx += 256) {
+			const int sq = idx / N, k = idx % N;
+			double2 z = make_double2(0, 0);
+			if (k == 0) {
+				const double2 A = bufB[(sq * (L + 1)) * 2], B = bufB[(sq * (L + 1)) * 2 + 1];
+				z = make_double2(A.x, B.x);
+			} else if (k <= L) {
+				const double2 A = bufB[(sq * (L + 1) + k) * 2], B = bufB[(sq * (L + 1) + k) * 2 + 1];
+				z = make_double2(A.x - B.y, A.y + B.x);
+			} else if (k >= N - L) {
+				const int m = N - k;
+				const double2 A = bufB[(sq * (L + 1) + m) * 2], B = bufB[(sq * (L + 1) + m) * 2 + 1];
+				z = make_double2(A.x + B.y, -A.y + B.x);
+			}
+			bufA[CPad<N>::at(sq, k)] = z;
+		}
+		__syncthreads();
+		const double2 *res = PassesCT<N, NSEQ, +1, 0>::run(bufA, bufB, tw);
+		double *dst = (double *)out + f * nspat;
+		const double *rl = (const double *)res;
+		constexpr int TOT = N * RB;
+		for (int idx = tid; idx < TOT; idx += 256) {
+			const int i = idx / RB, r = idx % RB;
+			const int j = j0 + r;
+			if (j < nlat) dst[(size_t)i * nlat + j] = rl[2 * CPad<N>::at(r >> 1, i) + (r & 1)];
+		}
+	}
+}
+
+template <int MODE, int N, int NSEQ>
+int launch_real_ct(sphswe_ctx *ctx, const void *in, void *out, int nf)
+{
+	constexpr int RB = 2 * NSEQ;
+	const int nblk = (ctx->nlat + RB - 1) / RB;
+	const int nblk_pad = (nblk + 7) / 8 * 8;
+	const size_t lds = 2 * (size_t)NSEQ * CPad<N>::len * sizeof(double2);
+	if (lds > 65536)
+		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_real<MODE, N, NSEQ>,
+						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	dim3 grid(nblk_pad, nf);
+	k_fft_real<MODE, N, NSEQ><<<grid, 256, lds, ctx->stream>>>(in, out, ctx->d_tw, ctx->nlat, ctx->L, nblk, nblk_pad,
+								    2.0 * M_PI / N);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return SPHSWE_OK;
+}
+
+// the compile-time kernel for the grid sizes of the usual truncations, or -1
+template <int MODE>
+int launch_real_fixed(sphswe_ctx *ctx, const void *in, void *out, int nf)
+{
+	switch (ctx->plan.n) {
+	case 96: return launch_real_ct<MODE, 96, 8>(ctx, in, out, nf);
+	case 196: return launch_real_ct<MODE, 196, 8>(ctx, in, out, nf);
+	case 384: return launch_real_ct<MODE, 384, 4>(ctx, in, out, nf);
+	case 784: return launch_real_ct<MODE, 784, 2>(ctx, in, out, nf);
+	case 1536: return launch_real_ct<MODE, 1536, 1>(ctx, in, out, nf);
+	case 3072: return launch_real_ct<MODE, 3072, 1>(ctx, in, out, nf);
+	}
+	return -1;
 }
 
 int seqs_per_block(int N)
@@ -329,13 +649,27 @@ int launch(sphswe_ctx *ctx, const void *in, void *out, int nf)
 	FftArgs a;
 	a.N = ctx->plan.n;
 	a.npass = ctx->plan.npass;
-	for (int i = 0; i < SPHSWE_MAX_RADIX; i++) a.radix[i] = ctx->plan.radix[i];
+	int Ns = 1;
+	for (int i = 0; i < SPHSWE_MAX_RADIX; i++) {
+		a.radix[i] = ctx->plan.radix[i];
+		if (i < a.npass) {
+			a.nbq[i] = qdiv_of(a.N / a.radix[i]);
+			a.nsq[i] = qdiv_of(Ns);
+			Ns *= a.radix[i];
+		} else {
+			a.nbq[i] = a.nsq[i] = qdiv_of(1);
+		}
+	}
 	int nseq = seqs_per_block(a.N);
 	int rings = (MODE == MODE_R2C || MODE == MODE_C2R) ? 2 * nseq : nseq;
+	a.Nq = qdiv_of(a.N);
+	a.RBq = qdiv_of(rings);
+	a.Lq = qdiv_of(ctx->L + 1);
 	int nblk = (ctx->nlat + rings - 1) / rings;
 	int nblk_pad = (nblk + 7) / 8 * 8;
-	// + 2*nseq slack: the c2r staging of G uses 2*(L+1) <= N+1 entries per sequence
-	size_t lds = (2 * (size_t)nseq * a.N + 2 * nseq) * sizeof(double2);
+	// two padded buffers of nseq sequences (the c2r staging of G uses
+	// 2 (L + 1) <= N + 1 < padded_len(N) entries per sequence)
+	size_t lds = 2 * (size_t)nseq * padded_len(a.N) * sizeof(double2);
 	double norm = 2.0 * M_PI / a.N;
 	dim3 grid(nblk_pad, nf);
 	if (lds > 65536)
@@ -349,11 +683,15 @@ int launch(sphswe_ctx *ctx, const void *in, void *out, int nf)
 
 int sphswe_fft_r2c(sphswe_ctx *ctx, const double *spat, double2 *four, int nf)
 {
-	return launch<MODE_R2C>(ctx, spat, four, nf);
+	if (nf <= 0) return SPHSWE_OK;
+	const int rc = launch_real_fixed<MODE_R2C>(ctx, spat, four, nf);
+	return rc >= 0 ? rc : launch<MODE_R2C>(ctx, spat, four, nf);
 }
 int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf)
 {
-	return launch<MODE_C2R>(ctx, four, spat, nf);
+	if (nf <= 0) return SPHSWE_OK;
+	const int rc = launch_real_fixed<MODE_C2R>(ctx, four, spat, nf);
+	return rc >= 0 ? rc : launch<MODE_C2R>(ctx, four, spat, nf);
 }
 int sphswe_fft_c2c_fwd(sphswe_ctx *ctx, const double2 *spat, double2 *four, int nf)
 {
