@@ -225,13 +225,24 @@ Geo geo_of(sphswe_ctx *ctx)
 // E[ring][column] = w_j (F_N + F_S) (even parity of l - m) and w_j (F_N - F_S)
 // (odd), for the 16 MFMA columns of a slot group, written once per (order,
 // 64-ring chunk) in the order in which the analysis lanes consume them, so
-// that the analysis reads them with coalesced loads and every 32-degree
-// segment of the order reuses them.  Rings past NJ get weight 0.
-//   FP64 / FP32: per (m, chunk, parity) 16 steps x 64 lanes of one value;
-//                step s, lane n + 16 kq holds ring 4 s + kq of column n.
-//   bf16 (NT terms): per (m, chunk, parity) 4 steps x NT terms x 64 lanes of
-//                four values; step s, lane n + 16 kq holds rings
-//                16 s + 4 kq + e (e = 0..3) of column n.
+// that every 32-degree segment of the order reuses them with coalesced loads.
+// Rings past NJ get weight 0.  Lane n + 16 kq (column n) holds QN values q,
+// stored at [q][lane]; the columns of absent fields are neither written nor
+// read (their bytes are never requested).
+//   FP64 / FP32: q = 16 parity + s, s = 0..15: ring 4 s + kq;
+//   bf16 (NT terms): q = (4 parity + s) NT + term, s = 0..3: four values for
+//                rings 16 s + 4 kq + e (e = 0..3).
+template <int PM>
+__host__ __device__ constexpr int eop_qn()
+{
+	return (PM == PM_F64 || PM == PM_F32) ? 32 : 8 * Terms<PM>::n;
+}
+template <int PM>
+__device__ __forceinline__ int eop_at(int n, int q, int kq)
+{
+	return q * 64 + kq * 16 + n;
+}
+
 template <int PM>
 __host__ __device__ inline size_t eop_bytes_per_chunk()
 {
@@ -255,7 +266,7 @@ __global__ __launch_bounds__(256) void k_leg_eprep(const double2 *__restrict__ f
 	const double *prow = (const double *)four + 2 * column_fb<LAY, NS>(col, g) + (n & 1);
 	const double cmask = col.valid ? 1.0 : 0.0;
 	const int NJ = g.NJ, nlat = g.nlat;
-	double be[16], bo[16];
+	double be[16], bo[16];   // computed for every lane, stored for present columns
 #pragma unroll
 	for (int s = 0; s < 16; s++) {
 		const int rr = (PM == PM_F64 || PM == PM_F32) ? 4 * s + kq : 16 * (s >> 2) + 4 * kq + (s & 3);
@@ -268,19 +279,20 @@ __global__ __launch_bounds__(256) void k_leg_eprep(const double2 *__restrict__ f
 	}
 	const size_t base = (size_t)blockIdx.y * (g.L + 1) * nch * eop_bytes_per_chunk<PM>() + mc * eop_bytes_per_chunk<PM>();
 	char *eb = (char *)E + base;
+	if (!col.valid) return;
 	if (PM == PM_F64) {
 		double *e = (double *)eb;
 #pragma unroll
 		for (int s = 0; s < 16; s++) {
-			e[s * 64 + lane] = be[s];
-			e[(16 + s) * 64 + lane] = bo[s];
+			e[eop_at<PM>(n, s, kq)] = be[s];
+			e[eop_at<PM>(n, 16 + s, kq)] = bo[s];
 		}
 	} else if (PM == PM_F32) {
 		float *e = (float *)eb;
 #pragma unroll
 		for (int s = 0; s < 16; s++) {
-			e[s * 64 + lane] = (float)be[s];
-			e[(16 + s) * 64 + lane] = (float)bo[s];
+			e[eop_at<PM>(n, s, kq)] = (float)be[s];
+			e[eop_at<PM>(n, 16 + s, kq)] = (float)bo[s];
 		}
 	} else {
 		constexpr int NT = Terms<PM>::n;
@@ -295,7 +307,7 @@ __global__ __launch_bounds__(256) void k_leg_eprep(const double2 *__restrict__ f
 				s4v t[NT];
 				split4<NT>(v4, t);
 #pragma unroll
-				for (int k = 0; k < NT; k++) e[((p * 4 + s) * NT + k) * 64 + lane] = t[k];
+				for (int k = 0; k < NT; k++) e[eop_at<PM>(n, (p * 4 + s) * NT + k, kq)] = t[k];
 			}
 	}
 }

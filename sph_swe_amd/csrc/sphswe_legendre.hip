@@ -243,7 +243,7 @@ __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__re
 // crosses wavefronts.  For each 64-ring chunk the wavefront
 //   - loads its B operands, the weighted hemisphere combinations E of the
 //     chunk's rings for its column (written once per (m, chunk) by the
-//     operand pass, coalesced);
+//     operand pass; columns of absent fields read as 0);
 //   - runs the recurrence with lane = ring over the 32 degrees of its segment
 //     and writes the Pbar tile to its own LDS region, split by parity of l-m;
 //   - contracts over rings with v_mfma_f64_16x16x4f64:
@@ -290,8 +290,8 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double *__rest
 		double be[16], bo[16];
 #pragma unroll
 		for (int s = 0; s < 16; s++) {
-			be[s] = Ec[s * 64 + lane];
-			bo[s] = Ec[(16 + s) * 64 + lane];
+			be[s] = col.valid ? Ec[eop_at<PM_F64>(n, s, kq)] : 0.0;
+			bo[s] = col.valid ? Ec[eop_at<PM_F64>(n, 16 + s, kq)] : 0.0;
 		}
 		// Pbar tile: lane = ring (rings past NJ give finite values that meet
 		// weight 0; rows past se, of a short last segment, are never stored)

@@ -257,8 +257,8 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_anal_mx(const void *__res
 			const float *e = (const float *)Ec;
 #pragma unroll
 			for (int s = 0; s < 16; s++) {
-				be[s] = e[s * 64 + lane];
-				bo[s] = e[(16 + s) * 64 + lane];
+				be[s] = col.valid ? e[eop_at<PM>(n, s, kq)] : 0.0f;
+				bo[s] = col.valid ? e[eop_at<PM>(n, 16 + s, kq)] : 0.0f;
 			}
 		} else {
 			const s4v *e = (const s4v *)Ec;
@@ -267,7 +267,8 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_anal_mx(const void *__res
 #pragma unroll
 				for (int s = 0; s < 4; s++)
 #pragma unroll
-					for (int k = 0; k < NT; k++) bs[p][s][k] = e[((p * 4 + s) * NT + k) * 64 + lane];
+					for (int k = 0; k < NT; k++)
+						bs[p][s][k] = col.valid ? e[eop_at<PM>(n, (p * 4 + s) * NT + k, kq)] : s4v{0, 0, 0, 0};
 		}
 		// Pbar tile: lane = ring
 		{
