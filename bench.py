@@ -10,7 +10,7 @@ same round trip with the Legendre contractions in FP32, BF16 (hi/lo) and BF16
 (Gaussian dam, dt = 0.001 a / nlat = 8.296 s) in FP64 and in each reduced mode
 with the geopotential error after 100 steps,
 the linear REXI rate at T256 (200 poles, dt = 3600 s, seeded (phi, vort, div)
-start state), the roofline of the dominant kernel (the Legendre synthesis)
+start state), the roofline of the dominant kernel (the Legendre analysis)
 measured with HIP events on the library's stream, and the CPU oracle timed on a
 bounded sample.
 """
@@ -304,7 +304,7 @@ def cpu_baseline(L, ctx_nlat, ctx_nphi):
             "sample": "1 round trip of 3 fields at T511 (768x1536), oracle/sphswe_oracle.c, %.1f s" % dt}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leg_synth_T511_3fields_hbm.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leg_anal_T511_3fields_hbm.json")
 
 
 def pmc_kernels():
@@ -329,8 +329,9 @@ def pmc_kernels():
 
 
 def measured_traffic():
-    """HBM bytes per launch of the Legendre synthesis from the committed counter
-    summary (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or None."""
+    """HBM bytes per Legendre analysis (operand pass + contraction) from the
+    committed counter summary (FETCH_SIZE and WRITE_SIZE scaled by the copy
+    calibration), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             return json.load(f)["bytes_per_launch"]
@@ -365,6 +366,7 @@ def main():
     ms_syn, ms_ana, flops = bench_legendre_kernel(ctx)
     swe_rate, swe_dt = bench_swe(ctx, args.swe_steps, 2)
     achieved = flops / (ms_syn * 1e-3) / 1e12
+    achieved_ana = flops / (ms_ana * 1e-3) / 1e12
     mixed_sht = bench_sht_modes(ctx, max(args.steps // 4, 10), 3)
     mixed_swe = bench_swe_modes(ctx) if args.swe_steps > 0 else None
     ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
@@ -395,12 +397,13 @@ def main():
         "ensemble_T127_64members": ensemble,
         "rexi_steps_per_s": rexi_rate,
         "rexi_config": rexi_cfg,
-        "roofline": {"bound": "mfma", "engine": "fp64 VALU (fp64 vector peak = fp64 matrix peak on MI355X)",
-                     "kernel": "k_leg_synth (Legendre synthesis, 3 fields)",
-                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
-                     "avg_ms": ms_syn, "analysis_avg_ms": ms_ana,
-                     "analysis_achieved": flops / (ms_ana * 1e-3) / 1e12},
+        "roofline": {"bound": "mfma", "engine": "fp64 matrix cores (v_mfma_f64_16x16x4f64)",
+                     "kernel": "k_leg_eprep + k_leg_anal (Legendre analysis, 3 fields, T511)",
+                     "achieved": achieved_ana, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_ana / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
+                     "avg_ms": ms_ana, "flops_per_launch": flops,
+                     "synthesis": {"kernel": "k_leg_synth (FP64 VALU)", "avg_ms": ms_syn, "achieved": achieved,
+                                   "frac": achieved / FP64_PEAK_TFLOPS}},
         "cpu_baseline": cpu,
     }
     ctx.close()
