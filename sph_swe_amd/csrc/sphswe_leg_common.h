@@ -227,8 +227,8 @@ Geo geo_of(sphswe_ctx *ctx)
 // 64-ring chunk) in the order in which the analysis lanes consume them, so
 // that every 32-degree segment of the order reuses them with coalesced loads.
 // Rings past NJ get weight 0.  Lane n + 16 kq (column n) holds QN values q,
-// stored at [q][lane]; the columns of absent fields are neither written nor
-// read (their bytes are never requested).
+// stored at [q][lane]; the columns of absent fields are not written (a stale
+// value there reaches only its own MFMA output column, which is not stored).
 //   FP64 / FP32: q = 16 parity + s, s = 0..15: ring 4 s + kq;
 //   bf16 (NT terms): q = (4 parity + s) NT + term, s = 0..3: four values for
 //                rings 16 s + 4 kq + e (e = 0..3).

@@ -285,20 +285,30 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double *__rest
 	const double *Em = E + ((size_t)blockIdx.y * (L + 1) + am) * nch * (2 * 16 * 64);
 	d4v acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
 	double (*P)[16][AN_PS] = Pt[wave];
-	for (int c = 0; c < nch; c++) {
+	// operands of chunk c: E values (B) and the Pbar tile (lane = ring; rings
+	// past NJ give finite values that meet weight 0; rows past se, of a short
+	// last segment, are never stored)
+	auto load_e = [&](int c, double (&be)[16], double (&bo)[16]) __attribute__((always_inline)) {
 		const double *Ec = Em + (size_t)c * (2 * 16 * 64);
-		double be[16], bo[16];
+		// unconditional loads (the buffer spans all 16 columns): the columns of
+		// absent fields hold stale values, which reach only their own output
+		// column of the MFMA, and that column is never stored
 #pragma unroll
 		for (int s = 0; s < 16; s++) {
-			be[s] = col.valid ? Ec[eop_at<PM_F64>(n, s, kq)] : 0.0;
-			bo[s] = col.valid ? Ec[eop_at<PM_F64>(n, 16 + s, kq)] : 0.0;
+			be[s] = Ec[eop_at<PM_F64>(n, s, kq)];
+			bo[s] = Ec[eop_at<PM_F64>(n, 16 + s, kq)];
 		}
-		// Pbar tile: lane = ring (rings past NJ give finite values that meet
-		// weight 0; rows past se, of a short last segment, are never stored)
-		const int j = c * 64 + lane;
+	};
+	// seed and mu of this lane's ring in chunk c (chunks past the last reload
+	// the last); they are loaded two chunks ahead, so that the recurrence never
+	// waits for the E loads issued after them (loads complete in order)
+	auto ring_of = [&](int c, double2 &sd, double &mu) __attribute__((always_inline)) {
+		const int j = min(c, nch - 1) * 64 + lane;
 		const int jc = j < NJ ? j : NJ - 1;
-		const double mu = g.ct[jc];
-		const double2 sd = g.seed[(long)u * NJ + jc];
+		mu = g.ct[jc];
+		sd = g.seed[(long)u * NJ + jc];
+	};
+	auto make_tile = [&](double2 sd, double mu) __attribute__((always_inline)) {
 		double p0 = sd.x, p1 = sd.y;
 		P[0][0][lane] = p0;
 		P[1][0][lane] = p1;
@@ -309,14 +319,40 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double *__rest
 			p1 = Pv;
 			P[t & 1][t >> 1][lane] = Pv;
 		}
-		wave_sync();
+	};
+	// software pipeline: the tile of chunk c is read into registers, its MFMAs
+	// are issued, and while the matrix core works through them the recurrence
+	// of chunk c + 1 rewrites the tile and its E values are loaded
+	double be[16], bo[16];
+	double2 sd, sdn;
+	double mu, mun;
+	ring_of(0, sd, mu);
+	ring_of(1, sdn, mun);
+	load_e(0, be, bo);
+	make_tile(sd, mu);
+	wave_sync();
+	for (int c = 0; c < nch; c++) {
+		double a0[16], a1[16];
+#pragma unroll
+		for (int s = 0; s < 16; s++) {
+			a0[s] = P[0][n][4 * s + kq];
+			a1[s] = P[1][n][4 * s + kq];
+		}
+		wave_sync();   // every lane has its A operands: the tile is free
 		// contraction over the 64 rings of the chunk, 4 per MFMA
 #pragma unroll
 		for (int s = 0; s < 16; s++) {
-			acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(P[0][n][4 * s + kq], be[s], acc0, 0, 0, 0);
-			acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(P[1][n][4 * s + kq], bo[s], acc1, 0, 0, 0);
+			acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[s], be[s], acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[s], bo[s], acc1, 0, 0, 0);
 		}
-		wave_sync();
+		if (c + 1 < nch) {
+			sd = sdn;
+			mu = mun;
+			ring_of(c + 2, sdn, mun);
+			load_e(c + 1, be, bo);
+			make_tile(sd, mu);
+			wave_sync();
+		}
 	}
 	// D[i][n] in lane n + 16 (i % 4), element i / 4; row i of parity p is degree s0 + 2i + p
 	if (!col.valid) return;
