@@ -92,6 +92,7 @@ __device__ inline long slot_qaddr(int ss, int am, int l, int group, int nfields,
 }
 
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
 typedef short s4v __attribute__((ext_vector_type(4)));
 
 enum { PM_F64 = SPHSWE_PREC_FP64, PM_F32 = SPHSWE_PREC_FP32, PM_BF16 = SPHSWE_PREC_BF16, PM_BF16X3 = SPHSWE_PREC_BF16X3 };

@@ -256,8 +256,6 @@ constexpr int AN_WAVES = 4;       // independent wavefronts per workgroup
 constexpr int AN_PS = 65;         // LDS row stride (doubles) of the Pbar tile
 constexpr int AN_NS = 8;          // slots per group: 16 MFMA columns
 
-typedef double d4v __attribute__((ext_vector_type(4)));
-
 template <int LAY, int NS>
 __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double *__restrict__ E, double2 *__restrict__ spec, Geo g,
 							    const int *__restrict__ unit_m, int nunits, int group0,
@@ -463,10 +461,8 @@ int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, in
 int sphswe_leg_synth_pole(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
 			  const double *oscale)
 {
-	Geo g = geo_of(ctx);
-	g.moff = d_moff;
-	g.P = P;
-	return launch_synth<LAY_POLE, 4>(ctx, g, spec, four, nf, oscale);
+	// hundreds of columns: the contraction runs on the FP64 matrix cores
+	return sphswe_leg_synth_pole_mfma(ctx, spec, four, nf, P, d_moff, oscale);
 }
 
 int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff)

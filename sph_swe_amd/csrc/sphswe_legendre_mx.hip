@@ -28,6 +28,8 @@
 // element k % 4; both D[i][n] in lane n + 16 (i / 4) element i % 4.
 #include "sphswe_leg_common.h"
 
+#include <type_traits>
+
 namespace {
 
 // acc += A * B over the split terms of total order below NT
@@ -53,19 +55,28 @@ constexpr int MX_NS = 8;        // slots per group: 16 MFMA columns
 // synthesis
 // ----------------------------------------------------------------------------
 
-// row stride (floats) of the Pbar tile: conflict-free A-operand reads
-// (16 mod 32 for one value per lane, 12 mod 32 for four degrees per lane)
+// row stride (elements) of the Pbar tile: conflict-free A-operand reads
+// (FP32: 16 mod 32 for one value per lane, bf16: 12 mod 32 for four degrees
+// per lane; FP64: 16 consecutive doubles per lane group in any case)
 template <int PM>
 struct SxStride {
-	static constexpr int v = PM == PM_F32 ? 80 : 76;
+	static constexpr int v = PM == PM_F64 ? 65 : (PM == PM_F32 ? 80 : 76);
 };
+// the tile and accumulator types of a mode: FP64 for PM_F64
+// (v_mfma_f64_16x16x4f64, D[i][n] in lane n + 16 (i % 4), element i / 4),
+// FP32 otherwise
+template <int PM>
+using TileT = std::conditional_t<PM == PM_F64, double, float>;
+template <int PM>
+using AccT = std::conditional_t<PM == PM_F64, d4v, f4v>;
 
 template <int LAY, int NS, int PM>
 __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *__restrict__ spec, double2 *__restrict__ four,
 								Geo g, int nfields, const double *__restrict__ oscale)
 {
 	constexpr int PS = SxStride<PM>::v;
-	__shared__ float Pt[MX_WAVES][2][16][PS];
+	typedef TileT<PM> T;
+	__shared__ T Pt[MX_WAVES][2][16][PS];
 	__shared__ double2 Cfs[MX_WAVES][MX_SEG];
 	const int L = g.L, NJ = g.NJ;
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -88,21 +99,21 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *_
 	const int comp = n & 1;
 	const double *sd_ = (const double *)spec;
 
-	f4v acc[2][4];
+	AccT<PM> acc[2][4];
 #pragma unroll
 	for (int p = 0; p < 2; p++)
 #pragma unroll
-		for (int t = 0; t < 4; t++) acc[p][t] = f4v{0, 0, 0, 0};
-	float (*P)[16][PS] = Pt[wave];
+		for (int t = 0; t < 4; t++) acc[p][t] = AccT<PM>{0, 0, 0, 0};
+	T (*P)[16][PS] = Pt[wave];
 	double2 *cf = Cfs[wave];
 
 	for (int k = sa; k < sb; k++) {
 		const int s0 = am + k * MX_SEG, se = min(s0 + MX_SEG - 1, L);
 		if (lane < MX_SEG) cf[lane] = g.coef[lm_index(L, min(s0 + lane, se), am)];
 		// B operands: Q[degree][column]; degrees past se are 0
-		float bq[2][4];
+		T bq[2][4];
 		s4v bs[2][Terms<PM>::n];
-		if (PM == PM_F32) {
+		if (PM == PM_F32 || PM == PM_F64) {
 #pragma unroll
 			for (int p = 0; p < 2; p++)
 #pragma unroll
@@ -110,7 +121,7 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *_
 					const int l = s0 + 2 * (4 * s + kq) + p;
 					const int lc = min(l, se);
 					const double v = sd_[2 * column_q<LAY, NS>(col, am, lc, g) + comp];
-					bq[p][s] = (col.valid && l <= se) ? (float)(col.sgn * v) : 0.0f;
+					bq[p][s] = (col.valid && l <= se) ? (T)(col.sgn * v) : (T)0;
 				}
 		} else {
 #pragma unroll
@@ -131,18 +142,27 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *_
 		{
 			const double2 sdv = g.seed[((long)so + k) * NJ + jc];
 			double p0 = sdv.x, p1 = sdv.y;
-			P[0][0][lane] = (float)p0;
-			P[1][0][lane] = (float)p1;
+			P[0][0][lane] = (T)p0;
+			P[1][0][lane] = (T)p1;
 #pragma unroll
 			for (int t = 2; t < MX_SEG; t++) {
 				const double Pv = rec_step(cf[t], mu, p1, p0);
 				p0 = p1;
 				p1 = Pv;
-				P[t & 1][t >> 1][lane] = (float)Pv;
+				P[t & 1][t >> 1][lane] = (T)Pv;
 			}
 		}
 		wave_sync();
-		if (PM == PM_F32) {
+		if constexpr (PM == PM_F64) {
+#pragma unroll
+			for (int p = 0; p < 2; p++)
+#pragma unroll
+				for (int s = 0; s < 4; s++)
+#pragma unroll
+					for (int t = 0; t < 4; t++)
+						acc[p][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(P[p][4 * s + kq][16 * t + n], bq[p][s],
+												 acc[p][t], 0, 0, 0);
+		} else if constexpr (PM == PM_F32) {
 #pragma unroll
 			for (int p = 0; p < 2; p++)
 #pragma unroll
@@ -169,10 +189,10 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *_
 
 	// reduce the wavefronts of each order (LDS reused), leaders write
 	__syncthreads();
-	float *red = &Pt[0][0][0][0];
+	T *red = &Pt[0][0][0][0];
 	const bool leader = wi == 0;
 	if (!leader) {
-		float *rw = red + (size_t)(wave - 1) * 32 * 64;
+		T *rw = red + (size_t)(wave - 1) * 32 * 64;
 #pragma unroll
 		for (int p = 0; p < 2; p++)
 #pragma unroll
@@ -183,7 +203,7 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *_
 	__syncthreads();
 	if (!leader) return;
 	for (int w = wave + 1; w < wave + nw; w++) {
-		const float *rw = red + (size_t)(w - 1) * 32 * 64;
+		const T *rw = red + (size_t)(w - 1) * 32 * 64;
 #pragma unroll
 		for (int p = 0; p < 2; p++)
 #pragma unroll
@@ -199,7 +219,8 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *_
 	for (int t = 0; t < 4; t++)
 #pragma unroll
 		for (int r = 0; r < 4; r++) {
-			const int jj = blockIdx.x * 64 + 16 * t + 4 * kq + r;
+			// ring of element r: D row i = 4 kq + r (FP32 layouts), kq + 4 r (FP64)
+			const int jj = blockIdx.x * 64 + 16 * t + (PM == PM_F64 ? kq + 4 * r : 4 * kq + r);
 			if (jj >= NJ) continue;
 			const double sc = zero_im ? 0.0 : (oscale ? oscale[jj] : 1.0);
 			const double E = acc[0][t][r], O = acc[1][t][r];
@@ -372,6 +393,17 @@ int anal_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int pre
 }
 
 }  // namespace
+
+// FP64 synthesis on the FP64 matrix cores in the pole layout (REXI): nf
+// columns of row stride P, m-block offsets moff; output times oscale[j]
+int sphswe_leg_synth_pole_mfma(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
+			       const double *oscale)
+{
+	Geo g = geo_of(ctx);
+	g.moff = d_moff;
+	g.P = P;
+	return launch_synth_mx<LAY_POLE, PM_F64>(ctx, g, spec, four, nf, oscale);
+}
 
 int sphswe_leg_synth_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
 {
