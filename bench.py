@@ -130,6 +130,7 @@ def bench_swe(ctx, steps, warmup):
 def bench_rexi(steps, L=256, M=188):
     """Linear SWE by REXI: seeded start state, 200 poles, dt = 3600 s."""
     ctx = capi.Context(L)
+    tuning = ctx.autotune()
     state = ctx.to_device(workloads.rexi_state(ctx))
     prm = capi.rexi_params(dt=3600.0, rexi_h=0.2, rexi_M=M, half=1, visc=1e5)
     t0 = time.perf_counter()
@@ -146,7 +147,7 @@ def bench_rexi(steps, L=256, M=188):
     if not np.all(np.isfinite(out)):
         raise RuntimeError("REXI state became non-finite")
     cfg = {"lmax": L, "nlat": ctx.nlat, "nphi": ctx.nphi, "rexi_M": M, "rexi_h": 0.2, "poles": rx.npoles,
-           "dt": 3600.0, "steps": steps, "viscosity": 1e5, "setup_s": setup_s}
+           "dt": 3600.0, "steps": steps, "viscosity": 1e5, "setup_s": setup_s, "autotune": tuning}
     rx.close()
     ctx.close()
     return steps / (ms / 1000.0), cfg
@@ -245,6 +246,7 @@ def bench_ensemble(nsteps=200, nmem=64, L=127):
     (2.83), and every dt from 120 s up overflows within 200 steps (the m = 1
     polar mode of the scalar-wind formulation)."""
     ctx = capi.Context(L)
+    tuning = ctx.autotune()
     dt = 0.002 * 6.37122e6 / ctx.nlat
     rng = np.random.default_rng(2024)
     lon = np.arange(ctx.nphi) / ctx.nphi * 2 * np.pi
@@ -284,7 +286,7 @@ def bench_ensemble(nsteps=200, nmem=64, L=127):
             float(np.linalg.norm(finals[key][3 * k] - ref[3 * k]) / np.linalg.norm(ref[3 * k])) for k in range(nmem))
         res[key]["speedup_vs_fp64_eager"] = res[key]["member_steps_per_s"] / res["fp64_eager"]["member_steps_per_s"]
     cfg = {"lmax": L, "nlat": ctx.nlat, "nphi": ctx.nphi, "members": nmem, "dt": dt, "steps": nsteps,
-           "viscosity": 1e5}
+           "viscosity": 1e5, "autotune": tuning}
     ctx.close()
     return {"config": cfg, "runs": res}
 
@@ -348,6 +350,7 @@ def main():
     ap.add_argument("--rexi-steps", type=int, default=10)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch shapes)")
     ap.add_argument("--pmc-kernels", action="store_true",
                     help="only launch the kernels used for counter collection")
     args = ap.parse_args()
@@ -360,6 +363,9 @@ def main():
         return
 
     ctx = capi.Context(L_BENCH)
+    t0 = time.perf_counter()
+    tuning = ctx.autotune() if not args.no_tune else ctx.tuning()
+    tune_s = time.perf_counter() - t0
     wall, dev_ms, rt_err = bench_sht(ctx, args.steps, args.warmup)
     ms_step = dev_ms / args.steps
     value = args.steps / (dev_ms / 1000.0)
@@ -388,6 +394,7 @@ def main():
         "config": {"workload": "sht_round_trip_T511_3fields", "lmax": L_BENCH, "nlat": ctx.nlat,
                    "nphi": ctx.nphi, "fields": NFIELDS, "precision": "fp64"},
         "wall_s": wall,
+        "autotune": dict(tuning, setup_s=tune_s),
         "round_trip_rel_l2": rt_err,
         "swe_steps_per_s": swe_rate,
         "swe_config": {"lmax": L_BENCH, "dt": swe_dt, "scheme": "RK4 nonlinear", "steps": args.swe_steps,

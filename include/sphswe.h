@@ -240,6 +240,20 @@ int sphswe_band_add(sphswe_band *band, int component, double s_re, double s_im, 
  * zgbsv) on the first solve after a change; SPHSWE_ERR_SINGULAR on a zero pivot. */
 int sphswe_band_solve(sphswe_ctx *ctx, sphswe_band *band, const double *d_rhs, double *d_x);
 
+/* ---- autotuner (no counterpart in the reference: launch shapes are this
+ * library's).  sphswe_autotune times the variants available for the context's
+ * sizes on synthetic data (about a second at T511; allocates and frees its own
+ * buffers, synchronises the stream) and keeps the fastest:
+ *   fft_nseq_r2c / _c2r  complex sequences per workgroup of the real FFTs
+ *                        (0: built-in default);
+ *   synth_mfma_min       FP64 real syntheses of at least this many fields run
+ *                        on the FP64 matrix cores (0: never).
+ * All variants are FP64 and differ only in the order of the sums.
+ * sphswe_set_tuning pins the choices (e.g. from an earlier run). */
+int sphswe_autotune(sphswe_ctx *ctx);
+int sphswe_set_tuning(sphswe_ctx *ctx, int fft_nseq_r2c, int fft_nseq_c2r, int synth_mfma_min);
+int sphswe_get_tuning(const sphswe_ctx *ctx, int *fft_nseq_r2c, int *fft_nseq_c2r, int *synth_mfma_min, int *tuned);
+
 /* ---- timing on the context stream: ms between two sphswe_timer_mark calls */
 int sphswe_timer_mark(sphswe_ctx *ctx, int slot);
 int sphswe_timer_elapsed(sphswe_ctx *ctx, int slot0, int slot1, float *ms);

@@ -73,6 +73,9 @@ _SIGS = {
     "sphswe_reduce_minmax": [_vp, _vp, _sz, _dp, _dp, _dp],
     "sphswe_swe_rk4": [_vp, ctypes.POINTER(SweParams), _vp, _i, _i],
     "sphswe_swe_rk4_graph": [_vp, ctypes.POINTER(SweParams), _vp, _i, _i],
+    "sphswe_autotune": [_vp],
+    "sphswe_set_tuning": [_vp, _i, _i, _i],
+    "sphswe_get_tuning": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
     "sphswe_rexi_create": [_vp, ctypes.POINTER(RexiParams), ctypes.POINTER(_vp)],
     "sphswe_rexi_destroy": [_vp],
     "sphswe_rexi_poles": [_vp, _ip, _dp, _dp],
@@ -278,6 +281,22 @@ class Context:
     def swe_rk4_graph(self, params, state, nmembers, nsteps):
         """As swe_rk4, one captured step replayed nsteps times."""
         self._check(self.lib.sphswe_swe_rk4_graph(self.h, ctypes.byref(params), state.ptr, nmembers, nsteps))
+
+    # autotuner ----------------------------------------------------------------
+    def autotune(self):
+        """Times the launch variants of this context's sizes and keeps the
+        fastest; returns the choices (see tuning())."""
+        self._check(self.lib.sphswe_autotune(self.h))
+        return self.tuning()
+
+    def tuning(self):
+        v = [_i(0) for _ in range(4)]
+        self._check(self.lib.sphswe_get_tuning(self.h, *[ctypes.byref(x) for x in v]))
+        return {"fft_nseq_r2c": v[0].value, "fft_nseq_c2r": v[1].value, "synth_mfma_min": v[2].value,
+                "tuned": bool(v[3].value)}
+
+    def set_tuning(self, fft_nseq_r2c=0, fft_nseq_c2r=0, synth_mfma_min=0):
+        self._check(self.lib.sphswe_set_tuning(self.h, fft_nseq_r2c, fft_nseq_c2r, synth_mfma_min))
 
     # REXI -------------------------------------------------------------------
     def rexi(self, params, alpha=None, beta=None):

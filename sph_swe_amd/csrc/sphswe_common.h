@@ -42,6 +42,11 @@ struct sphswe_ctx {
 	int seg = 16;                 // degrees per segment
 	int nseg_total = 0;
 	int max_lds = 65536;          // LDS bytes a workgroup may allocate
+	// launch choices of the autotuner (0: the built-in default)
+	int fft_nseq_r2c = 0;         // complex sequences per workgroup of the real FFTs
+	int fft_nseq_c2r = 0;
+	int synth_mfma_min = 0;       // FP64 real synthesis on the matrix cores from this many fields (0: never)
+	int tuned = 0;
 	int *d_mofidx = nullptr;      // m of real spectral index [nlm]
 	int *d_lofcidx = nullptr;     // l of complex spectral index [(L+1)^2]
 	double2 *d_tw = nullptr;      // e^{-2 pi i k / nphi} [nphi]
@@ -86,6 +91,8 @@ int sphswe_escratch(sphswe_ctx *ctx, size_t bytes, void **p);
 // internal launchers ----------------------------------------------------------
 
 int sphswe_fft_r2c(sphswe_ctx *ctx, const double *spat, double2 *four, int nf);
+int sphswe_fft_nseq_candidates(const sphswe_ctx *ctx, int *out, int max);
+int sphswe_leg_synth_f64_mfma(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf);
 int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf);
 int sphswe_fft_c2c_fwd(sphswe_ctx *ctx, const double2 *spat, double2 *four, int nf);
 int sphswe_fft_c2c_bwd(sphswe_ctx *ctx, const double2 *four, double2 *spat, int nf);

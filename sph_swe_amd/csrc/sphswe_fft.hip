@@ -619,16 +619,30 @@ int launch_real_ct(sphswe_ctx *ctx, const void *in, void *out, int nf)
 	return SPHSWE_OK;
 }
 
-// the compile-time kernel for the grid sizes of the usual truncations, or -1
+// the compile-time kernel for the grid sizes of the usual truncations, or -1.
+// nseq: complex sequences per workgroup (2 nseq rings); 0 takes the default,
+// a value without an instantiation for this size falls back to it
 template <int MODE>
-int launch_real_fixed(sphswe_ctx *ctx, const void *in, void *out, int nf)
+int launch_real_fixed(sphswe_ctx *ctx, const void *in, void *out, int nf, int nseq)
 {
 	switch (ctx->plan.n) {
-	case 96: return launch_real_ct<MODE, 96, 8>(ctx, in, out, nf);
-	case 196: return launch_real_ct<MODE, 196, 8>(ctx, in, out, nf);
-	case 384: return launch_real_ct<MODE, 384, 4>(ctx, in, out, nf);
-	case 784: return launch_real_ct<MODE, 784, 2>(ctx, in, out, nf);
-	case 1536: return launch_real_ct<MODE, 1536, 1>(ctx, in, out, nf);
+	case 96:
+		if (nseq == 16) return launch_real_ct<MODE, 96, 16>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 96, 8>(ctx, in, out, nf);
+	case 196:
+		if (nseq == 4) return launch_real_ct<MODE, 196, 4>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 196, 8>(ctx, in, out, nf);
+	case 384:
+		if (nseq == 2) return launch_real_ct<MODE, 384, 2>(ctx, in, out, nf);
+		if (nseq == 8) return launch_real_ct<MODE, 384, 8>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 384, 4>(ctx, in, out, nf);
+	case 784:
+		if (nseq == 1) return launch_real_ct<MODE, 784, 1>(ctx, in, out, nf);
+		if (nseq == 4) return launch_real_ct<MODE, 784, 4>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 784, 2>(ctx, in, out, nf);
+	case 1536:
+		if (nseq == 2) return launch_real_ct<MODE, 1536, 2>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 1536, 1>(ctx, in, out, nf);
 	case 3072: return launch_real_ct<MODE, 3072, 1>(ctx, in, out, nf);
 	}
 	return -1;
@@ -684,15 +698,33 @@ int launch(sphswe_ctx *ctx, const void *in, void *out, int nf)
 int sphswe_fft_r2c(sphswe_ctx *ctx, const double *spat, double2 *four, int nf)
 {
 	if (nf <= 0) return SPHSWE_OK;
-	const int rc = launch_real_fixed<MODE_R2C>(ctx, spat, four, nf);
+	const int rc = launch_real_fixed<MODE_R2C>(ctx, spat, four, nf, ctx->fft_nseq_r2c);
 	return rc >= 0 ? rc : launch<MODE_R2C>(ctx, spat, four, nf);
 }
 int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf)
 {
 	if (nf <= 0) return SPHSWE_OK;
-	const int rc = launch_real_fixed<MODE_C2R>(ctx, four, spat, nf);
+	const int rc = launch_real_fixed<MODE_C2R>(ctx, four, spat, nf, ctx->fft_nseq_c2r);
 	return rc >= 0 ? rc : launch<MODE_C2R>(ctx, four, spat, nf);
 }
+// the sequences-per-workgroup candidates of the compile-time real FFT of this
+// grid (empty for other sizes), for the autotuner
+int sphswe_fft_nseq_candidates(const sphswe_ctx *ctx, int *out, int max)
+{
+	static const int c96[] = {8, 16}, c196[] = {4, 8}, c384[] = {2, 4, 8}, c784[] = {1, 2, 4}, c1536[] = {1, 2};
+	const int *c = nullptr;
+	int n = 0;
+	switch (ctx->plan.n) {
+	case 96: c = c96; n = 2; break;
+	case 196: c = c196; n = 2; break;
+	case 384: c = c384; n = 3; break;
+	case 784: c = c784; n = 3; break;
+	case 1536: c = c1536; n = 2; break;
+	}
+	for (int i = 0; i < n && i < max; i++) out[i] = c[i];
+	return n < max ? n : max;
+}
+
 int sphswe_fft_c2c_fwd(sphswe_ctx *ctx, const double2 *spat, double2 *four, int nf)
 {
 	return launch<MODE_C2C_FWD>(ctx, spat, four, nf);

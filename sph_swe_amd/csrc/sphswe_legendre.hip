@@ -408,6 +408,7 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_synth_mx(ctx, spec, four, nf, prec);
+	if (ctx->synth_mfma_min > 0 && nf >= ctx->synth_mfma_min) return sphswe_leg_synth_f64_mfma(ctx, spec, four, nf);
 	const long nlm = ctx->nlm;
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
 	int rc = SPHSWE_OK;

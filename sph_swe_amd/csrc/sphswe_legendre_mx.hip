@@ -405,6 +405,13 @@ int sphswe_leg_synth_pole_mfma(sphswe_ctx *ctx, const double2 *spec, double2 *fo
 	return launch_synth_mx<LAY_POLE, PM_F64>(ctx, g, spec, four, nf, oscale);
 }
 
+// FP64 real synthesis on the FP64 matrix cores (chosen by the autotuner for
+// many fields)
+int sphswe_leg_synth_f64_mfma(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf)
+{
+	return launch_synth_mx<LAY_REAL, PM_F64>(ctx, geo_of(ctx), spec, four, nf, nullptr);
+}
+
 int sphswe_leg_synth_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
 {
 	return synth_mx<LAY_REAL>(ctx, spec, four, nf, prec);
