@@ -181,6 +181,33 @@ __device__ inline double rec_step(double2 c, double mu, double p1, double p0)
 }
 
 
+// degrees 2..SEG-1 of a segment from (p0, p1) = degrees 0 and 1, with the
+// coefficients cf[t] in LDS; store(t, value) writes the tile.  In batches of 8:
+// 8 coefficient reads, 8 steps in registers, 8 stores (a coefficient read that
+// follows a tile store would otherwise wait its full LDS latency every step).
+template <int SEG, typename F>
+__device__ __forceinline__ void rec_tile(const double2 *cf, double mu, double p0, double p1, F store)
+{
+#pragma unroll
+	for (int tb = 2; tb < SEG; tb += 8) {
+		double2 c[8];
+		double v[8];
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+			if (tb + k < SEG) c[k] = cf[tb + k];
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+			if (tb + k < SEG) {
+				v[k] = rec_step(c[k], mu, p1, p0);
+				p0 = p1;
+				p1 = v[k];
+			}
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+			if (tb + k < SEG) store(tb + k, v[k]);
+	}
+}
+
 // W wavefronts of a workgroup share a pair of orders with nseg0 and nseg1
 // degree segments: the number given to the first order that minimises the
 // largest share

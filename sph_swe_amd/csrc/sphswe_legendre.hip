@@ -307,16 +307,9 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double *__rest
 		sd = g.seed[(long)u * NJ + jc];
 	};
 	auto make_tile = [&](double2 sd, double mu) __attribute__((always_inline)) {
-		double p0 = sd.x, p1 = sd.y;
-		P[0][0][lane] = p0;
-		P[1][0][lane] = p1;
-#pragma unroll
-		for (int t = 2; t < AN_SEG; t++) {
-			const double Pv = rec_step(cf[t], mu, p1, p0);
-			p0 = p1;
-			p1 = Pv;
-			P[t & 1][t >> 1][lane] = Pv;
-		}
+		P[0][0][lane] = sd.x;
+		P[1][0][lane] = sd.y;
+		rec_tile<AN_SEG>(cf, mu, sd.x, sd.y, [&](int t, double v) { P[t & 1][t >> 1][lane] = v; });
 	};
 	// software pipeline: the tile of chunk c is read into registers, its MFMAs
 	// are issued, and while the matrix core works through them the recurrence
@@ -411,17 +404,28 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 	if (ctx->synth_mfma_min > 0 && nf >= ctx->synth_mfma_min) return sphswe_leg_synth_f64_mfma(ctx, spec, four, nf);
 	const long nlm = ctx->nlm;
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
+	// groups of 4 and 3 fields, no group of one where avoidable: a group
+	// reruns the recurrence, so a lone field costs nearly as much as four
+	int n4 = nf / 4, n3 = 0, n2 = 0, n1 = 0;
+	switch (nf % 4) {
+	case 1:
+		if (nf >= 9) { n4 -= 2; n3 = 3; } else if (nf == 5) { n4 = 0; n3 = 1; n2 = 1; } else n1 = 1;
+		break;
+	case 2:
+		if (nf >= 6) { n4 -= 1; n3 = 2; } else n2 = 1;
+		break;
+	case 3: n3 = 1; break;
+	}
+	const Geo g = geo_of(ctx);
 	int rc = SPHSWE_OK;
 	int f = 0;
-	if (nf - f >= 4) {
-		int g = (nf - f) / 4;
-		if ((rc = launch_synth<LAY_REAL, 4>(ctx, geo_of(ctx), spec + f * nlm, four + f * nfour, g * 4, nullptr))) return rc;
-		f += g * 4;
-	}
-	int r = nf - f;
-	if (r == 3) rc = launch_synth<LAY_REAL, 3>(ctx, geo_of(ctx), spec + f * nlm, four + f * nfour, 3, nullptr);
-	else if (r == 2) rc = launch_synth<LAY_REAL, 2>(ctx, geo_of(ctx), spec + f * nlm, four + f * nfour, 2, nullptr);
-	else if (r == 1) rc = launch_synth<LAY_REAL, 1>(ctx, geo_of(ctx), spec + f * nlm, four + f * nfour, 1, nullptr);
+	if (n4 && (rc = launch_synth<LAY_REAL, 4>(ctx, g, spec + f * nlm, four + f * nfour, 4 * n4, nullptr))) return rc;
+	f += 4 * n4;
+	if (n3 && (rc = launch_synth<LAY_REAL, 3>(ctx, g, spec + f * nlm, four + f * nfour, 3 * n3, nullptr))) return rc;
+	f += 3 * n3;
+	if (n2 && (rc = launch_synth<LAY_REAL, 2>(ctx, g, spec + f * nlm, four + f * nfour, 2 * n2, nullptr))) return rc;
+	f += 2 * n2;
+	if (n1) rc = launch_synth<LAY_REAL, 1>(ctx, g, spec + f * nlm, four + f * nfour, 1, nullptr);
 	return rc;
 }
 

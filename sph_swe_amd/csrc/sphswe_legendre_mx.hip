@@ -144,13 +144,7 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_synth_mx(const double2 *_
 			double p0 = sdv.x, p1 = sdv.y;
 			P[0][0][lane] = (T)p0;
 			P[1][0][lane] = (T)p1;
-#pragma unroll
-			for (int t = 2; t < MX_SEG; t++) {
-				const double Pv = rec_step(cf[t], mu, p1, p0);
-				p0 = p1;
-				p1 = Pv;
-				P[t & 1][t >> 1][lane] = (T)Pv;
-			}
+			rec_tile<MX_SEG>(cf, mu, p0, p1, [&](int t, double v) { P[t & 1][t >> 1][lane] = (T)v; });
 		}
 		wave_sync();
 		if constexpr (PM == PM_F64) {
@@ -300,13 +294,7 @@ __global__ __launch_bounds__(64 * MX_WAVES) void k_leg_anal_mx(const void *__res
 			double p0 = sdv.x, p1 = sdv.y;
 			P[0][0][lane] = (float)p0;
 			P[1][0][lane] = (float)p1;
-#pragma unroll
-			for (int t = 2; t < MX_SEG; t++) {
-				const double Pv = rec_step(cf[t], mu, p1, p0);
-				p0 = p1;
-				p1 = Pv;
-				P[t & 1][t >> 1][lane] = (float)Pv;
-			}
+			rec_tile<MX_SEG>(cf, mu, p0, p1, [&](int t, double v) { P[t & 1][t >> 1][lane] = (float)v; });
 		}
 		wave_sync();
 		if (PM == PM_F32) {
