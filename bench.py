@@ -155,6 +155,48 @@ def bench_rexi(steps, L=256, M=188):
 
 MODES = ("fp32", "bf16", "bf16x3")
 
+# REXI precision forms: (Legendre precision of the per-pole transforms, storage
+# precision of the band LU factors)
+REXI_MODES = (("fp64", "fp32"), ("fp32", "fp64"), ("fp32", "fp32"), ("bf16x3", "fp32"), ("bf16", "fp32"))
+
+
+def bench_rexi_modes(steps, L=256, M=188):
+    """Config 1 (T256, 200 poles, dt = 3600 s, seeded start state) with the
+    per-pole Legendre transforms in reduced precision and/or the band LU factors
+    stored in FP32 (solves in FP64): steps/s and the relative L2 error of h, u
+    and v after the run against the FP64 run from the same start."""
+    ctx = capi.Context(L)
+    ctx.autotune()
+    x0 = workloads.rexi_state(ctx)
+    res, finals = {}, {}
+    for leg, band in (("fp64", "fp64"),) + REXI_MODES:
+        name = leg + "_legendre_" + band + "_factors"
+        prm = capi.rexi_params(dt=3600.0, rexi_h=0.2, rexi_M=M, half=1, visc=1e5,
+                               precision=capi.PRECISIONS[leg], band_precision=capi.PRECISIONS[band])
+        rx = ctx.rexi(prm)
+        state = ctx.to_device(x0)
+        rx.step(state, 1)
+        state = ctx.to_device(x0)
+        ctx.sync()
+        ctx.mark(8)
+        rx.step(state, steps)
+        ctx.mark(9)
+        ms = ctx.elapsed_ms(8, 9)
+        rx.close()
+        finals[name] = ctx.to_host(state, np.complex128, (3, ctx.nlm))
+        if not np.all(np.isfinite(finals[name])):
+            raise RuntimeError("REXI state became non-finite (%s)" % name)
+        res[name] = {"steps_per_s": steps / (ms / 1000.0)}
+    ref = "fp64_legendre_fp64_factors"
+    for name in res:
+        if name == ref:
+            continue
+        for k, f in enumerate("huv"):
+            res[name][f + "_rel_l2"] = _rel(finals[name][k], finals[ref][k])[0]
+        res[name]["speedup_vs_fp64"] = res[name]["steps_per_s"] / res[ref]["steps_per_s"]
+    ctx.close()
+    return {"lmax": L, "poles_M": M, "steps": steps, "dt": 3600.0, "modes": res}
+
 
 def _rel(x, ref):
     return float(np.linalg.norm(x - ref) / np.linalg.norm(ref)), float(np.abs(x - ref).max() / np.abs(ref).max())
@@ -377,6 +419,7 @@ def main():
     mixed_swe = bench_swe_modes(ctx) if args.swe_steps > 0 else None
     ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
     rexi_rate, rexi_cfg = bench_rexi(args.rexi_steps) if args.rexi_steps > 0 else (None, None)
+    rexi_modes = bench_rexi_modes(args.rexi_steps) if args.rexi_steps > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(L_BENCH, ctx.nlat, ctx.nphi)
     line = {
         "metric": "SHT round-trips/sec at T511 (3 fields, FP64); SWE steps/sec at T511 in swe_steps_per_s",
@@ -404,6 +447,7 @@ def main():
         "ensemble_T127_64members": ensemble,
         "rexi_steps_per_s": rexi_rate,
         "rexi_config": rexi_cfg,
+        "mixed_rexi_T256_200poles": rexi_modes,
         "roofline": {"bound": "mfma", "engine": "fp64 matrix cores (v_mfma_f64_16x16x4f64)",
                      "kernel": "k_leg_eprep + k_leg_anal (Legendre analysis, 3 fields, T511)",
                      "achieved": achieved_ana, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

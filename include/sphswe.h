@@ -35,7 +35,8 @@ enum {
 	SPHSWE_ERR_HIP = 2,       /* HIP runtime error */
 	SPHSWE_ERR_NOMEM = 3,     /* device allocation failed */
 	SPHSWE_ERR_SINGULAR = 4,  /* banded factorisation met a zero pivot (zgbsv INFO > 0) */
-	SPHSWE_ERR_NAN = 5        /* NaN/Inf detected */
+	SPHSWE_ERR_NAN = 5,       /* NaN/Inf detected */
+	SPHSWE_ERR_INTERNAL = 6   /* a structural check of the library failed */
 };
 
 /* precision of the Legendre contraction; the associated-Legendre recurrence is
@@ -188,6 +189,11 @@ typedef struct {
 	double rexi_h;           /* REXI sampling width (default 0.2) */
 	int rexi_M;              /* REXI approximation area (default 128) */
 	int rexi_half;           /* 1: half the poles (default) */
+	/* the fields below have no counterpart in the reference (0 = FP64 = its arithmetic) */
+	int precision;           /* SPHSWE_PREC_* of the per-pole Legendre transforms */
+	int band_precision;      /* SPHSWE_PREC_FP64 or SPHSWE_PREC_FP32: storage of the LU factors
+				  * (factorised in FP64 on the host; the solves run in FP64) */
+	int refine;              /* FP32 factors: sweeps of FP64 iterative refinement (0 or 1) */
 } sphswe_rexi_params;
 
 /* REXI::setup for (h, M, half) and SWERexiSPH::setup for every pole: pole

@@ -106,11 +106,13 @@ int sphswe_leg_anal_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int 
 int sphswe_leg_synth_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
 int sphswe_leg_anal_cplx_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
 int sphswe_leg_synth_cplx_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
-int sphswe_leg_synth_pole_mfma(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
-			       const double *oscale);
+int sphswe_leg_synth_pole_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
+			     const double *oscale, int prec);
+int sphswe_leg_anal_pole_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff,
+			    int prec);
 int sphswe_leg_synth_pole(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
-			  const double *oscale);
-int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff);
+			  const double *oscale, int prec);
+int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff, int prec);
 
 // device helpers --------------------------------------------------------------
 

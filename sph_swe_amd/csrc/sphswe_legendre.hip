@@ -462,16 +462,20 @@ int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, in
 }
 
 // pole layout (REXI): nfields columns, row stride P, m-block offsets moff;
-// synthesis output multiplied by oscale[j] when given
+// synthesis output multiplied by oscale[j] when given.  Hundreds of columns:
+// the synthesis contraction runs on the matrix cores in every precision, the
+// FP64 analysis on the FP64 analysis kernel.
 int sphswe_leg_synth_pole(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
-			  const double *oscale)
+			  const double *oscale, int prec)
 {
-	// hundreds of columns: the contraction runs on the FP64 matrix cores
-	return sphswe_leg_synth_pole_mfma(ctx, spec, four, nf, P, d_moff, oscale);
+	if (int rc = check_prec(ctx, prec)) return rc;
+	return sphswe_leg_synth_pole_mx(ctx, spec, four, nf, P, d_moff, oscale, prec);
 }
 
-int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff)
+int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff, int prec)
 {
+	if (int rc = check_prec(ctx, prec)) return rc;
+	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_pole_mx(ctx, four, spec, nf, P, d_moff, prec);
 	Geo g = geo_of(ctx);
 	g.moff = d_moff;
 	g.P = P;

@@ -382,15 +382,36 @@ int anal_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int pre
 
 }  // namespace
 
-// FP64 synthesis on the FP64 matrix cores in the pole layout (REXI): nf
-// columns of row stride P, m-block offsets moff; output times oscale[j]
-int sphswe_leg_synth_pole_mfma(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
-			       const double *oscale)
+// pole layout (REXI): nf columns of row stride P, m-block offsets moff, on
+// the matrix cores in any precision (FP64: v_mfma_f64_16x16x4f64); synthesis
+// output times oscale[j]
+int sphswe_leg_synth_pole_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
+			     const double *oscale, int prec)
 {
 	Geo g = geo_of(ctx);
 	g.moff = d_moff;
 	g.P = P;
-	return launch_synth_mx<LAY_POLE, PM_F64>(ctx, g, spec, four, nf, oscale);
+	switch (prec) {
+	case PM_F64: return launch_synth_mx<LAY_POLE, PM_F64>(ctx, g, spec, four, nf, oscale);
+	case PM_F32: return launch_synth_mx<LAY_POLE, PM_F32>(ctx, g, spec, four, nf, oscale);
+	case PM_BF16: return launch_synth_mx<LAY_POLE, PM_BF16>(ctx, g, spec, four, nf, oscale);
+	case PM_BF16X3: return launch_synth_mx<LAY_POLE, PM_BF16X3>(ctx, g, spec, four, nf, oscale);
+	}
+	return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "unknown precision %d", prec);
+}
+
+int sphswe_leg_anal_pole_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff,
+			    int prec)
+{
+	Geo g = geo_of(ctx);
+	g.moff = d_moff;
+	g.P = P;
+	switch (prec) {
+	case PM_F32: return launch_anal_mx<LAY_POLE, PM_F32>(ctx, g, four, spec, nf);
+	case PM_BF16: return launch_anal_mx<LAY_POLE, PM_BF16>(ctx, g, four, spec, nf);
+	case PM_BF16X3: return launch_anal_mx<LAY_POLE, PM_BF16X3>(ctx, g, four, spec, nf);
+	}
+	return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "unknown precision %d", prec);
 }
 
 // FP64 real synthesis on the FP64 matrix cores (chosen by the autotuner for

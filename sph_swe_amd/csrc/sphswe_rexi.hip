@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 
 using sphswe_host::cplx;
 
@@ -42,7 +43,8 @@ struct sphswe_rexi {
 	std::vector<cplx> alpha0, beta0;   // as returned by REXI::setup
 	double2 *d_alpha = nullptr, *d_beta = nullptr;   // divided by dt (SWERexiSPH.hpp:82-83)
 	int *d_moff = nullptr;
-	double2 *d_fphi = nullptr, *d_fvel = nullptr;
+	void *d_fphi = nullptr, *d_fvel = nullptr;   // parity-compact LU factors, double2 or float2
+	int fac32 = 0;                     // factors stored in FP32
 	unsigned char *d_pphi = nullptr, *d_pvel = nullptr;
 	double *d_oscale = nullptr;        // 2 pi / sqrt(1 - mu^2)
 	// work
@@ -80,17 +82,36 @@ __device__ inline double2 wave_sum(double2 v)
 	return v;
 }
 
+// R independent wavefront sums, interleaved so that their cross-lane
+// exchanges wait together
+template <int R>
+__device__ inline void wave_sum_n(double2 (&v)[R])
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+		for (int r = 0; r < R; r++) {
+			v[r].x += __shfl_xor(v[r].x, o);
+			v[r].y += __shfl_xor(v[r].y, o);
+		}
+}
+
+// rows of a band sweep whose loads are issued together: the sweeps are
+// sequential in the row, so without this every row waits a full memory latency
+constexpr int RD = 4;
+
 // m of workgroup row y: 0, -1, 1, -2, 2, ... (longest systems first)
 __device__ inline int m_of(int y) { return (y & 1) ? -((y + 1) >> 1) : (y >> 1); }
 
-// swap window[0] <-> window[k] for a runtime k in [0, K], as K pairwise
+// swap window[0] <-> window[k] for a runtime even k in [0, K], as pairwise
 // conditional swaps on registers (a select chain reading w[k] is turned into an
-// indexed access by the compiler, which moves the window to scratch memory)
+// indexed access by the compiler, which moves the window to scratch memory).
+// The REXI bands couple degrees of equal parity only, so pivots are even.
 template <int K>
 __device__ inline void pivot_swap(double2 (&w)[K + 1], int k)
 {
 #pragma unroll
-	for (int i = 1; i <= K; i++) {
+	for (int i = 2; i <= K; i += 2) {
 		const bool s = (k == i);
 		const double2 a = w[0], b = w[i];
 		w[0] = make_double2(s ? b.x : a.x, s ? b.y : a.y);
@@ -98,10 +119,23 @@ __device__ inline void pivot_swap(double2 (&w)[K + 1], int k)
 	}
 }
 
+// LU factors of the REXI bands, parity-compact (only the even offsets are
+// nonzero): per row j of an m block, for halo h, entry 0 = 1 / U(j,j), entries
+// 1..h = U(j-2e, j), entries h+1..h+h/2 = L(j+2e, j), e = 1, 2, ...; stored in
+// FP64 (double2) or rounded to FP32 (float2), the arithmetic is FP64
+constexpr int rexi_nec(int halo) { return 1 + halo + halo / 2; }
+__device__ inline double2 ldf(const double2 *p) { return *p; }
+__device__ inline double2 ldf(const float2 *p)
+{
+	const float2 v = *p;
+	return make_double2(v.x, v.y);
+}
+
+template <typename FT>
 struct PhiArgs {
 	const double2 *alpha, *beta;
 	const int *moff;
-	const double2 *fac;
+	const FT *fac;
 	const unsigned char *piv;
 	const double2 *basis;     // NBASIS complex spectral fields, standard layout
 	double2 *Y;               // forward-substitution scratch, pole layout
@@ -114,10 +148,10 @@ struct PhiArgs {
 };
 
 // phi: assemble the right-hand side, solve, emit gradient inputs, reduce beta*phi
-template <int KL>
-__global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs a)
+template <int KL, typename FT>
+__global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 {
-	constexpr int KV = 2 * KL, NE = 1 + KV + KL;
+	constexpr int KV = 2 * KL, NE = rexi_nec(KL);
 	const int m = m_of(blockIdx.y);
 	const int am = abs(m);
 	const int L = a.L, P = a.P;
@@ -160,7 +194,7 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs a)
 		// diagonal Helmholtz: phi = rhs / (alpha^2 + (Phi / r^2) n (n+1)) is done by
 		// the caller through factors holding 1 / diag; rows independent
 		for (int j = 0; j < N; j++) {
-			const double2 x = cm(rhs_at(j), a.fac[(row0 + j) * (size_t)NE * P + pc]);
+			const double2 x = cm(rhs_at(j), ldf(a.fac + (row0 + j) * (size_t)NE * P + pc));
 			if (pv) a.X[(row0 + j) * 2 * P + p] = make_double2(-m * x.y, m * x.x);
 			a.Y[(row0 + j) * P + pc] = x;
 			double2 bx = wave_sum(pv ? cm(be, x) : make_double2(0, 0));
@@ -182,43 +216,78 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs a)
 	double2 w[KL + 1];
 #pragma unroll
 	for (int i = 0; i <= KL; i++) w[i] = (i < N) ? rhs_at(i) : make_double2(0, 0);
-	for (int j = 0; j < N; j++) {
-		const double2 *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
-		const int k = a.piv[(row0 + j) * P + pc];
-		pivot_swap<KL>(w, k);
+	for (int jb = 0; jb < N; jb += RD) {
+		double2 lf[RD][KL / 2];
+		int kp[RD];
 #pragma unroll
-		for (int i = 1; i <= KL; i++) w[i] = cb(w[i], cm(f[(size_t)(KV + i) * P], w[0]));
-		a.Y[(row0 + j) * P + pc] = w[0];
+		for (int d = 0; d < RD; d++) {
+			const int j = min(jb + d, N - 1);
+			const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
 #pragma unroll
-		for (int i = 0; i < KL; i++) w[i] = w[i + 1];
-		w[KL] = (j + KL + 1 < N) ? rhs_at(j + KL + 1) : make_double2(0, 0);
+			for (int e = 0; e < KL / 2; e++) lf[d][e] = ldf(f + (size_t)(KL + 1 + e) * P);
+			kp[d] = a.piv[(row0 + j) * P + pc];
+		}
+#pragma unroll
+		for (int d = 0; d < RD; d++) {
+			const int j = jb + d;
+			if (j < N) {
+				pivot_swap<KL>(w, kp[d]);
+#pragma unroll
+				for (int e = 0; e < KL / 2; e++) w[2 * e + 2] = cb(w[2 * e + 2], cm(lf[d][e], w[0]));
+				a.Y[(row0 + j) * P + pc] = w[0];
+#pragma unroll
+				for (int i = 0; i < KL; i++) w[i] = w[i + 1];
+				w[KL] = (j + KL + 1 < N) ? rhs_at(j + KL + 1) : make_double2(0, 0);
+			}
+		}
 	}
 	// backward substitution with U (KV super-diagonals)
 	double2 v[KV + 1];
 #pragma unroll
 	for (int i = 0; i <= KV; i++) v[i] = (N - 1 - i >= 0) ? a.Y[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
-	for (int j = N - 1; j >= 0; j--) {
-		const double2 *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
-		const double2 x = cm(v[0], f[0]);
+	for (int jt = N - 1; jt >= 0; jt -= RD) {
+		double2 uf[RD][KL + 1], yn[RD], bx[RD];
 #pragma unroll
-		for (int i = 1; i <= KV; i++) v[i] = cb(v[i], cm(f[(size_t)i * P], x));
-		// outputs: i m phi at j, D' phi at j+1 (needs phi(j) and phi(j+2))
-		if (pv) a.X[(row0 + j) * 2 * P + p] = make_double2(-m * x.y, m * x.x);
-		phi[2] = phi[1];
-		phi[1] = phi[0];
-		phi[0] = x;
-		if (j + 1 < N) {
-			// row j+1: phi(j) = x, phi(j+2) = phi[2]
-			const int n = am + j + 1;
-			const double dn = n, dm = m;
-			const double c1 = (-dn + 1.0) * idR(dn - 1, dm), c2 = (dn + 2.0) * idS(dn + 1, dm);
-			if (pv) a.X[(row0 + j + 1) * 2 * P + P + p] = ca(cs(x, c1), cs(phi[2], c2));
+		for (int d = 0; d < RD; d++) {
+			const int j = max(jt - d, 0);
+			const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+#pragma unroll
+			for (int e = 0; e <= KL; e++) uf[d][e] = ldf(f + (size_t)e * P);
+			const int jy = jt - d - KV - 1;
+			yn[d] = (jy >= 0) ? a.Y[(row0 + jy) * P + pc] : make_double2(0, 0);
 		}
-		double2 bx = wave_sum(pv ? cm(be, x) : make_double2(0, 0));
-		if (threadIdx.x == 0) a.part[((size_t)blockIdx.x * 3 + 0) * a.rows + row0 + j] = bx;
 #pragma unroll
-		for (int i = 0; i < KV; i++) v[i] = v[i + 1];
-		v[KV] = (j - KV - 1 >= 0) ? a.Y[(row0 + j - KV - 1) * P + pc] : make_double2(0, 0);
+		for (int d = 0; d < RD; d++) {
+			const int j = jt - d;
+			bx[d] = make_double2(0, 0);
+			if (j >= 0) {
+				const double2 x = cm(v[0], uf[d][0]);
+#pragma unroll
+				for (int e = 1; e <= KL; e++) v[2 * e] = cb(v[2 * e], cm(uf[d][e], x));
+				// outputs: i m phi at j, D' phi at j+1 (needs phi(j) and phi(j+2))
+				if (pv) a.X[(row0 + j) * 2 * P + p] = make_double2(-m * x.y, m * x.x);
+				phi[2] = phi[1];
+				phi[1] = phi[0];
+				phi[0] = x;
+				if (j + 1 < N) {
+					// row j+1: phi(j) = x, phi(j+2) = phi[2]
+					const int n = am + j + 1;
+					const double dn = n, dm = m;
+					const double c1 = (-dn + 1.0) * idR(dn - 1, dm), c2 = (dn + 2.0) * idS(dn + 1, dm);
+					if (pv) a.X[(row0 + j + 1) * 2 * P + P + p] = ca(cs(x, c1), cs(phi[2], c2));
+				}
+				if (pv) bx[d] = cm(be, x);
+#pragma unroll
+				for (int i = 0; i < KV; i++) v[i] = v[i + 1];
+				v[KV] = yn[d];
+			}
+		}
+		// pole sums of the batch's rows
+		wave_sum_n<RD>(bx);
+		if (threadIdx.x == 0)
+#pragma unroll
+			for (int d = 0; d < RD; d++)
+				if (jt - d >= 0) a.part[((size_t)blockIdx.x * 3 + 0) * a.rows + row0 + jt - d] = bx[d];
 	}
 	// row 0: phi(-1) = 0, phi(1) = phi[1]
 	{
@@ -229,10 +298,11 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs a)
 	}
 }
 
+template <typename FT>
 struct VelArgs {
 	const double2 *alpha, *beta;
 	const int *moff;
-	const double2 *fac;
+	const FT *fac;
 	const unsigned char *piv;
 	const double2 *basis;
 	const double2 *G;         // [rows][2P]: G_lon | G_lat
@@ -244,10 +314,10 @@ struct VelArgs {
 	size_t rows;
 };
 
-template <int KL>
-__global__ __launch_bounds__(64) void k_rexi_vel(VelArgs a)
+template <int KL, typename FT>
+__global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 {
-	constexpr int KV = 2 * KL, NE = 1 + KV + KL;
+	constexpr int KV = 2 * KL, NE = rexi_nec(KL);
 	const int m = m_of(blockIdx.y);
 	const int am = abs(m);
 	const int L = a.L, P = a.P;
@@ -285,13 +355,21 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs a)
 		}
 		return;
 	}
+	// a, b at row j from G values already loaded
+	auto ab_pre = [&](int j, double2 gl, double2 gt, double2 &av, double2 &bv) __attribute__((always_inline)) {
+		if (j < 0 || j >= N) { av = bv = make_double2(0, 0); return; }
+		const int n = am + j;
+		const long k = (long)n * n + n + m;
+		av = ca(B[B_U0 * (long)a.ncplx + k], cs(gl, a.ir));
+		bv = ca(B[B_V0 * (long)a.ncplx + k], cs(gt, a.ir));
+	};
 	// rhs_a = alpha a - W mu(b), rhs_b = W mu(a) + alpha b, mu = R(n-1) x_{n-1} + S(n+1) x_{n+1}
 	double2 aw[3], bw[3];   // (j-1, j, j+1)
 	aw[0] = bw[0] = make_double2(0, 0);
 	ab_at(0, aw[1], bw[1]);
 	ab_at(1, aw[2], bw[2]);
-	auto rhs_next = [&](int j, double2 &ra, double2 &rb) __attribute__((always_inline)) {
-		// uses aw/bw = values at (j-1, j, j+1), then shifts in j+2
+	// uses aw/bw = values at (j-1, j, j+1), then shifts in (an, bn) of row j+2
+	auto rhs_next = [&](int j, double2 &ra, double2 &rb, double2 an, double2 bn) __attribute__((always_inline)) {
 		const int n = am + j;
 		const double dn = n, dm = m;
 		const double r = idR(dn - 1, dm), s = idS(dn + 1, dm);
@@ -299,35 +377,62 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs a)
 		const double2 mua = ca(cs(aw[0], r), cs(aw[2], s));
 		ra = cb(cm(al, aw[1]), cs(mub, W));
 		rb = ca(cs(mua, W), cm(al, bw[1]));
-		aw[0] = aw[1]; aw[1] = aw[2];
-		bw[0] = bw[1]; bw[1] = bw[2];
-		ab_at(j + 2, aw[2], bw[2]);
+		aw[0] = aw[1]; aw[1] = aw[2]; aw[2] = an;
+		bw[0] = bw[1]; bw[1] = bw[2]; bw[2] = bn;
 	};
 	double2 wa[KL + 1], wb[KL + 1];
 #pragma unroll
 	for (int i = 0; i <= KL; i++) {
-		if (i < N) rhs_next(i, wa[i], wb[i]);
-		else wa[i] = wb[i] = make_double2(0, 0);
-	}
-	int jn = KL + 1;   // next rhs row to produce
-	double2 *Ya = a.Y, *Yb = a.Y + a.rows * P;
-	for (int j = 0; j < N; j++) {
-		const double2 *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
-		const int k = a.piv[(row0 + j) * P + pc];
-		pivot_swap<KL>(wa, k);
-		pivot_swap<KL>(wb, k);
-#pragma unroll
-		for (int i = 1; i <= KL; i++) {
-			const double2 lf = f[(size_t)(KV + i) * P];
-			wa[i] = cb(wa[i], cm(lf, wa[0]));
-			wb[i] = cb(wb[i], cm(lf, wb[0]));
+		if (i < N) {
+			double2 an, bn;
+			ab_at(i + 2, an, bn);
+			rhs_next(i, wa[i], wb[i], an, bn);
+		} else {
+			wa[i] = wb[i] = make_double2(0, 0);
 		}
-		Ya[(row0 + j) * P + pc] = wa[0];
-		Yb[(row0 + j) * P + pc] = wb[0];
+	}
+	// forward substitution; row j makes the right-hand side of row j + KL + 1,
+	// which shifts in a, b of row j + KL + 3
+	double2 *Ya = a.Y, *Yb = a.Y + a.rows * P;
+	for (int jb = 0; jb < N; jb += RD) {
+		double2 lf[RD][KL / 2], gl[RD], gt[RD];
+		int kp[RD];
 #pragma unroll
-		for (int i = 0; i < KL; i++) { wa[i] = wa[i + 1]; wb[i] = wb[i + 1]; }
-		if (jn < N) { rhs_next(jn, wa[KL], wb[KL]); jn++; }
-		else wa[KL] = wb[KL] = make_double2(0, 0);
+		for (int d = 0; d < RD; d++) {
+			const int j = min(jb + d, N - 1);
+			const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+#pragma unroll
+			for (int e = 0; e < KL / 2; e++) lf[d][e] = ldf(f + (size_t)(KL + 1 + e) * P);
+			kp[d] = a.piv[(row0 + j) * P + pc];
+			const int jr = min(jb + d + KL + 3, N - 1);
+			gl[d] = a.G[(row0 + jr) * 2 * P + pc];
+			gt[d] = a.G[(row0 + jr) * 2 * P + P + pc];
+		}
+#pragma unroll
+		for (int d = 0; d < RD; d++) {
+			const int j = jb + d;
+			if (j < N) {
+				pivot_swap<KL>(wa, kp[d]);
+				pivot_swap<KL>(wb, kp[d]);
+#pragma unroll
+				for (int e = 0; e < KL / 2; e++) {
+					wa[2 * e + 2] = cb(wa[2 * e + 2], cm(lf[d][e], wa[0]));
+					wb[2 * e + 2] = cb(wb[2 * e + 2], cm(lf[d][e], wb[0]));
+				}
+				Ya[(row0 + j) * P + pc] = wa[0];
+				Yb[(row0 + j) * P + pc] = wb[0];
+#pragma unroll
+				for (int i = 0; i < KL; i++) { wa[i] = wa[i + 1]; wb[i] = wb[i + 1]; }
+				const int jn = j + KL + 1;
+				if (jn < N) {
+					double2 an, bn;
+					ab_pre(jn + 2, gl[d], gt[d], an, bn);
+					rhs_next(jn, wa[KL], wb[KL], an, bn);
+				} else {
+					wa[KL] = wb[KL] = make_double2(0, 0);
+				}
+			}
+		}
 	}
 	double2 va[KV + 1], vb[KV + 1];
 #pragma unroll
@@ -336,26 +441,48 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs a)
 		va[i] = in ? Ya[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
 		vb[i] = in ? Yb[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
 	}
-	for (int j = N - 1; j >= 0; j--) {
-		const double2 *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
-		const double2 xa = cm(va[0], f[0]), xb = cm(vb[0], f[0]);
+	for (int jt = N - 1; jt >= 0; jt -= RD) {
+		double2 uf[RD][KL + 1], ya[RD], yb[RD], sx[2 * RD];
 #pragma unroll
-		for (int i = 1; i <= KV; i++) {
-			const double2 uf = f[(size_t)i * P];
-			va[i] = cb(va[i], cm(uf, xa));
-			vb[i] = cb(vb[i], cm(uf, xb));
-		}
-		double2 su = wave_sum(pv ? cm(be, xa) : make_double2(0, 0));
-		double2 sv = wave_sum(pv ? cm(be, xb) : make_double2(0, 0));
-		if (threadIdx.x == 0) {
-			a.part[((size_t)blockIdx.x * 3 + 1) * a.rows + row0 + j] = su;
-			a.part[((size_t)blockIdx.x * 3 + 2) * a.rows + row0 + j] = sv;
+		for (int d = 0; d < RD; d++) {
+			const int j = max(jt - d, 0);
+			const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+#pragma unroll
+			for (int e = 0; e <= KL; e++) uf[d][e] = ldf(f + (size_t)e * P);
+			const int jy = jt - d - KV - 1;
+			ya[d] = (jy >= 0) ? Ya[(row0 + jy) * P + pc] : make_double2(0, 0);
+			yb[d] = (jy >= 0) ? Yb[(row0 + jy) * P + pc] : make_double2(0, 0);
 		}
 #pragma unroll
-		for (int i = 0; i < KV; i++) { va[i] = va[i + 1]; vb[i] = vb[i + 1]; }
-		const bool in = j - KV - 1 >= 0;
-		va[KV] = in ? Ya[(row0 + j - KV - 1) * P + pc] : make_double2(0, 0);
-		vb[KV] = in ? Yb[(row0 + j - KV - 1) * P + pc] : make_double2(0, 0);
+		for (int d = 0; d < RD; d++) {
+			const int j = jt - d;
+			sx[2 * d] = sx[2 * d + 1] = make_double2(0, 0);
+			if (j >= 0) {
+				const double2 xa = cm(va[0], uf[d][0]), xb = cm(vb[0], uf[d][0]);
+#pragma unroll
+				for (int e = 1; e <= KL; e++) {
+					va[2 * e] = cb(va[2 * e], cm(uf[d][e], xa));
+					vb[2 * e] = cb(vb[2 * e], cm(uf[d][e], xb));
+				}
+				if (pv) {
+					sx[2 * d] = cm(be, xa);
+					sx[2 * d + 1] = cm(be, xb);
+				}
+#pragma unroll
+				for (int i = 0; i < KV; i++) { va[i] = va[i + 1]; vb[i] = vb[i + 1]; }
+				va[KV] = ya[d];
+				vb[KV] = yb[d];
+			}
+		}
+		// pole sums of the batch's rows
+		wave_sum_n<2 * RD>(sx);
+		if (threadIdx.x == 0)
+#pragma unroll
+			for (int d = 0; d < RD; d++)
+				if (jt - d >= 0) {
+					a.part[((size_t)blockIdx.x * 3 + 1) * a.rows + row0 + jt - d] = sx[2 * d];
+					a.part[((size_t)blockIdx.x * 3 + 2) * a.rows + row0 + jt - d] = sx[2 * d + 1];
+				}
 	}
 }
 
@@ -445,6 +572,10 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	*out = nullptr;
 	if (!(prm->dt > 0)) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI needs a positive time step (AppSWE.hpp:185-190)");
 	if (alpha0.empty() || alpha0.size() != beta0.size()) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: empty pole set");
+	if (prm->band_precision != SPHSWE_PREC_FP64 && prm->band_precision != SPHSWE_PREC_FP32)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: band_precision must be FP64 or FP32");
+	if (prm->refine != 0 || prm->precision < SPHSWE_PREC_FP64 || prm->precision > SPHSWE_PREC_BF16X3)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: invalid precision or refine");
 	sphswe_rexi *r = new sphswe_rexi();
 	r->ctx = ctx;
 	r->prm = *prm;
@@ -467,20 +598,39 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	for (int m = -L; m <= L; m++) moff[m + L] = ncompact(L, std::abs(m), m);
 
 	// assemble and factor the band matrices of every pole (SWERexiSPH::setup,
-	// SWERexiSPH.hpp:104-129), in parallel over poles
+	// SWERexiSPH.hpp:104-129), in parallel over poles.  Every component couples
+	// degrees of equal parity (offsets 0, +-2, +-4), so the LU with partial
+	// pivoting keeps exact zeros at odd offsets and even pivot offsets; only
+	// the even entries are kept (checked here).
 	const double rr = prm->earth_radius, W = 2.0 * prm->coriolis_omega, G = prm->h0 * prm->gravitation;
-	const int NEp = 1 + 8 + 4, NEv = 1 + 4 + 2;
+	const int NEp = 1 + 8 + 4, NEv = 1 + 4 + 2;           // full LU entries per row (halo 4, 2)
+	const int NCp = rexi_nec(4), NCv = rexi_nec(2);       // kept entries per row
 	std::vector<double2> hfphi, hfvel;
 	std::vector<unsigned char> hpphi, hpvel;
-	int info_all = 0;
+	int info_all = 0, parity_bad = 0;
+	// full entry of compact entry c for halo h
+	auto full_of = [](int c, int h) { return c == 0 ? 0 : (c <= h ? 2 * c : 2 * h + 2 * (c - h)); };
 	if (r->coriolis) {
-		hfphi.assign(r->rows * NEp * P, make_double2(0, 0));
-		hfvel.assign(r->rows * NEv * P, make_double2(0, 0));
+		hfphi.assign(r->rows * NCp * P, make_double2(0, 0));
+		hfvel.assign(r->rows * NCv * P, make_double2(0, 0));
 		hpphi.assign(r->rows * P, 0);
 		hpvel.assign(r->rows * P, 0);
 		unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
 		std::vector<std::thread> th;
-		std::vector<int> infos(nth, 0);
+		std::vector<int> infos(nth, 0), bad(nth, 0);
+		auto pack = [&](const std::vector<cplx> &fac, const std::vector<unsigned char> &piv, int NE, int NC, int h,
+				size_t r0, int pp, std::vector<double2> &hf, std::vector<unsigned char> &hp, int &badv) {
+			for (size_t j = 0; j < piv.size(); j++) {
+				for (int e = 1; e < NE; e++)
+					if (e != 0 && (e <= 2 * h ? (e & 1) : ((e - 2 * h) & 1)) && fac[j * NE + e] != cplx(0, 0)) badv = 1;
+				if (piv[j] & 1) badv = 1;
+				for (int c = 0; c < NC; c++) {
+					const cplx v = fac[j * NE + full_of(c, h)];
+					hf[((r0 + j) * NC + c) * P + pp] = make_double2(v.real(), v.imag());
+				}
+				hp[(r0 + j) * P + pp] = piv[j];
+			}
+		};
 		for (unsigned t = 0; t < nth; t++)
 			th.emplace_back([&, t]() {
 				std::vector<cplx> fac;
@@ -503,31 +653,20 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 						const size_t r0 = moff[m + L];
 						int info = band_factor_block(Bp, m, fac, piv);
 						if (info && !infos[t]) infos[t] = info;
-						for (size_t j = 0; j < piv.size(); j++) {
-							for (int e = 0; e < NEp; e++) {
-								const cplx v = fac[j * NEp + e];
-								hfphi[((r0 + j) * NEp + e) * P + pp] = make_double2(v.real(), v.imag());
-							}
-							hpphi[(r0 + j) * P + pp] = piv[j];
-						}
+						pack(fac, piv, NEp, NCp, 4, r0, pp, hfphi, hpphi, bad[t]);
 						info = band_factor_block(Bv, m, fac, piv);
 						if (info && !infos[t]) infos[t] = info;
-						for (size_t j = 0; j < piv.size(); j++) {
-							for (int e = 0; e < NEv; e++) {
-								const cplx v = fac[j * NEv + e];
-								hfvel[((r0 + j) * NEv + e) * P + pp] = make_double2(v.real(), v.imag());
-							}
-							hpvel[(r0 + j) * P + pp] = piv[j];
-						}
+						pack(fac, piv, NEv, NCv, 2, r0, pp, hfvel, hpvel, bad[t]);
 					}
 				}
 			});
 		for (auto &x : th) x.join();
 		for (int v : infos)
 			if (v && !info_all) info_all = v;
+		for (int v : bad) parity_bad |= v;
 	} else {
 		// no Coriolis: diagonal Helmholtz 1 / (alpha^2 + (Phi/r^2) n(n+1)) (SWERexiSPH.hpp:230-231)
-		hfphi.assign(r->rows * NEp * P, make_double2(0, 0));
+		hfphi.assign(r->rows * NCp * P, make_double2(0, 0));
 		for (int m = -L; m <= L; m++)
 			for (int n = std::abs(m); n <= L; n++)
 				for (int pp = 0; pp < P; pp++) {
@@ -535,13 +674,18 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 					const cplx bb = cplx(-G, 0) / (rr * rr);
 					const cplx d = a2 + (-bb * (double)n * ((double)n + 1.0));
 					const cplx inv = 1.0 / d;
-					hfphi[((moff[m + L] + n - std::abs(m)) * NEp) * P + pp] = make_double2(inv.real(), inv.imag());
+					hfphi[((moff[m + L] + n - std::abs(m)) * NCp) * P + pp] = make_double2(inv.real(), inv.imag());
 				}
 	}
 	if (info_all) {
 		delete r;
 		return sphswe_set_err(ctx, SPHSWE_ERR_SINGULAR, "zgbsv-equivalent factorisation: INFO = %d", info_all);
 	}
+	if (parity_bad) {
+		delete r;
+		return sphswe_set_err(ctx, SPHSWE_ERR_INTERNAL, "REXI band factors: nonzero entry at an odd offset");
+	}
+	r->fac32 = prm->band_precision == SPHSWE_PREC_FP32;
 	std::vector<double> osc(ctx->nlat);
 	// the metric division of grad_lon / grad_lat, times the 2 pi that the forward
 	// FFT (normalised by 2 pi / nphi) would give the Fourier coefficients the
@@ -555,9 +699,25 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 			return sphswe_set_err(ctx, SPHSWE_ERR_HIP, "rexi: upload failed");
 		return SPHSWE_OK;
 	};
+	// factors to the device, in FP64 or rounded to FP32
+	auto up_fac = [&](void **d, const std::vector<double2> &h) -> int {
+		int rc2;
+		if (!r->fac32) {
+			double2 *p2 = nullptr;
+			if ((rc2 = dev_alloc(ctx, &p2, h.size()))) return rc2;
+			*d = p2;
+			return up(p2, h.data(), sizeof(double2) * h.size());
+		}
+		std::vector<float2> h32(h.size());
+		for (size_t i = 0; i < h.size(); i++) h32[i] = make_float2((float)h[i].x, (float)h[i].y);
+		float2 *p2 = nullptr;
+		if ((rc2 = dev_alloc(ctx, &p2, h32.size()))) return rc2;
+		*d = p2;
+		return up(p2, h32.data(), sizeof(float2) * h32.size());
+	};
 	const size_t nsp = (size_t)ctx->nspat;
 	if ((rc = dev_alloc(ctx, &r->d_alpha, P)) || (rc = dev_alloc(ctx, &r->d_beta, P)) ||
-	    (rc = dev_alloc(ctx, &r->d_moff, 2 * L + 1)) || (rc = dev_alloc(ctx, &r->d_fphi, hfphi.size())) ||
+	    (rc = dev_alloc(ctx, &r->d_moff, 2 * L + 1)) || (rc = up_fac(&r->d_fphi, hfphi)) ||
 	    (rc = dev_alloc(ctx, &r->d_oscale, ctx->nlat)) ||
 	    (rc = dev_alloc(ctx, &r->d_X, r->rows * 2 * P)) || (rc = dev_alloc(ctx, &r->d_G, r->rows * 2 * P)) ||
 	    (rc = dev_alloc(ctx, &r->d_F, (size_t)2 * P * (2 * L + 1) * ctx->nlat)) ||
@@ -572,15 +732,13 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	}
 	if ((rc = up(r->d_alpha, al.data(), sizeof(double2) * P)) || (rc = up(r->d_beta, be.data(), sizeof(double2) * P)) ||
 	    (rc = up(r->d_moff, moff.data(), sizeof(int) * moff.size())) ||
-	    (rc = up(r->d_fphi, hfphi.data(), sizeof(double2) * hfphi.size())) ||
 	    (rc = up(r->d_oscale, osc.data(), sizeof(double) * osc.size()))) {
 		sphswe_rexi_destroy(r);
 		return rc;
 	}
 	if (r->coriolis) {
-		if ((rc = dev_alloc(ctx, &r->d_fvel, hfvel.size())) || (rc = dev_alloc(ctx, &r->d_pphi, hpphi.size())) ||
+		if ((rc = up_fac(&r->d_fvel, hfvel)) || (rc = dev_alloc(ctx, &r->d_pphi, hpphi.size())) ||
 		    (rc = dev_alloc(ctx, &r->d_pvel, hpvel.size())) ||
-		    (rc = up(r->d_fvel, hfvel.data(), sizeof(double2) * hfvel.size())) ||
 		    (rc = up(r->d_pphi, hpphi.data(), hpphi.size())) || (rc = up(r->d_pvel, hpvel.data(), hpvel.size()))) {
 			sphswe_rexi_destroy(r);
 			return rc;
@@ -715,29 +873,40 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 	k_real_to_cplx<<<g256(3 * nsp), 256, 0, st>>>(r->d_phys, r->d_cphys, 3 * nsp);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	if ((rc = rexi_basis(ctx, r))) return rc;
-	// phi solves over (m, pole)
-	PhiArgs pa;
-	pa.alpha = r->d_alpha; pa.beta = r->d_beta; pa.moff = r->d_moff;
-	pa.fac = r->d_fphi; pa.piv = r->d_pphi; pa.basis = r->d_basis;
-	pa.Y = r->d_Y; pa.X = r->d_X; pa.part = r->d_part;
-	pa.L = L; pa.P = P; pa.ncplx = (int)nc;
-	pa.two_omega = 2.0 * r->prm.coriolis_omega; pa.avg_geo = r->prm.h0 * g;
-	pa.coriolis = r->coriolis; pa.rows = r->rows;
 	dim3 grid(r->ngroups, 2 * L + 1);
-	k_rexi_phi<4><<<grid, 64, 0, st>>>(pa);
+	// phi solves over (m, pole)
+	auto phi = [&](auto *fac) {
+		typedef std::remove_const_t<std::remove_pointer_t<decltype(fac)>> FT;
+		PhiArgs<FT> pa;
+		pa.alpha = r->d_alpha; pa.beta = r->d_beta; pa.moff = r->d_moff;
+		pa.fac = fac; pa.piv = r->d_pphi; pa.basis = r->d_basis;
+		pa.Y = r->d_Y; pa.X = r->d_X; pa.part = r->d_part;
+		pa.L = L; pa.P = P; pa.ncplx = (int)nc;
+		pa.two_omega = 2.0 * r->prm.coriolis_omega; pa.avg_geo = r->prm.h0 * g;
+		pa.coriolis = r->coriolis; pa.rows = r->rows;
+		k_rexi_phi<4, FT><<<grid, 64, 0, st>>>(pa);
+	};
+	if (r->fac32) phi((const float2 *)r->d_fphi);
+	else phi((const double2 *)r->d_fphi);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	// gradients of phi: A(S(x) / sqrt(1-mu^2)) per m, without FFT
-	if ((rc = sphswe_leg_synth_pole(ctx, r->d_X, r->d_F, 2 * P, 2 * P, r->d_moff, r->d_oscale))) return rc;
-	if ((rc = sphswe_leg_anal_pole(ctx, r->d_F, r->d_G, 2 * P, 2 * P, r->d_moff))) return rc;
+	const int prec = r->prm.precision;
+	if ((rc = sphswe_leg_synth_pole(ctx, r->d_X, r->d_F, 2 * P, 2 * P, r->d_moff, r->d_oscale, prec))) return rc;
+	if ((rc = sphswe_leg_anal_pole(ctx, r->d_F, r->d_G, 2 * P, 2 * P, r->d_moff, prec))) return rc;
 	// velocity solves
-	VelArgs va;
-	va.alpha = r->d_alpha; va.beta = r->d_beta; va.moff = r->d_moff;
-	va.fac = r->d_fvel; va.piv = r->d_pvel; va.basis = r->d_basis; va.G = r->d_G;
-	va.Y = r->d_Y; va.part = r->d_part;
-	va.L = L; va.P = P; va.ncplx = (int)nc;
-	va.two_omega = 2.0 * r->prm.coriolis_omega; va.ir = 1.0 / a;
-	va.coriolis = r->coriolis; va.rows = r->rows;
-	k_rexi_vel<2><<<grid, 64, 0, st>>>(va);
+	auto vel = [&](auto *fac) {
+		typedef std::remove_const_t<std::remove_pointer_t<decltype(fac)>> FT;
+		VelArgs<FT> va;
+		va.alpha = r->d_alpha; va.beta = r->d_beta; va.moff = r->d_moff;
+		va.fac = fac; va.piv = r->d_pvel; va.basis = r->d_basis; va.G = r->d_G;
+		va.Y = r->d_Y; va.part = r->d_part;
+		va.L = L; va.P = P; va.ncplx = (int)nc;
+		va.two_omega = 2.0 * r->prm.coriolis_omega; va.ir = 1.0 / a;
+		va.coriolis = r->coriolis; va.rows = r->rows;
+		k_rexi_vel<2, FT><<<grid, 64, 0, st>>>(va);
+	};
+	if (r->fac32) vel((const float2 *)r->d_fvel);
+	else vel((const double2 *)r->d_fvel);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	// pole sum, real part (spat_RealToSPHData + accumulation, AppSWE.hpp:519-521)
 	k_rexi_sum<<<g256(nc), 256, 0, st>>>(r->d_part, r->d_acc, ctx->d_lofcidx, r->d_moff, L, r->ngroups, r->rows,
