@@ -190,7 +190,8 @@ typedef struct {
 	int rexi_M;              /* REXI approximation area (default 128) */
 	int rexi_half;           /* 1: half the poles (default) */
 	/* the fields below have no counterpart in the reference (0 = FP64 = its arithmetic) */
-	int precision;           /* SPHSWE_PREC_* of the per-pole Legendre transforms */
+	int precision;           /* SPHSWE_PREC_FP64 or SPHSWE_PREC_FP32: the per-pole gradient
+				  * operator (a GEMM on the FP64 or FP32 matrix cores) */
 	int band_precision;      /* SPHSWE_PREC_FP64 or SPHSWE_PREC_FP32: storage of the LU factors
 				  * (factorised in FP64 on the host; the solves run in FP64) */
 	int refine;              /* FP32 factors: sweeps of FP64 iterative refinement (0 or 1) */

@@ -48,7 +48,14 @@ struct sphswe_rexi {
 	unsigned char *d_pphi = nullptr, *d_pvel = nullptr;
 	double *d_oscale = nullptr;        // 2 pi / sqrt(1 - mu^2)
 	// work
-	double2 *d_X = nullptr, *d_G = nullptr, *d_F = nullptr, *d_Y = nullptr;
+	double2 *d_X = nullptr, *d_G = nullptr, *d_Y = nullptr;
+	// the gradient operator per |m| and parity of n - |m| (pole independent):
+	// symmetric n_p x n_p, FP64 and rounded to FP32
+	double *d_gop = nullptr;
+	float *d_gop32 = nullptr;
+	long *d_goff = nullptr;            // offset of the (|m|, parity) matrix, index 2|m| + parity
+	int4 *d_gtiles = nullptr;          // (m, parity, first row, -) of each output block of the apply kernel
+	int ngtiles = 0;
 	double2 *d_part = nullptr, *d_basis = nullptr, *d_acc = nullptr;
 	double *d_phys = nullptr;
 	double2 *d_cphys = nullptr, *d_cspec = nullptr;
@@ -131,6 +138,46 @@ __device__ inline double2 ldf(const float2 *p)
 	return make_double2(v.x, v.y);
 }
 
+// right-hand side of the phi system of one pole at degree n, order m
+// (SWERexiSPH.hpp:162-195, with the pole-independent fields in the basis B)
+__device__ inline double2 phi_rhs(const double2 *__restrict__ B, long ncplx, int coriolis, double G, double W, int m, int n,
+				  double2 al)
+{
+	const long k = (long)n * n + n + m;
+	const double2 b1 = B[B_DIV * ncplx + k], b3 = B[B_PHI * ncplx + k];
+	if (!coriolis) {
+		// rhs = Phi div0 + alpha phi0 (SWERexiSPH.hpp:230)
+		return ca(cs(b1, G), cm(al, b3));
+	}
+	const double W2 = W * W;
+	const double2 ial = cinv(al), al2 = cm(al, al);
+	const double2 b2 = B[B_MUETA * ncplx + k], b4 = B[B_MU2PHI * ncplx + k];
+	const double2 b5 = B[B_MU2DIV * ncplx + k], b6 = B[B_MU2MUETA * ncplx + k];
+	const double2 b7 = B[B_MU2MU2PHI * ncplx + k];
+	const double2 w1 = B[B_W1 * ncplx + k], w2 = B[B_W2 * ncplx + k];
+	const double2 w3 = B[B_W3 * ncplx + k];
+	const double2 wa = cs(ial, W), w2a = cs(ial, W2);
+	// foo = G (div0 - (W/alpha) mu(eta0)) + (alpha phi0 + (W^2/alpha) mu2(phi0))
+	const double2 foo = ca(cs(cb(b1, cm(wa, b2)), G), ca(cm(al, b3), cm(w2a, b4)));
+	const double2 mfoo = ca(cs(cb(b5, cm(wa, b6)), G), ca(cm(al, b4), cm(w2a, b7)));
+	// Fc = -alpha^2 W1 + W^2 W2 + 2 alpha W W3 (the W's carry the factor W/a)
+	const double2 fc = ca(cb(cs(w2, W2), cm(al2, w1)), cm(cs(al, 2.0 * W), w3));
+	return ca(ca(cm(al2, foo), cs(mfoo, W2)), cm(cs(ial, G), fc));
+}
+
+// the phi right-hand sides of every (row, pole) into the pole layout (the
+// forward sweep then streams them like the factors)
+__global__ __launch_bounds__(256) void k_rexi_phi_rhs(const double2 *__restrict__ alpha, const int *__restrict__ moff,
+						      const double2 *__restrict__ B, double2 *__restrict__ out, int L, int P,
+						      long ncplx, double G, double W)
+{
+	const int m = (int)blockIdx.y - L, am = abs(m), N = L + 1 - am;
+	const long t = blockIdx.x * 256L + threadIdx.x;
+	if (t >= (long)N * P) return;
+	const int j = (int)(t / P), p = (int)(t % P);
+	out[((long)moff[m + L] + j) * P + p] = phi_rhs(B, ncplx, 1, G, W, m, am + j, alpha[p]);
+}
+
 template <typename FT>
 struct PhiArgs {
 	const double2 *alpha, *beta;
@@ -161,31 +208,11 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 	const int pc = pv ? p : P - 1;
 	const size_t row0 = a.moff[m + L];
 	const double2 al = a.alpha[pc], be = a.beta[pc];
-	const double2 ial = cinv(al);
-	const double G = a.avg_geo, W = a.two_omega, W2 = W * W;
-	const double2 al2 = cm(al, al);
+	const double G = a.avg_geo, W = a.two_omega;
 	const double2 *B = a.basis;
 
 	auto rhs_at = [&](int j) __attribute__((always_inline)) -> double2 {
-		const int n = am + j;
-		const long k = (long)n * n + n + m;
-		const double2 b1 = B[B_DIV * (long)a.ncplx + k], b3 = B[B_PHI * (long)a.ncplx + k];
-		if (!a.coriolis) {
-			// rhs = Phi div0 + alpha phi0 (SWERexiSPH.hpp:230)
-			return ca(cs(b1, G), cm(al, b3));
-		}
-		const double2 b2 = B[B_MUETA * (long)a.ncplx + k], b4 = B[B_MU2PHI * (long)a.ncplx + k];
-		const double2 b5 = B[B_MU2DIV * (long)a.ncplx + k], b6 = B[B_MU2MUETA * (long)a.ncplx + k];
-		const double2 b7 = B[B_MU2MU2PHI * (long)a.ncplx + k];
-		const double2 w1 = B[B_W1 * (long)a.ncplx + k], w2 = B[B_W2 * (long)a.ncplx + k];
-		const double2 w3 = B[B_W3 * (long)a.ncplx + k];
-		const double2 wa = cs(ial, W), w2a = cs(ial, W2);
-		// foo = G (div0 - (W/alpha) mu(eta0)) + (alpha phi0 + (W^2/alpha) mu2(phi0))
-		const double2 foo = ca(cs(cb(b1, cm(wa, b2)), G), ca(cm(al, b3), cm(w2a, b4)));
-		const double2 mfoo = ca(cs(cb(b5, cm(wa, b6)), G), ca(cm(al, b4), cm(w2a, b7)));
-		// Fc = -alpha^2 W1 + W^2 W2 + 2 alpha W W3 (the W's carry the factor W/a)
-		const double2 fc = ca(cb(cs(w2, W2), cm(al2, w1)), cm(cs(al, 2.0 * W), w3));
-		return ca(ca(cm(al2, foo), cs(mfoo, W2)), cm(cs(ial, G), fc));
+		return phi_rhs(B, a.ncplx, a.coriolis, G, W, m, am + j, al);
 	};
 
 	double2 phi[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};   // phi(j), phi(j+1), phi(j+2)
@@ -212,12 +239,14 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 		return;
 	}
 
-	// forward substitution with row interchanges (L of the pivoted LU)
+	// forward substitution with row interchanges (L of the pivoted LU), in
+	// place: Y holds the right-hand sides (k_rexi_phi_rhs) ahead of row j and
+	// the forward-substituted values up to row j
 	double2 w[KL + 1];
 #pragma unroll
-	for (int i = 0; i <= KL; i++) w[i] = (i < N) ? rhs_at(i) : make_double2(0, 0);
+	for (int i = 0; i <= KL; i++) w[i] = (i < N) ? a.Y[(row0 + i) * P + pc] : make_double2(0, 0);
 	for (int jb = 0; jb < N; jb += RD) {
-		double2 lf[RD][KL / 2];
+		double2 lf[RD][KL / 2], rn[RD];
 		int kp[RD];
 #pragma unroll
 		for (int d = 0; d < RD; d++) {
@@ -226,6 +255,8 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 #pragma unroll
 			for (int e = 0; e < KL / 2; e++) lf[d][e] = ldf(f + (size_t)(KL + 1 + e) * P);
 			kp[d] = a.piv[(row0 + j) * P + pc];
+			const int jn = jb + d + KL + 1;
+			rn[d] = (jn < N) ? a.Y[(row0 + jn) * P + pc] : make_double2(0, 0);
 		}
 #pragma unroll
 		for (int d = 0; d < RD; d++) {
@@ -237,7 +268,7 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 				a.Y[(row0 + j) * P + pc] = w[0];
 #pragma unroll
 				for (int i = 0; i < KL; i++) w[i] = w[i + 1];
-				w[KL] = (j + KL + 1 < N) ? rhs_at(j + KL + 1) : make_double2(0, 0);
+				w[KL] = rn[d];
 			}
 		}
 	}
@@ -540,6 +571,129 @@ __global__ void k_cmul_bcast(const double2 *__restrict__ g, const double2 *__res
 
 inline int g256(long n) { return (int)((n + 255) / 256); }
 
+// ---- gradient operator ------------------------------------------------------
+// grad_lon / grad_lat of a single-m field without the FFT pair: synthesis to
+// (m, ring), scaling by 2 pi / sqrt(1 - mu^2), analysis with the Gauss weights.
+// For each m that is the real matrix
+//   M_m[l][l'] = sum_j w_j (2 pi / sqrt(1 - mu_j^2)) Pbar_l^m(mu_j) Pbar_l'^m(mu_j),
+// symmetric, zero unless l - l' is even (the rings pair up as mu, -mu), and the
+// same for -m (the (-1)^m of the negative-m slots cancels).  It is built once
+// per REXI object by running the pole-layout synthesis / analysis on unit
+// vectors, and applied per step as a batched GEMM over the 4P real columns of
+// the poles (columns i m phi | D' phi, re / im).
+
+constexpr int GOP_RT = 8;   // 16-row tiles per wavefront of the apply kernel
+
+typedef double gd4v __attribute__((ext_vector_type(4)));
+typedef float gf4v __attribute__((ext_vector_type(4)));
+
+// unit vectors: column cc of a chunk is e_{c0 + cc} in every m >= 0 block
+__global__ void k_gop_unit(double2 *__restrict__ x, const int *__restrict__ moff, int L, int CB, int c0)
+{
+	const int m = (int)blockIdx.y - L, am = abs(m);
+	const int N = L + 1 - am;
+	const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (t >= (long)N * CB) return;
+	const int i = (int)(t / CB), cc = (int)(t % CB);
+	x[((long)moff[m + L] + i) * CB + cc] = make_double2((m >= 0 && i == c0 + cc) ? 1.0 : 0.0, 0.0);
+}
+
+// the chunk's columns of M(|m|, parity) from the analysed unit vectors
+__global__ void k_gop_pack(const double2 *__restrict__ g, double *__restrict__ M, const long *__restrict__ goff,
+			   const int *__restrict__ moff, int L, int CB, int c0)
+{
+	const int am = blockIdx.y;
+	const int N = L + 1 - am;
+	const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (t >= (long)N * CB) return;
+	const int i = (int)(t / CB), cc = (int)(t % CB), c = c0 + cc;
+	if (c >= N || ((i ^ c) & 1)) return;
+	const int par = i & 1, np = (N + 1 - par) / 2;
+	M[goff[2 * am + par] + (long)(i >> 1) * np + (c >> 1)] = g[((long)moff[am + L] + i) * CB + cc].x;
+}
+
+// symmetrise (the pair (a, b), a <= b, handled by one thread) and round to FP32
+__global__ void k_gop_sym(double *__restrict__ M, float *__restrict__ M32, const long *__restrict__ goff, int L)
+{
+	const int am = blockIdx.y >> 1, par = blockIdx.y & 1;
+	const int N = L + 1 - am, np = (N + 1 - par) / 2;
+	const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (t >= (long)np * np) return;
+	const int a = (int)(t / np), b = (int)(t % np);
+	if (a > b) return;
+	double *Mm = M + goff[2 * am + par];
+	float *Mm32 = M32 + goff[2 * am + par];
+	const double v = 0.5 * (Mm[(long)a * np + b] + Mm[(long)b * np + a]);
+	Mm[(long)a * np + b] = v;
+	Mm[(long)b * np + a] = v;
+	Mm32[(long)a * np + b] = (float)v;
+	Mm32[(long)b * np + a] = (float)v;
+}
+
+// Y[rows of (m, parity)][cols] = M(|m|, parity) X[...]: one wavefront per
+// (m, parity, up to GOP_RT 16-row tiles, 16 real columns), so that every X row
+// is read once per column strip and each k step issues GOP_RT MFMAs on one B
+// operand.  A = M (symmetric: the 16 rows of one column k are contiguous),
+// B = X rows.  FP64: v_mfma_f64_16x16x4f64, D[i][n] in lane n + 16 (i % 4),
+// element i / 4.  FP32: v_mfma_f32_16x16x4f32 with M and X rounded, D[i][n] in
+// lane n + 16 (i / 4), element i % 4.  A[i][k] in lane i + 16 k and B[k][n] in
+// lane n + 16 k for both.
+template <typename T>
+__global__ __launch_bounds__(64) void k_gop_apply(const T *__restrict__ M, const long *__restrict__ goff,
+						  const int4 *__restrict__ tiles, const int *__restrict__ moff,
+						  const double *__restrict__ X, double *__restrict__ Y, int L, int ncol)
+{
+	constexpr bool F64 = sizeof(T) == 8;
+	typedef std::conditional_t<F64, gd4v, gf4v> acc_t;
+	const int4 tl = tiles[blockIdx.x];
+	const int m = tl.x, par = tl.y, r0 = tl.z;
+	const int am = abs(m), N = L + 1 - am, np = (N + 1 - par) / 2;
+	const int nt = min(GOP_RT, (np - r0 + 15) / 16);   // row tiles of this wavefront (uniform)
+	const T *Mm = M + goff[2 * am + par];
+	const long xrow0 = (long)moff[m + L] + par;   // row of parity index 0; parity index q is row xrow0 + 2 q
+	const int lane = threadIdx.x, i = lane & 15, k = lane >> 4;
+	const int col = blockIdx.y * 16 + i;
+	const bool cv = col < ncol;
+	acc_t acc[GOP_RT];
+#pragma unroll
+	for (int t = 0; t < GOP_RT; t++) acc[t] = acc_t{0, 0, 0, 0};
+	// UN k steps per pass: their loads are issued together
+	constexpr int UN = F64 ? 4 : 2;
+	for (int kk = 0; kk < np; kk += 4 * UN) {
+		T b[UN], av[UN][GOP_RT];
+#pragma unroll
+		for (int u = 0; u < UN; u++) {
+			const int q = kk + 4 * u + k;
+			const bool qv = q < np;
+			b[u] = (qv && cv) ? (T)X[(xrow0 + 2 * (long)q) * ncol + col] : (T)0;
+			const T *mq = Mm + (long)(qv ? q : 0) * np + r0 + i;
+#pragma unroll
+			for (int t = 0; t < GOP_RT; t++)
+				av[u][t] = (t < nt && qv && r0 + 16 * t + i < np) ? mq[16 * t] : (T)0;
+		}
+#pragma unroll
+		for (int u = 0; u < UN; u++)
+#pragma unroll
+			for (int t = 0; t < GOP_RT; t++) {
+				if (t < nt) {
+					if constexpr (F64) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][t], b[u], acc[t], 0, 0, 0);
+					else acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u][t], b[u], acc[t], 0, 0, 0);
+				}
+			}
+	}
+	if (!cv) return;
+#pragma unroll
+	for (int t = 0; t < GOP_RT; t++) {
+		if (t >= nt) break;
+#pragma unroll
+		for (int e = 0; e < 4; e++) {
+			const int row = r0 + 16 * t + (F64 ? k + 4 * e : 4 * k + e);
+			if (row < np) Y[(xrow0 + 2 * (long)row) * ncol + col] = (double)acc[t][e];
+		}
+	}
+}
+
+
 template <typename T>
 int dev_alloc(sphswe_ctx *ctx, T **p, size_t n)
 {
@@ -555,11 +709,75 @@ extern "C" int sphswe_rexi_destroy(sphswe_rexi *r)
 	if (!r) return SPHSWE_OK;
 	if (r->ctx) hipStreamSynchronize(r->ctx->stream);
 	void *ptrs[] = {r->d_alpha, r->d_beta, r->d_moff, r->d_fphi, r->d_fvel, r->d_pphi, r->d_pvel, r->d_oscale,
-			r->d_X, r->d_G, r->d_F, r->d_Y, r->d_part, r->d_basis, r->d_acc, r->d_phys, r->d_cphys, r->d_cspec,
+			r->d_X, r->d_G, r->d_Y, r->d_part, r->d_basis, r->d_acc, r->d_phys, r->d_cphys, r->d_cspec,
+			r->d_gop, r->d_gop32, r->d_goff, r->d_gtiles,
 			r->d_in};
 	for (void *p : ptrs)
 		if (p) hipFree(p);
 	delete r;
+	return SPHSWE_OK;
+}
+
+// build the gradient operator of a REXI object (d_moff and d_oscale on the
+// device): unit vectors through the FP64 pole-layout synthesis and analysis in
+// chunks of up to 64 columns; the tile table of the per-step GEMM
+static int rexi_gradient_operator(sphswe_ctx *ctx, sphswe_rexi *r)
+{
+	const int L = ctx->L;
+	hipStream_t st = ctx->stream;
+	std::vector<long> goff(2 * (L + 1));
+	long tot = 0;
+	for (int am = 0; am <= L; am++)
+		for (int par = 0; par < 2; par++) {
+			const long np = (L + 1 - am + 1 - par) / 2;
+			goff[2 * am + par] = tot;
+			tot += np * np;
+		}
+	std::vector<int4> tiles;
+	for (int am = 0; am <= L; am++)
+		for (int sg = (am ? -1 : 1); sg <= 1; sg += 2)
+			for (int par = 0; par < 2; par++) {
+				const int np = (L + 1 - am + 1 - par) / 2;
+				for (int r0 = 0; r0 < np; r0 += 16 * GOP_RT) tiles.push_back(make_int4(sg * am, par, r0, 0));
+			}
+	r->ngtiles = (int)tiles.size();
+	int rc;
+	if ((rc = dev_alloc(ctx, &r->d_gop, tot)) || (rc = dev_alloc(ctx, &r->d_gop32, tot)) ||
+	    (rc = dev_alloc(ctx, &r->d_goff, goff.size())) || (rc = dev_alloc(ctx, &r->d_gtiles, tiles.size())))
+		return rc;
+	SPHSWE_HIP(ctx, hipMemcpy(r->d_goff, goff.data(), sizeof(long) * goff.size(), hipMemcpyHostToDevice));
+	SPHSWE_HIP(ctx, hipMemcpy(r->d_gtiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+	const int CB = std::min(L + 1, 64);
+	double2 *xid = nullptr, *four = nullptr, *gout = nullptr;
+	auto release = [&]() {
+		hipStreamSynchronize(st);
+		if (xid) hipFree(xid);
+		if (four) hipFree(four);
+		if (gout) hipFree(gout);
+	};
+	if ((rc = dev_alloc(ctx, &xid, r->rows * CB)) || (rc = dev_alloc(ctx, &gout, r->rows * CB)) ||
+	    (rc = dev_alloc(ctx, &four, (size_t)CB * (2 * L + 1) * ctx->nlat))) {
+		release();
+		return rc;
+	}
+	for (int c0 = 0; c0 <= L; c0 += CB) {
+		k_gop_unit<<<dim3((unsigned)(((long)(L + 1) * CB + 255) / 256), 2 * L + 1), 256, 0, st>>>(xid, r->d_moff, L,
+													   CB, c0);
+		if ((rc = sphswe_leg_synth_pole(ctx, xid, four, CB, CB, r->d_moff, r->d_oscale, SPHSWE_PREC_FP64)) ||
+		    (rc = sphswe_leg_anal_pole(ctx, four, gout, CB, CB, r->d_moff, SPHSWE_PREC_FP64))) {
+			release();
+			return rc;
+		}
+		k_gop_pack<<<dim3((unsigned)(((long)(L + 1) * CB + 255) / 256), L + 1), 256, 0, st>>>(gout, r->d_gop,
+												       r->d_goff, r->d_moff,
+												       L, CB, c0);
+	}
+	const long npmax = (L + 2) / 2;
+	k_gop_sym<<<dim3((unsigned)((npmax * npmax + 255) / 256), 2 * (L + 1)), 256, 0, st>>>(r->d_gop, r->d_gop32,
+											      r->d_goff, L);
+	hipError_t e = hipGetLastError();
+	release();
+	if (e != hipSuccess) return sphswe_set_err(ctx, SPHSWE_ERR_HIP, "rexi: gradient operator: %s", hipGetErrorString(e));
 	return SPHSWE_OK;
 }
 
@@ -574,8 +792,9 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	if (alpha0.empty() || alpha0.size() != beta0.size()) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: empty pole set");
 	if (prm->band_precision != SPHSWE_PREC_FP64 && prm->band_precision != SPHSWE_PREC_FP32)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: band_precision must be FP64 or FP32");
-	if (prm->refine != 0 || prm->precision < SPHSWE_PREC_FP64 || prm->precision > SPHSWE_PREC_BF16X3)
-		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: invalid precision or refine");
+	if (prm->precision != SPHSWE_PREC_FP64 && prm->precision != SPHSWE_PREC_FP32)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: precision must be FP64 or FP32");
+	if (prm->refine != 0) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: invalid refine");
 	sphswe_rexi *r = new sphswe_rexi();
 	r->ctx = ctx;
 	r->prm = *prm;
@@ -720,7 +939,6 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	    (rc = dev_alloc(ctx, &r->d_moff, 2 * L + 1)) || (rc = up_fac(&r->d_fphi, hfphi)) ||
 	    (rc = dev_alloc(ctx, &r->d_oscale, ctx->nlat)) ||
 	    (rc = dev_alloc(ctx, &r->d_X, r->rows * 2 * P)) || (rc = dev_alloc(ctx, &r->d_G, r->rows * 2 * P)) ||
-	    (rc = dev_alloc(ctx, &r->d_F, (size_t)2 * P * (2 * L + 1) * ctx->nlat)) ||
 	    (rc = dev_alloc(ctx, &r->d_Y, r->rows * 2 * P)) ||
 	    (rc = dev_alloc(ctx, &r->d_part, (size_t)r->ngroups * 3 * r->rows)) ||
 	    (rc = dev_alloc(ctx, &r->d_basis, (size_t)NBASIS * ctx->ncplx)) ||
@@ -743,6 +961,10 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 			sphswe_rexi_destroy(r);
 			return rc;
 		}
+	}
+	if ((rc = rexi_gradient_operator(ctx, r))) {
+		sphswe_rexi_destroy(r);
+		return rc;
 	}
 	*out = r;
 	return SPHSWE_OK;
@@ -886,13 +1108,28 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 		pa.coriolis = r->coriolis; pa.rows = r->rows;
 		k_rexi_phi<4, FT><<<grid, 64, 0, st>>>(pa);
 	};
+	if (r->coriolis) {
+		const long nc2 = ctx->ncplx;
+		k_rexi_phi_rhs<<<dim3((unsigned)(((long)(L + 1) * P + 255) / 256), 2 * L + 1), 256, 0, st>>>(
+			r->d_alpha, r->d_moff, r->d_basis, r->d_Y, L, P, nc2, r->prm.h0 * g, 2.0 * r->prm.coriolis_omega);
+		SPHSWE_LAUNCH_CHECK(ctx);
+	}
 	if (r->fac32) phi((const float2 *)r->d_fphi);
 	else phi((const double2 *)r->d_fphi);
 	SPHSWE_LAUNCH_CHECK(ctx);
-	// gradients of phi: A(S(x) / sqrt(1-mu^2)) per m, without FFT
-	const int prec = r->prm.precision;
-	if ((rc = sphswe_leg_synth_pole(ctx, r->d_X, r->d_F, 2 * P, 2 * P, r->d_moff, r->d_oscale, prec))) return rc;
-	if ((rc = sphswe_leg_anal_pole(ctx, r->d_F, r->d_G, 2 * P, 2 * P, r->d_moff, prec))) return rc;
+	// gradients of phi, A(S(x) / sqrt(1-mu^2)) per m: the precomputed operator
+	// applied to the 2P columns (i m phi | D' phi) of every pole at once
+	{
+		const int ncol = 4 * P;   // real columns
+		dim3 gg(r->ngtiles, (ncol + 15) / 16);
+		if (r->prm.precision == SPHSWE_PREC_FP64)
+			k_gop_apply<double><<<gg, 64, 0, st>>>(r->d_gop, r->d_goff, r->d_gtiles, r->d_moff,
+							       (const double *)r->d_X, (double *)r->d_G, L, ncol);
+		else
+			k_gop_apply<float><<<gg, 64, 0, st>>>(r->d_gop32, r->d_goff, r->d_gtiles, r->d_moff,
+							      (const double *)r->d_X, (double *)r->d_G, L, ncol);
+		SPHSWE_LAUNCH_CHECK(ctx);
+	}
 	// velocity solves
 	auto vel = [&](auto *fac) {
 		typedef std::remove_const_t<std::remove_pointer_t<decltype(fac)>> FT;
