@@ -276,6 +276,61 @@ def bench_swe_modes(ctx, nsteps=100):
     return {"steps": nsteps, "dt": dt, "modes": res}
 
 
+def bench_swe_batched(ctx, nmem=16, nsteps=100):
+    """The north star's batched form at T511: nmem nonlinear SWE members
+    (Gaussian dams, centres offset by U(-0.2, 0.2) rad, amplitudes U(300, 700) m,
+    seeded) advanced together by the captured RK4 step, in FP64 and in each
+    reduced mode, against the plain form: one FP64 state stepped eagerly.
+    Reports member-steps/s, the speedup over the plain form's steps/s, and the
+    largest member's relative L2 geopotential error after nsteps against the
+    batched FP64 run (member 0 of which equals the plain run).  dt = 0.0005 a /
+    nlat = 4.148 s, as the mixed T511 leg."""
+    dt = 0.0005 * 6.37122e6 / ctx.nlat
+    rng = np.random.default_rng(4242)
+    lon = np.arange(ctx.nphi) / ctx.nphi * 2 * np.pi
+    x0 = []
+    for _ in range(nmem):
+        h = gaussian_dam(lon, ctx.ct, lat_c=np.pi / 4 + rng.uniform(-0.2, 0.2), lon_c=np.pi + rng.uniform(-0.2, 0.2),
+                         amp=rng.uniform(300, 700))
+        z = np.zeros_like(h)
+        x0.append(ctx.analysis(np.stack([h, z, z])))
+    x0 = np.concatenate(x0)
+    # plain form: member 0 alone, FP64, eager
+    prm = capi.swe_params(dt=dt, precision=capi.PREC_FP64)
+    one = ctx.to_device(x0[:3])
+    ctx.swe_rk4(prm, one, 1, 2)
+    one = ctx.to_device(x0[:3])
+    ctx.sync()
+    ctx.mark(12)
+    ctx.swe_rk4(prm, one, 1, nsteps)
+    ctx.mark(13)
+    plain = nsteps / (ctx.elapsed_ms(12, 13) / 1000.0)
+    plain_final = ctx.to_host(one, np.complex128, (3, ctx.nlm))
+    del one
+    res, finals = {}, {}
+    for name in ("fp64",) + MODES:
+        prm = capi.swe_params(dt=dt, precision=capi.PRECISIONS[name])
+        state = ctx.to_device(x0)
+        ctx.swe_rk4_graph(prm, state, nmem, 2)
+        ctx._check(ctx.lib.sphswe_memcpy_h2d(ctx.h, state.ptr, np.ascontiguousarray(x0).ctypes.data, x0.nbytes))
+        ctx.sync()
+        ctx.mark(12)
+        ctx.swe_rk4_graph(prm, state, nmem, nsteps)
+        ctx.mark(13)
+        ms = ctx.elapsed_ms(12, 13)
+        finals[name] = ctx.to_host(state, np.complex128, (nmem * 3, ctx.nlm))
+        del state
+        if not np.all(np.isfinite(finals[name])):
+            raise RuntimeError("batched SWE state became non-finite (%s)" % name)
+        rate = nmem * nsteps / (ms / 1000.0)
+        res[name] = {"member_steps_per_s": rate, "ms_per_step": ms / nsteps, "speedup_vs_plain_fp64": rate / plain}
+    ref = finals["fp64"]
+    res["fp64"]["member0_vs_plain_phi_rel_l2"] = _rel(ref[0], plain_final[0])[0]
+    for name in MODES:
+        res[name]["phi_rel_l2_max_member"] = max(_rel(finals[name][3 * k], ref[3 * k])[0] for k in range(nmem))
+    return {"members": nmem, "steps": nsteps, "dt": dt, "plain_fp64_steps_per_s": plain, "runs": res}
+
+
 def bench_ensemble(nsteps=200, nmem=64, L=127):
     """Config 3: 64 nonlinear SWE members at T127 (192 x 384), 200 steps, each
     a Gaussian dam with centre offsets U(-0.2, 0.2) rad and amplitude
@@ -417,6 +472,7 @@ def main():
     achieved_ana = flops / (ms_ana * 1e-3) / 1e12
     mixed_sht = bench_sht_modes(ctx, max(args.steps // 4, 10), 3)
     mixed_swe = bench_swe_modes(ctx) if args.swe_steps > 0 else None
+    batched_swe = bench_swe_batched(ctx) if args.swe_steps > 0 else None
     ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
     rexi_rate, rexi_cfg = bench_rexi(args.rexi_steps) if args.rexi_steps > 0 else (None, None)
     rexi_modes = bench_rexi_modes(args.rexi_steps) if args.rexi_steps > 0 else None
@@ -444,6 +500,7 @@ def main():
                        "viscosity": 1e5},
         "mixed_sht_T511_3fields": mixed_sht,
         "mixed_swe_T511_dam_100steps": mixed_swe,
+        "batched_mixed_swe_T511_16members_100steps": batched_swe,
         "ensemble_T127_64members": ensemble,
         "rexi_steps_per_s": rexi_rate,
         "rexi_config": rexi_cfg,

@@ -261,6 +261,18 @@ int sphswe_autotune(sphswe_ctx *ctx);
 int sphswe_set_tuning(sphswe_ctx *ctx, int fft_nseq_r2c, int fft_nseq_c2r, int synth_mfma_min);
 int sphswe_get_tuning(const sphswe_ctx *ctx, int *fft_nseq_r2c, int *fft_nseq_c2r, int *synth_mfma_min, int *tuned);
 
+/* FP32 real Legendre syntheses / analyses (SPHSWE_PREC_FP32) of at least
+ * synth_min_fields / anal_min_fields fields run as batched GEMMs on the FP32
+ * matrix cores from stored FP32 associated-Legendre tables (two tables of
+ * nlm * nlat / 2 floats each, 201 MB apiece at T511, allocated on the first
+ * such transform); fewer fields use the on-the-fly recurrence kernels.  0 (the
+ * default) disables a direction.  Both forms round the same FP64 recurrence
+ * values to FP32 and differ in the order of the FP32 sums only.
+ * sphswe_autotune sets them (16 where the tables are faster at 16 fields).
+ * No counterpart in the reference. */
+int sphswe_set_fp32_tables(sphswe_ctx *ctx, int synth_min_fields, int anal_min_fields);
+int sphswe_get_fp32_tables(const sphswe_ctx *ctx, int *synth_min_fields, int *anal_min_fields);
+
 /* ---- timing on the context stream: ms between two sphswe_timer_mark calls */
 int sphswe_timer_mark(sphswe_ctx *ctx, int slot);
 int sphswe_timer_elapsed(sphswe_ctx *ctx, int slot0, int slot1, float *ms);

@@ -77,6 +77,8 @@ _SIGS = {
     "sphswe_autotune": [_vp],
     "sphswe_set_tuning": [_vp, _i, _i, _i],
     "sphswe_get_tuning": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "sphswe_set_fp32_tables": [_vp, _i, _i],
+    "sphswe_get_fp32_tables": [_vp, _ip, _ip],
     "sphswe_rexi_create": [_vp, ctypes.POINTER(RexiParams), ctypes.POINTER(_vp)],
     "sphswe_rexi_destroy": [_vp],
     "sphswe_rexi_poles": [_vp, _ip, _dp, _dp],
@@ -293,8 +295,9 @@ class Context:
     def tuning(self):
         v = [_i(0) for _ in range(4)]
         self._check(self.lib.sphswe_get_tuning(self.h, *[ctypes.byref(x) for x in v]))
+        ts, ta = self.fp32_tables()
         return {"fft_nseq_r2c": v[0].value, "fft_nseq_c2r": v[1].value, "synth_mfma_min": v[2].value,
-                "tuned": bool(v[3].value)}
+                "fp32_table_synth_min": ts, "fp32_table_anal_min": ta, "tuned": bool(v[3].value)}
 
     def set_tuning(self, fft_nseq_r2c=0, fft_nseq_c2r=0, synth_mfma_min=0):
         self._check(self.lib.sphswe_set_tuning(self.h, fft_nseq_r2c, fft_nseq_c2r, synth_mfma_min))
@@ -305,6 +308,18 @@ class Context:
 
     def band(self, halo):
         return Band(self, halo)
+
+    def set_fp32_tables(self, synth_min_fields, anal_min_fields=None):
+        """FP32 real syntheses / analyses of at least that many fields use the
+        stored FP32 Legendre tables (0: never); one value sets both."""
+        if anal_min_fields is None:
+            anal_min_fields = synth_min_fields
+        self._check(self.lib.sphswe_set_fp32_tables(self.h, int(synth_min_fields), int(anal_min_fields)))
+
+    def fp32_tables(self):
+        a, b = _i(), _i()
+        self._check(self.lib.sphswe_get_fp32_tables(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     # timing ----------------------------------------------------------------
     def mark(self, slot):

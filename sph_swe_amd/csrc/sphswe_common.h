@@ -46,6 +46,14 @@ struct sphswe_ctx {
 	int fft_nseq_r2c = 0;         // complex sequences per workgroup of the real FFTs
 	int fft_nseq_c2r = 0;
 	int synth_mfma_min = 0;       // FP64 real synthesis on the matrix cores from this many fields (0: never)
+	int tab_min_synth = 0;        // FP32 real syntheses from stored tables from this many fields (0: never)
+	int tab_min_anal = 0;         // FP32 real analyses likewise
+	// FP32 associated-Legendre tables (built on first use, sphswe_legendre_tab.hip)
+	float *d_tab_s = nullptr;     // per (m, parity): [degree][ring]
+	float *d_tab_a = nullptr;     // per (m, parity): [ring][degree]
+	long *d_tab_off = nullptr;    // offset of (m, parity), index 2m + parity
+	int2 *d_tab_tiles_s = nullptr, *d_tab_tiles_a = nullptr;
+	int tab_ntiles_s = 0, tab_ntiles_a = 0;
 	int tuned = 0;
 	int *d_mofidx = nullptr;      // m of real spectral index [nlm]
 	int *d_lofcidx = nullptr;     // l of complex spectral index [(L+1)^2]
@@ -106,6 +114,10 @@ int sphswe_leg_anal_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int 
 int sphswe_leg_synth_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
 int sphswe_leg_anal_cplx_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
 int sphswe_leg_synth_cplx_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
+int sphswe_leg_tables(sphswe_ctx *ctx);
+void sphswe_leg_tables_free(sphswe_ctx *ctx);
+int sphswe_leg_synth_tab(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf);
+int sphswe_leg_anal_tab(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf);
 int sphswe_leg_synth_pole_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
 			     const double *oscale, int prec);
 int sphswe_leg_anal_pole_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int P, const int *d_moff,

@@ -270,6 +270,7 @@ extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 	hipSetDevice(ctx->device);
 	if (ctx->stream) hipStreamSynchronize(ctx->stream);
 	sphswe_swe_release(ctx);
+	sphswe_leg_tables_free(ctx);
 	void *ptrs[] = {ctx->d_ct, ctx->d_st, ctx->d_sq, ctx->d_coslat, ctx->d_wg, ctx->d_coef,
 			ctx->d_seed, ctx->d_segoff, ctx->d_unit_m, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_escratch, ctx->d_flag};
 	for (void *p : ptrs)

@@ -400,6 +400,8 @@ int check_prec(sphswe_ctx *ctx, int prec)
 int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
+	if (prec == SPHSWE_PREC_FP32 && ctx->tab_min_synth > 0 && nf >= ctx->tab_min_synth)
+		return sphswe_leg_synth_tab(ctx, spec, four, nf);
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_synth_mx(ctx, spec, four, nf, prec);
 	if (ctx->synth_mfma_min > 0 && nf >= ctx->synth_mfma_min) return sphswe_leg_synth_f64_mfma(ctx, spec, four, nf);
 	const long nlm = ctx->nlm;
@@ -432,6 +434,8 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
+	if (prec == SPHSWE_PREC_FP32 && ctx->tab_min_anal > 0 && nf >= ctx->tab_min_anal)
+		return sphswe_leg_anal_tab(ctx, four, spec, nf);
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_mx(ctx, four, spec, nf, prec);
 	return launch_anal<LAY_REAL>(ctx, geo_of(ctx), four, spec, nf);
 }

@@ -332,6 +332,8 @@ extern "C" int sphswe_swe_rk4_graph(sphswe_ctx *ctx, const sphswe_swe_params *p,
 		SweBufs b;
 		if (int rc = swe_buffers(ctx, nmem, b)) return rc;
 		if (int rc = sphswe_leg_anal_reserve(ctx, 4 * nmem, p->precision)) return rc;
+		if (p->precision == SPHSWE_PREC_FP32 && (ctx->tab_min_synth > 0 || ctx->tab_min_anal > 0))
+			if (int rc = sphswe_leg_tables(ctx)) return rc;
 		SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
 		hipGraph_t graph;
 		SPHSWE_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));

@@ -1,11 +1,13 @@
 // Autotuner: times the launch variants available for this context's sizes on
 // synthetic data and keeps the fastest in the context.  Every variant computes
-// the same transform in FP64 (they differ in the order of the sums only), so
-// tuning changes results by round-off at most.
+// the same transform in the same precision (they differ in the order of the
+// sums only), so tuning changes results by round-off of that precision at most.
 //   - real FFTs: complex sequences per workgroup (rings per workgroup, LDS per
 //     workgroup), per direction;
 //   - FP64 real synthesis: the vector-unit kernel against the FP64 matrix-core
-//     kernel, as the number of fields from which the latter is used.
+//     kernel, as the number of fields from which the latter is used;
+//   - FP32 real synthesis and analysis: the stored-table GEMMs against the
+//     recurrence kernels, per direction, at 16 fields.
 #include "sphswe_common.h"
 
 namespace {
@@ -48,6 +50,22 @@ extern "C" int sphswe_get_tuning(const sphswe_ctx *ctx, int *fft_nseq_r2c, int *
 	if (fft_nseq_c2r) *fft_nseq_c2r = ctx->fft_nseq_c2r;
 	if (synth_mfma_min) *synth_mfma_min = ctx->synth_mfma_min;
 	if (tuned) *tuned = ctx->tuned;
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_set_fp32_tables(sphswe_ctx *ctx, int synth_min_fields, int anal_min_fields)
+{
+	if (!ctx || synth_min_fields < 0 || anal_min_fields < 0) return SPHSWE_ERR_ARG;
+	ctx->tab_min_synth = synth_min_fields;
+	ctx->tab_min_anal = anal_min_fields;
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_get_fp32_tables(const sphswe_ctx *ctx, int *synth_min_fields, int *anal_min_fields)
+{
+	if (!ctx) return SPHSWE_ERR_ARG;
+	if (synth_min_fields) *synth_min_fields = ctx->tab_min_synth;
+	if (anal_min_fields) *anal_min_fields = ctx->tab_min_anal;
 	return SPHSWE_OK;
 }
 
@@ -109,6 +127,27 @@ extern "C" int sphswe_autotune(sphswe_ctx *ctx)
 		else break;
 	}
 	ctx->synth_mfma_min = threshold;
+	// FP32 real transforms of NF fields: stored tables or the recurrence kernels,
+	// per direction; the tables are released when neither direction uses them
+	for (int dir = 0; dir < 2 && !rc; dir++) {
+		float t_rec = 0, t_tab = 0;
+		int &thr = dir ? ctx->tab_min_anal : ctx->tab_min_synth;
+		auto run = [&]() {
+			return dir ? sphswe_leg_anal(ctx, (const double2 *)four, (double2 *)spec, NF, SPHSWE_PREC_FP32)
+				   : sphswe_leg_synth(ctx, (const double2 *)spec, (double2 *)four, NF, SPHSWE_PREC_FP32);
+		};
+		thr = 0;
+		rc = time_ms(ctx, REPS, run, &t_rec);
+		if (rc) break;
+		thr = NF;
+		rc = time_ms(ctx, REPS, run, &t_tab);
+		if (rc) break;
+		thr = t_tab < t_rec ? NF : 0;
+	}
+	if (!rc && !ctx->tab_min_synth && !ctx->tab_min_anal) {
+		(void)hipStreamSynchronize(ctx->stream);
+		sphswe_leg_tables_free(ctx);
+	}
 	if (spat) (void)hipFree(spat);
 	if (four) (void)hipFree(four);
 	if (spec) (void)hipFree(spec);

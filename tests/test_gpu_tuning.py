@@ -59,11 +59,21 @@ def test_autotune_choices_and_results(pair):
     assert t["tuned"]
     assert t["fft_nseq_r2c"] in NSEQ[ctx.L] and t["fft_nseq_c2r"] in NSEQ[ctx.L]
     assert t["synth_mfma_min"] in (0, 1, 3, 8, 16)
+    assert t["fp32_table_synth_min"] in (0, 16) and t["fp32_table_anal_min"] in (0, 16)
     f1 = ctx.synthesis(qs)
     q1 = ctx.analysis(f1)
     for k in range(16):
         assert rel_l2(f1[k], f0[k]) < TOL
         assert rel_l2(q1[k], q0[k]) < TOL
+    # FP32 transforms with the tuned choice against the recurrence form
+    g1 = ctx.synthesis(qs, capi.PREC_FP32)
+    r1 = ctx.analysis(f0, capi.PREC_FP32)
+    ctx.set_fp32_tables(0)
+    g0 = ctx.synthesis(qs, capi.PREC_FP32)
+    r0 = ctx.analysis(f0, capi.PREC_FP32)
+    for k in range(16):
+        assert rel_l2(g1[k], g0[k]) < 2e-6
+        assert rel_l2(r1[k], r0[k]) < 2e-6
     ctx.set_tuning()
 
 
