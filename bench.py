@@ -157,22 +157,26 @@ MODES = ("fp32", "bf16", "bf16x3")
 
 # REXI precision forms: (precision of the per-pole gradient operator, storage
 # precision of the band LU factors)
-REXI_MODES = (("fp64", "fp32"), ("fp32", "fp64"), ("fp32", "fp32"))
+REXI_MODES = (("fp64", "fp32"), ("fp64", "fp32+refine"), ("fp32", "fp64"), ("fp32", "fp32"),
+              ("fp32", "fp32+refine"))
 
 
 def bench_rexi_modes(steps, L=256, M=188):
     """Config 1 (T256, 200 poles, dt = 3600 s, seeded start state) with the
     per-pole gradient operator on the FP32 matrix cores and/or the band LU
     factors stored in FP32 (solves in FP64): steps/s and the relative L2 error
-    of h, u and v after the run against the FP64 run from the same start."""
+    of h, u and v after the run against the FP64 run from the same start.
+    "+refine": FP32 factors with one sweep of FP64 iterative refinement."""
     ctx = capi.Context(L)
     ctx.autotune()
     x0 = workloads.rexi_state(ctx)
     res, finals = {}, {}
     for leg, band in (("fp64", "fp64"),) + REXI_MODES:
         name = leg + "_gradient_" + band + "_factors"
+        bp = band.split("+")[0]
         prm = capi.rexi_params(dt=3600.0, rexi_h=0.2, rexi_M=M, half=1, visc=1e5,
-                               precision=capi.PRECISIONS[leg], band_precision=capi.PRECISIONS[band])
+                               precision=capi.PRECISIONS[leg], band_precision=capi.PRECISIONS[bp],
+                               refine=int(band.endswith("+refine")))
         rx = ctx.rexi(prm)
         state = ctx.to_device(x0)
         rx.step(state, 1)

@@ -45,6 +45,9 @@ struct sphswe_rexi {
 	int *d_moff = nullptr;
 	void *d_fphi = nullptr, *d_fvel = nullptr;   // parity-compact LU factors, double2 or float2
 	int fac32 = 0;                     // factors stored in FP32
+	int refine = 0;                    // one sweep of FP64 iterative refinement per solve
+	double2 *d_kphi = nullptr;         // refinement: pole-independent pieces of the phi bands [rows][4][5]
+	double2 *d_kvel = nullptr;         // refinement: W^2 mu^2 band of the velocity systems [rows][3]
 	unsigned char *d_pphi = nullptr, *d_pvel = nullptr;
 	double *d_oscale = nullptr;        // 2 pi / sqrt(1 - mu^2)
 	// work
@@ -169,13 +172,13 @@ __device__ inline double2 phi_rhs(const double2 *__restrict__ B, long ncplx, int
 // forward sweep then streams them like the factors)
 __global__ __launch_bounds__(256) void k_rexi_phi_rhs(const double2 *__restrict__ alpha, const int *__restrict__ moff,
 						      const double2 *__restrict__ B, double2 *__restrict__ out, int L, int P,
-						      long ncplx, double G, double W)
+						      long rs, long ncplx, double G, double W)
 {
 	const int m = (int)blockIdx.y - L, am = abs(m), N = L + 1 - am;
 	const long t = blockIdx.x * 256L + threadIdx.x;
 	if (t >= (long)N * P) return;
 	const int j = (int)(t / P), p = (int)(t % P);
-	out[((long)moff[m + L] + j) * P + p] = phi_rhs(B, ncplx, 1, G, W, m, am + j, alpha[p]);
+	out[((long)moff[m + L] + j) * rs + p] = phi_rhs(B, ncplx, 1, G, W, m, am + j, alpha[p]);
 }
 
 template <typename FT>
@@ -192,10 +195,19 @@ struct PhiArgs {
 	double two_omega, avg_geo;
 	int coriolis;
 	size_t rows;
+	// iterative refinement: right-hand sides, then residuals, in R (row stride
+	// 2P, first half); the pole-independent pieces of A per row and even
+	// offset -4..4: K[row][kind][5], kinds K0, K2 (times alpha^2), K1 (alpha),
+	// Km1 (1/alpha); A = alpha^4 I + K0 + alpha^2 K2 + alpha K1 + Km1 / alpha
+	double2 *R;
+	const double2 *K;
 };
 
-// phi: assemble the right-hand side, solve, emit gradient inputs, reduce beta*phi
-template <int KL, typename FT>
+// phi: assemble the right-hand side, solve, emit gradient inputs, reduce
+// beta*phi.  REF: the solve with the stored (FP32) factors is refined once:
+// x0 (kept in the first half of X), r = b - A x0 in FP64, the correction
+// solve, x = x0 + dx.
+template <int KL, typename FT, bool REF>
 __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 {
 	constexpr int KV = 2 * KL, NE = rexi_nec(KL);
@@ -239,86 +251,142 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 		return;
 	}
 
-	// forward substitution with row interchanges (L of the pivoted LU), in
-	// place: Y holds the right-hand sides (k_rexi_phi_rhs) ahead of row j and
-	// the forward-substituted values up to row j
-	double2 w[KL + 1];
+	// forward substitution with row interchanges (L of the pivoted LU): the
+	// right-hand sides of R (row stride rs) into Y.  R may be Y itself: row j
+	// is stored after the rows ahead of it have been loaded.
+	auto fwd = [&](const double2 *Rr, size_t rs) __attribute__((always_inline)) {
+		double2 w[KL + 1];
 #pragma unroll
-	for (int i = 0; i <= KL; i++) w[i] = (i < N) ? a.Y[(row0 + i) * P + pc] : make_double2(0, 0);
-	for (int jb = 0; jb < N; jb += RD) {
-		double2 lf[RD][KL / 2], rn[RD];
-		int kp[RD];
+		for (int i = 0; i <= KL; i++) w[i] = (i < N) ? Rr[(row0 + i) * rs + pc] : make_double2(0, 0);
+		for (int jb = 0; jb < N; jb += RD) {
+			double2 lf[RD][KL / 2], rn[RD];
+			int kp[RD];
 #pragma unroll
-		for (int d = 0; d < RD; d++) {
-			const int j = min(jb + d, N - 1);
-			const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+			for (int d = 0; d < RD; d++) {
+				const int j = min(jb + d, N - 1);
+				const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
 #pragma unroll
-			for (int e = 0; e < KL / 2; e++) lf[d][e] = ldf(f + (size_t)(KL + 1 + e) * P);
-			kp[d] = a.piv[(row0 + j) * P + pc];
-			const int jn = jb + d + KL + 1;
-			rn[d] = (jn < N) ? a.Y[(row0 + jn) * P + pc] : make_double2(0, 0);
-		}
-#pragma unroll
-		for (int d = 0; d < RD; d++) {
-			const int j = jb + d;
-			if (j < N) {
-				pivot_swap<KL>(w, kp[d]);
-#pragma unroll
-				for (int e = 0; e < KL / 2; e++) w[2 * e + 2] = cb(w[2 * e + 2], cm(lf[d][e], w[0]));
-				a.Y[(row0 + j) * P + pc] = w[0];
-#pragma unroll
-				for (int i = 0; i < KL; i++) w[i] = w[i + 1];
-				w[KL] = rn[d];
+				for (int e = 0; e < KL / 2; e++) lf[d][e] = ldf(f + (size_t)(KL + 1 + e) * P);
+				kp[d] = a.piv[(row0 + j) * P + pc];
+				const int jn = jb + d + KL + 1;
+				rn[d] = (jn < N) ? Rr[(row0 + jn) * rs + pc] : make_double2(0, 0);
 			}
-		}
-	}
-	// backward substitution with U (KV super-diagonals)
-	double2 v[KV + 1];
 #pragma unroll
-	for (int i = 0; i <= KV; i++) v[i] = (N - 1 - i >= 0) ? a.Y[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
-	for (int jt = N - 1; jt >= 0; jt -= RD) {
-		double2 uf[RD][KL + 1], yn[RD], bx[RD];
+			for (int d = 0; d < RD; d++) {
+				const int j = jb + d;
+				if (j < N) {
+					pivot_swap<KL>(w, kp[d]);
 #pragma unroll
-		for (int d = 0; d < RD; d++) {
-			const int j = max(jt - d, 0);
-			const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+					for (int e = 0; e < KL / 2; e++) w[2 * e + 2] = cb(w[2 * e + 2], cm(lf[d][e], w[0]));
+					a.Y[(row0 + j) * P + pc] = w[0];
 #pragma unroll
-			for (int e = 0; e <= KL; e++) uf[d][e] = ldf(f + (size_t)e * P);
-			const int jy = jt - d - KV - 1;
-			yn[d] = (jy >= 0) ? a.Y[(row0 + jy) * P + pc] : make_double2(0, 0);
-		}
-#pragma unroll
-		for (int d = 0; d < RD; d++) {
-			const int j = jt - d;
-			bx[d] = make_double2(0, 0);
-			if (j >= 0) {
-				const double2 x = cm(v[0], uf[d][0]);
-#pragma unroll
-				for (int e = 1; e <= KL; e++) v[2 * e] = cb(v[2 * e], cm(uf[d][e], x));
-				// outputs: i m phi at j, D' phi at j+1 (needs phi(j) and phi(j+2))
-				if (pv) a.X[(row0 + j) * 2 * P + p] = make_double2(-m * x.y, m * x.x);
-				phi[2] = phi[1];
-				phi[1] = phi[0];
-				phi[0] = x;
-				if (j + 1 < N) {
-					// row j+1: phi(j) = x, phi(j+2) = phi[2]
-					const int n = am + j + 1;
-					const double dn = n, dm = m;
-					const double c1 = (-dn + 1.0) * idR(dn - 1, dm), c2 = (dn + 2.0) * idS(dn + 1, dm);
-					if (pv) a.X[(row0 + j + 1) * 2 * P + P + p] = ca(cs(x, c1), cs(phi[2], c2));
+					for (int i = 0; i < KL; i++) w[i] = w[i + 1];
+					w[KL] = rn[d];
 				}
-				if (pv) bx[d] = cm(be, x);
-#pragma unroll
-				for (int i = 0; i < KV; i++) v[i] = v[i + 1];
-				v[KV] = yn[d];
 			}
 		}
-		// pole sums of the batch's rows
-		wave_sum_n<RD>(bx);
-		if (threadIdx.x == 0)
+	};
+	// backward substitution with U (KV super-diagonals) from Y; emit(j, x)
+	// per row, j = N-1 .. 0, returns beta x for the pole sums when SUM
+	auto bwd = [&](auto emit, auto sum_tag) __attribute__((always_inline)) {
+		constexpr bool SUM = decltype(sum_tag)::value;
+		double2 v[KV + 1];
 #pragma unroll
-			for (int d = 0; d < RD; d++)
-				if (jt - d >= 0) a.part[((size_t)blockIdx.x * 3 + 0) * a.rows + row0 + jt - d] = bx[d];
+		for (int i = 0; i <= KV; i++) v[i] = (N - 1 - i >= 0) ? a.Y[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
+		for (int jt = N - 1; jt >= 0; jt -= RD) {
+			double2 uf[RD][KL + 1], yn[RD], bx[RD];
+#pragma unroll
+			for (int d = 0; d < RD; d++) {
+				const int j = max(jt - d, 0);
+				const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+#pragma unroll
+				for (int e = 0; e <= KL; e++) uf[d][e] = ldf(f + (size_t)e * P);
+				const int jy = jt - d - KV - 1;
+				yn[d] = (jy >= 0) ? a.Y[(row0 + jy) * P + pc] : make_double2(0, 0);
+			}
+#pragma unroll
+			for (int d = 0; d < RD; d++) {
+				const int j = jt - d;
+				bx[d] = make_double2(0, 0);
+				if (j >= 0) {
+					const double2 x = cm(v[0], uf[d][0]);
+#pragma unroll
+					for (int e = 1; e <= KL; e++) v[2 * e] = cb(v[2 * e], cm(uf[d][e], x));
+					bx[d] = emit(j, x);
+#pragma unroll
+					for (int i = 0; i < KV; i++) v[i] = v[i + 1];
+					v[KV] = yn[d];
+				}
+			}
+			if constexpr (SUM) {
+				// pole sums of the batch's rows
+				wave_sum_n<RD>(bx);
+				if (threadIdx.x == 0)
+#pragma unroll
+					for (int d = 0; d < RD; d++)
+						if (jt - d >= 0) a.part[((size_t)blockIdx.x * 3 + 0) * a.rows + row0 + jt - d] = bx[d];
+			}
+		}
+	};
+	// outputs of phi(j): i m phi at j, D' phi at j+1 (needs phi(j) and phi(j+2)); beta phi
+	auto out = [&](int j, double2 x) __attribute__((always_inline)) -> double2 {
+		if (pv) a.X[(row0 + j) * 2 * P + p] = make_double2(-m * x.y, m * x.x);
+		phi[2] = phi[1];
+		phi[1] = phi[0];
+		phi[0] = x;
+		if (j + 1 < N) {
+			// row j+1: phi(j) = x, phi(j+2) = phi[2]
+			const int n = am + j + 1;
+			const double dn = n, dm = m;
+			const double c1 = (-dn + 1.0) * idR(dn - 1, dm), c2 = (dn + 2.0) * idS(dn + 1, dm);
+			if (pv) a.X[(row0 + j + 1) * 2 * P + P + p] = ca(cs(x, c1), cs(phi[2], c2));
+		}
+		return pv ? cm(be, x) : make_double2(0, 0);
+	};
+	if constexpr (!REF) {
+		fwd(a.Y, (size_t)P);
+		bwd(out, std::true_type{});
+	} else {
+		const size_t rs = 2 * (size_t)P;
+		const double2 al2 = cm(al, al), al4 = cm(al2, al2), ial = cinv(al);
+		// r(row q) = b(q) - sum over d = -4, -2, .., 4 of A(q, q+d) x0(q+d), in place in R;
+		// xw[k] = x0(q - KL + k)
+		double2 xw[KV + 1];
+#pragma unroll
+		for (int k = 0; k <= KV; k++) xw[k] = make_double2(0, 0);
+		auto resid = [&](int q) __attribute__((always_inline)) {
+			if (q < 0 || q >= N) return;
+			double2 r = a.R[(row0 + q) * rs + pc];
+			const double2 *Kq = a.K + (row0 + q) * 20;
+#pragma unroll
+			for (int e = 0; e <= KL; e++) {
+				const int c = q + 2 * e - KL;
+				if (c < 0 || c >= N) continue;
+				double2 A = ca(ca(Kq[e], cm(al2, Kq[5 + e])), ca(cm(al, Kq[10 + e]), cm(ial, Kq[15 + e])));
+				if (2 * e == KL) A = ca(A, al4);
+				r = cb(r, cm(A, xw[2 * e]));
+			}
+			a.R[(row0 + q) * rs + pc] = r;
+		};
+		fwd(a.R, rs);
+		bwd([&](int j, double2 x) __attribute__((always_inline)) -> double2 {
+			if (pv) a.X[(row0 + j) * 2 * P + p] = x;
+#pragma unroll
+			for (int k = KV; k > 0; k--) xw[k] = xw[k - 1];
+			xw[0] = x;
+			resid(j + KL);
+			return make_double2(0, 0);
+		}, std::false_type{});
+		for (int j = -1; j >= -KL; j--) {
+#pragma unroll
+			for (int k = KV; k > 0; k--) xw[k] = xw[k - 1];
+			xw[0] = make_double2(0, 0);
+			resid(j + KL);
+		}
+		fwd(a.R, rs);
+		bwd([&](int j, double2 dx) __attribute__((always_inline)) -> double2 {
+			return out(j, ca(a.X[(row0 + j) * 2 * P + pc], dx));
+		}, std::true_type{});
 	}
 	// row 0: phi(-1) = 0, phi(1) = phi[1]
 	{
@@ -343,9 +411,14 @@ struct VelArgs {
 	double two_omega, ir;
 	int coriolis;
 	size_t rows;
+	// iterative refinement: right-hand sides, then residuals, of both systems
+	// in R ([rows][2P]: a | b); x0 in X ([rows][2P], over G once the forward
+	// sweep has read it); A = alpha^2 I + Kv, Kv[row][3] at offsets -2, 0, 2
+	double2 *R, *X;
+	const double2 *Kv;
 };
 
-template <int KL, typename FT>
+template <int KL, typename FT, bool REF>
 __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 {
 	constexpr int KV = 2 * KL, NE = rexi_nec(KL);
@@ -423,8 +496,19 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 		}
 	}
 	// forward substitution; row j makes the right-hand side of row j + KL + 1,
-	// which shifts in a, b of row j + KL + 3
+	// which shifts in a, b of row j + KL + 3 (REF: the right-hand sides are
+	// also stored in R)
+	const size_t rs = 2 * (size_t)P;
 	double2 *Ya = a.Y, *Yb = a.Y + a.rows * P;
+	auto keep = [&](int q, double2 ra, double2 rb) __attribute__((always_inline)) {
+		if constexpr (REF) {
+			a.R[(row0 + q) * rs + pc] = ra;
+			a.R[(row0 + q) * rs + P + pc] = rb;
+		}
+	};
+#pragma unroll
+	for (int i = 0; i <= KL; i++)
+		if (i < N) keep(i, wa[i], wb[i]);
 	for (int jb = 0; jb < N; jb += RD) {
 		double2 lf[RD][KL / 2], gl[RD], gt[RD];
 		int kp[RD];
@@ -459,61 +543,161 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 					double2 an, bn;
 					ab_pre(jn + 2, gl[d], gt[d], an, bn);
 					rhs_next(jn, wa[KL], wb[KL], an, bn);
+					keep(jn, wa[KL], wb[KL]);
 				} else {
 					wa[KL] = wb[KL] = make_double2(0, 0);
 				}
 			}
 		}
 	}
-	double2 va[KV + 1], vb[KV + 1];
+	// backward substitution with U from Ya, Yb; emit(j, xa, xb, sa, sb) per row,
+	// j = N-1 .. 0, sets beta x for the pole sums when SUM
+	auto bwd = [&](auto emit, auto sum_tag) __attribute__((always_inline)) {
+		constexpr bool SUM = decltype(sum_tag)::value;
+		double2 va[KV + 1], vb[KV + 1];
 #pragma unroll
-	for (int i = 0; i <= KV; i++) {
-		const bool in = N - 1 - i >= 0;
-		va[i] = in ? Ya[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
-		vb[i] = in ? Yb[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
-	}
-	for (int jt = N - 1; jt >= 0; jt -= RD) {
-		double2 uf[RD][KL + 1], ya[RD], yb[RD], sx[2 * RD];
-#pragma unroll
-		for (int d = 0; d < RD; d++) {
-			const int j = max(jt - d, 0);
-			const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
-#pragma unroll
-			for (int e = 0; e <= KL; e++) uf[d][e] = ldf(f + (size_t)e * P);
-			const int jy = jt - d - KV - 1;
-			ya[d] = (jy >= 0) ? Ya[(row0 + jy) * P + pc] : make_double2(0, 0);
-			yb[d] = (jy >= 0) ? Yb[(row0 + jy) * P + pc] : make_double2(0, 0);
+		for (int i = 0; i <= KV; i++) {
+			const bool in = N - 1 - i >= 0;
+			va[i] = in ? Ya[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
+			vb[i] = in ? Yb[(row0 + N - 1 - i) * P + pc] : make_double2(0, 0);
 		}
+		for (int jt = N - 1; jt >= 0; jt -= RD) {
+			double2 uf[RD][KL + 1], ya[RD], yb[RD], sx[2 * RD];
 #pragma unroll
-		for (int d = 0; d < RD; d++) {
-			const int j = jt - d;
-			sx[2 * d] = sx[2 * d + 1] = make_double2(0, 0);
-			if (j >= 0) {
-				const double2 xa = cm(va[0], uf[d][0]), xb = cm(vb[0], uf[d][0]);
+			for (int d = 0; d < RD; d++) {
+				const int j = max(jt - d, 0);
+				const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
 #pragma unroll
-				for (int e = 1; e <= KL; e++) {
-					va[2 * e] = cb(va[2 * e], cm(uf[d][e], xa));
-					vb[2 * e] = cb(vb[2 * e], cm(uf[d][e], xb));
+				for (int e = 0; e <= KL; e++) uf[d][e] = ldf(f + (size_t)e * P);
+				const int jy = jt - d - KV - 1;
+				ya[d] = (jy >= 0) ? Ya[(row0 + jy) * P + pc] : make_double2(0, 0);
+				yb[d] = (jy >= 0) ? Yb[(row0 + jy) * P + pc] : make_double2(0, 0);
+			}
+#pragma unroll
+			for (int d = 0; d < RD; d++) {
+				const int j = jt - d;
+				sx[2 * d] = sx[2 * d + 1] = make_double2(0, 0);
+				if (j >= 0) {
+					const double2 xa = cm(va[0], uf[d][0]), xb = cm(vb[0], uf[d][0]);
+#pragma unroll
+					for (int e = 1; e <= KL; e++) {
+						va[2 * e] = cb(va[2 * e], cm(uf[d][e], xa));
+						vb[2 * e] = cb(vb[2 * e], cm(uf[d][e], xb));
+					}
+					emit(j, xa, xb, sx[2 * d], sx[2 * d + 1]);
+#pragma unroll
+					for (int i = 0; i < KV; i++) { va[i] = va[i + 1]; vb[i] = vb[i + 1]; }
+					va[KV] = ya[d];
+					vb[KV] = yb[d];
 				}
-				if (pv) {
-					sx[2 * d] = cm(be, xa);
-					sx[2 * d + 1] = cm(be, xb);
-				}
+			}
+			if constexpr (SUM) {
+				// pole sums of the batch's rows
+				wave_sum_n<2 * RD>(sx);
+				if (threadIdx.x == 0)
 #pragma unroll
-				for (int i = 0; i < KV; i++) { va[i] = va[i + 1]; vb[i] = vb[i + 1]; }
-				va[KV] = ya[d];
-				vb[KV] = yb[d];
+					for (int d = 0; d < RD; d++)
+						if (jt - d >= 0) {
+							a.part[((size_t)blockIdx.x * 3 + 1) * a.rows + row0 + jt - d] = sx[2 * d];
+							a.part[((size_t)blockIdx.x * 3 + 2) * a.rows + row0 + jt - d] = sx[2 * d + 1];
+						}
 			}
 		}
-		// pole sums of the batch's rows
-		wave_sum_n<2 * RD>(sx);
-		if (threadIdx.x == 0)
+	};
+	auto sums = [&](int, double2 xa, double2 xb, double2 &sa, double2 &sb) __attribute__((always_inline)) {
+		if (pv) {
+			sa = cm(be, xa);
+			sb = cm(be, xb);
+		}
+	};
+	if constexpr (!REF) {
+		bwd(sums, std::true_type{});
+	} else {
+		const double2 al2 = cm(al, al);
+		// x0 into X; r = b - A x0 in place in R, row q once x0(q-2 .. q+2) is known;
+		// xw[k] = x0(q - KL + k)
+		double2 xwa[KV + 1], xwb[KV + 1];
 #pragma unroll
-			for (int d = 0; d < RD; d++)
-				if (jt - d >= 0) {
-					a.part[((size_t)blockIdx.x * 3 + 1) * a.rows + row0 + jt - d] = sx[2 * d];
-					a.part[((size_t)blockIdx.x * 3 + 2) * a.rows + row0 + jt - d] = sx[2 * d + 1];
+		for (int k = 0; k <= KV; k++) xwa[k] = xwb[k] = make_double2(0, 0);
+		auto resid = [&](int q) __attribute__((always_inline)) {
+			if (q < 0 || q >= N) return;
+			double2 ra = a.R[(row0 + q) * rs + pc], rb = a.R[(row0 + q) * rs + P + pc];
+			const double2 *Kq = a.Kv + (row0 + q) * 3;
+#pragma unroll
+			for (int e = 0; e <= 2; e++) {
+				const int c = q + 2 * e - KL;
+				if (c < 0 || c >= N) continue;
+				const double2 A = (2 * e == KL) ? ca(Kq[e], al2) : Kq[e];
+				ra = cb(ra, cm(A, xwa[2 * e]));
+				rb = cb(rb, cm(A, xwb[2 * e]));
+			}
+			a.R[(row0 + q) * rs + pc] = ra;
+			a.R[(row0 + q) * rs + P + pc] = rb;
+		};
+		bwd([&](int j, double2 xa, double2 xb, double2 &, double2 &) __attribute__((always_inline)) {
+			if (pv) {
+				a.X[(row0 + j) * rs + p] = xa;
+				a.X[(row0 + j) * rs + P + p] = xb;
+			}
+#pragma unroll
+			for (int k = KV; k > 0; k--) { xwa[k] = xwa[k - 1]; xwb[k] = xwb[k - 1]; }
+			xwa[0] = xa;
+			xwb[0] = xb;
+			resid(j + KL);
+		}, std::false_type{});
+		for (int j = -1; j >= -KL; j--) {
+#pragma unroll
+			for (int k = KV; k > 0; k--) { xwa[k] = xwa[k - 1]; xwb[k] = xwb[k - 1]; }
+			xwa[0] = xwb[0] = make_double2(0, 0);
+			resid(j + KL);
+		}
+		// correction: forward substitution of the residuals, then x = x0 + dx
+		{
+			double2 ca_[KL + 1], cb_[KL + 1];
+#pragma unroll
+			for (int i = 0; i <= KL; i++) {
+				ca_[i] = (i < N) ? a.R[(row0 + i) * rs + pc] : make_double2(0, 0);
+				cb_[i] = (i < N) ? a.R[(row0 + i) * rs + P + pc] : make_double2(0, 0);
+			}
+			for (int jb = 0; jb < N; jb += RD) {
+				double2 lf[RD][KL / 2], rna[RD], rnb[RD];
+				int kp[RD];
+#pragma unroll
+				for (int d = 0; d < RD; d++) {
+					const int j = min(jb + d, N - 1);
+					const FT *f = a.fac + (row0 + j) * (size_t)NE * P + pc;
+#pragma unroll
+					for (int e = 0; e < KL / 2; e++) lf[d][e] = ldf(f + (size_t)(KL + 1 + e) * P);
+					kp[d] = a.piv[(row0 + j) * P + pc];
+					const int jn = jb + d + KL + 1;
+					rna[d] = (jn < N) ? a.R[(row0 + jn) * rs + pc] : make_double2(0, 0);
+					rnb[d] = (jn < N) ? a.R[(row0 + jn) * rs + P + pc] : make_double2(0, 0);
 				}
+#pragma unroll
+				for (int d = 0; d < RD; d++) {
+					const int j = jb + d;
+					if (j < N) {
+						pivot_swap<KL>(ca_, kp[d]);
+						pivot_swap<KL>(cb_, kp[d]);
+#pragma unroll
+						for (int e = 0; e < KL / 2; e++) {
+							ca_[2 * e + 2] = cb(ca_[2 * e + 2], cm(lf[d][e], ca_[0]));
+							cb_[2 * e + 2] = cb(cb_[2 * e + 2], cm(lf[d][e], cb_[0]));
+						}
+						Ya[(row0 + j) * P + pc] = ca_[0];
+						Yb[(row0 + j) * P + pc] = cb_[0];
+#pragma unroll
+						for (int i = 0; i < KL; i++) { ca_[i] = ca_[i + 1]; cb_[i] = cb_[i + 1]; }
+						ca_[KL] = rna[d];
+						cb_[KL] = rnb[d];
+					}
+				}
+			}
+		}
+		bwd([&](int j, double2 dxa, double2 dxb, double2 &sa, double2 &sb) __attribute__((always_inline)) {
+			const double2 xa = ca(a.X[(row0 + j) * rs + pc], dxa), xb = ca(a.X[(row0 + j) * rs + P + pc], dxb);
+			sums(j, xa, xb, sa, sb);
+		}, std::true_type{});
 	}
 }
 
@@ -710,7 +894,7 @@ extern "C" int sphswe_rexi_destroy(sphswe_rexi *r)
 	if (r->ctx) hipStreamSynchronize(r->ctx->stream);
 	void *ptrs[] = {r->d_alpha, r->d_beta, r->d_moff, r->d_fphi, r->d_fvel, r->d_pphi, r->d_pvel, r->d_oscale,
 			r->d_X, r->d_G, r->d_Y, r->d_part, r->d_basis, r->d_acc, r->d_phys, r->d_cphys, r->d_cspec,
-			r->d_gop, r->d_gop32, r->d_goff, r->d_gtiles,
+			r->d_gop, r->d_gop32, r->d_goff, r->d_gtiles, r->d_kphi, r->d_kvel,
 			r->d_in};
 	for (void *p : ptrs)
 		if (p) hipFree(p);
@@ -794,7 +978,7 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: band_precision must be FP64 or FP32");
 	if (prm->precision != SPHSWE_PREC_FP64 && prm->precision != SPHSWE_PREC_FP32)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: precision must be FP64 or FP32");
-	if (prm->refine != 0) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: invalid refine");
+	if (prm->refine != 0 && prm->refine != 1) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: refine must be 0 or 1");
 	sphswe_rexi *r = new sphswe_rexi();
 	r->ctx = ctx;
 	r->prm = *prm;
@@ -905,6 +1089,37 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 		return sphswe_set_err(ctx, SPHSWE_ERR_INTERNAL, "REXI band factors: nonzero entry at an odd offset");
 	}
 	r->fac32 = prm->band_precision == SPHSWE_PREC_FP32;
+	r->refine = prm->refine && r->coriolis;   // without rotation the solves are diagonal
+	// refinement: A(alpha) = alpha^4 I + K0 + alpha^2 K2 + alpha K1 + K-1 / alpha for
+	// phi and alpha^2 I + W^2 mu^2 for the winds, the pole-independent pieces at the
+	// even offsets of every row (the components of SWERexiSPH::setup grouped by
+	// their power of alpha)
+	std::vector<double2> hK, hKv;
+	if (r->refine) {
+		BandRows K0(L, 4), K2(L, 4), K1(L, 4), Km1(L, 4), Kv(L, 2);
+		band_add(K0, 13, (W * W) * (W * W), rr);
+		band_add(K0, 16, G * 2.0 * W * W, rr);
+		band_add(K0, 18, -G * W * W, rr);
+		band_add(K2, 12, 2.0 * W * W, rr);
+		band_add(K2, 17, -G, rr);
+		band_add(K1, 14, -G * W, rr);
+		band_add(Km1, 15, G * W * W * W, rr);
+		band_add(Kv, 12, W * W, rr);
+		hK.resize(r->rows * 20);
+		hKv.resize(r->rows * 3);
+		const BandRows *kinds[4] = {&K0, &K2, &K1, &Km1};
+		for (size_t q = 0; q < r->rows; q++) {
+			for (int k = 0; k < 4; k++)
+				for (int e = 0; e < 5; e++) {
+					const cplx v = kinds[k]->data[q * 9 + 2 * e];
+					hK[q * 20 + k * 5 + e] = make_double2(v.real(), v.imag());
+				}
+			for (int e = 0; e < 3; e++) {
+				const cplx v = Kv.data[q * 5 + 2 * e];
+				hKv[q * 3 + e] = make_double2(v.real(), v.imag());
+			}
+		}
+	}
 	std::vector<double> osc(ctx->nlat);
 	// the metric division of grad_lon / grad_lat, times the 2 pi that the forward
 	// FFT (normalised by 2 pi / nphi) would give the Fourier coefficients the
@@ -961,6 +1176,12 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 			sphswe_rexi_destroy(r);
 			return rc;
 		}
+	}
+	if (r->refine && ((rc = dev_alloc(ctx, &r->d_kphi, hK.size())) || (rc = dev_alloc(ctx, &r->d_kvel, hKv.size())) ||
+			  (rc = up(r->d_kphi, hK.data(), sizeof(double2) * hK.size())) ||
+			  (rc = up(r->d_kvel, hKv.data(), sizeof(double2) * hKv.size())))) {
+		sphswe_rexi_destroy(r);
+		return rc;
 	}
 	if ((rc = rexi_gradient_operator(ctx, r))) {
 		sphswe_rexi_destroy(r);
@@ -1106,12 +1327,17 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 		pa.L = L; pa.P = P; pa.ncplx = (int)nc;
 		pa.two_omega = 2.0 * r->prm.coriolis_omega; pa.avg_geo = r->prm.h0 * g;
 		pa.coriolis = r->coriolis; pa.rows = r->rows;
-		k_rexi_phi<4, FT><<<grid, 64, 0, st>>>(pa);
+		pa.R = r->d_G; pa.K = r->d_kphi;
+		if (r->refine) k_rexi_phi<4, FT, true><<<grid, 64, 0, st>>>(pa);
+		else k_rexi_phi<4, FT, false><<<grid, 64, 0, st>>>(pa);
 	};
 	if (r->coriolis) {
+		// right-hand sides into Y (solved in place), or into the first half of G
+		// (kept for the residual) when refining
 		const long nc2 = ctx->ncplx;
 		k_rexi_phi_rhs<<<dim3((unsigned)(((long)(L + 1) * P + 255) / 256), 2 * L + 1), 256, 0, st>>>(
-			r->d_alpha, r->d_moff, r->d_basis, r->d_Y, L, P, nc2, r->prm.h0 * g, 2.0 * r->prm.coriolis_omega);
+			r->d_alpha, r->d_moff, r->d_basis, r->refine ? r->d_G : r->d_Y, L, P, r->refine ? 2L * P : (long)P, nc2,
+			r->prm.h0 * g, 2.0 * r->prm.coriolis_omega);
 		SPHSWE_LAUNCH_CHECK(ctx);
 	}
 	if (r->fac32) phi((const float2 *)r->d_fphi);
@@ -1140,7 +1366,9 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 		va.L = L; va.P = P; va.ncplx = (int)nc;
 		va.two_omega = 2.0 * r->prm.coriolis_omega; va.ir = 1.0 / a;
 		va.coriolis = r->coriolis; va.rows = r->rows;
-		k_rexi_vel<2, FT><<<grid, 64, 0, st>>>(va);
+		va.R = r->d_X; va.X = r->d_G; va.Kv = r->d_kvel;
+		if (r->refine) k_rexi_vel<2, FT, true><<<grid, 64, 0, st>>>(va);
+		else k_rexi_vel<2, FT, false><<<grid, 64, 0, st>>>(va);
 	};
 	if (r->fac32) vel((const float2 *)r->d_fvel);
 	else vel((const double2 *)r->d_fvel);

@@ -190,6 +190,15 @@ struct SweBufs {
 	double *P9, *P4, *P2, *P1;
 };
 
+// one captured step per context, rebuilt when its arguments change
+struct SweGraph {
+	hipGraphExec_t exec = nullptr;
+	sphswe_swe_params prm;
+	double *state = nullptr;
+	int nmem = 0;
+};
+std::unordered_map<const sphswe_ctx *, SweGraph> g_graph;
+
 int swe_buffers(sphswe_ctx *ctx, int nmem, SweBufs &b)
 {
 	const long nlm = ctx->nlm, nspat = ctx->nspat;
@@ -204,6 +213,12 @@ int swe_buffers(sphswe_ctx *ctx, int nmem, SweBufs &b)
 	if (ws.bytes < bytes) {
 		if (ws.base) {
 			SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+			// a captured step points into the old workspace: drop it
+			auto gi = g_graph.find(ctx);
+			if (gi != g_graph.end() && gi->second.exec) {
+				SPHSWE_HIP(ctx, hipGraphExecDestroy(gi->second.exec));
+				gi->second.exec = nullptr;
+			}
 			SPHSWE_HIP(ctx, hipFree(ws.base));
 			ws.base = nullptr;
 			ws.bytes = 0;
@@ -278,15 +293,6 @@ int check_params(sphswe_ctx *ctx, const sphswe_swe_params *p)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: precision %d not available", p->precision);
 	return SPHSWE_OK;
 }
-
-// one captured step per context, rebuilt when its arguments change
-struct SweGraph {
-	hipGraphExec_t exec = nullptr;
-	sphswe_swe_params prm;
-	double *state = nullptr;
-	int nmem = 0;
-};
-std::unordered_map<const sphswe_ctx *, SweGraph> g_graph;
 
 }  // namespace
 

@@ -22,6 +22,11 @@ BOUNDS = {
     ("fp32", "fp64"): 2e-6,
     ("fp32", "fp32"): 2e-6,
 }
+# FP32 factors with one sweep of FP64 iterative refinement (residual with the
+# FP64 band rebuilt from its pole-independent pieces): the first solve's error
+# eps ~ 3e-8 becomes ~eps^2 plus FP64 round-off of the residual, i.e. the FP64
+# path's agreement with the oracle (1e-12 there, ~1e-15 measured)
+REFINED_BOUND = 1e-11
 
 
 def _state(L, seed):
@@ -61,10 +66,32 @@ def test_rexi_modes_vs_oracle(L, M, leg, band):
     ctx.close()
 
 
+@pytest.mark.parametrize("L,M", [(31, 8), (64, 32)])
+def test_rexi_refined_fp32_factors_vs_oracle(L, M):
+    print(f"rexi refined FP32 factors vs oracle L={L} M={M}", flush=True)
+    ctx = capi.Context(L)
+    orc = O.Oracle(L, ctx.nlat, ctx.nphi)
+    h, u, v = _state(L, 11)
+    dt, nsteps = 3600.0, 2
+    prm = dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, visc=1e5, rexi_h=0.2, rexi_M=M, half=1, dt=dt)
+    ref = orc.swe_rexi_run(prm, h, u, v, nsteps)
+    params = capi.rexi_params(omega=OMEGA, visc=1e5, dt=dt, rexi_h=0.2, rexi_M=M, half=1,
+                              band_precision=capi.PREC_FP32, refine=1)
+    out = _run(ctx, params, (h, u, v), nsteps)
+    unref = _run(ctx, capi.rexi_params(omega=OMEGA, visc=1e5, dt=dt, rexi_h=0.2, rexi_M=M, half=1,
+                                       band_precision=capi.PREC_FP32), (h, u, v), nsteps)
+    for k in range(3):
+        e, e0 = rel_l2(out[k], ref[k]), rel_l2(unref[k], ref[k])
+        print(L, "refined", "huv"[k], e, "unrefined", e0)
+        assert e < REFINED_BOUND
+        assert e < 1e-3 * e0   # the refinement removes the FP32 factor error
+    ctx.close()
+
+
 def test_rexi_rejects_bad_precision():
     ctx = capi.Context(16)
     for kw in (dict(band_precision=capi.PREC_BF16), dict(precision=capi.PREC_BF16X3), dict(precision=7),
-               dict(refine=1)):
+               dict(refine=2)):
         with pytest.raises(RuntimeError):
             ctx.rexi(capi.rexi_params(dt=3600.0, rexi_M=8, **kw))
     ctx.close()
@@ -116,3 +143,18 @@ def test_rexi_full_size_modes_vs_fp64(full, leg, band):
         e = rel_l2(out[k], ref[k])
         print(leg, band, "huv"[k], e)
         assert e < 4 * BOUNDS[(leg, band)]
+
+
+def test_rexi_full_size_refined_fp32_factors_vs_fp64(full):
+    # T256, 200 poles, rotation and viscosity, two steps: FP32 factors refined once
+    ctx = full
+    print("rexi full-size refined FP32 factors vs fp64", flush=True)
+    x = _state(L_FULL, 21)
+    ref = _run(ctx, capi.rexi_params(omega=OMEGA, visc=1e5, dt=3600.0, rexi_h=0.2, rexi_M=M_FULL, half=1), x, 2)
+    params = capi.rexi_params(omega=OMEGA, visc=1e5, dt=3600.0, rexi_h=0.2, rexi_M=M_FULL, half=1,
+                              band_precision=capi.PREC_FP32, refine=1)
+    out = _run(ctx, params, x, 2)
+    for k in range(3):
+        e = rel_l2(out[k], ref[k])
+        print("refined", "huv"[k], e)
+        assert e < REFINED_BOUND

@@ -147,3 +147,25 @@ def test_ensemble_config3_members_match_single_runs():
                 assert rel_l2(out[3 * k + f], ref[f]) < 1e-13, (k, f)
     finally:
         ctx.close()
+
+
+def test_graph_rebuilt_after_workspace_growth():
+    # regression: a step captured for 1 member, then an eager call for 4
+    # members (the workspace is reallocated), then the 1-member graph again
+    # with unchanged arguments: the captured step must be rebuilt on the new
+    # workspace, and equal the eager steps bit for bit
+    L = 31
+    ctx = capi.Context(L)
+    try:
+        x0 = _members(ctx, 4)
+        prm = capi.swe_params(dt=300.0)
+        g = ctx.to_device(x0[:3])
+        e = ctx.to_device(x0[:3])
+        ctx.swe_rk4_graph(prm, g, 1, 1)
+        big = ctx.to_device(x0)
+        ctx.swe_rk4(prm, big, 4, 1)
+        ctx.swe_rk4_graph(prm, g, 1, 2)
+        ctx.swe_rk4(prm, e, 1, 3)
+        assert np.array_equal(ctx.to_host(g, np.complex128, (3, ctx.nlm)), ctx.to_host(e, np.complex128, (3, ctx.nlm)))
+    finally:
+        ctx.close()
