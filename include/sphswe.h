@@ -162,7 +162,9 @@ typedef struct {
 	double h0;               /* average height */
 	double viscosity;        /* nu (explicit nu*laplace/a^2) */
 	double dt;
-	int nonlinear;           /* 1: nonlinear equations, 0: linear */
+	int nonlinear;           /* 1: nonlinear equations, 0: linear; 2: the nonlinear remainder
+				  * (full minus linear tendency), for a split scheme whose linear
+				  * part is integrated by REXI (no counterpart in the reference) */
 	int precision;           /* SPHSWE_PREC_* for the Legendre stage */
 } sphswe_swe_params;
 

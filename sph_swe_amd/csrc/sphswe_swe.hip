@@ -11,7 +11,10 @@
 //   level 0  synthesis of 9 fields:  h, u, v, im*h, D'h, im*u, D'u, im*v, D'v
 //            (D' = (1-mu^2) d/dmu, SPHOperators.hpp:234-260)
 //   pointwise h*u, h*v*st (div_lat, SPHOperators.hpp:411-439) and the u, v
-//            tendencies (grad_lon/grad_lat divide by sqrt(1-mu^2), f = 2 Omega mu)
+//            tendencies (grad_lon/grad_lat divide by sqrt(1-mu^2), f = 2 Omega mu);
+//            nonlinear = 0 the linear equations (h0 for h in the fluxes, no
+//            advection), nonlinear = 2 the difference of the two (flux of
+//            h - h0 and advection only: the part a split scheme leaves to RK4)
 //   level 1  analysis of 4 fields, im* and D' on the two flux fields
 //   level 2  synthesis of 2 fields, divide by sqrt(1-mu^2) (and st)
 //   level 3  analysis of 1 field (the h tendency)
@@ -62,7 +65,7 @@ __global__ void k_prep9(const double2 *__restrict__ X, double2 *__restrict__ S9,
 // pointwise level 0 -> 4 fields per member
 __global__ void k_phys1(const double *__restrict__ P9, double *__restrict__ P4, const double *__restrict__ ct,
 			const double *__restrict__ st, const double *__restrict__ sq, int nlat, long nspat, int nmem,
-			double ga, double two_omega, double ia, int nonlinear, double h0a)
+			double ga, double two_omega, double ia, int nonlinear, double h0)
 {
 	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
 	if (i >= nspat * nmem) return;
@@ -75,20 +78,26 @@ __global__ void k_phys1(const double *__restrict__ P9, double *__restrict__ P4, 
 	double Ul = p[5 * nspat], Um = p[6 * nspat], Vl = p[7 * nspat], Vm = p[8 * nspat];
 	const double c = sq[j], mu = ct[j];
 	double *o = P4 + (long)mem * 4 * nspat + k;
-	if (nonlinear) {
+	if (nonlinear == 1) {
 		o[0] = U * H;                         // h*u
 		o[nspat] = (V * H) * st[j];           // (h*v)*cos(phi)
 		double tu = -ga * (Hl / c) + two_omega * (V * mu) - ((Ul / c) * U + (Um / c) * V) * ia;
 		double tv = -ga * (Hm / c) - two_omega * (U * mu) - ((Vl / c) * U + (Vm / c) * V) * ia;
 		o[2 * nspat] = tu;
 		o[3 * nspat] = tv;
+	} else if (nonlinear == 2) {
+		// the full tendency minus its linear part: flux of h - h0, advection
+		const double Hp = H - h0;
+		o[0] = U * Hp;
+		o[nspat] = (V * Hp) * st[j];
+		o[2 * nspat] = -((Ul / c) * U + (Um / c) * V) * ia;
+		o[3 * nspat] = -((Vl / c) * U + (Vm / c) * V) * ia;
 	} else {
 		o[0] = U;
 		o[nspat] = V * st[j];
 		o[2 * nspat] = -ga * (Hl / c) + two_omega * (V * mu);
 		o[3 * nspat] = -ga * (Hm / c) - two_omega * (U * mu);
 	}
-	(void)h0a;
 }
 
 // spectral level 1: im * A(hu), D' A(hv st)
@@ -266,7 +275,7 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 		if ((rc = sphswe_leg_synth(ctx, b.S9, b.F, 9 * nm, prec))) return rc;
 		if ((rc = sphswe_fft_c2r(ctx, b.F, b.P9, 9 * nm))) return rc;
 		k_phys1<<<g1(nspat * nm), 256, 0, st>>>(b.P9, b.P4, ctx->d_ct, ctx->d_st, ctx->d_sq, nlat, nspat, nm, ga,
-							 two_omega, ia, p->nonlinear, p->h0 / a);
+							 two_omega, ia, p->nonlinear, p->h0);
 		if ((rc = sphswe_fft_r2c(ctx, b.P4, b.F, 4 * nm))) return rc;
 		if ((rc = sphswe_leg_anal(ctx, b.F, b.S4, 4 * nm, prec))) return rc;
 		k_prep2<<<g1(nlm * nm), 256, 0, st>>>(b.S4, b.S2, ctx->d_mofidx, L, (int)nlm, nm);
@@ -291,6 +300,8 @@ int check_params(sphswe_ctx *ctx, const sphswe_swe_params *p)
 {
 	if (p->precision < SPHSWE_PREC_FP64 || p->precision > SPHSWE_PREC_BF16X3)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: precision %d not available", p->precision);
+	if (p->nonlinear < 0 || p->nonlinear > 2)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: nonlinear must be 0, 1 or 2");
 	return SPHSWE_OK;
 }
 

@@ -74,3 +74,70 @@ def rexi_state(ctx, seed=2024, a=A_EARTH, g=GRAV):
     u = (gl(chi) - gt(psi)) / a
     v = (gl(psi) + gt(chi)) / a
     return np.stack([phi[None, :] / g, u, v]).reshape(3, -1)
+
+
+def galewsky_state(ctx, seed=None, pert_amp=1.0, a=A_EARTH, g=GRAV, omega=OMEGA, h_avg=10000.0):
+    """(h, u, v) spectra of the Galewsky et al. (2004) balanced zonal jet without
+    its bump, set up as the reference does (src/include/benchmarks/
+    BenchmarkGalewsky.hpp:101-246):
+      u(phi) = u_max / e_n * exp(1 / ((phi - phi0)(phi - phi1))) on phi0 < phi < phi1
+      (u_max = 80 m/s, phi0 = pi/7, phi1 = pi/2 - phi0, e_n = exp(-4 / (phi1 - phi0)^2)), v = 0;
+      g h(lat_j) = - integral from -pi/2 to lat_j of a u (2 Omega sin phi + tan(phi) u / a),
+      accumulated from the south pole through the Gauss latitudes, then shifted so
+      that its area mean, with weights 2 pi (sin(end) - sin(start)) per interval
+      and the value at each interval's end, is h_avg.
+    With a seed, a height perturbation with spectrum ~ l^-2 (l = 0 zeroed), scaled
+    to a largest value of pert_amp metres on the grid, is added."""
+    from scipy.integrate import quad
+
+    phi0 = np.pi / 7.0
+    phi1 = np.pi / 2.0 - phi0
+    u_max = 80.0
+    en = np.exp(-4.0 / (phi1 - phi0) ** 2)
+
+    def u_of(phi):
+        if phi <= phi0 or phi >= phi1:
+            return 0.0
+        return u_max / en * np.exp(1.0 / ((phi - phi0) * (phi - phi1)))
+
+    def integrand(phi):
+        up = u_of(phi)
+        return a * up * (2.0 * omega * np.sin(phi) + (np.tan(phi) / a) * up)
+
+    def integral(s, e):
+        # the integrand vanishes outside (phi0, phi1) and is smooth inside
+        s2, e2 = max(s, phi0), min(e, phi1)
+        if e2 <= s2:
+            return 0.0
+        return quad(integrand, s2, e2, epsabs=1e-13, epsrel=1e-13, limit=200)[0]
+
+    lat = np.arcsin(np.asarray(ctx.ct))   # j = 0 northernmost
+    nlat = lat.size
+    hg = np.empty(nlat)
+    hg_sum = area = 0.0
+    start = -np.pi / 2.0
+    prev = 0.0
+    for j in range(nlat - 1, -1, -1):
+        end = lat[j]
+        prev = prev - integral(start, end)
+        hg[j] = prev
+        mterm = (np.sin(end) - np.sin(start)) * 2.0 * np.pi
+        hg_sum += prev * mterm
+        area += mterm
+        start = end
+    last = prev - integral(start, np.pi / 2.0)
+    mterm = (np.sin(np.pi / 2.0) - np.sin(start)) * 2.0 * np.pi
+    hg_sum += last * mterm
+    area += mterm
+    h_lat = hg / g + (h_avg - (hg_sum / g) / area)
+    u_lat = np.array([u_of(p) for p in lat])
+    nphi = ctx.nphi
+    h = np.tile(h_lat, nphi)      # [i * nlat + j]
+    u = np.tile(u_lat, nphi)
+    v = np.zeros_like(h)
+    x = ctx.analysis(np.stack([h, u, v]))
+    if seed is not None:
+        q = seeded_spectra(ctx.L, 1, seed, power=1.0)   # amplitude l^-1: spectrum ~ l^-2
+        pert = ctx.synthesis(q)[0]
+        x[0] += q[0] * (pert_amp / np.max(np.abs(pert)))
+    return x
