@@ -501,7 +501,10 @@ struct PassesCT {
 
 // MODE_R2C: nf real fields -> Fourier rows (L + 1, nlat); MODE_C2R: back.
 // Workgroup = 2 NSEQ adjacent rings packed into NSEQ complex sequences.
-template <int MODE, int N, int NSEQ>
+// RM: the physical fields are ring-major, [j * N + i] (the SWE stepper's
+// internal layout: every ring is one contiguous row), instead of the
+// reference's [i * nlat + j].
+template <int MODE, int N, int NSEQ, bool RM>
 __global__ __launch_bounds__(256) void k_fft_real(const void *__restrict__ in, void *__restrict__ out,
 						  const double2 *__restrict__ tw, int nlat, int L, int nblk, int nblk_pad,
 						  double norm)
@@ -525,14 +528,14 @@ __global__ __launch_bounds__(256) void k_fft_real(const void *__restrict__ in, v
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
 				const int idx = min(b + u * 256 + tid, TOT - 1);
-				const int i = idx / RB, r = idx % RB;
-				v[u] = src[(size_t)i * nlat + min(j0 + r, nlat - 1)];
+				const int i = RM ? idx % N : idx / RB, r = RM ? idx / N : idx % RB;
+				v[u] = RM ? src[(size_t)min(j0 + r, nlat - 1) * N + i] : src[(size_t)i * nlat + min(j0 + r, nlat - 1)];
 			}
 #pragma unroll
 			for (int u = 0; u < KB; u++) {
 				const int idx = b + u * 256 + tid;
 				if (TOT % (KB * 256) != 0 && idx >= TOT) break;
-				const int i = idx / RB, r = idx % RB;
+				const int i = RM ? idx % N : idx / RB, r = RM ? idx / N : idx % RB;
 				lds[2 * CPad<N>::at(r >> 1, i) + (r & 1)] = (j0 + r < nlat) ? v[u] : 0.0;
 			}
 		}
@@ -595,14 +598,14 @@ __global__ __launch_bounds__(256) void k_fft_real(const void *__restrict__ in, v
 		const double *rl = (const double *)res;
 		constexpr int TOT = N * RB;
 		for (int idx = tid; idx < TOT; idx += 256) {
-			const int i = idx / RB, r = idx % RB;
+			const int i = RM ? idx % N : idx / RB, r = RM ? idx / N : idx % RB;
 			const int j = j0 + r;
-			if (j < nlat) dst[(size_t)i * nlat + j] = rl[2 * CPad<N>::at(r >> 1, i) + (r & 1)];
+			if (j < nlat) dst[RM ? (size_t)j * N + i : (size_t)i * nlat + j] = rl[2 * CPad<N>::at(r >> 1, i) + (r & 1)];
 		}
 	}
 }
 
-template <int MODE, int N, int NSEQ>
+template <int MODE, int N, int NSEQ, bool RM>
 int launch_real_ct(sphswe_ctx *ctx, const void *in, void *out, int nf)
 {
 	constexpr int RB = 2 * NSEQ;
@@ -610,10 +613,10 @@ int launch_real_ct(sphswe_ctx *ctx, const void *in, void *out, int nf)
 	const int nblk_pad = (nblk + 7) / 8 * 8;
 	const size_t lds = 2 * (size_t)NSEQ * CPad<N>::len * sizeof(double2);
 	if (lds > 65536)
-		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_real<MODE, N, NSEQ>,
+		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_real<MODE, N, NSEQ, RM>,
 						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 	dim3 grid(nblk_pad, nf);
-	k_fft_real<MODE, N, NSEQ><<<grid, 256, lds, ctx->stream>>>(in, out, ctx->d_tw, ctx->nlat, ctx->L, nblk, nblk_pad,
+	k_fft_real<MODE, N, NSEQ, RM><<<grid, 256, lds, ctx->stream>>>(in, out, ctx->d_tw, ctx->nlat, ctx->L, nblk, nblk_pad,
 								    2.0 * M_PI / N);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
@@ -622,28 +625,28 @@ int launch_real_ct(sphswe_ctx *ctx, const void *in, void *out, int nf)
 // the compile-time kernel for the grid sizes of the usual truncations, or -1.
 // nseq: complex sequences per workgroup (2 nseq rings); 0 takes the default,
 // a value without an instantiation for this size falls back to it
-template <int MODE>
+template <int MODE, bool RM = false>
 int launch_real_fixed(sphswe_ctx *ctx, const void *in, void *out, int nf, int nseq)
 {
 	switch (ctx->plan.n) {
 	case 96:
-		if (nseq == 16) return launch_real_ct<MODE, 96, 16>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 96, 8>(ctx, in, out, nf);
+		if (nseq == 16) return launch_real_ct<MODE, 96, 16, RM>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 96, 8, RM>(ctx, in, out, nf);
 	case 196:
-		if (nseq == 4) return launch_real_ct<MODE, 196, 4>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 196, 8>(ctx, in, out, nf);
+		if (nseq == 4) return launch_real_ct<MODE, 196, 4, RM>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 196, 8, RM>(ctx, in, out, nf);
 	case 384:
-		if (nseq == 2) return launch_real_ct<MODE, 384, 2>(ctx, in, out, nf);
-		if (nseq == 8) return launch_real_ct<MODE, 384, 8>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 384, 4>(ctx, in, out, nf);
+		if (nseq == 2) return launch_real_ct<MODE, 384, 2, RM>(ctx, in, out, nf);
+		if (nseq == 8) return launch_real_ct<MODE, 384, 8, RM>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 384, 4, RM>(ctx, in, out, nf);
 	case 784:
-		if (nseq == 1) return launch_real_ct<MODE, 784, 1>(ctx, in, out, nf);
-		if (nseq == 4) return launch_real_ct<MODE, 784, 4>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 784, 2>(ctx, in, out, nf);
+		if (nseq == 1) return launch_real_ct<MODE, 784, 1, RM>(ctx, in, out, nf);
+		if (nseq == 4) return launch_real_ct<MODE, 784, 4, RM>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 784, 2, RM>(ctx, in, out, nf);
 	case 1536:
-		if (nseq == 2) return launch_real_ct<MODE, 1536, 2>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 1536, 1>(ctx, in, out, nf);
-	case 3072: return launch_real_ct<MODE, 3072, 1>(ctx, in, out, nf);
+		if (nseq == 2) return launch_real_ct<MODE, 1536, 2, RM>(ctx, in, out, nf);
+		return launch_real_ct<MODE, 1536, 1, RM>(ctx, in, out, nf);
+	case 3072: return launch_real_ct<MODE, 3072, 1, RM>(ctx, in, out, nf);
 	}
 	return -1;
 }
@@ -707,6 +710,27 @@ int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf)
 	const int rc = launch_real_fixed<MODE_C2R>(ctx, four, spat, nf, ctx->fft_nseq_c2r);
 	return rc >= 0 ? rc : launch<MODE_C2R>(ctx, four, spat, nf);
 }
+// ring-major physical fields ([j * nphi + i]), for buffers internal to the
+// library; -1 when the grid size has no compile-time kernel (the caller then
+// keeps the reference's layout)
+int sphswe_fft_r2c_rm(sphswe_ctx *ctx, const double *spat, double2 *four, int nf)
+{
+	if (nf <= 0) return SPHSWE_OK;
+	return launch_real_fixed<MODE_R2C, true>(ctx, spat, four, nf, ctx->fft_nseq_r2c);
+}
+int sphswe_fft_c2r_rm(sphswe_ctx *ctx, const double2 *four, double *spat, int nf)
+{
+	if (nf <= 0) return SPHSWE_OK;
+	return launch_real_fixed<MODE_C2R, true>(ctx, four, spat, nf, ctx->fft_nseq_c2r);
+}
+bool sphswe_fft_has_rm(const sphswe_ctx *ctx)
+{
+	switch (ctx->plan.n) {
+	case 96: case 196: case 384: case 784: case 1536: case 3072: return true;
+	}
+	return false;
+}
+
 // the sequences-per-workgroup candidates of the compile-time real FFT of this
 // grid (empty for other sizes), for the autotuner
 int sphswe_fft_nseq_candidates(const sphswe_ctx *ctx, int *out, int max)
