@@ -107,6 +107,32 @@ def bench_legendre_kernel(ctx, reps=20):
     return ms, ms_a, flops
 
 
+def bench_fft_kernels(ctx, nf=9, reps=20):
+    """Longitude FFTs of nf fields at the context's grid (the reference's
+    [i * nlat + j] layout, through the C ABI): average launch time with HIP
+    events and achieved HBM bandwidth on the algorithmic bytes, 8 nlat nphi
+    (physical) + 16 (L + 1) nlat (Fourier rows) per field."""
+    lib, h = ctx.lib, ctx.h
+    spat = ctx.empty(nf * ctx.nspat * 8)
+    four = ctx.empty(nf * (ctx.L + 1) * ctx.nlat * 16)
+    ctx._check(lib.sphswe_fill_zero(h, spat.ptr, nf * ctx.nspat * 8))
+    ctx._check(lib.sphswe_fill_zero(h, four.ptr, nf * (ctx.L + 1) * ctx.nlat * 16))
+    bytes_per = float(nf * (8 * ctx.nspat + 16 * (ctx.L + 1) * ctx.nlat))
+    out = {"fields": nf, "bytes_per_launch": bytes_per, "peak_gbs": HBM_PEAK_GBS}
+    for name, fn in (("r2c", lambda: lib.sphswe_fft_analysis(h, spat.ptr, four.ptr, nf)),
+                     ("c2r", lambda: lib.sphswe_fft_synthesis(h, four.ptr, spat.ptr, nf))):
+        for _ in range(3):
+            ctx._check(fn())
+        ctx.mark(2)
+        for _ in range(reps):
+            ctx._check(fn())
+        ctx.mark(3)
+        ms = ctx.elapsed_ms(2, 3) / reps
+        gbs = bytes_per / (ms * 1e-3) / 1e9
+        out[name] = {"avg_ms": ms, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return out
+
+
 def bench_swe(ctx, steps, warmup):
     lon = np.arange(ctx.nphi) / ctx.nphi * 2 * np.pi
     hgt = gaussian_dam(lon, ctx.ct)
@@ -471,6 +497,7 @@ def main():
     ms_step = dev_ms / args.steps
     value = args.steps / (dev_ms / 1000.0)
     ms_syn, ms_ana, flops = bench_legendre_kernel(ctx)
+    fft = bench_fft_kernels(ctx)
     swe_rate, swe_dt = bench_swe(ctx, args.swe_steps, 2)
     achieved = flops / (ms_syn * 1e-3) / 1e12
     achieved_ana = flops / (ms_ana * 1e-3) / 1e12
@@ -515,7 +542,8 @@ def main():
                      "frac": achieved_ana / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "avg_ms": ms_ana, "flops_per_launch": flops,
                      "synthesis": {"kernel": "k_leg_synth (FP64 VALU)", "avg_ms": ms_syn, "achieved": achieved,
-                                   "frac": achieved / FP64_PEAK_TFLOPS}},
+                                   "frac": achieved / FP64_PEAK_TFLOPS},
+                     "fft_T511_9fields": dict(fft, bound="hbm", kernel="k_fft_real (reference layout)")},
         "cpu_baseline": cpu,
     }
     ctx.close()
