@@ -468,6 +468,102 @@ def measured_traffic():
         return None
 
 
+# split-scheme forms at T1023: (RK4 Legendre precision, REXI gradient-operator
+# precision, REXI factor storage, FP64 refinement sweep)
+SPLIT_FORMS = (("fp64", "fp64", "fp64", "fp64", 0),
+               ("mixed", "bf16x3", "fp32", "fp32", 0),
+               ("mixed_refined", "bf16x3", "fp32", "fp32", 1))
+
+
+def bench_split(nsteps=50, L=1023, M=188, dt=300.0, seg=2):
+    """Config 4: nonlinear SWE at T1023 on 1536 x 3072, Galewsky balanced jet
+    (no bump) plus a seeded height perturbation of 1 m with spectrum ~ l^-2,
+    dt = 300 s, split scheme per step: the linear equations by REXI (200 poles,
+    M = 188, h = 0.2, implicit viscosity 1e5) and the nonlinear remainder by one
+    RK4 step.  FP64 form, and mixed forms (BF16X3 Legendre contractions in the
+    RK4 part, FP32 REXI gradient operator, FP32 band factors with and without
+    one sweep of FP64 refinement).
+
+    The formulation diverges at this size: a straight FP64 run overflows within
+    a few steps (reported as finite_steps), from an m = 1 mode that the REXI
+    step of the scalar-wind equations amplifies (about 80x per step at T1023;
+    the oracle's REXI shows the same top-degree m = 1 growth at T63), and at
+    300 s the jet's advection of m > 530 is outside RK4's stability interval.
+    The nsteps timed steps therefore run as nsteps / seg segments of seg steps,
+    each from the initial state (a device-to-device copy of 8 MB inside the
+    timed region, against ~60 ms of work per step), so that every timed step
+    works on finite values.  Errors: relative L2 of h (= phi / g), u and v
+    after seg steps against the FP64 form."""
+    ctx = capi.Context(L)
+    tuning = ctx.autotune()
+    x0 = np.ascontiguousarray(workloads.galewsky_state(ctx, seed=1023, pert_amp=1.0))
+    d_x0 = ctx.to_device(x0)
+    res, finals = {}, {}
+    for name, sht, grad, band, refine in SPLIT_FORMS:
+        t0 = time.perf_counter()
+        rx = ctx.rexi(capi.rexi_params(dt=dt, rexi_h=0.2, rexi_M=M, half=1, visc=1e5,
+                                       precision=capi.PRECISIONS[grad], band_precision=capi.PRECISIONS[band],
+                                       refine=refine))
+        ctx.sync()
+        setup_s = time.perf_counter() - t0
+        prm = capi.swe_params(dt=dt, precision=capi.PRECISIONS[sht])
+        state = ctx.to_device(x0)
+        rx.split_step(prm, state, 1)
+        ctx.sync()
+        ctx.mark(12)
+        for _ in range(nsteps // seg):
+            ctx._check(ctx.lib.sphswe_memcpy_d2d(ctx.h, state.ptr, d_x0.ptr, x0.nbytes))
+            rx.split_step(prm, state, seg)
+        ctx.mark(13)
+        ms = ctx.elapsed_ms(12, 13)
+        steps = seg * (nsteps // seg)
+        finals[name] = ctx.to_host(state, np.complex128, (3, ctx.nlm))
+        if not np.all(np.isfinite(finals[name])):
+            raise RuntimeError("split SWE state became non-finite within %d steps (%s)" % (seg, name))
+        res[name] = {"steps_per_s": steps / (ms / 1000.0), "ms_per_step": ms / steps, "rexi_setup_s": setup_s,
+                     "rk4_legendre": sht, "rexi_gradient": grad, "rexi_factors": band, "refine": refine}
+        if name == "fp64":
+            # the two parts of a step, timed separately from the initial state
+            nl = capi.swe_params(dt=dt, visc=0.0, nonlinear=2, precision=capi.PREC_FP64)
+            ctx._check(ctx.lib.sphswe_memcpy_d2d(ctx.h, state.ptr, d_x0.ptr, x0.nbytes))
+            ctx.mark(12)
+            rx.step(state, seg)
+            ctx.mark(13)
+            ctx._check(ctx.lib.sphswe_memcpy_d2d(ctx.h, state.ptr, d_x0.ptr, x0.nbytes))
+            ctx.mark(14)
+            ctx.swe_rk4(nl, state, 1, seg)
+            ctx.mark(15)
+            res[name]["rexi_ms_per_step"] = ctx.elapsed_ms(12, 13) / seg
+            res[name]["rk4_ms_per_step"] = ctx.elapsed_ms(14, 15) / seg
+            # a straight run: steps until the state is non-finite
+            ctx._check(ctx.lib.sphswe_memcpy_d2d(ctx.h, state.ptr, d_x0.ptr, x0.nbytes))
+            finite = 0
+            for n in range(nsteps):
+                rx.split_step(prm, state, 1)
+                if not np.all(np.isfinite(ctx.to_host(state, np.complex128, (3, ctx.nlm)))):
+                    break
+                finite = n + 1
+            res[name]["finite_steps"] = finite
+        del state
+        rx.close()
+    ref = finals["fp64"]
+    for name in res:
+        if name == "fp64":
+            continue
+        for k, f in enumerate("huv"):
+            res[name][f + "_rel_l2_after_%d_steps" % seg] = _rel(finals[name][k], ref[k])[0]
+        # v is zero in the jet and small after seg steps: also against the wind scale |u|
+        res[name]["v_err_over_u_l2_after_%d_steps" % seg] = float(
+            np.linalg.norm(finals[name][2] - ref[2]) / np.linalg.norm(ref[1]))
+        res[name]["speedup_vs_fp64"] = res[name]["steps_per_s"] / res["fp64"]["steps_per_s"]
+    cfg = {"lmax": L, "nlat": ctx.nlat, "nphi": ctx.nphi, "poles": M + 12, "rexi_M": M, "rexi_h": 0.2, "dt": dt,
+           "steps": nsteps, "segment_steps": seg, "viscosity": 1e5,
+           "scheme": "Lie split: REXI (linear) then RK4 (nonlinear remainder)",
+           "initial": "Galewsky jet without bump + 1 m seeded perturbation ~ l^-2", "autotune": tuning}
+    ctx.close()
+    return {"config": cfg, "forms": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -476,6 +572,7 @@ def main():
     ap.add_argument("--swe-steps", type=int, default=50)
     ap.add_argument("--rexi-steps", type=int, default=10)
     ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--split-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch shapes)")
     ap.add_argument("--pmc-kernels", action="store_true",
@@ -507,6 +604,7 @@ def main():
     ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
     rexi_rate, rexi_cfg = bench_rexi(args.rexi_steps) if args.rexi_steps > 0 else (None, None)
     rexi_modes = bench_rexi_modes(args.rexi_steps) if args.rexi_steps > 0 else None
+    split = bench_split(args.split_steps) if args.split_steps > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(L_BENCH, ctx.nlat, ctx.nphi)
     line = {
         "metric": "SHT round-trips/sec at T511 (3 fields, FP64); SWE steps/sec at T511 in swe_steps_per_s",
@@ -536,6 +634,7 @@ def main():
         "rexi_steps_per_s": rexi_rate,
         "rexi_config": rexi_cfg,
         "mixed_rexi_T256_200poles": rexi_modes,
+        "split_T1023_galewsky": split,
         "roofline": {"bound": "mfma", "engine": "fp64 matrix cores (v_mfma_f64_16x16x4f64)",
                      "kernel": "k_leg_eprep + k_leg_anal (Legendre analysis, 3 fields, T511)",
                      "achieved": achieved_ana, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

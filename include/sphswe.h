@@ -230,6 +230,19 @@ int sphswe_rexi_create_poles(sphswe_ctx *ctx, const sphswe_rexi_params *p, int n
  * may be the same buffer. */
 int sphswe_rexi_apply(sphswe_ctx *ctx, sphswe_rexi *rexi, const double *d_in, double *d_out);
 
+/* nsteps steps of a split scheme for the nonlinear SWE: per step, the REXI
+ * step of the linear equations (sphswe_rexi_step: pole sum and implicit
+ * viscosity), then one RK4 step of the nonlinear remainder (sphswe_swe_rk4
+ * with nonlinear = 2 and no explicit viscosity; p->nonlinear and p->viscosity
+ * are ignored).  p->dt must be the REXI object's dt and a, g, Omega and h0 must
+ * agree (SPHSWE_ERR_ARG otherwise); p->precision selects the Legendre stage of
+ * the RK4 part, the REXI object its own precision forms.  No counterpart in
+ * the reference, whose REXI integrates the linear equations only
+ * (AppSWE.hpp:185-192, 336-565); its parts replace SWERexiSPH::solve and
+ * p_run_euler_timestep_update (AppSWE.hpp:574-659).  d_state: (h, u, v)
+ * spectral, 3*nlm complex, in place. */
+int sphswe_swe_split(sphswe_ctx *ctx, sphswe_rexi *rexi, const sphswe_swe_params *p, double *d_state, int nsteps);
+
 /* ---- banded complex solver (SPHSolverComplex<T> with SPHMatrixComplex storage,
  *      SPHSolverComplex.hpp:50-511; BandedMatrixSolver::solve_diagBandedInverse_Carray
  *      + zgbsv, BandedMatrixSolver.hpp:235-434).  Rows in NCompact order, halo 1..4. */

@@ -86,6 +86,7 @@ _SIGS = {
     "sphswe_rexi_coefficients": [_d, _i, _i, _ip, _dp, _dp, _dp],
     "sphswe_rexi_create_poles": [_vp, ctypes.POINTER(RexiParams), _i, _dp, _dp, ctypes.POINTER(_vp)],
     "sphswe_rexi_apply": [_vp, _vp, _vp, _vp],
+    "sphswe_swe_split": [_vp, _vp, ctypes.POINTER(SweParams), _vp, _i],
     "sphswe_spat_real_to_cplx": [_vp, _vp, _vp, _i],
     "sphswe_spat_cplx_part": [_vp, _vp, _vp, _i, _i],
     "sphswe_band_create": [_vp, _i, ctypes.POINTER(_vp)],
@@ -378,6 +379,11 @@ class Rexi:
 
     def step(self, state, nsteps=1):
         self.ctx._check(self.ctx.lib.sphswe_rexi_step(self.ctx.h, self.h, state.ptr, nsteps))
+
+    def split_step(self, params, state, nsteps=1):
+        """nsteps split steps: this REXI step of the linear equations, then one
+        RK4 step of the nonlinear remainder with params (SweParams)."""
+        self.ctx._check(self.ctx.lib.sphswe_swe_split(self.ctx.h, self.h, ctypes.byref(params), state.ptr, nsteps))
 
     def apply(self, d_in, d_out):
         """sum_n Re(beta_n x_n) for real spectra (phi0, u0, v0) -> (phi, u, v)."""

@@ -1413,3 +1413,25 @@ extern "C" int sphswe_rexi_step(sphswe_ctx *ctx, sphswe_rexi *r, double *d_state
 	}
 	return SPHSWE_OK;
 }
+
+// Lie splitting: the linear equations by REXI (with its implicit viscosity),
+// then the nonlinear remainder (flux of h - h0, advection) by one RK4 step
+extern "C" int sphswe_swe_split(sphswe_ctx *ctx, sphswe_rexi *r, const sphswe_swe_params *p, double *d_state,
+				int nsteps)
+{
+	if (!ctx || !r || !p || !d_state || nsteps < 0 || r->ctx != ctx) return SPHSWE_ERR_ARG;
+	const sphswe_rexi_params &q = r->prm;
+	if (p->dt != q.dt)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_split: RK4 dt %g differs from the REXI dt %g", p->dt, q.dt);
+	if (p->earth_radius != q.earth_radius || p->gravitation != q.gravitation ||
+	    p->coriolis_omega != q.coriolis_omega || p->h0 != q.h0)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_split: a, g, Omega and h0 must equal the REXI object's");
+	sphswe_swe_params nl = *p;
+	nl.nonlinear = 2;
+	nl.viscosity = 0.0;
+	for (int step = 0; step < nsteps; step++) {
+		if (int rc = sphswe_rexi_step(ctx, r, d_state, 1)) return rc;
+		if (int rc = sphswe_swe_rk4(ctx, &nl, d_state, 1, 1)) return rc;
+	}
+	return SPHSWE_OK;
+}
