@@ -815,19 +815,25 @@ __global__ void k_gop_sym(double *__restrict__ M, float *__restrict__ M32, const
 }
 
 // Y[rows of (m, parity)][cols] = M(|m|, parity) X[...]: one wavefront per
-// (m, parity, up to GOP_RT 16-row tiles, 16 real columns), so that every X row
-// is read once per column strip and each k step issues GOP_RT MFMAs on one B
-// operand.  A = M (symmetric: the 16 rows of one column k are contiguous),
-// B = X rows.  FP64: v_mfma_f64_16x16x4f64, D[i][n] in lane n + 16 (i % 4),
-// element i / 4.  FP32: v_mfma_f32_16x16x4f32 with M and X rounded, D[i][n] in
-// lane n + 16 (i / 4), element i % 4.  A[i][k] in lane i + 16 k and B[k][n] in
-// lane n + 16 k for both.
+// (m, parity, up to GOP_RT 16-row tiles, CS strips of 16 real columns).  Each
+// k step loads GOP_RT A values and CS B values per lane and issues
+// GOP_RT * CS = 16 MFMAs on them (FP64: 128 accumulator registers, 2 waves
+// per SIMD; FP32: 64, 3 waves), so M is read once per CS strips and every X
+// row once per wavefront.  A = M (symmetric: the 16 rows of one column k are
+// contiguous), B = X rows.  FP64: v_mfma_f64_16x16x4f64, D[i][n] in lane
+// n + 16 (i % 4), element i / 4.  FP32: v_mfma_f32_16x16x4f32 with M and X
+// rounded, D[i][n] in lane n + 16 (i / 4), element i % 4.  A[i][k] in lane
+// i + 16 k and B[k][n] in lane n + 16 k for both.
+template <typename T>
+constexpr int gop_cs() { return 2; }
+
 template <typename T>
 __global__ __launch_bounds__(64) void k_gop_apply(const T *__restrict__ M, const long *__restrict__ goff,
 						  const int4 *__restrict__ tiles, const int *__restrict__ moff,
 						  const double *__restrict__ X, double *__restrict__ Y, int L, int ncol)
 {
 	constexpr bool F64 = sizeof(T) == 8;
+	constexpr int CS = gop_cs<T>();
 	typedef std::conditional_t<F64, gd4v, gf4v> acc_t;
 	const int4 tl = tiles[blockIdx.x];
 	const int m = tl.x, par = tl.y, r0 = tl.z;
@@ -836,20 +842,29 @@ __global__ __launch_bounds__(64) void k_gop_apply(const T *__restrict__ M, const
 	const T *Mm = M + goff[2 * am + par];
 	const long xrow0 = (long)moff[m + L] + par;   // row of parity index 0; parity index q is row xrow0 + 2 q
 	const int lane = threadIdx.x, i = lane & 15, k = lane >> 4;
-	const int col = blockIdx.y * 16 + i;
-	const bool cv = col < ncol;
-	acc_t acc[GOP_RT];
+	int col[CS];
+	bool cv[CS];
 #pragma unroll
-	for (int t = 0; t < GOP_RT; t++) acc[t] = acc_t{0, 0, 0, 0};
+	for (int c = 0; c < CS; c++) {
+		col[c] = (blockIdx.y * CS + c) * 16 + i;
+		cv[c] = col[c] < ncol;
+	}
+	acc_t acc[GOP_RT][CS];
+#pragma unroll
+	for (int t = 0; t < GOP_RT; t++)
+#pragma unroll
+		for (int c = 0; c < CS; c++) acc[t][c] = acc_t{0, 0, 0, 0};
 	// UN k steps per pass: their loads are issued together
-	constexpr int UN = F64 ? 4 : 2;
+	constexpr int UN = 2;
 	for (int kk = 0; kk < np; kk += 4 * UN) {
-		T b[UN], av[UN][GOP_RT];
+		T b[UN][CS], av[UN][GOP_RT];
 #pragma unroll
 		for (int u = 0; u < UN; u++) {
 			const int q = kk + 4 * u + k;
 			const bool qv = q < np;
-			b[u] = (qv && cv) ? (T)X[(xrow0 + 2 * (long)q) * ncol + col] : (T)0;
+			const double *xq = X + (xrow0 + 2 * (long)(qv ? q : 0)) * ncol;
+#pragma unroll
+			for (int c = 0; c < CS; c++) b[u][c] = (qv && cv[c]) ? (T)xq[col[c]] : (T)0;
 			const T *mq = Mm + (long)(qv ? q : 0) * np + r0 + i;
 #pragma unroll
 			for (int t = 0; t < GOP_RT; t++)
@@ -860,19 +875,27 @@ __global__ __launch_bounds__(64) void k_gop_apply(const T *__restrict__ M, const
 #pragma unroll
 			for (int t = 0; t < GOP_RT; t++) {
 				if (t < nt) {
-					if constexpr (F64) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][t], b[u], acc[t], 0, 0, 0);
-					else acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u][t], b[u], acc[t], 0, 0, 0);
+#pragma unroll
+					for (int c = 0; c < CS; c++) {
+						if constexpr (F64)
+							acc[t][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][t], b[u][c], acc[t][c], 0, 0, 0);
+						else
+							acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u][t], b[u][c], acc[t][c], 0, 0, 0);
+					}
 				}
 			}
 	}
-	if (!cv) return;
 #pragma unroll
 	for (int t = 0; t < GOP_RT; t++) {
 		if (t >= nt) break;
 #pragma unroll
 		for (int e = 0; e < 4; e++) {
 			const int row = r0 + 16 * t + (F64 ? k + 4 * e : 4 * k + e);
-			if (row < np) Y[(xrow0 + 2 * (long)row) * ncol + col] = (double)acc[t][e];
+			if (row >= np) continue;
+			double *yr = Y + (xrow0 + 2 * (long)row) * ncol;
+#pragma unroll
+			for (int c = 0; c < CS; c++)
+				if (cv[c]) yr[col[c]] = (double)acc[t][c][e];
 		}
 	}
 }
@@ -1347,7 +1370,8 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 	// applied to the 2P columns (i m phi | D' phi) of every pole at once
 	{
 		const int ncol = 4 * P;   // real columns
-		dim3 gg(r->ngtiles, (ncol + 15) / 16);
+		const int cs = r->prm.precision == SPHSWE_PREC_FP64 ? gop_cs<double>() : gop_cs<float>();
+		dim3 gg(r->ngtiles, (ncol + 16 * cs - 1) / (16 * cs));
 		if (r->prm.precision == SPHSWE_PREC_FP64)
 			k_gop_apply<double><<<gg, 64, 0, st>>>(r->d_gop, r->d_goff, r->d_gtiles, r->d_moff,
 							       (const double *)r->d_X, (double *)r->d_G, L, ncol);
