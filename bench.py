@@ -10,7 +10,9 @@ same round trip with the Legendre contractions in FP32, BF16 (hi/lo) and BF16
 (Gaussian dam, dt = 0.001 a / nlat = 8.296 s) in FP64 and in each reduced mode
 with the geopotential error after 100 steps,
 the linear REXI rate at T256 (200 poles, dt = 3600 s, seeded (phi, vort, div)
-start state), the roofline of the dominant kernel (the Legendre analysis)
+start state) in FP64 and its precision forms, the T127 ensemble and the batched
+T511 form, the split scheme at T1023 (REXI linear part, RK4 nonlinear
+remainder, Galewsky jet), the roofline of the dominant kernel (the Legendre analysis)
 measured with HIP events on the library's stream, and the CPU oracle timed on a
 bounded sample.
 """
@@ -573,6 +575,8 @@ def main():
     ap.add_argument("--rexi-steps", type=int, default=10)
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--split-steps", type=int, default=50)
+    ap.add_argument("--batched-members", type=int, default=16,
+                    help="members of the batched T511 leg (0: skip the leg)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch shapes)")
     ap.add_argument("--pmc-kernels", action="store_true",
@@ -600,7 +604,8 @@ def main():
     achieved_ana = flops / (ms_ana * 1e-3) / 1e12
     mixed_sht = bench_sht_modes(ctx, max(args.steps // 4, 10), 3)
     mixed_swe = bench_swe_modes(ctx) if args.swe_steps > 0 else None
-    batched_swe = bench_swe_batched(ctx) if args.swe_steps > 0 else None
+    batched_swe = (bench_swe_batched(ctx, nmem=args.batched_members)
+                   if args.swe_steps > 0 and args.batched_members > 0 else None)
     ensemble = bench_ensemble(args.ensemble_steps) if args.ensemble_steps > 0 else None
     rexi_rate, rexi_cfg = bench_rexi(args.rexi_steps) if args.rexi_steps > 0 else (None, None)
     rexi_modes = bench_rexi_modes(args.rexi_steps) if args.rexi_steps > 0 else None
@@ -629,7 +634,7 @@ def main():
                        "viscosity": 1e5},
         "mixed_sht_T511_3fields": mixed_sht,
         "mixed_swe_T511_dam_100steps": mixed_swe,
-        "batched_mixed_swe_T511_16members_100steps": batched_swe,
+        "batched_mixed_swe_T511_100steps": batched_swe,
         "ensemble_T127_64members": ensemble,
         "rexi_steps_per_s": rexi_rate,
         "rexi_config": rexi_cfg,
