@@ -142,7 +142,9 @@ enum {
 	SPHSWE_SPAT_DIV_ST = 3,       /* out = a / st */
 	SPHSWE_SPAT_MUL_MU = 4,       /* out = a * mu */
 	SPHSWE_SPAT_SCALE = 5,        /* out = s * a */
-	SPHSWE_SPAT_DIV_COSLAT = 6    /* out = a / cos(lat[j]) */
+	SPHSWE_SPAT_DIV_COSLAT = 6,   /* out = a / cos(lat[j]) */
+	SPHSWE_SPAT_DIV_ST2 = 7       /* out = a / (st * st): the cos^2 phi of the Robert divergences
+				       * (SPHOperators.hpp:129-172) */
 };
 int sphswe_spat_op(sphswe_ctx *ctx, int op, const double *d_a, const double *d_b, double *d_out,
 		   double s, int nfields);
@@ -166,6 +168,9 @@ typedef struct {
 				  * (full minus linear tendency), for a split scheme whose linear
 				  * part is integrated by REXI (no counterpart in the reference) */
 	int precision;           /* SPHSWE_PREC_* for the Legendre stage */
+	int robert_functions;    /* SimVars::use_robert_functions: with nonlinear = 0, u and v are
+				  * Robert functions (u cos phi, v cos phi) and the tendency is
+				  * AppSWE.hpp:606-620; ignored by the nonlinear equations, as there */
 } sphswe_swe_params;
 
 /* Advance nmembers independent states by nsteps RK4 steps.  d_state holds, per

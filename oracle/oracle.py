@@ -181,9 +181,10 @@ class Oracle:
         return out
 
     def swe_rk4_run(self, params, h, u, v, nsteps):
-        """params dict: a, g, omega, h0, visc, nonlinear, dt; h/u/v physical."""
+        """params dict: a, g, omega, h0, visc, nonlinear, dt, and optionally robert
+        (1: the linear equations with Robert-function velocities); h/u/v physical."""
         p = np.array([params["a"], params["g"], params["omega"], params["h0"], params["visc"],
-                      float(params["nonlinear"]), params["dt"]])
+                      float(params["nonlinear"]), params["dt"], float(params.get("robert", 0))])
         outs = [np.empty(self.nlm, dtype=np.complex128) for _ in range(6)]
         h, u, v = (np.ascontiguousarray(x, dtype=np.float64).reshape(self.nspat) for x in (h, u, v))
         lib().orc_swe_rk4_run(self.h, _d(p), _d(h), _d(u), _d(v), nsteps,
@@ -192,7 +193,7 @@ class Oracle:
 
     def swe_tendency(self, params, h, u, v):
         p = np.array([params["a"], params["g"], params["omega"], params["h0"], params["visc"],
-                      float(params["nonlinear"]), params.get("dt", 0.0)])
+                      float(params["nonlinear"]), params.get("dt", 0.0), float(params.get("robert", 0))])
         ins = [np.ascontiguousarray(x, dtype=np.complex128) for x in (h, u, v)]
         outs = [np.empty(self.nlm, dtype=np.complex128) for _ in range(3)]
         lib().orc_swe_tendency(self.h, _d(p), *[_d(x.view(np.float64)) for x in ins],

@@ -641,7 +641,9 @@ static void real_part_to(ofield *cf, ofield *r)
 
 /* spat_update_lambda* helpers: kind 0 = divide by sqrt(1-mu^2) (gaussian
  * grid, mu = ct), 1 = multiply by st, 2 = divide by st, 3 = divide by
- * cos(lat[j]) (SPHOperatorsComplex.hpp:326-333) */
+ * cos(lat[j]) (SPHOperatorsComplex.hpp:326-333), 4 = divide by st*st (the
+ * cos-phi grid of the Robert divergences, SPHOperators.hpp:129-172;
+ * spat_update_lambda_cosphi_grid is the cogaussian grid, SPHData.hpp:756-761) */
 static void spat_metric(ofield *f, int kind)
 {
 	ocfg *c = f->c;
@@ -654,8 +656,9 @@ static void spat_metric(ofield *f, int kind)
 			else if (kind == 1) s = c->st[j];
 			else if (kind == 2) s = 1.0 / c->st[j];
 			else s = 1.0 / cos(c->lat[j]);
-			if (kind == 0 || kind == 2 || kind == 3) {
-				double d = (kind == 0) ? sqrt(1.0 - mu * mu) : (kind == 2 ? c->st[j] : cos(c->lat[j]));
+			if (kind == 0 || kind == 2 || kind == 3 || kind == 4) {
+				double d = (kind == 0) ? sqrt(1.0 - mu * mu)
+					   : (kind == 2 ? c->st[j] : (kind == 4 ? c->st[j] * c->st[j] : cos(c->lat[j])));
 				if (f->is_cplx) f->cspat[k] /= d; else f->spat[k] /= d;
 			} else {
 				if (f->is_cplx) f->cspat[k] *= s; else f->spat[k] *= s;
@@ -742,6 +745,27 @@ static ofield *div_lat(ofield *a)
 
 static ofield *div_lon(ofield *a) { return grad_lon(a); }
 
+/* Robert-function operators (SPHOperators.hpp:124-208): the divergences
+ * divide by cos^2 phi in physical space, the gradients stay spectral */
+static ofield *robert_div_lon(ofield *a)
+{
+	ofield *o = spec_op(a, 0);
+	req_spat(o);
+	spat_metric(o, 4);
+	return o;
+}
+
+static ofield *robert_div_lat(ofield *a)
+{
+	ofield *o = spec_op(a, 1);
+	req_spat(o);
+	spat_metric(o, 4);
+	return o;
+}
+
+static ofield *robert_grad_lon(ofield *a) { return spec_op(a, 0); }
+static ofield *robert_grad_lat(ofield *a) { return spec_op(a, 1); }
+
 /* vort = grad_lon(v) - grad_lat(u) (SPHOperators.hpp:507-513) */
 static ofield *op_vort(ofield *u, ofield *v)
 {
@@ -803,6 +827,7 @@ void orc_phys_op(ocfg *c, int which, const double *in, double *out)
 typedef struct {
 	double a, g, omega, h0, visc;
 	int nonlinear;
+	int robert;    /* SimVars::use_robert_functions; the linear equations only */
 } oswe;
 
 static ofield *g_fdata;
@@ -847,6 +872,15 @@ static void tendency(const oswe *P, ofield *ih, ofield *iu, ofield *iv, ofield *
 			ofield *g4 = grad_lat(iv);
 			ofield *p4 = mul(iv, g4);
 			isub(ov, scale(add(p3, p4), 1.0 / P->a));
+		}
+	} else if (P->robert) {
+		/* linear, velocities as Robert functions (AppSWE.hpp:606-620) */
+		assign(oh, scale(neg(add(robert_div_lon(iu), robert_div_lat(iv))), P->h0 / P->a));
+		assign(ou, scale(neg(robert_grad_lon(ih)), P->g / P->a));
+		assign(ov, scale(neg(robert_grad_lat(ih)), P->g / P->a));
+		if (P->omega != 0) {
+			iadd(ou, coriolis(iv, P->omega));
+			isub(ov, coriolis(iu, P->omega));
 		}
 	} else {
 		/* linear (AppSWE.hpp:591-603) */
@@ -928,7 +962,7 @@ static void make_fdata(ocfg *c)
 void orc_swe_rk4_run(ocfg *c, const double *params, const double *h0s, const double *u0s, const double *v0s,
 		     int nsteps, cplx *out_h, cplx *out_u, cplx *out_v, cplx *out_phi, cplx *out_vort, cplx *out_div)
 {
-	oswe P = {params[0], params[1], params[2], params[3], params[4], (int)params[5]};
+	oswe P = {params[0], params[1], params[2], params[3], params[4], (int)params[5], (int)params[7]};
 	double dt = params[6];
 	make_fdata(c);
 	ofield *h = field_alloc(c, 0, 0), *u = field_alloc(c, 0, 0), *v = field_alloc(c, 0, 0);
@@ -960,7 +994,7 @@ void orc_swe_rk4_run(ocfg *c, const double *params, const double *h0s, const dou
 void orc_swe_tendency(ocfg *c, const double *params, const cplx *h, const cplx *u, const cplx *v,
 		      cplx *oh, cplx *ou, cplx *ov)
 {
-	oswe P = {params[0], params[1], params[2], params[3], params[4], (int)params[5]};
+	oswe P = {params[0], params[1], params[2], params[3], params[4], (int)params[5], (int)params[7]};
 	make_fdata(c);
 	ofield *fh = T(c), *fu = T(c), *fv = T(c);
 	memcpy(fh->spec, h, sizeof(cplx) * c->nlm); fh->spec_valid = 1;

@@ -19,9 +19,7 @@ PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16, "bf16x3":
 # spectral operators (sphswe_spec_op / _cplx)
 OP_IDENTITY, OP_DIFF_LON, OP_ONE_MINUS_MU2_DMU, OP_MU, OP_MU2, OP_LAPLACE, OP_HELMHOLTZ = range(7)
 # physical operators (sphswe_spat_op / _cplx)
-SPAT_MUL, SPAT_DIV_COSPHI, SPAT_MUL_ST, SPAT_DIV_ST, SPAT_MUL_MU, SPAT_SCALE, SPAT_DIV_COSLAT = range(7)
-(OP_IDENTITY, OP_DIFF_LON, OP_ONE_MINUS_MU2_DMU, OP_MU, OP_MU2, OP_LAPLACE, OP_HELMHOLTZ) = range(7)
-(SPAT_MUL, SPAT_DIV_COSPHI, SPAT_MUL_ST, SPAT_DIV_ST, SPAT_MUL_MU, SPAT_SCALE, SPAT_DIV_COSLAT) = range(7)
+SPAT_MUL, SPAT_DIV_COSPHI, SPAT_MUL_ST, SPAT_DIV_ST, SPAT_MUL_MU, SPAT_SCALE, SPAT_DIV_COSLAT, SPAT_DIV_ST2 = range(8)
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -34,7 +32,7 @@ _fp = ctypes.POINTER(ctypes.c_float)
 
 class SweParams(ctypes.Structure):
     _fields_ = [("earth_radius", _d), ("gravitation", _d), ("coriolis_omega", _d), ("h0", _d),
-                ("viscosity", _d), ("dt", _d), ("nonlinear", _i), ("precision", _i)]
+                ("viscosity", _d), ("dt", _d), ("nonlinear", _i), ("precision", _i), ("robert_functions", _i)]
 
 
 class RexiParams(ctypes.Structure):
@@ -407,8 +405,8 @@ def rexi_params(a=6.37122e6, g=9.80616, omega=7.292e-5, h0=10000.0, visc=1e5, dt
 
 
 def swe_params(a=6.37122e6, g=9.80616, omega=7.292e-5, h0=10000.0, visc=1e5, dt=120.0, nonlinear=1,
-               precision=PREC_FP64):
-    return SweParams(a, g, omega, h0, visc, dt, nonlinear, precision)
+               precision=PREC_FP64, robert=0):
+    return SweParams(a, g, omega, h0, visc, dt, nonlinear, precision, robert)
 
 
 class Band:

@@ -235,6 +235,7 @@ __global__ void k_spat_op(int op, const T *__restrict__ a, const T *__restrict__
 	case SPHSWE_SPAT_DIV_ST: out[i] = tdiv(x, st[j]); break;
 	case SPHSWE_SPAT_MUL_MU: out[i] = tmul(x, ct[j]); break;
 	case SPHSWE_SPAT_SCALE: out[i] = tmul(x, s); break;
+	case SPHSWE_SPAT_DIV_ST2: out[i] = tdiv(x, st[j] * st[j]); break;
 	default: out[i] = tdiv(x, coslat[j]); break;
 	}
 }
@@ -244,7 +245,7 @@ __global__ void k_spat_op(int op, const T *__restrict__ a, const T *__restrict__
 extern "C" int sphswe_spat_op(sphswe_ctx *ctx, int op, const double *d_a, const double *d_b, double *d_out, double s,
 			      int nfields)
 {
-	if (!ctx || !d_a || !d_out || nfields < 0 || op < 0 || op > SPHSWE_SPAT_DIV_COSLAT) return SPHSWE_ERR_ARG;
+	if (!ctx || !d_a || !d_out || nfields < 0 || op < 0 || op > SPHSWE_SPAT_DIV_ST2) return SPHSWE_ERR_ARG;
 	if (op == SPHSWE_SPAT_MUL && !d_b) return SPHSWE_ERR_ARG;
 	long total = (long)ctx->nspat * nfields;
 	if (total == 0) return SPHSWE_OK;
@@ -257,7 +258,7 @@ extern "C" int sphswe_spat_op(sphswe_ctx *ctx, int op, const double *d_a, const 
 extern "C" int sphswe_spat_op_cplx(sphswe_ctx *ctx, int op, const double *d_a, const double *d_b, double *d_out,
 				   double s, int nfields)
 {
-	if (!ctx || !d_a || !d_out || nfields < 0 || op < 0 || op > SPHSWE_SPAT_DIV_COSLAT) return SPHSWE_ERR_ARG;
+	if (!ctx || !d_a || !d_out || nfields < 0 || op < 0 || op > SPHSWE_SPAT_DIV_ST2) return SPHSWE_ERR_ARG;
 	if (op == SPHSWE_SPAT_MUL && !d_b) return SPHSWE_ERR_ARG;
 	long total = (long)ctx->nspat * nfields;
 	if (total == 0) return SPHSWE_OK;

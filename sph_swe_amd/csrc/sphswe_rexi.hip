@@ -1453,6 +1453,7 @@ extern "C" int sphswe_swe_split(sphswe_ctx *ctx, sphswe_rexi *r, const sphswe_sw
 	sphswe_swe_params nl = *p;
 	nl.nonlinear = 2;
 	nl.viscosity = 0.0;
+	nl.robert_functions = 0;
 	for (int step = 0; step < nsteps; step++) {
 		if (int rc = sphswe_rexi_step(ctx, r, d_state, 1)) return rc;
 		if (int rc = sphswe_swe_rk4(ctx, &nl, d_state, 1, 1)) return rc;

@@ -14,7 +14,10 @@
 //            tendencies (grad_lon/grad_lat divide by sqrt(1-mu^2), f = 2 Omega mu);
 //            nonlinear = 0 the linear equations (h0 for h in the fluxes, no
 //            advection), nonlinear = 2 the difference of the two (flux of
-//            h - h0 and advection only: the part a split scheme leaves to RK4)
+//            h - h0 and advection only: the part a split scheme leaves to RK4);
+//            linear with Robert functions (AppSWE.hpp:606-620): the h
+//            tendency -(h0/a)(im U + D'V)/cos^2 phi is complete here, the u, v
+//            ones have no metric division, and levels 2-3 are skipped
 //   level 1  analysis of 4 fields, im* and D' on the two flux fields
 //   level 2  synthesis of 2 fields, divide by sqrt(1-mu^2) (and st)
 //   level 3  analysis of 1 field (the h tendency)
@@ -85,6 +88,13 @@ __global__ void k_phys1(const double *__restrict__ P9, double *__restrict__ P4, 
 		double tv = -ga * (Hm / c) - two_omega * (U * mu) - ((Vl / c) * U + (Vm / c) * V) * ia;
 		o[2 * nspat] = tu;
 		o[3 * nspat] = tv;
+	} else if (nonlinear == 3) {
+		// linear, Robert functions: robert_div_lon(U) + robert_div_lat(V) divide
+		// i m U and D'V by cos^2 phi (= st^2), robert_grad_lon/lat(h) = i m h, D'h
+		o[0] = -((Ul + Vm) / (st[j] * st[j])) * h0;   // h0 carries 1/a here
+		o[nspat] = 0.0;
+		o[2 * nspat] = -ga * Hl + two_omega * (V * mu);
+		o[3 * nspat] = -ga * Hm - two_omega * (U * mu);
 	} else if (nonlinear == 2) {
 		// the full tendency minus its linear part: flux of h - h0, advection
 		const double Hp = H - h0;
@@ -142,7 +152,8 @@ __global__ void k_phys2(const double *__restrict__ P2, double *__restrict__ P1, 
 // tendencies k = (T_h, T_u, T_v) + nu/a^2 laplace(Y) and the RK4 bookkeeping
 //   stage < 3: acc (=|+=) b_s k ; Ynext = X + dt*(a_{s+1} k)
 //   stage 3:   X += dt*(acc + b_3 k)
-__global__ void k_stage(const double2 *__restrict__ TH, const double2 *__restrict__ S4, const double2 *__restrict__ Y,
+__global__ void k_stage(const double2 *__restrict__ TH, int th_stride, const double2 *__restrict__ S4,
+			const double2 *__restrict__ Y,
 			double2 *__restrict__ X, double2 *__restrict__ ACC, double2 *__restrict__ YN,
 			const int *__restrict__ mof, int L, int nlm, int nmem, double visc, int stage, double bs,
 			double anext, double dt)
@@ -160,7 +171,7 @@ __global__ void k_stage(const double2 *__restrict__ TH, const double2 *__restric
 	double2 *yn = YN + (long)mem * 3 * nlm;
 	const double2 *s4 = S4 + (long)mem * 4 * nlm;
 	double2 t[3];
-	t[0] = TH[(long)mem * nlm + idx];
+	t[0] = TH[(long)mem * th_stride * nlm + idx];
 	t[1] = s4[2 * nlm + idx];
 	t[2] = s4[3 * nlm + idx];
 #pragma unroll
@@ -265,6 +276,9 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 	const double bcoef[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
 	const double anext[3] = {0.5, 0.5, 1.0};
 	const int prec = p->precision;   // of the Legendre contractions; everything else is FP64
+	// 3: the linear equations with Robert functions (the reference reads its flag
+	// in the linear branch only)
+	const int mode = (p->nonlinear == 0 && p->robert_functions) ? 3 : p->nonlinear;
 	const int L = ctx->L, nlat = ctx->nlat;
 	hipStream_t st = ctx->stream;
 	int rc;
@@ -284,18 +298,26 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 		if ((rc = sphswe_leg_synth(ctx, b.S9, b.F, 9 * nm, prec))) return rc;
 		if ((rc = c2r(b.F, b.P9, 9 * nm))) return rc;
 		k_phys1<<<g1(nspat * nm), 256, 0, st>>>(b.P9, b.P4, ctx->d_ct, ctx->d_st, ctx->d_sq, nlat, ctx->nphi, rm,
-							 nspat, nm, ga, two_omega, ia, p->nonlinear, p->h0);
+							 nspat, nm, ga, two_omega, ia, mode, mode == 3 ? p->h0 / a : p->h0);
 		if ((rc = r2c(b.P4, b.F, 4 * nm))) return rc;
 		if ((rc = sphswe_leg_anal(ctx, b.F, b.S4, 4 * nm, prec))) return rc;
-		k_prep2<<<g1(nlm * nm), 256, 0, st>>>(b.S4, b.S2, ctx->d_mofidx, L, (int)nlm, nm);
-		if ((rc = sphswe_leg_synth(ctx, b.S2, b.F, 2 * nm, prec))) return rc;
-		if ((rc = c2r(b.F, b.P2, 2 * nm))) return rc;
-		k_phys2<<<g1(nspat * nm), 256, 0, st>>>(b.P2, b.P1, ctx->d_st, ctx->d_sq, nlat, ctx->nphi, rm, nspat, nm,
-							 p->nonlinear ? ia : p->h0 / a);
-		if ((rc = r2c(b.P1, b.F, nm))) return rc;
-		if ((rc = sphswe_leg_anal(ctx, b.F, b.TH, nm, prec))) return rc;
-		k_stage<<<g1(nlm * nm), 256, 0, st>>>(b.TH, b.S4, Y, X, b.ACC, YN, ctx->d_mofidx, L, (int)nlm, nm, visc, s,
-						       bcoef[s], s < 3 ? anext[s] : 0.0, dt);
+		// the h tendency: field 0 of each member's four (Robert functions: complete
+		// after level 1), else levels 2-3
+		const double2 *TH = b.S4;
+		int th_stride = 4;
+		if (mode != 3) {
+			k_prep2<<<g1(nlm * nm), 256, 0, st>>>(b.S4, b.S2, ctx->d_mofidx, L, (int)nlm, nm);
+			if ((rc = sphswe_leg_synth(ctx, b.S2, b.F, 2 * nm, prec))) return rc;
+			if ((rc = c2r(b.F, b.P2, 2 * nm))) return rc;
+			k_phys2<<<g1(nspat * nm), 256, 0, st>>>(b.P2, b.P1, ctx->d_st, ctx->d_sq, nlat, ctx->nphi, rm, nspat,
+								 nm, p->nonlinear ? ia : p->h0 / a);
+			if ((rc = r2c(b.P1, b.F, nm))) return rc;
+			if ((rc = sphswe_leg_anal(ctx, b.F, b.TH, nm, prec))) return rc;
+			TH = b.TH;
+			th_stride = 1;
+		}
+		k_stage<<<g1(nlm * nm), 256, 0, st>>>(TH, th_stride, b.S4, Y, X, b.ACC, YN, ctx->d_mofidx, L, (int)nlm, nm,
+						       visc, s, bcoef[s], s < 3 ? anext[s] : 0.0, dt);
 		SPHSWE_LAUNCH_CHECK(ctx);
 		if (s < 3) {
 			Y = YN;
@@ -311,6 +333,8 @@ int check_params(sphswe_ctx *ctx, const sphswe_swe_params *p)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: precision %d not available", p->precision);
 	if (p->nonlinear < 0 || p->nonlinear > 2)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: nonlinear must be 0, 1 or 2");
+	if (p->robert_functions != 0 && p->robert_functions != 1)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: robert_functions must be 0 or 1");
 	return SPHSWE_OK;
 }
 

@@ -107,8 +107,9 @@ public:
 			}
 			simVars.use_nonlinear_equations = 0;
 		}
-		if (simVars.use_robert_functions || simVars.rexi_use_extended_modes) {
-			std::cerr << "Robert functions and extended REXI modes are not available in this build" << std::endl;
+		if ((simVars.use_robert_functions && simVars.use_rexi) || simVars.rexi_use_extended_modes) {
+			std::cerr << "Robert functions with REXI and extended REXI modes are not available in this build"
+				  << std::endl;
 			exit(1);
 		}
 
@@ -164,7 +165,16 @@ public:
 		(void)i_simulation_timestamp;
 		o_dt = simVars.timecontrol.current_timestep_size;
 		const double a = simVars.earth_radius, g = simVars.gravitation;
-		if (!simVars.use_nonlinear_equations) {
+		if (!simVars.use_nonlinear_equations && simVars.use_robert_functions) {
+			// linear equations, velocities as Robert functions (AppSWE.hpp:606-620)
+			o_h_t = -(op.robert_div_lon(i_u) + op.robert_div_lat(i_v)) * (simVars.h0 / a);
+			o_u_t = -op.robert_grad_lon(i_h) * (g / a);
+			o_v_t = -op.robert_grad_lat(i_h) * (g / a);
+			if (simVars.coriolis_omega != 0) {
+				o_u_t += f(i_v);
+				o_v_t -= f(i_u);
+			}
+		} else if (!simVars.use_nonlinear_equations) {
 			o_h_t = -(op.div_lon(i_u) + op.div_lat(i_v)) * (simVars.h0 / a);
 			o_u_t = -op.grad_lon(i_h) * (g / a);
 			o_v_t = -op.grad_lat(i_h) * (g / a);
@@ -241,7 +251,8 @@ private:
 					       (void **)&d_state), "state");
 		sphswe_swe_params p = {simVars.earth_radius, simVars.gravitation, simVars.coriolis_omega, simVars.h0,
 				       simVars.viscosity2, simVars.timecontrol.current_timestep_size,
-				       simVars.use_nonlinear_equations ? 1 : 0, SPHSWE_PREC_FP64};
+				       simVars.use_nonlinear_equations ? 1 : 0, SPHSWE_PREC_FP64,
+				       simVars.use_robert_functions ? 1 : 0};
 		simVars.timecontrol.current_simulation_time = 0;
 		while (simVars.timecontrol.current_simulation_time < simVars.timecontrol.max_simulation_time) {
 			output_if_due();

@@ -2,7 +2,9 @@
 // operators (diff_lon :30-54, spec_one_minus_mu_squared_diff_lat_mu :234-260,
 // mu :268-297, mu2 :303-333, laplace :482-498) and the composite operators with
 // their projection points (grad_lon :100-119, grad_lat :339-388, div_lon
-// :397-402, div_lat :411-439, vort :507-513, div :523-529).
+// :397-402, div_lat :411-439, vort :507-513, div :523-529) and the
+// Robert-function operators (robert_div_lon :129-145, robert_div_lat :155-172,
+// robert_grad_lon :186-193, robert_grad_lat :203-210).
 #ifndef SPHSWE_HOST_SPHOPERATORS_HPP
 #define SPHSWE_HOST_SPHOPERATORS_HPP
 
@@ -34,6 +36,17 @@ public:
 	// grad_lon(v) - grad_lat(u), without the metric term of the true vorticity
 	SPHData vort(const SPHData &u, const SPHData &v) const { return grad_lon(v) - grad_lat(u); }
 	SPHData div(const SPHData &u, const SPHData &v) const { return div_lon(u) + div_lat(v); }
+
+	// Robert functions U = u cos phi, V = v cos phi: the divergences divide by
+	// cos^2 phi in physical space (the cos-phi grid is the cogaussian st), the
+	// gradients stay spectral
+	SPHData robert_div_lon(const SPHData &i) const { return metric(diff_lon(i), SPHSWE_SPAT_DIV_ST2); }
+	SPHData robert_div_lat(const SPHData &i) const
+	{
+		return metric(spec_one_minus_mu_squared_diff_lat_mu(i), SPHSWE_SPAT_DIV_ST2);
+	}
+	SPHData robert_grad_lon(const SPHData &i) const { return diff_lon(i); }
+	SPHData robert_grad_lat(const SPHData &i) const { return spec_one_minus_mu_squared_diff_lat_mu(i); }
 
 private:
 	static SPHData spec(const SPHData &i, int op)

@@ -55,6 +55,30 @@ def test_app_rk4_vs_oracle(tmp_path, fused):
         assert f.readline().startswith("#TI")
 
 
+@pytest.mark.parametrize("fused", [1, 0])
+def test_app_rk4_robert_functions_vs_oracle(tmp_path, fused):
+    # N0 S1: the linear equations with Robert-function velocities (AppSWE.hpp:606-620)
+    L, dt, nsteps = 31, 120.0, 4
+    nlat, nphi = capi.grid_auto(L)
+    orc = O.Oracle(L, nlat, nphi)
+    dump = str(tmp_path / "state.bin")
+    _run(["T%d" % L, "P0", "Q4", "N0", "S1", "R%g" % dt, "E%g" % (dt * nsteps), "O1e12", "V100000",
+          "G%d" % fused, "Z" + dump], str(tmp_path))
+    out = _load(dump, orc.nlm)
+    h = _dam(orc)
+    z = np.zeros_like(h)
+    ref = orc.swe_rk4_run(dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, visc=1e5, nonlinear=0, robert=1, dt=dt),
+                          h, z, z, nsteps)
+    for k, name in enumerate("huv"):
+        e = rel_l2(out[k], ref[name])
+        print("rk4 robert", fused, name, e)
+        assert e < 1e-10
+    # REXI with Robert functions is refused by the program, as documented
+    r = subprocess.run([APP, "T%d" % L, "P0", "N0", "S1", "X0", "M8", "R3600", "E3600", "O1e12"], cwd=str(tmp_path),
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+
+
 @pytest.mark.parametrize("fused,omega", [(1, OMEGA), (0, OMEGA), (0, 0.0)])
 def test_app_rexi_vs_oracle(tmp_path, fused, omega):
     L, dt, nsteps, M = 31, 3600.0, 2, 8

@@ -213,3 +213,41 @@ def test_nonlinear_remainder_vs_oracle():
     with pytest.raises(capi.SphSweError):
         ctx.swe_rk4(capi.swe_params(dt=dt, nonlinear=3), st, 1, 1)
     ctx.close()
+
+
+@pytest.mark.parametrize("L,omega,visc", [(31, 7.292e-5, 1e5), (64, 7.292e-5, 0.0), (31, 0.0, 0.0)])
+def test_rk4_robert_functions_vs_oracle(L, omega, visc):
+    # linear equations with Robert-function velocities (AppSWE.hpp:606-620):
+    # h tendency complete after the first analysis level, no metric division in
+    # the u, v tendencies; 5 steps from a height bump with U, V bumps
+    print(f"rk4 Robert functions vs oracle L={L} omega={omega} visc={visc}", flush=True)
+    from tests.test_oracle_robert import _bump
+    ctx = capi.Context(L)
+    orc = O.Oracle(L, ctx.nlat, ctx.nphi)
+    h = (1e4 + 50.0 * _bump(orc, 0.2, 1.0, 0.3)).reshape(-1)
+    U = (10.0 * _bump(orc, -0.1, 2.5, 0.35)).reshape(-1)
+    V = (8.0 * _bump(orc, 0.3, 4.0, 0.3)).reshape(-1)
+    dt, nsteps = 120.0, 5
+    ref = orc.swe_rk4_run(dict(swe_prm(dt, visc=visc, omega=omega, nonlinear=0), robert=1), h, U, V, nsteps)
+    x0 = ctx.analysis(np.stack([h, U, V]))
+    st = ctx.to_device(x0)
+    ctx.swe_rk4(capi.swe_params(dt=dt, visc=visc, omega=omega, nonlinear=0, robert=1), st, 1, nsteps)
+    out = ctx.to_host(st, np.complex128, (3, ctx.nlm))
+    for k, f in enumerate("huv"):
+        e = rel_l2(out[k], ref[f])
+        print("robert", f, e)
+        assert e < TOL_SHORT, f
+    # the standard formulation differs (the flag is not ignored) ...
+    st = ctx.to_device(x0)
+    ctx.swe_rk4(capi.swe_params(dt=dt, visc=visc, omega=omega, nonlinear=0), st, 1, nsteps)
+    assert rel_l2(ctx.to_host(st, np.complex128, (3, ctx.nlm))[0], ref["h"]) > 1e-9
+    # ... and the nonlinear equations read no flag, as in the reference
+    runs = []
+    for rb in (0, 1):
+        st = ctx.to_device(x0)
+        ctx.swe_rk4(capi.swe_params(dt=dt, visc=visc, omega=omega, nonlinear=1, robert=rb), st, 1, 2)
+        runs.append(ctx.to_host(st, np.complex128, (3, ctx.nlm)))
+    assert np.array_equal(runs[0], runs[1])
+    with pytest.raises(capi.SphSweError):
+        ctx.swe_rk4(capi.swe_params(dt=dt, nonlinear=0, robert=2), st, 1, 1)
+    ctx.close()
