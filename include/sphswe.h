@@ -202,6 +202,9 @@ typedef struct {
 	int band_precision;      /* SPHSWE_PREC_FP64 or SPHSWE_PREC_FP32: storage of the LU factors
 				  * (factorised in FP64 on the host; the solves run in FP64) */
 	int refine;              /* FP32 factors: sweeps of FP64 iterative refinement (0 or 1) */
+	int robert_functions;    /* 1: SWERexiSPHRobert (SWERexiSPHRobert.hpp): u, v are Robert
+				  * functions; z4..z6robert, robert_div / robert_vort right-hand
+				  * sides, spectral gradients (no gradient operator); refine = 0 */
 } sphswe_rexi_params;
 
 /* REXI::setup for (h, M, half) and SWERexiSPH::setup for every pole: pole
@@ -256,7 +259,8 @@ enum {
 	SPHSWE_BAND_SCALAR = 0,           /* solver_component_scalar_phi */
 	SPHSWE_BAND_MU = 1,               /* solver_component_mu_phi */
 	SPHSWE_BAND_ONE_MINUS_MU2_DMU = 2, /* solver_component_one_minus_mu_mu_diff_mu_phi */
-	SPHSWE_BAND_REXI_Z1 = 11          /* solver_component_rexi_z1 .. z8 = 11 .. 18 */
+	SPHSWE_BAND_REXI_Z1 = 11,         /* solver_component_rexi_z1 .. z8 = 11 .. 18 */
+	SPHSWE_BAND_REXI_Z4ROBERT = 19    /* solver_component_rexi_z4robert .. z6robert = 19 .. 21 */
 };
 int sphswe_band_create(sphswe_ctx *ctx, int halo, sphswe_band **band);
 int sphswe_band_destroy(sphswe_band *band);

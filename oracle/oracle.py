@@ -203,7 +203,8 @@ class Oracle:
     def swe_rexi_run(self, params, h, u, v, nsteps):
         """params dict: a, g, omega, h0, visc, rexi_h, rexi_M, half, dt; h/u/v spectral."""
         p = np.array([params["a"], params["g"], params["omega"], params["h0"], params["visc"],
-                      params["rexi_h"], float(params["rexi_M"]), float(params["half"]), params["dt"]])
+                      params["rexi_h"], float(params["rexi_M"]), float(params["half"]), params["dt"],
+                      float(params.get("robert", 0))])
         ins = [np.ascontiguousarray(x, dtype=np.complex128) for x in (h, u, v)]
         outs = [np.empty(self.nlm, dtype=np.complex128) for _ in range(3)]
         lib().orc_swe_rexi_run(self.h, _d(p), *[_d(x.view(np.float64)) for x in ins], nsteps,
@@ -278,7 +279,7 @@ def gbsv(AB, kl, ku, b):
 def identities(n, m):
     out = np.empty(9)
     lib().orc_identities(float(n), float(m), _d(out))
-    return dict(zip("RSABCDE", out[:7]))
+    return dict(zip("RSABCDEGH", out))
 
 
 def ref_available():

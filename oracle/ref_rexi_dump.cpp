@@ -5,7 +5,8 @@
 //
 // usage: ref_rexi_dump rexi <h> <M> <half>     -> "re_alpha im_alpha re_beta_re im_beta_re re_beta_im im_beta_im"
 //                                                  per pole
-//        ref_rexi_dump ident <L>               -> "n m R S A B C D E" for 0<=m<=n<=L (and m<0)
+//        ref_rexi_dump ident <L>               -> "n m R S A B C D E G H" for 0<=m<=n<=L (and m<0);
+//                                                 H(0, m) is printed as nan (the reference asserts there)
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -28,10 +29,12 @@ int main(int argc, char **argv)
 		int L = std::atoi(argv[2]);
 		for (int n = 0; n <= L; n++)
 			for (int m = -n; m <= n; m++)
-				std::printf("%d %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", n, m,
+				std::printf("%d %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", n, m,
 					    SPHIdentities::R(n, m), SPHIdentities::S(n, m), SPHIdentities::A(n, m),
 					    SPHIdentities::B(n, m), SPHIdentities::C(n, m),
-					    n + 1 >= 0 ? SPHIdentities::D(n, m) : 0.0, SPHIdentities::E(n, m));
+					    n + 1 >= 0 ? SPHIdentities::D(n, m) : 0.0, SPHIdentities::E(n, m),
+					    SPHIdentities::G(n, m),
+					    n >= 1 ? SPHIdentities::H(n, m) : NAN);   // H(0, m) asserts on E(-1, m)
 		return 0;
 	}
 	std::fprintf(stderr, "usage: %s rexi <h> <M> <half> | ident <L>\n", argv[0]);

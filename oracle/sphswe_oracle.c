@@ -458,12 +458,14 @@ static double idB(double n, double m) { return idR(n - 1, m) * idS(n, m) + idS(n
 static double idC(double k, double m) { double n = k - 2.0; return idS(n + 1, m) * idS(n + 2, m); }
 static double idD(double k, double m) { double n = k + 1; return (2.0 * n + 1.0) * sqrt((n * n - m * m) / (4.0 * n * n - 1.0)); }
 static double idE(double n, double m) { (void)m; return -n; }
+static double idG(double k, double m) { double n = k + 1; return idD(n - 1, m) + idE(n, m) * idR(n - 1, m); }
+static double idH(double k, double m) { double n = k - 1; return idE(n, m) * idS(n + 1, m); }
 
 void orc_identities(double n, double m, double *out9)
 {
 	out9[0] = idR(n, m); out9[1] = idS(n, m); out9[2] = idA(n, m);
 	out9[3] = idB(n, m); out9[4] = idC(n, m); out9[5] = idD(n, m); out9[6] = idE(n, m);
-	out9[7] = 0; out9[8] = 0;
+	out9[7] = idG(n, m); out9[8] = idH(n, m);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1163,6 +1165,42 @@ void orc_mat_add(omat *M, int comp, double sre, double sim, double r)
 				row_add(M, n, m, 0, s * fac * idB(dn, dm));
 				row_add(M, n, m, +2, s * fac * idC(dn + 2, dm)); break;
 			}
+			/* Robert-function components (SPHSolverComplex.hpp:241-370) */
+			case 19: {   /* z4robert */
+				cplx fac = ir2 * s * (I * dm);
+				row_add(M, n, m, -2, -fac * idA(dn - 2, dm));
+				row_add(M, n, m, 0, -fac * idB(dn, dm) + fac);
+				row_add(M, n, m, +2, -fac * idC(dn + 2, dm)); break;
+			}
+			case 20: {   /* z5robert: every term on the -2 diagonal, as the reference adds them (:305-309) */
+				cplx fac = ir2 * s * (I * dm);
+				row_add(M, n, m, -2, -fac * (idA(dn - 2, dm) * idA(dn - 4, dm)));
+				row_add(M, n, m, -2, -fac * (idA(dn - 2, dm) * idB(dn - 2, dm) + idB(dn, dm) * idA(dn - 2, dm))
+						      + fac * idA(dn - 2, dm));
+				row_add(M, n, m, -2, -fac * (idA(dn - 2, dm) * idC(dn, dm) + idB(dn, dm) * idB(dn, dm)
+							     + idC(dn + 2, dm) * idA(dn, dm)) + fac * idB(dn, dm));
+				row_add(M, n, m, -2, -fac * (idB(dn, dm) * idC(dn + 2, dm) + idC(dn + 2, dm) * idB(dn + 2, dm))
+						      + fac * idC(dn + 2, dm));
+				row_add(M, n, m, -2, -fac * (idC(dn + 2, dm) * idC(dn + 4, dm))); break;
+			}
+			case 21: {   /* z6robert */
+				cplx s6 = ir2 * s * dm;
+				double gr = idG(dn - 1, dm) * idR(dn - 2, dm);
+				double gs = idG(dn - 1, dm) * idS(dn, dm) + idH(dn + 1, dm) * idR(dn, dm);
+				double hs = idH(dn + 1, dm) * idS(dn + 2, dm);
+				row_add(M, n, m, -2, s6 * gr);
+				row_add(M, n, m, 0, s6 * gs);
+				row_add(M, n, m, +2, s6 * hs);
+				row_add(M, n, m, -4, s6 * (-idA(dn - 4, dm) * gr));
+				row_add(M, n, m, -2, s6 * (-idB(dn - 2, dm) * gr));
+				row_add(M, n, m, 0, s6 * (-idC(dn, dm) * gr));
+				row_add(M, n, m, -2, s6 * (-idA(dn - 2, dm) * gs));
+				row_add(M, n, m, 0, s6 * (-idB(dn, dm) * gs));
+				row_add(M, n, m, +2, s6 * (-idC(dn + 2, dm) * gs));
+				row_add(M, n, m, 0, s6 * (-idA(dn, dm) * hs));
+				row_add(M, n, m, +2, s6 * (-idB(dn + 2, dm) * hs));
+				row_add(M, n, m, +4, s6 * (-idC(dn + 4, dm) * hs)); break;
+			}
 			default: break;
 			}
 		}
@@ -1275,13 +1313,15 @@ typedef struct {
 	cplx alpha, beta;
 	double r, ir, two_omega, avg_geo;
 	int coriolis;
+	int robert;    /* SWERexiSPHRobert (SWERexiSPHRobert.hpp) */
 	omat *Mphi, *Mvel;
 } orexi_pole;
 
 static void pole_setup(orexi_pole *p, ocfg *c, cplx alpha, cplx beta, double r, double omega,
-		       double avg_geo, double dt, int coriolis)
+		       double avg_geo, double dt, int coriolis, int robert)
 {
 	p->coriolis = coriolis;
+	p->robert = robert;
 	p->alpha = alpha / dt;
 	p->beta = beta / dt;
 	p->r = r; p->ir = 1.0 / r;
@@ -1296,9 +1336,10 @@ static void pole_setup(orexi_pole *p, ocfg *c, cplx alpha, cplx beta, double r, 
 	if (coriolis) {
 		ADD(p->Mphi, 12, 2.0 * to * to * a * a);
 		ADD(p->Mphi, 13, (to * to) * (to * to));
-		ADD(p->Mphi, 14, -G * a * to);
-		ADD(p->Mphi, 15, G / a * to * to * to);
-		ADD(p->Mphi, 16, G * 2.0 * to * to);
+		/* z4..z6, or their Robert forms (SWERexiSPHRobert.hpp setup) */
+		ADD(p->Mphi, robert ? 19 : 14, -G * a * to);
+		ADD(p->Mphi, robert ? 20 : 15, G / a * to * to * to);
+		ADD(p->Mphi, robert ? 21 : 16, G * 2.0 * to * to);
 	}
 	ADD(p->Mphi, 17, -G * a * a);
 	if (coriolis) ADD(p->Mphi, 18, -G * to * to);
@@ -1344,8 +1385,19 @@ static void pole_solve_c(orexi_pole *p, ofield *i_phi0, ofield *i_u0, ofield *i_
 	ofield *phi0 = cplx_from_real(i_phi0);
 	ofield *u0 = cplx_from_real(i_u0);
 	ofield *v0 = cplx_from_real(i_v0);
-	ofield *div0 = scale(op_div(u0, v0), p->ir);
-	ofield *eta0 = scale(op_vort(u0, v0), p->ir);
+	ofield *div0, *eta0;
+	if (p->robert) {
+		/* robert_div: one division by cos^2 phi of the spectral sum
+		 * (SPHOperatorsComplex.hpp:530-550); robert_vort stays spectral (:496-502) */
+		ofield *d = add(spec_op(u0, 0), spec_op(v0, 1));
+		req_spat(d);
+		spat_metric(d, 4);
+		div0 = scale(d, p->ir);
+		eta0 = scale(sub(spec_op(v0, 0), spec_op(u0, 1)), p->ir);
+	} else {
+		div0 = scale(op_div(u0, v0), p->ir);
+		eta0 = scale(op_vort(u0, v0), p->ir);
+	}
 	cplx a = p->alpha;
 	double to = p->two_omega, G = p->avg_geo;
 	ofield *phi, *u, *v;
@@ -1353,7 +1405,7 @@ static void pole_solve_c(orexi_pole *p, ofield *i_phi0, ofield *i_u0, ofield *i_
 		/* Fc_k = ((two_omega*ir)*grad_lat(mu)) * ( -(a^2 u0 - to^2 mu2(u0)) + 2 a to mu(v0) ):
 		 * the scalar product projects grad_lat(mu) to spectral space before the
 		 * physical product (left-associative evaluation of SWERexiSPH.hpp:183-186) */
-		ofield *gm = scale(grad_lat(mu), to * p->ir);
+		ofield *gm = scale(p->robert ? robert_grad_lat(mu) : grad_lat(mu), to * p->ir);
 		ofield *br = add(neg(sub(scale(u0, a * a), scale(spec_op(u0, 3), to * to))),
 				 scale(spec_op(v0, 2), 2.0 * a * to));
 		ofield *Fc = mul(gm, br);
@@ -1364,8 +1416,8 @@ static void pole_solve_c(orexi_pole *p, ofield *i_phi0, ofield *i_u0, ofield *i_
 		ofield *foo = add(t1, t2);
 		ofield *rhs = add(add(scale(foo, a * a), scale(spec_op(foo, 3), to * to)), scale(Fc, G / a));
 		phi = solve_field(p->Mphi, rhs);
-		ofield *A_ = add(u0, scale(grad_lon(phi), p->ir));
-		ofield *B_ = add(v0, scale(grad_lat(phi), p->ir));
+		ofield *A_ = add(u0, scale(p->robert ? robert_grad_lon(phi) : grad_lon(phi), p->ir));
+		ofield *B_ = add(v0, scale(p->robert ? robert_grad_lat(phi) : grad_lat(phi), p->ir));
 		ofield *rhsa = sub(scale(A_, a), scale(spec_op(B_, 2), to));
 		ofield *rhsb = add(scale(spec_op(A_, 2), to), scale(B_, a));
 		u = solve_field(p->Mvel, rhsa);
@@ -1373,8 +1425,8 @@ static void pole_solve_c(orexi_pole *p, ofield *i_phi0, ofield *i_u0, ofield *i_
 	} else {
 		ofield *rhs = add(scale(div0, G), scale(cplx_from_real(i_phi0), a));
 		phi = helmholtz_c(rhs, a * a, -G, p->r);
-		u = scale(add(u0, scale(grad_lon(phi), p->ir)), 1.0 / a);
-		v = scale(add(v0, scale(grad_lat(phi), p->ir)), 1.0 / a);
+		u = scale(add(u0, scale(p->robert ? robert_grad_lon(phi) : grad_lon(phi), p->ir)), 1.0 / a);
+		v = scale(add(v0, scale(p->robert ? robert_grad_lat(phi) : grad_lat(phi), p->ir)), 1.0 / a);
 	}
 	(void)phi0;
 	*o_phi = phi; *o_u = u; *o_v = v;
@@ -1405,7 +1457,7 @@ void orc_rexi_pole_solve(ocfg *c, const double *params, double alpha_re, double 
 	double a = params[0], g = params[1], omega = params[2], h0 = params[3];
 	orexi_pole pole;
 	memset(&pole, 0, sizeof(pole));
-	pole_setup(&pole, c, alpha_re + I * alpha_im, 1.0, a, omega, h0 * g, 1.0, omega != 0);
+	pole_setup(&pole, c, alpha_re + I * alpha_im, 1.0, a, omega, h0 * g, 1.0, omega != 0, 0);
 	ofield *f[3];
 	const cplx *in[3] = {phi_in, u_in, v_in};
 	for (int k = 0; k < 3; k++) {
@@ -1427,7 +1479,7 @@ void orc_rexi_pole_solve(ocfg *c, const double *params, double alpha_re, double 
 
 /*
  * REXI time stepping (AppSWE.hpp:336-565), linear equations.
- * params: [a, g, omega, h0, visc, rexi_h, rexi_M, half, dt]
+ * params: [a, g, omega, h0, visc, rexi_h, rexi_M, half, dt, robert]
  * Input state: spectral h, u, v (nlm each).  Output spectral after nsteps.
  */
 int orc_swe_rexi_run(ocfg *c, const double *params, const cplx *h_in, const cplx *u_in, const cplx *v_in,
@@ -1436,6 +1488,7 @@ int orc_swe_rexi_run(ocfg *c, const double *params, const cplx *h_in, const cplx
 	double a = params[0], g = params[1], omega = params[2], h0 = params[3], visc = params[4];
 	double rh = params[5]; int rM = (int)params[6], half = (int)params[7];
 	double dt = params[8];
+	int robert = (int)params[9];
 	int maxp = 2 * (rM + 11) + 1;
 	double *al = (double *)malloc(sizeof(double) * 2 * maxp), *be = (double *)malloc(sizeof(double) * 2 * maxp);
 	int np = orc_rexi_coeffs(rh, rM, half, al, be);
@@ -1443,7 +1496,7 @@ int orc_swe_rexi_run(ocfg *c, const double *params, const cplx *h_in, const cplx
 	orexi_pole *poles = (orexi_pole *)calloc(np, sizeof(orexi_pole));
 	for (int i = 0; i < np; i++)
 		pole_setup(&poles[i], c, al[2 * i] + I * al[2 * i + 1], be[2 * i] + I * be[2 * i + 1],
-			   a, omega, h0 * g, dt, coriolis);
+			   a, omega, h0 * g, dt, coriolis, robert);
 	ofield *h = field_alloc(c, 0, 0), *u = field_alloc(c, 0, 0), *v = field_alloc(c, 0, 0);
 	memcpy(h->spec, h_in, sizeof(cplx) * c->nlm); h->spec_valid = 1;
 	memcpy(u->spec, u_in, sizeof(cplx) * c->nlm); u->spec_valid = 1;

@@ -38,7 +38,7 @@ class SweParams(ctypes.Structure):
 class RexiParams(ctypes.Structure):
     _fields_ = [("earth_radius", _d), ("gravitation", _d), ("coriolis_omega", _d), ("h0", _d),
                 ("viscosity", _d), ("dt", _d), ("rexi_h", _d), ("rexi_M", _i), ("rexi_half", _i),
-                ("precision", _i), ("band_precision", _i), ("refine", _i)]
+                ("precision", _i), ("band_precision", _i), ("refine", _i), ("robert_functions", _i)]
 
 
 _SIGS = {
@@ -400,8 +400,8 @@ class Rexi:
 
 
 def rexi_params(a=6.37122e6, g=9.80616, omega=7.292e-5, h0=10000.0, visc=1e5, dt=3600.0, rexi_h=0.2, rexi_M=128,
-                half=1, precision=PREC_FP64, band_precision=PREC_FP64, refine=0):
-    return RexiParams(a, g, omega, h0, visc, dt, rexi_h, rexi_M, half, precision, band_precision, refine)
+                half=1, precision=PREC_FP64, band_precision=PREC_FP64, refine=0, robert=0):
+    return RexiParams(a, g, omega, h0, visc, dt, rexi_h, rexi_M, half, precision, band_precision, refine, robert)
 
 
 def swe_params(a=6.37122e6, g=9.80616, omega=7.292e-5, h0=10000.0, visc=1e5, dt=120.0, nonlinear=1,

@@ -143,6 +143,46 @@ void band_add(BandRows &B, int comp, cplx s, double r)
 				row_add(B, n, m, +2, s * fac * idC(dn + 2, dm));
 				break;
 			}
+			// Robert-function components (SPHSolverComplex.hpp:241-370)
+			case 19: { // z4robert: grad_j(mu) grad_i(phi) = (1 - mu^2) i m phi / r^2
+				const cplx fac = ir2 * s * (I * dm);
+				row_add(B, n, m, -2, -fac * idA(dn - 2, dm));
+				row_add(B, n, m, 0, -fac * idB(dn, dm) + fac);
+				row_add(B, n, m, +2, -fac * idC(dn + 2, dm));
+				break;
+			}
+			case 20: { // z5robert: the five entries of (mu^2 - mu^4) i m / r^2, all on
+				   // the -2 diagonal as the reference adds them (:305-309)
+				const cplx fac = ir2 * s * (I * dm);
+				row_add(B, n, m, -2, -fac * (idA(dn - 2, dm) * idA(dn - 4, dm)));
+				row_add(B, n, m, -2, -fac * (idA(dn - 2, dm) * idB(dn - 2, dm) + idB(dn, dm) * idA(dn - 2, dm)) +
+							     fac * idA(dn - 2, dm));
+				row_add(B, n, m, -2, -fac * (idA(dn - 2, dm) * idC(dn, dm) + idB(dn, dm) * idB(dn, dm) +
+							      idC(dn + 2, dm) * idA(dn, dm)) + fac * idB(dn, dm));
+				row_add(B, n, m, -2, -fac * (idB(dn, dm) * idC(dn + 2, dm) + idC(dn + 2, dm) * idB(dn + 2, dm)) +
+							     fac * idC(dn + 2, dm));
+				row_add(B, n, m, -2, -fac * (idC(dn + 2, dm) * idC(dn + 4, dm)));
+				break;
+			}
+			case 21: { // z6robert: grad_j(mu) mu grad_j(phi), times m / r^2
+				const cplx s6 = ir2 * s * dm;
+				const double gr = idG(dn - 1, dm) * idR(dn - 2, dm);
+				const double gs = idG(dn - 1, dm) * idS(dn, dm) + idH(dn + 1, dm) * idR(dn, dm);
+				const double hs = idH(dn + 1, dm) * idS(dn + 2, dm);
+				row_add(B, n, m, -2, s6 * gr);
+				row_add(B, n, m, 0, s6 * gs);
+				row_add(B, n, m, +2, s6 * hs);
+				row_add(B, n, m, -4, s6 * (-idA(dn - 4, dm) * gr));
+				row_add(B, n, m, -2, s6 * (-idB(dn - 2, dm) * gr));
+				row_add(B, n, m, 0, s6 * (-idC(dn, dm) * gr));
+				row_add(B, n, m, -2, s6 * (-idA(dn - 2, dm) * gs));
+				row_add(B, n, m, 0, s6 * (-idB(dn, dm) * gs));
+				row_add(B, n, m, +2, s6 * (-idC(dn + 2, dm) * gs));
+				row_add(B, n, m, 0, s6 * (-idA(dn, dm) * hs));
+				row_add(B, n, m, +2, s6 * (-idB(dn + 2, dm) * hs));
+				row_add(B, n, m, +4, s6 * (-idC(dn + 4, dm) * hs));
+				break;
+			}
 			default: break;
 			}
 		}
@@ -153,8 +193,8 @@ int band_component_width(int comp)
 	switch (comp) {
 	case 0: case 11: case 14: case 17: return 0;
 	case 1: case 2: return 1;
-	case 12: case 15: case 16: case 18: return 2;
-	case 13: return 4;
+	case 12: case 15: case 16: case 18: case 19: case 20: return 2;
+	case 13: case 21: return 4;
 	default: return -1;
 	}
 }

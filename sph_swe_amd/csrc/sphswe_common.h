@@ -169,6 +169,8 @@ __host__ __device__ inline double idD(double k, double m)
 	return (2.0 * n + 1.0) * sqrt((n * n - m * m) / (4.0 * n * n - 1.0));
 }
 __host__ __device__ inline double idE(double n, double) { return -n; }
+__host__ __device__ inline double idG(double k, double m) { double n = k + 1; return idD(n - 1, m) + idE(n, m) * idR(n - 1, m); }
+__host__ __device__ inline double idH(double k, double m) { double n = k - 1; return idE(n, m) * idS(n + 1, m); }
 
 // XCD-aware block remap: hardware places block b on XCD (b mod 8); consecutive
 // logical blocks are gathered on one XCD so that neighbours share its L2.

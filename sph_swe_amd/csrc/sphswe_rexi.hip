@@ -45,6 +45,7 @@ struct sphswe_rexi {
 	int *d_moff = nullptr;
 	void *d_fphi = nullptr, *d_fvel = nullptr;   // parity-compact LU factors, double2 or float2
 	int fac32 = 0;                     // factors stored in FP32
+	int robert = 0;                    // SWERexiSPHRobert: Robert-function velocities
 	int refine = 0;                    // one sweep of FP64 iterative refinement per solve
 	double2 *d_kphi = nullptr;         // refinement: pole-independent pieces of the phi bands [rows][4][5]
 	double2 *d_kvel = nullptr;         // refinement: W^2 mu^2 band of the velocity systems [rows][3]
@@ -1002,10 +1003,15 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	if (prm->precision != SPHSWE_PREC_FP64 && prm->precision != SPHSWE_PREC_FP32)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: precision must be FP64 or FP32");
 	if (prm->refine != 0 && prm->refine != 1) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: refine must be 0 or 1");
+	if (prm->robert_functions != 0 && prm->robert_functions != 1)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: robert_functions must be 0 or 1");
+	if (prm->robert_functions && prm->refine)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: the refinement sweep is not available with Robert functions");
 	sphswe_rexi *r = new sphswe_rexi();
 	r->ctx = ctx;
 	r->prm = *prm;
 	r->coriolis = prm->coriolis_omega != 0.0;
+	r->robert = prm->robert_functions;
 	r->alpha0 = alpha0;
 	r->beta0 = beta0;
 	const int P = r->P = (int)r->alpha0.size();
@@ -1067,9 +1073,10 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 					band_add(Bp, 11, (a * a) * (a * a), rr);
 					band_add(Bp, 12, 2.0 * W * W * a * a, rr);
 					band_add(Bp, 13, (W * W) * (W * W), rr);
-					band_add(Bp, 14, -G * a * W, rr);
-					band_add(Bp, 15, G / a * W * W * W, rr);
-					band_add(Bp, 16, G * 2.0 * W * W, rr);
+					// z4..z6, or their Robert forms (SWERexiSPHRobert.hpp setup)
+					band_add(Bp, r->robert ? 19 : 14, -G * a * W, rr);
+					band_add(Bp, r->robert ? 20 : 15, G / a * W * W * W, rr);
+					band_add(Bp, r->robert ? 21 : 16, G * 2.0 * W * W, rr);
 					band_add(Bp, 17, -G * a * a, rr);
 					band_add(Bp, 18, -G * W * W, rr);
 					BandRows Bv(L, 2);
@@ -1206,7 +1213,9 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 		sphswe_rexi_destroy(r);
 		return rc;
 	}
-	if ((rc = rexi_gradient_operator(ctx, r))) {
+	// Robert functions: the gradients of phi are the spectral i m phi and D' phi
+	// that the phi sweep writes, so there is no gradient operator
+	if (!r->robert && (rc = rexi_gradient_operator(ctx, r))) {
 		sphswe_rexi_destroy(r);
 		return rc;
 	}
@@ -1292,6 +1301,22 @@ static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_DIFF_LON, SPEC(2), SPEC(7), 1, 1, 0, 0, 0, 0))) return rc;
 	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_ONE_MINUS_MU2_DMU, SPEC(1), SPEC(8), 1, 1, 0, 0, 0, 0))) return rc;
 	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_ONE_MINUS_MU2_DMU, SPEC(4), SPEC(9), 1, 1, 0, 0, 0, 0))) return rc;
+	if (r->robert) {
+		// SWERexiSPHRobert.hpp solve: div0 = ir robert_div(u0, v0), one division by
+		// cos^2 phi of i m u0c + D' v0c (SPHOperatorsComplex.hpp:530-550); eta0 =
+		// ir robert_vort = ir (i m v0c - D' u0c), spectral; the Coriolis multiplier
+		// robert_grad_lat(mu) = D' A(mu), spectral
+		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_ONE_MINUS_MU2_DMU, SPEC(2), SPEC(6), 1, 1, 0, 0, 0, 0))) return rc;
+		k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 5 * nc, cs_ + 6 * nc, cs_ + 10 * nc, 1.0, 1.0, nc);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if ((rc = sphswe_sht_synthesis_cplx(ctx, SPEC(10), PHYS(0), 1, SPHSWE_PREC_FP64))) return rc;
+		if ((rc = sphswe_spat_op_cplx(ctx, SPHSWE_SPAT_DIV_ST2, PHYS(0), nullptr, PHYS(0), 1.0, 1))) return rc;
+		if ((rc = sphswe_sht_analysis_cplx(ctx, PHYS(0), SPEC(11), 1, SPHSWE_PREC_FP64))) return rc;
+		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_IDENTITY, SPEC(11), BF(B_DIV), 1, ir, 0, 0, 0, 0))) return rc;
+		k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 7 * nc, cs_ + 8 * nc, cs_ + 15 * nc, -1.0, ir, nc);   // eta0 in spec[15]
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_IDENTITY, SPEC(9), SPEC(5), 1, W * ir, 0, 0, 0, 0))) return rc;   // gms
+	} else {
 	// synthesis, metric divisions: grad_lon / grad_lat divide by sqrt(1-mu^2); the
 	// complex div_lat further divides by cos(lat) (SPHOperatorsComplex.hpp:326-333)
 	if ((rc = sphswe_sht_synthesis_cplx(ctx, SPEC(5), PHYS(0), 5, SPHSWE_PREC_FP64))) return rc;
@@ -1304,6 +1329,7 @@ static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 	k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 12 * nc, cs_ + 13 * nc, cs_ + 15 * nc, -1.0, ir, nc);   // eta0 in spec[15]
 	SPHSWE_LAUNCH_CHECK(ctx);
 	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_IDENTITY, SPEC(14), SPEC(5), 1, W * ir, 0, 0, 0, 0))) return rc;   // gms -> spec[5]
+	}
 	// mu / mu2 images
 	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU, SPEC(15), BF(B_MUETA), 1, 1, 0, 0, 0, 0))) return rc;
 	if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU2, BF(B_PHI), BF(B_MU2PHI), 1, 1, 0, 0, 0, 0))) return rc;
@@ -1367,8 +1393,9 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 	else phi((const double2 *)r->d_fphi);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	// gradients of phi, A(S(x) / sqrt(1-mu^2)) per m: the precomputed operator
-	// applied to the 2P columns (i m phi | D' phi) of every pole at once
-	{
+	// applied to the 2P columns (i m phi | D' phi) of every pole at once (Robert
+	// functions: the columns themselves are the gradients)
+	if (!r->robert) {
 		const int ncol = 4 * P;   // real columns
 		const int cs = r->prm.precision == SPHSWE_PREC_FP64 ? gop_cs<double>() : gop_cs<float>();
 		dim3 gg(r->ngtiles, (ncol + 16 * cs - 1) / (16 * cs));
@@ -1385,7 +1412,7 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 		typedef std::remove_const_t<std::remove_pointer_t<decltype(fac)>> FT;
 		VelArgs<FT> va;
 		va.alpha = r->d_alpha; va.beta = r->d_beta; va.moff = r->d_moff;
-		va.fac = fac; va.piv = r->d_pvel; va.basis = r->d_basis; va.G = r->d_G;
+		va.fac = fac; va.piv = r->d_pvel; va.basis = r->d_basis; va.G = r->robert ? r->d_X : r->d_G;
 		va.Y = r->d_Y; va.part = r->d_part;
 		va.L = L; va.P = P; va.ncplx = (int)nc;
 		va.two_omega = 2.0 * r->prm.coriolis_omega; va.ir = 1.0 / a;

@@ -107,9 +107,10 @@ public:
 			}
 			simVars.use_nonlinear_equations = 0;
 		}
-		if ((simVars.use_robert_functions && simVars.use_rexi) || simVars.rexi_use_extended_modes) {
-			std::cerr << "Robert functions with REXI and extended REXI modes are not available in this build"
-				  << std::endl;
+		if ((simVars.use_robert_functions && simVars.use_rexi && !simVars.gpu_fused_step) ||
+		    simVars.rexi_use_extended_modes) {
+			std::cerr << "Robert functions with the class-by-class REXI loop (G0) and extended REXI modes "
+				     "are not available in this build" << std::endl;
 			exit(1);
 		}
 
@@ -293,7 +294,8 @@ private:
 		if (simVars.gpu_fused_step) {
 			sphswe_rexi_params p = {simVars.earth_radius, simVars.gravitation, with_coriolis ? simVars.coriolis_omega : 0.0,
 						simVars.h0, simVars.viscosity2, dt, simVars.rexi_h, simVars.rexi_M,
-						simVars.rexi_use_half_poles ? 1 : 0};
+						simVars.rexi_use_half_poles ? 1 : 0, SPHSWE_PREC_FP64, SPHSWE_PREC_FP64, 0,
+						simVars.use_robert_functions ? 1 : 0};
 			sphConfig->check(sphswe_rexi_create(sphConfig->gpu, &p, &batched), "sphswe_rexi_create");
 			sphConfig->check(sphswe_malloc(sphConfig->gpu, 3 * sizeof(std::complex<double>) * sphConfig->spec_num_elems,
 						       (void **)&d_state), "state");

@@ -7,7 +7,7 @@ SPHIdentities headers, compiled from the reference tree into oracle/_ref
 Outputs (plain text, one row per entry, full double precision):
   rexi_h<h>_M<M>_<half|full>.txt   Re alpha, Im alpha, Re beta_re, Im beta_re, Re beta_im,
                                    Im beta_im per pole
-  identities_L20.txt               n, m, R, S, A, B, C, D, E of SPHIdentities
+  identities_L20.txt               n, m, R, S, A, B, C, D, E, G, H of SPHIdentities
 """
 import os
 import subprocess

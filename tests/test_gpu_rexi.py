@@ -210,3 +210,31 @@ def test_rexi_top_degree_m1_growth_matches_oracle():
     assert amp_orc[11] > 10 * amp_orc[3] and amp_gpu[11] > 10 * amp_gpu[3]
     rx.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("L,M,omega,visc,nsteps", [(16, 8, OMEGA, 1e5, 2), (31, 8, OMEGA, 0.0, 2), (31, 8, 0.0, 1e5, 2),
+                                                    (64, 32, OMEGA, 1e5, 1)])
+def test_rexi_robert_functions_vs_oracle(L, M, omega, visc, nsteps):
+    # SWERexiSPHRobert (SWERexiSPHRobert.hpp): z4..z6robert bands, robert_div and
+    # robert_vort right-hand sides, spectral gradients in place of the gradient
+    # operator; u, v are Robert functions
+    print(f"rexi Robert vs oracle L={L} M={M} omega={omega} visc={visc} nsteps={nsteps}", flush=True)
+    ctx = capi.Context(L)
+    orc = O.Oracle(L, ctx.nlat, ctx.nphi)
+    h, u, v = _state(L, 17)
+    prm = dict(a=A_EARTH, g=G, omega=omega, h0=H0, visc=visc, rexi_h=0.2, rexi_M=M, half=1, dt=3600.0, robert=1)
+    ref = orc.swe_rexi_run(prm, h, u, v, nsteps)
+    params = capi.rexi_params(omega=omega, visc=visc, dt=3600.0, rexi_h=0.2, rexi_M=M, half=1, robert=1)
+    out = _run_gpu(ctx, params, (h, u, v), nsteps)
+    for k in range(3):
+        e = rel_l2(out[k], ref[k])
+        print("robert", "huv"[k], e)
+        assert e < TOL
+    # the standard formulation gives another result; refinement is refused
+    std = _run_gpu(ctx, capi.rexi_params(omega=omega, visc=visc, dt=3600.0, rexi_h=0.2, rexi_M=M, half=1),
+                   (h, u, v), nsteps)
+    assert rel_l2(std[1], ref[1]) > 1e-6
+    with pytest.raises(capi.SphSweError):
+        ctx.rexi(capi.rexi_params(omega=omega, dt=3600.0, rexi_M=M, band_precision=capi.PREC_FP32, refine=1,
+                                  robert=1))
+    ctx.close()

@@ -50,8 +50,10 @@ def test_identities_vs_golden():
     for row in ref:
         n, m = row[0], row[1]
         d = O.identities(n, m)
-        for k, name in enumerate("RSABCDE"):
+        for k, name in enumerate("RSABCDEGH"):
             want = row[2 + k]
+            if np.isnan(want) and name == "H":
+                continue   # H(0, m): the reference asserts on E(-1, m); z6robert uses H(n >= 1, m)
             if np.isnan(want):
                 assert np.isnan(d[name]), (n, m, name)
             else:
@@ -80,7 +82,9 @@ def test_identities_vs_reference_build():
     for row in ref[::7]:
         n, m = row[0], row[1]
         d = O.identities(n, m)
-        for k, name in enumerate("RSABCDE"):
+        for k, name in enumerate("RSABCDEGH"):
+            if np.isnan(row[2 + k]) and name == "H":
+                continue
             assert abs(d[name] - row[2 + k]) <= 1e-15 * max(1.0, abs(row[2 + k])), (n, m, name)
 
 
