@@ -177,6 +177,19 @@ def bench_rexi(steps, L=256, M=188):
     cfg = {"lmax": L, "nlat": ctx.nlat, "nphi": ctx.nphi, "rexi_M": M, "rexi_h": 0.2, "poles": rx.npoles,
            "dt": 3600.0, "steps": steps, "viscosity": 1e5, "setup_s": setup_s, "autotune": tuning}
     rx.close()
+    # the same workload with Robert-function velocities (SWERexiSPHRobert): the
+    # gradients are spectral, so the per-pole gradient operator is not applied
+    rxr = ctx.rexi(capi.rexi_params(dt=3600.0, rexi_h=0.2, rexi_M=M, half=1, visc=1e5, robert=1))
+    state = ctx.to_device(workloads.rexi_state(ctx))
+    rxr.step(state, 1)
+    ctx.sync()
+    ctx.mark(8)
+    rxr.step(state, steps)
+    ctx.mark(9)
+    cfg["robert_functions_steps_per_s"] = steps / (ctx.elapsed_ms(8, 9) / 1000.0)
+    if not np.all(np.isfinite(ctx.to_host(state, np.complex128, (3, ctx.nlm)))):
+        raise RuntimeError("Robert REXI state became non-finite")
+    rxr.close()
     ctx.close()
     return steps / (ms / 1000.0), cfg
 

@@ -23,6 +23,10 @@
  *     BandedMatrixSolver.hpp:235-434)
  *   - SWERexiSPH::solve (SWERexiSPH.hpp:137-244) and the REXI time step
  *     (AppSWE.hpp:415-565)
+ *   - the Robert-function variants: the linear tendency (AppSWE.hpp:606-620),
+ *     the robert_* operators (SPHOperators.hpp:124-208, SPHOperatorsComplex.hpp:
+ *     341-550), the z4..z6robert components (SPHSolverComplex.hpp:241-370) and
+ *     SWERexiSPHRobert::solve (SWERexiSPHRobert.hpp)
  *
  * Conventions (SHTNS sht_orthonormal, Condon-Shortley phase on):
  *   real field:    f = sum_l q_l0 Y_l0 + 2 Re sum_{m>0} q_lm Y_lm ; Im q_l0 ignored
@@ -36,7 +40,11 @@
  * Pbar vs scipy sph_legendre_p and closed forms, FFT vs numpy.fft, band LU vs
  * scipy zgbsv, REXI coefficients vs the reference's own headers compiled into
  * oracle/_ref, operators vs analytic derivatives, SWE linear mode vs closed
- * form frequency).
+ * form frequency; the Robert tendency vs the cos-weighted standard one, the
+ * Robert REXI step to first order in dt vs that tendency).
+ * Parity unpinned: component 21 (z6robert).  No closed form of it is known
+ * here, a sign change of it is invisible to the first-order check, and the
+ * reference's solver headers cannot be built without SHTNS.
  */
 #define _GNU_SOURCE
 #include <complex.h>
