@@ -767,7 +767,11 @@ inline int g256(long n) { return (int)((n + 255) / 256); }
 // vectors, and applied per step as a batched GEMM over the 4P real columns of
 // the poles (columns i m phi | D' phi, re / im).
 
-constexpr int GOP_RT = 8;   // 16-row tiles per wavefront of the apply kernel
+// 16-row tiles and 16-column strips per wavefront of the apply kernel: 4 x 4 in
+// FP64 (measured faster than 8 x 2: T1023 REXI step 44.4 -> 42.6 ms), 8 x 2 in
+// FP32 (4 x 4 measured slower there)
+template <typename T>
+constexpr int gop_rt() { return sizeof(T) == 8 ? 4 : 8; }
 
 typedef double gd4v __attribute__((ext_vector_type(4)));
 typedef float gf4v __attribute__((ext_vector_type(4)));
@@ -816,9 +820,9 @@ __global__ void k_gop_sym(double *__restrict__ M, float *__restrict__ M32, const
 }
 
 // Y[rows of (m, parity)][cols] = M(|m|, parity) X[...]: one wavefront per
-// (m, parity, up to GOP_RT 16-row tiles, CS strips of 16 real columns).  Each
-// k step loads GOP_RT A values and CS B values per lane and issues
-// GOP_RT * CS = 16 MFMAs on them (FP64: 128 accumulator registers, 2 waves
+// (m, parity, up to RT 16-row tiles, CS strips of 16 real columns).  Each
+// k step loads RT A values and CS B values per lane and issues
+// RT * CS = 16 MFMAs on them (FP64: 128 accumulator registers, 2 waves
 // per SIMD; FP32: 64, 3 waves), so M is read once per CS strips and every X
 // row once per wavefront.  A = M (symmetric: the 16 rows of one column k are
 // contiguous), B = X rows.  FP64: v_mfma_f64_16x16x4f64, D[i][n] in lane
@@ -826,7 +830,7 @@ __global__ void k_gop_sym(double *__restrict__ M, float *__restrict__ M32, const
 // rounded, D[i][n] in lane n + 16 (i / 4), element i % 4.  A[i][k] in lane
 // i + 16 k and B[k][n] in lane n + 16 k for both.
 template <typename T>
-constexpr int gop_cs() { return 2; }
+constexpr int gop_cs() { return sizeof(T) == 8 ? 4 : 2; }
 
 template <typename T>
 __global__ __launch_bounds__(64) void k_gop_apply(const T *__restrict__ M, const long *__restrict__ goff,
@@ -834,7 +838,7 @@ __global__ __launch_bounds__(64) void k_gop_apply(const T *__restrict__ M, const
 						  const double *__restrict__ X, double *__restrict__ Y, int L, int ncol)
 {
 	constexpr bool F64 = sizeof(T) == 8;
-	constexpr int CS = gop_cs<T>();
+	constexpr int CS = gop_cs<T>(), GOP_RT = gop_rt<T>();
 	typedef std::conditional_t<F64, gd4v, gf4v> acc_t;
 	const int4 tl = tiles[blockIdx.x];
 	const int m = tl.x, par = tl.y, r0 = tl.z;
@@ -941,12 +945,14 @@ static int rexi_gradient_operator(sphswe_ctx *ctx, sphswe_rexi *r)
 			goff[2 * am + par] = tot;
 			tot += np * np;
 		}
+	// row tiles per wavefront of the apply kernel this object will launch
+	const int rt = r->prm.precision == SPHSWE_PREC_FP64 ? gop_rt<double>() : gop_rt<float>();
 	std::vector<int4> tiles;
 	for (int am = 0; am <= L; am++)
 		for (int sg = (am ? -1 : 1); sg <= 1; sg += 2)
 			for (int par = 0; par < 2; par++) {
 				const int np = (L + 1 - am + 1 - par) / 2;
-				for (int r0 = 0; r0 < np; r0 += 16 * GOP_RT) tiles.push_back(make_int4(sg * am, par, r0, 0));
+				for (int r0 = 0; r0 < np; r0 += 16 * rt) tiles.push_back(make_int4(sg * am, par, r0, 0));
 			}
 	r->ngtiles = (int)tiles.size();
 	int rc;
