@@ -284,6 +284,20 @@ class Context:
         """As swe_rk4, one captured step replayed nsteps times."""
         self._check(self.lib.sphswe_swe_rk4_graph(self.h, ctypes.byref(params), state.ptr, nmembers, nsteps))
 
+    # checks ------------------------------------------------------------------
+    def any_nan(self, d, n):
+        """1 if any of the first n doubles of device buffer d is NaN or Inf."""
+        f = _i()
+        self._check(self.lib.sphswe_any_nan(self.h, d.ptr, n, ctypes.byref(f)))
+        return f.value
+
+    def minmax(self, d, n):
+        """(min, max, max |x|) of the first n doubles of device buffer d."""
+        a, b, c = _d(), _d(), _d()
+        self._check(self.lib.sphswe_reduce_minmax(self.h, d.ptr, n, ctypes.byref(a), ctypes.byref(b),
+                                                  ctypes.byref(c)))
+        return a.value, b.value, c.value
+
     # autotuner ----------------------------------------------------------------
     def autotune(self):
         """Times the launch variants of this context's sizes and keeps the

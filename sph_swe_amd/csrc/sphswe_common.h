@@ -69,6 +69,15 @@ struct sphswe_ctx {
 
 	std::unordered_map<void *, size_t> allocs;
 
+	// bumped whenever a buffer or kernel choice that a captured step may refer
+	// to changes (scratch regrowth, FP32 tables freed, tuning changed); a
+	// captured step records the value and is rebuilt when it differs
+	unsigned long buf_gen = 0;
+	// REXI and band objects created on this context: detached (ctx = null) by
+	// sphswe_destroy, so that a later destroy of the object touches no freed context
+	std::vector<sphswe_rexi *> rexis;
+	std::vector<sphswe_band *> bands;
+
 	hipEvent_t ev[SPHSWE_MAX_TIMERS];
 };
 
@@ -113,6 +122,8 @@ int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, in
 int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
 int sphswe_leg_anal_reserve(sphswe_ctx *ctx, int nf, int prec);
 void sphswe_swe_release(sphswe_ctx *ctx);
+void sphswe_rexi_detach(sphswe_rexi *r);
+void sphswe_band_detach(sphswe_band *b);
 int sphswe_leg_anal_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
 int sphswe_leg_synth_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
 int sphswe_leg_anal_cplx_mx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);

@@ -264,6 +264,8 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	return SPHSWE_OK;
 }
 
+static void sphswe_pool_release(const sphswe_ctx *ctx);
+
 extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 {
 	if (!ctx) return SPHSWE_OK;
@@ -271,11 +273,18 @@ extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 	if (ctx->stream) hipStreamSynchronize(ctx->stream);
 	sphswe_swe_release(ctx);
 	sphswe_leg_tables_free(ctx);
+	// objects still alive on this context keep their own memory and are freed
+	// by their destroy calls; they no longer refer to the context
+	for (sphswe_rexi *r : ctx->rexis) sphswe_rexi_detach(r);
+	for (sphswe_band *b : ctx->bands) sphswe_band_detach(b);
+	ctx->rexis.clear();
+	ctx->bands.clear();
 	void *ptrs[] = {ctx->d_ct, ctx->d_st, ctx->d_sq, ctx->d_coslat, ctx->d_wg, ctx->d_coef,
 			ctx->d_seed, ctx->d_segoff, ctx->d_unit_m, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_escratch, ctx->d_flag};
 	for (void *p : ptrs)
 		if (p) hipFree(p);
 	for (auto &kv : ctx->allocs) hipFree(kv.first);
+	sphswe_pool_release(ctx);
 	for (int i = 0; i < SPHSWE_MAX_TIMERS; i++)
 		if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
 	if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -320,6 +329,8 @@ extern "C" void *sphswe_stream(sphswe_ctx *ctx) { return ctx ? (void *)ctx->stre
 static int grow(sphswe_ctx *ctx, void **buf, size_t *have, size_t bytes, void **p)
 {
 	if (bytes > *have) {
+		// a captured step that points into the old buffer must be rebuilt
+		ctx->buf_gen++;
 		if (*buf) {
 			SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
 			SPHSWE_HIP(ctx, hipFree(*buf));
@@ -353,6 +364,17 @@ struct Pool {
 	std::multimap<size_t, void *> free_blocks;
 };
 std::unordered_map<const sphswe_ctx *, Pool> g_pools;
+}
+
+// frees the blocks cached for a context and forgets it (sphswe_destroy, after
+// the stream has drained), so that a context created later at the same host
+// address starts with an empty pool
+static void sphswe_pool_release(const sphswe_ctx *ctx)
+{
+	auto it = g_pools.find(ctx);
+	if (it == g_pools.end()) return;
+	for (auto &kv : it->second.free_blocks) (void)hipFree(kv.second);
+	g_pools.erase(it);
 }
 
 extern "C" int sphswe_malloc(sphswe_ctx *ctx, size_t bytes, void **d_ptr)

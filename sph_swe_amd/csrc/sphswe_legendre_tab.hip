@@ -314,6 +314,7 @@ int sphswe_leg_tables(sphswe_ctx *ctx)
 
 void sphswe_leg_tables_free(sphswe_ctx *ctx)
 {
+	ctx->buf_gen++;   // a captured FP32 step may read the tables
 	for (void *p : {(void *)ctx->d_tab_s, (void *)ctx->d_tab_a, (void *)ctx->d_tab_off, (void *)ctx->d_tab_tiles_s,
 			(void *)ctx->d_tab_tiles_a})
 		if (p) hipFree(p);

@@ -916,10 +916,19 @@ int dev_alloc(sphswe_ctx *ctx, T **p, size_t n)
 
 }  // namespace
 
+// the context is being destroyed: the object keeps its own memory (freed by
+// sphswe_rexi_destroy, which must still be called) and accepts no other call
+void sphswe_rexi_detach(sphswe_rexi *r) { r->ctx = nullptr; }
+
 extern "C" int sphswe_rexi_destroy(sphswe_rexi *r)
 {
 	if (!r) return SPHSWE_OK;
-	if (r->ctx) hipStreamSynchronize(r->ctx->stream);
+	if (r->ctx) {
+		hipStreamSynchronize(r->ctx->stream);
+		auto &v = r->ctx->rexis;
+		for (size_t i = 0; i < v.size(); i++)
+			if (v[i] == r) { v.erase(v.begin() + i); break; }
+	}
 	void *ptrs[] = {r->d_alpha, r->d_beta, r->d_moff, r->d_fphi, r->d_fvel, r->d_pphi, r->d_pvel, r->d_oscale,
 			r->d_X, r->d_G, r->d_Y, r->d_part, r->d_basis, r->d_acc, r->d_phys, r->d_cphys, r->d_cspec,
 			r->d_gop, r->d_gop32, r->d_goff, r->d_gtiles, r->d_kphi, r->d_kvel,
@@ -1015,6 +1024,7 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "REXI: the refinement sweep is not available with Robert functions");
 	sphswe_rexi *r = new sphswe_rexi();
 	r->ctx = ctx;
+	ctx->rexis.push_back(r);
 	r->prm = *prm;
 	r->coriolis = prm->coriolis_omega != 0.0;
 	r->robert = prm->robert_functions;

@@ -136,14 +136,24 @@ extern "C" int sphswe_band_create(sphswe_ctx *ctx, int halo, sphswe_band **band)
 	if (halo < 1 || halo > 4) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "band: halo %d outside 1..4", halo);
 	sphswe_band *b = new sphswe_band(ctx->L, halo);
 	b->ctx = ctx;
+	ctx->bands.push_back(b);
 	*band = b;
 	return SPHSWE_OK;
 }
 
+// the context is being destroyed: the object keeps its own memory (freed by
+// sphswe_band_destroy) and accepts no other call
+void sphswe_band_detach(sphswe_band *b) { b->ctx = nullptr; }
+
 extern "C" int sphswe_band_destroy(sphswe_band *band)
 {
 	if (!band) return SPHSWE_OK;
-	if (band->ctx) hipStreamSynchronize(band->ctx->stream);
+	if (band->ctx) {
+		hipStreamSynchronize(band->ctx->stream);
+		auto &v = band->ctx->bands;
+		for (size_t i = 0; i < v.size(); i++)
+			if (v[i] == band) { v.erase(v.begin() + i); break; }
+	}
 	if (band->d_fac) hipFree(band->d_fac);
 	if (band->d_piv) hipFree(band->d_piv);
 	if (band->d_moff) hipFree(band->d_moff);
@@ -153,7 +163,7 @@ extern "C" int sphswe_band_destroy(sphswe_band *band)
 
 extern "C" int sphswe_band_add(sphswe_band *band, int component, double s_re, double s_im, double r)
 {
-	if (!band) return SPHSWE_ERR_ARG;
+	if (!band || !band->ctx) return SPHSWE_ERR_ARG;
 	const int w = sphswe_host::band_component_width(component);
 	if (w < 0) return sphswe_set_err(band->ctx, SPHSWE_ERR_ARG, "band: unknown component %d", component);
 	// rowElement_add only asserts this (SPHMatrixComplex.hpp:200-204); an entry

@@ -216,6 +216,7 @@ struct SweGraph {
 	sphswe_swe_params prm;
 	double *state = nullptr;
 	int nmem = 0;
+	unsigned long gen = 0;   // ctx->buf_gen at capture
 };
 std::unordered_map<const sphswe_ctx *, SweGraph> g_graph;
 
@@ -372,18 +373,20 @@ extern "C" int sphswe_swe_rk4_graph(sphswe_ctx *ctx, const sphswe_swe_params *p,
 	if (!ctx || !p || !d_state || nmem < 1 || nsteps < 0) return SPHSWE_ERR_ARG;
 	if (int rc = check_params(ctx, p)) return rc;
 	SweGraph &G = g_graph[ctx];
-	if (!G.exec || G.state != d_state || G.nmem != nmem || memcmp(&G.prm, p, sizeof *p) != 0) {
+	// every buffer the step uses exists before the check below, so that a
+	// regrowth it causes is seen by the generation test
+	SweBufs b;
+	if (int rc = swe_buffers(ctx, nmem, b)) return rc;
+	if (int rc = sphswe_leg_anal_reserve(ctx, 4 * nmem, p->precision)) return rc;
+	if (p->precision == SPHSWE_PREC_FP32 && (ctx->tab_min_synth > 0 || ctx->tab_min_anal > 0))
+		if (int rc = sphswe_leg_tables(ctx)) return rc;
+	if (!G.exec || G.state != d_state || G.nmem != nmem || G.gen != ctx->buf_gen ||
+	    memcmp(&G.prm, p, sizeof *p) != 0) {
 		if (G.exec) {
 			SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
 			SPHSWE_HIP(ctx, hipGraphExecDestroy(G.exec));
 			G.exec = nullptr;
 		}
-		// every buffer the step uses exists before the capture
-		SweBufs b;
-		if (int rc = swe_buffers(ctx, nmem, b)) return rc;
-		if (int rc = sphswe_leg_anal_reserve(ctx, 4 * nmem, p->precision)) return rc;
-		if (p->precision == SPHSWE_PREC_FP32 && (ctx->tab_min_synth > 0 || ctx->tab_min_anal > 0))
-			if (int rc = sphswe_leg_tables(ctx)) return rc;
 		SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
 		hipGraph_t graph;
 		SPHSWE_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -402,6 +405,7 @@ extern "C" int sphswe_swe_rk4_graph(sphswe_ctx *ctx, const sphswe_swe_params *p,
 		G.prm = *p;
 		G.state = d_state;
 		G.nmem = nmem;
+		G.gen = ctx->buf_gen;
 	}
 	for (int n = 0; n < nsteps; n++) SPHSWE_HIP(ctx, hipGraphLaunch(G.exec, ctx->stream));
 	return SPHSWE_OK;

@@ -39,6 +39,7 @@ extern "C" int sphswe_set_tuning(sphswe_ctx *ctx, int fft_nseq_r2c, int fft_nseq
 	ctx->fft_nseq_r2c = fft_nseq_r2c;
 	ctx->fft_nseq_c2r = fft_nseq_c2r;
 	ctx->synth_mfma_min = synth_mfma_min;
+	ctx->buf_gen++;   // a captured step may use the previous choices
 	return SPHSWE_OK;
 }
 
@@ -58,6 +59,7 @@ extern "C" int sphswe_set_fp32_tables(sphswe_ctx *ctx, int synth_min_fields, int
 	if (!ctx || synth_min_fields < 0 || anal_min_fields < 0) return SPHSWE_ERR_ARG;
 	ctx->tab_min_synth = synth_min_fields;
 	ctx->tab_min_anal = anal_min_fields;
+	ctx->buf_gen++;
 	return SPHSWE_OK;
 }
 
@@ -152,5 +154,6 @@ extern "C" int sphswe_autotune(sphswe_ctx *ctx)
 	if (four) (void)hipFree(four);
 	if (spec) (void)hipFree(spec);
 	if (!rc) ctx->tuned = 1;
+	ctx->buf_gen++;   // the choices changed and the timing buffers were freed
 	return rc;
 }

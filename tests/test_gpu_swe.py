@@ -52,11 +52,43 @@ def test_rk4_no_rotation_no_viscosity():
         assert e < TOL_SHORT, name
 
 
+def _diagnostics(L, out, a=6.37122e6, g=9.80616):
+    """phi = g h, vort = (grad_lon v - grad_lat u)/a and div = (div_lon u +
+    div_lat v)/a of a spectral (h, u, v) on the GPU through the C ABI, with the
+    reference's projection points (AppSWE.hpp:85-95, SPHOperators.hpp:100-119,
+    339-439, 507-529)."""
+    ctx = capi.Context(L)
+    try:
+        h, u, v = out
+
+        def glon(x):
+            return ctx.spat_op(capi.SPAT_DIV_COSPHI, ctx.synthesis(ctx.spec_op(capi.OP_DIFF_LON, x))[0])
+
+        def glat(x):
+            return ctx.spat_op(capi.SPAT_DIV_COSPHI, ctx.synthesis(ctx.spec_op(capi.OP_ONE_MINUS_MU2_DMU, x))[0])
+
+        def dlat(x):
+            xs = ctx.analysis(ctx.spat_op(capi.SPAT_MUL_ST, ctx.synthesis(x)[0]))[0]
+            return ctx.spat_op(capi.SPAT_DIV_ST, glat(xs))
+
+        vort = ctx.analysis(glon(v) - glat(u))[0] / a
+        div = ctx.analysis(glon(u) + dlat(v))[0] / a
+        return g * h, vort, div
+    finally:
+        ctx.close()
+
+
 def test_rk4_config0_t64():
-    # nonlinear SWE, T64 on 98 x 196, Gaussian dam at (pi/4, pi), dt = 120 s, 100 steps
+    # nonlinear SWE, T64 on 98 x 196, Gaussian dam at (pi/4, pi), dt = 120 s,
+    # 100 steps; the configuration compares geopotential, vorticity and
+    # divergence spectra
     out, ref = _run(64, 100, 120.0)
     for k, name in enumerate("huv"):
         e = rel_l2(out[k], ref[name])
+        print(name, e)
+        assert e < TOL_LONG, name
+    for x, name in zip(_diagnostics(64, out), ("phi", "vort", "div")):
+        e = rel_l2(x, ref[name])
         print(name, e)
         assert e < TOL_LONG, name
 
@@ -169,6 +201,66 @@ def test_graph_rebuilt_after_workspace_growth():
         assert np.array_equal(ctx.to_host(g, np.complex128, (3, ctx.nlm)), ctx.to_host(e, np.complex128, (3, ctx.nlm)))
     finally:
         ctx.close()
+
+
+def test_graph_rebuilt_after_scratch_growth():
+    # regression: a step captured for 1 member, then library calls that
+    # reallocate the context's transform scratch (a 64-field analysis; a REXI
+    # object, whose gradient operator runs the pole-layout analysis; the
+    # autotuner), then the 1-member graph again with unchanged arguments: the
+    # captured step must be rebuilt on the new buffers and equal the eager
+    # steps bit for bit
+    L = 31
+    ctx = capi.Context(L)
+    try:
+        x0 = _members(ctx, 1)
+        prm = capi.swe_params(dt=300.0)
+        g = ctx.to_device(x0)
+        e = ctx.to_device(x0)
+        ctx.swe_rk4_graph(prm, g, 1, 1)
+        ctx.swe_rk4(prm, e, 1, 1)
+        rng = np.random.default_rng(5)
+        ctx.analysis(rng.standard_normal((64, ctx.nspat)))
+        ctx.swe_rk4_graph(prm, g, 1, 2)
+        ctx.swe_rk4(prm, e, 1, 2)
+        assert np.array_equal(ctx.to_host(g, np.complex128, (3, ctx.nlm)), ctx.to_host(e, np.complex128, (3, ctx.nlm)))
+        rx = ctx.rexi(capi.rexi_params(dt=600.0, rexi_M=32))
+        ctx.swe_rk4_graph(prm, g, 1, 1)
+        ctx.swe_rk4(prm, e, 1, 1)
+        assert np.array_equal(ctx.to_host(g, np.complex128, (3, ctx.nlm)), ctx.to_host(e, np.complex128, (3, ctx.nlm)))
+        ctx.autotune()
+        ctx.swe_rk4_graph(prm, g, 1, 2)
+        ctx.swe_rk4(prm, e, 1, 2)
+        assert np.array_equal(ctx.to_host(g, np.complex128, (3, ctx.nlm)), ctx.to_host(e, np.complex128, (3, ctx.nlm)))
+        rx.close()
+    finally:
+        ctx.close()
+
+
+def test_objects_outlive_their_context():
+    # a REXI object and a band solver whose context is destroyed first: their
+    # destroy calls free their own memory and touch no freed context; a
+    # context created afterwards works and starts with an empty block pool
+    ctx = capi.Context(15)
+    rx = ctx.rexi(capi.rexi_params(dt=600.0, rexi_M=16))
+    bd = ctx.band(2)
+    bd.add(0, 1.0)   # solver_component_scalar_phi
+    buf = ctx.empty(4096)
+    buf.free()
+    lib = ctx.lib
+    ctx.close()
+    assert lib.sphswe_rexi_destroy(rx.h) == 0
+    rx.h = None
+    assert lib.sphswe_band_destroy(bd.h) == 0
+    bd.h = None
+    ctx2 = capi.Context(15)
+    try:
+        q = np.zeros((1, ctx2.nlm), np.complex128)
+        q[0, 1] = 1.0
+        back = ctx2.analysis(ctx2.synthesis(q))
+        assert rel_l2(back, q) < 1e-13
+    finally:
+        ctx2.close()
 
 
 def test_nonlinear_remainder_vs_oracle():

@@ -304,7 +304,8 @@ def test_rexi_inertial_oscillation_closed_form(orc31, dt):
 def test_rexi_first_order_matches_linear_tendency(orc31, omega):
     # f(dt L) = C0 - dt L S0 + O(dt^2) with C0 = sum beta/alpha, S0 = sum beta/alpha^2:
     # the Richardson pair of (f(dt L) U - C0 U) / dt tends to -S0 L U, L the linear
-    # tendency (itself pinned by the RK4 checks), including every Coriolis term
+    # tendency (pinned against the analytic operators in test_oracle_swe.py),
+    # including every Coriolis term
     orc = orc31
     h = random_real_spec(31, 3, power=-2.0) * 50
     h[0] = 0

@@ -1,0 +1,231 @@
+"""Pins of the oracle's RK4 integrator and SWE tendency (CPU only).
+
+RK4 (TimesteppingRK.hpp:310-392, classic tableau :314-327): for a linear
+autonomous tendency x' = T x, one step of the classic RK4 is exactly the
+degree-4 Taylor polynomial R(dt T) x = sum_k (dt T)^k x / k!, k = 0..4.  The
+linear SWE tendency (nonlinear = 0, with rotation and viscosity) is linear, so
+one oracle RK4 step must equal R(dt T) x0 built from four applications of the
+oracle's own tendency.  A wrong stage weight or stage coefficient changes the
+polynomial (checked below on the same operator with mis-set tableaux).
+
+Tendency (AppSWE.hpp:574-659): on smooth, well-resolved fields the oracle's
+nonlinear h, u, v tendencies equal the pointwise analytic expressions
+  h_t = -(1/(a cos phi)) [d_lam(h u) + d_phi(h v cos phi)]
+  u_t = -(g/(a cos phi)) d_lam h + f v - (1/a)(u/cos phi d_lam u + v d_phi u)
+  v_t = -(g/a) d_phi h - f u - (1/a)(u/cos phi d_lam v + v d_phi v)
+with f = 2 Omega sin(phi) (the reference's grad_lon = d_lam / cos phi,
+grad_lat = d_phi and div_lat(x) = d_phi(x cos phi)/cos phi, SPHOperators.hpp:
+100-119,339-439), projected by the same Gauss analysis.  Every term is checked
+to contribute far above the tolerance, so a dropped term or a flipped sign
+fails.  The vorticity and divergence diagnostics (SPHOperators.hpp:507-529,
+with the reference's vort = grad_lon(v) - grad_lat(u), no metric term) are
+pinned the same way.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests._util import A_EARTH, G, H0, OMEGA, random_real_spec, rel_l2
+
+
+# ---------------------------------------------------------------- RK4 --------
+
+def _taylor4(orc, prm, x, dt):
+    out = [y.copy() for y in x]
+    term = [y.copy() for y in x]
+    for k in range(1, 5):
+        t = orc.swe_tendency(prm, *term)
+        term = [dt * y / k for y in t]
+        out = [o + y for o, y in zip(out, term)]
+    return out
+
+
+def _rk_generic(orc, prm, x, dt, A, b):
+    """Explicit Runge-Kutta with tableau (A, b) on the oracle tendency."""
+    ks = []
+    for s in range(len(b)):
+        xs = [y.copy() for y in x]
+        for j in range(s):
+            xs = [xs[f] + dt * A[s][j] * ks[j][f] for f in range(3)]
+        ks.append(orc.swe_tendency(prm, *xs))
+    return [x[f] + dt * sum(b[s] * ks[s][f] for s in range(len(b))) for f in range(3)]
+
+
+CLASSIC_A = [[], [0.5], [0.0, 0.5], [0.0, 0.0, 1.0]]
+CLASSIC_B = [1 / 6, 1 / 3, 1 / 3, 1 / 6]
+
+
+@pytest.mark.parametrize("omega,visc", [(OMEGA, 1e5), (0.0, 0.0)])
+def test_rk4_step_equals_taylor_polynomial_of_linear_tendency(omega, visc):
+    L = 31
+    orc = O.Oracle(L)
+    h = random_real_spec(L, 21, power=-2.0) * 50.0
+    h[0] = H0 * np.sqrt(4 * np.pi)
+    u, v = random_real_spec(L, 22, power=-2.0) * 5.0, random_real_spec(L, 23, power=-2.0) * 5.0
+    # band-limited physical start state: its analysis returns the spectra exactly
+    hs, us, vs = (orc.synthesis(x) for x in (h, u, v))
+    x0 = [orc.analysis(y) for y in (hs, us, vs)]
+    dt = 600.0
+    prm = dict(a=A_EARTH, g=G, omega=omega, h0=H0, visc=visc, nonlinear=0, dt=dt)
+    step = orc.swe_rk4_run(prm, hs, us, vs, 1)
+    poly = _taylor4(orc, prm, x0, dt)
+    # the same step through a generic RK driver with the classic tableau
+    gen = _rk_generic(orc, prm, x0, dt, CLASSIC_A, CLASSIC_B)
+    incr = [poly[f] - x0[f] for f in range(3)]
+    for f, name in enumerate("huv"):
+        e = rel_l2(step[name] - x0[f], incr[f])
+        assert e < 1e-12, (name, e)
+        assert rel_l2(gen[f] - x0[f], incr[f]) < 1e-12, name
+    # mis-set tableaux on the same operator move the increment far from the
+    # polynomial: a swapped weight pair, a stage coefficient of 1 instead of 1/2
+    bad_b = _rk_generic(orc, prm, x0, dt, CLASSIC_A, [1 / 6, 1 / 6, 1 / 2, 1 / 6])
+    bad_a = _rk_generic(orc, prm, x0, dt, [[], [1.0], [0.0, 0.5], [0.0, 0.0, 1.0]], CLASSIC_B)
+    for bad in (bad_b, bad_a):
+        worst = max(rel_l2(bad[f] - x0[f], incr[f]) for f in range(3))
+        assert worst > 1e-6, worst
+
+
+def test_rk4_two_steps_compose():
+    # the integrator carries its state between steps: two single steps equal
+    # one two-step run bit for bit (no hidden stage state)
+    L = 15
+    orc = O.Oracle(L)
+    h = orc.synthesis(random_real_spec(L, 7, power=-2.0) * 30.0 + 0)
+    h = h + H0
+    u = orc.synthesis(random_real_spec(L, 8, power=-2.0) * 3.0)
+    v = orc.synthesis(random_real_spec(L, 9, power=-2.0) * 3.0)
+    prm = dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, visc=1e5, nonlinear=1, dt=300.0)
+    two = orc.swe_rk4_run(prm, h, u, v, 2)
+    one = orc.swe_rk4_run(prm, h, u, v, 1)
+    again = orc.swe_rk4_run(prm, orc.synthesis(one["h"]), orc.synthesis(one["u"]), orc.synthesis(one["v"]), 1)
+    for name in "huv":
+        assert rel_l2(again[name], two[name]) < 1e-13, name
+
+
+# ---------------------------------------------------------- tendency ---------
+
+def _bump(orc, lat0, lon0, sig):
+    """exp((c - 1)/sig), c = cos of the great-circle angle to (lat0, lon0), and
+    its analytic d/dlam, d/dphi on the grid (shape nphi x nlat)."""
+    phi = np.arcsin(orc.ct)[None, :]
+    lam = orc.lon()[:, None]
+    c = np.sin(phi) * np.sin(lat0) + np.cos(phi) * np.cos(lat0) * np.cos(lam - lon0)
+    B = np.exp((c - 1.0) / sig)
+    c_lam = -np.cos(phi) * np.cos(lat0) * np.sin(lam - lon0)
+    c_phi = np.cos(phi) * np.sin(lat0) - np.sin(phi) * np.cos(lat0) * np.cos(lam - lon0)
+    return B, B * c_lam / sig, B * c_phi / sig
+
+
+def _fields(orc, with_h=True):
+    # widths about 0.25 rad: resolved at T127 and below 1e-10 of their peak
+    # at both poles, where the scalar-wind metric terms are not band-limited
+    Hb, Hl, Hp = _bump(orc, 0.2, 1.0, 0.03)
+    Ub, Ul, Up = _bump(orc, -0.1, 1.3, 0.03)
+    Vb, Vl, Vp = _bump(orc, 0.3, 0.7, 0.03)
+    ah = 50.0 if with_h else 0.0
+    h, h_l, h_p = H0 + ah * Hb, ah * Hl, ah * Hp
+    u, u_l, u_p = 10.0 * Ub, 10.0 * Ul, 10.0 * Up
+    v, v_l, v_p = 8.0 * Vb, 8.0 * Vl, 8.0 * Vp
+    return (h, h_l, h_p), (u, u_l, u_p), (v, v_l, v_p)
+
+
+def _terms(orc, fields, omega):
+    """The analytic tendency of every term of AppSWE.hpp:631-649, pointwise."""
+    (h, h_l, h_p), (u, u_l, u_p), (v, v_l, v_p) = fields
+    phi = np.arcsin(orc.ct)[None, :]
+    cphi, sphi = np.cos(phi), np.sin(phi)
+    a, g = A_EARTH, G
+    f = 2.0 * omega * sphi
+    T = {}
+    T["h_flux_lon"] = -(h_l * u + h * u_l) / (a * cphi)
+    T["h_flux_lat"] = -((h_p * v + h * v_p) * cphi - h * v * sphi) / (a * cphi)
+    T["u_grad"] = -g * h_l / (a * cphi)
+    T["u_cor"] = f * v
+    T["u_adv_lon"] = -(u * u_l / cphi) / a
+    T["u_adv_lat"] = -(v * u_p) / a
+    T["v_grad"] = -g * h_p / a
+    T["v_cor"] = -f * u
+    T["v_adv_lon"] = -(u * v_l / cphi) / a
+    T["v_adv_lat"] = -(v * v_p) / a
+    return T
+
+
+@pytest.mark.parametrize("omega,with_h", [(OMEGA, True), (0.0, True), (OMEGA, False)])
+def test_nonlinear_tendency_equals_analytic(omega, with_h):
+    L = 127
+    orc = O.Oracle(L)
+    fields = _fields(orc, with_h)
+    hs, us, vs = (orc.analysis(x[0].reshape(-1)) for x in fields)
+    prm = dict(a=A_EARTH, g=G, omega=omega, h0=H0, visc=0.0, nonlinear=1)
+    th, tu, tv = orc.swe_tendency(prm, hs, us, vs)
+    T = _terms(orc, fields, omega)
+    groups = {"h": ("h_flux_lon", "h_flux_lat"),
+              "u": ("u_grad", "u_cor", "u_adv_lon", "u_adv_lat"),
+              "v": ("v_grad", "v_cor", "v_adv_lon", "v_adv_lat")}
+    out = {"h": th, "u": tu, "v": tv}
+    tol = 1e-8
+    for name, keys in groups.items():
+        exact = sum(T[k] for k in keys)
+        ref = orc.analysis(exact.reshape(-1))
+        e = rel_l2(out[name], ref)
+        assert e < tol, (name, e)
+        # every term present in this case is far above the tolerance, so that a
+        # dropped term or a flipped sign (2x the term) fails
+        for k in keys:
+            t = orc.analysis(T[k].reshape(-1))
+            if np.linalg.norm(t) == 0.0:
+                continue
+            assert np.linalg.norm(t) / np.linalg.norm(ref) > 1e3 * tol, (name, k)
+
+
+def test_linear_tendency_equals_analytic():
+    # nonlinear = 0 (AppSWE.hpp:591-603): h0 for h in the fluxes, no advection
+    L = 127
+    orc = O.Oracle(L)
+    fields = _fields(orc)
+    (h, h_l, h_p), (u, u_l, u_p), (v, v_l, v_p) = fields
+    hs, us, vs = (orc.analysis(x[0].reshape(-1)) for x in fields)
+    prm = dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, visc=0.0, nonlinear=0)
+    th, tu, tv = orc.swe_tendency(prm, hs, us, vs)
+    phi = np.arcsin(orc.ct)[None, :]
+    cphi, sphi = np.cos(phi), np.sin(phi)
+    f = 2.0 * OMEGA * sphi
+    eh = -H0 * (u_l / cphi + (v_p * cphi - v * sphi) / cphi) / A_EARTH
+    eu = -G * h_l / (A_EARTH * cphi) + f * v
+    ev = -G * h_p / A_EARTH - f * u
+    for got, ex in ((th, eh), (tu, eu), (tv, ev)):
+        assert rel_l2(got, orc.analysis(ex.reshape(-1))) < 1e-8
+
+
+def test_viscosity_term_is_laplacian():
+    # nu * laplace / a^2 (AppSWE.hpp:651-658): the difference of the tendency
+    # with and without viscosity is -nu n(n+1)/a^2 times each field
+    L = 31
+    orc = O.Oracle(L)
+    h = random_real_spec(L, 31, power=-2.0) * 10
+    u, v = random_real_spec(L, 32, power=-2.0), random_real_spec(L, 33, power=-2.0)
+    base = dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, nonlinear=1)
+    t1 = orc.swe_tendency(dict(base, visc=1e5), h, u, v)
+    t0 = orc.swe_tendency(dict(base, visc=0.0), h, u, v)
+    ls = np.concatenate([np.arange(m, L + 1) for m in range(L + 1)])
+    lap = -ls * (ls + 1.0) * 1e5 / A_EARTH ** 2
+    for a_, b_, x in zip(t1, t0, (h, u, v)):
+        assert rel_l2(a_ - b_, lap * x) < 1e-12
+
+
+def test_vorticity_divergence_diagnostics_equal_analytic():
+    # config 0's diagnostics: vort = (grad_lon v - grad_lat u)/a and
+    # div = (div_lon u + div_lat v)/a after zero steps (orc_swe_rk4_run outputs)
+    L = 127
+    orc = O.Oracle(L)
+    fields = _fields(orc)
+    (h, _, _), (u, u_l, u_p), (v, v_l, v_p) = fields
+    prm = dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, visc=0.0, nonlinear=1, dt=1.0)
+    r = orc.swe_rk4_run(prm, h.reshape(-1), u.reshape(-1), v.reshape(-1), 0)
+    phi = np.arcsin(orc.ct)[None, :]
+    cphi, sphi = np.cos(phi), np.sin(phi)
+    vort = (v_l / cphi - u_p) / A_EARTH
+    div = (u_l / cphi + (v_p * cphi - v * sphi) / cphi) / A_EARTH
+    assert rel_l2(r["vort"], orc.analysis(vort.reshape(-1))) < 1e-8
+    assert rel_l2(r["div"], orc.analysis(div.reshape(-1))) < 1e-8
+    assert rel_l2(r["phi"], G * orc.analysis(h.reshape(-1))) < 1e-14
