@@ -96,6 +96,21 @@ int sphswe_sht_synthesis(sphswe_ctx *ctx, const double *d_spec, double *d_spat, 
 int sphswe_sht_analysis_cplx(sphswe_ctx *ctx, const double *d_spat, double *d_spec, int nfields, int precision);
 int sphswe_sht_synthesis_cplx(sphswe_ctx *ctx, const double *d_spec, double *d_spat, int nfields, int precision);
 
+/* sphswe_sht_analysis that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The Fourier rows
+ * and the Legendre analysis operands live in one fixed device array of the
+ * library, sphswe_noalloc_scratch_bytes() long (96 MiB), held on every device
+ * the library runs on until the process ends; a call whose nfields at this
+ * truncation need more returns SPHSWE_ERR_ARG (FP64: up to 7 fields at T511,
+ * none at T1023).  The array is shared by every context on a device and keeps
+ * the last call's contents: calls that overlap on one device race.  FP64
+ * results equal sphswe_sht_analysis bit for bit; FP32 never uses the stored
+ * tables (sphswe_set_fp32_tables), so it may differ from sphswe_sht_analysis
+ * in the order of its FP32 sums. */
+int sphswe_sht_analysis_noalloc(sphswe_ctx *ctx, const double *d_spat, double *d_spec, int nfields, int precision);
+size_t sphswe_noalloc_scratch_bytes(void);
+
 /* the two stages of the real pair on their own (for measurement and fusion).
  * Fourier layout: complex double[nfields][L+1][nlat], entry (m, j) is
  * (2 pi / nphi) sum_i f(i,j) e^{-2 pi i m i / nphi}. */

@@ -58,6 +58,7 @@ _SIGS = {
     "sphswe_sht_synthesis": [_vp, _vp, _vp, _i, _i],
     "sphswe_sht_analysis_cplx": [_vp, _vp, _vp, _i, _i],
     "sphswe_sht_synthesis_cplx": [_vp, _vp, _vp, _i, _i],
+    "sphswe_sht_analysis_noalloc": [_vp, _vp, _vp, _i, _i],
     "sphswe_fft_analysis": [_vp, _vp, _vp, _i],
     "sphswe_fft_synthesis": [_vp, _vp, _vp, _i],
     "sphswe_legendre_analysis": [_vp, _vp, _vp, _i, _i],
@@ -112,6 +113,8 @@ def load():
         fn.restype = _i
     lib.sphswe_last_error.argtypes = [_vp]
     lib.sphswe_last_error.restype = ctypes.c_char_p
+    lib.sphswe_noalloc_scratch_bytes.argtypes = []
+    lib.sphswe_noalloc_scratch_bytes.restype = _sz
     lib.sphswe_stream.argtypes = [_vp]
     lib.sphswe_stream.restype = _vp
     _lib = lib

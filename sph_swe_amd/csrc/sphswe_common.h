@@ -66,6 +66,14 @@ struct sphswe_ctx {
 	void *d_escratch = nullptr;   // analysis operands (weighted Fourier rows, operand order)
 	size_t escratch_bytes = 0;
 	double *d_flag = nullptr;     // small reduction scratch (64 doubles)
+	// while set (by the no-allocation entry points), the Legendre analyses take
+	// their operand buffer from here, never grow d_escratch, and skip the
+	// stored-table path (whose tables are allocated on first use)
+	void *ext_escratch = nullptr;
+	size_t ext_escratch_bytes = 0;
+	// device address of the library's fixed scratch array on this context's
+	// device (sphswe_noalloc.hip), looked up at create
+	unsigned char *noalloc_scratch = nullptr;
 
 	std::unordered_map<void *, size_t> allocs;
 
@@ -121,6 +129,9 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 int sphswe_leg_anal_cplx(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
 int sphswe_leg_synth_cplx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec);
 int sphswe_leg_anal_reserve(sphswe_ctx *ctx, int nf, int prec);
+// bytes of analysis operands per group of eight real fields in a precision
+size_t sphswe_leg_anal_group_bytes(const sphswe_ctx *ctx, int prec);
+int sphswe_noalloc_init(sphswe_ctx *ctx);
 void sphswe_swe_release(sphswe_ctx *ctx);
 void sphswe_rexi_detach(sphswe_rexi *r);
 void sphswe_band_detach(sphswe_band *b);

@@ -260,6 +260,7 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	if ((rc = upload(ctx, &ctx->d_tw, tw))) return fail(rc);
 	if (hipMalloc((void **)&ctx->d_flag, 64 * sizeof(double)) != hipSuccess)
 		return fail(sphswe_set_err(ctx, SPHSWE_ERR_NOMEM, "hipMalloc flag"));
+	if ((rc = sphswe_noalloc_init(ctx))) return fail(rc);
 	*out = ctx;
 	return SPHSWE_OK;
 }

@@ -352,7 +352,17 @@ int analysis_batches(sphswe_ctx *ctx, const Geo &g, const double2 *four, int nf,
 	const size_t gbytes = (size_t)(g.L + 1) * nch * eop_bytes_per_chunk<PM>();
 	int gb = (int)std::max<size_t>(1, std::min<size_t>(groups, ANALYSIS_MAX_EBYTES / gbytes));
 	void *E = nullptr;
-	if (int rc = sphswe_escratch(ctx, gbytes * gb, &E)) return rc;
+	if (ctx->ext_escratch) {
+		// a caller-provided buffer: as many groups per batch as it holds
+		const size_t fit = ctx->ext_escratch_bytes / gbytes;
+		if (fit < 1)
+			return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "analysis operands need %zu bytes, the scratch holds %zu",
+					      gbytes, ctx->ext_escratch_bytes);
+		gb = (int)std::min<size_t>(groups, fit);
+		E = ctx->ext_escratch;
+	} else if (int rc = sphswe_escratch(ctx, gbytes * gb, &E)) {
+		return rc;
+	}
 	for (int g0 = 0; g0 < groups; g0 += gb) {
 		const int ng = std::min(gb, groups - g0);
 		const long nthr = (long)(g.L + 1) * nch * 64;

@@ -434,7 +434,7 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
 {
 	if (int rc = check_prec(ctx, prec)) return rc;
-	if (prec == SPHSWE_PREC_FP32 && ctx->tab_min_anal > 0 && nf >= ctx->tab_min_anal)
+	if (prec == SPHSWE_PREC_FP32 && ctx->tab_min_anal > 0 && nf >= ctx->tab_min_anal && !ctx->ext_escratch)
 		return sphswe_leg_anal_tab(ctx, four, spec, nf);
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_mx(ctx, four, spec, nf, prec);
 	return launch_anal<LAY_REAL>(ctx, geo_of(ctx), four, spec, nf);
@@ -484,6 +484,19 @@ int sphswe_leg_anal_pole(sphswe_ctx *ctx, const double2 *four, double2 *spec, in
 	g.moff = d_moff;
 	g.P = P;
 	return launch_anal<LAY_POLE>(ctx, g, four, spec, nf);
+}
+
+size_t sphswe_leg_anal_group_bytes(const sphswe_ctx *ctx, int prec)
+{
+	const int nch = (ctx->NJ + 63) / 64;
+	size_t per_chunk = 0;
+	switch (prec) {
+	case SPHSWE_PREC_FP64: per_chunk = eop_bytes_per_chunk<PM_F64>(); break;
+	case SPHSWE_PREC_FP32: per_chunk = eop_bytes_per_chunk<PM_F32>(); break;
+	case SPHSWE_PREC_BF16: per_chunk = eop_bytes_per_chunk<PM_BF16>(); break;
+	default: per_chunk = eop_bytes_per_chunk<PM_BF16X3>(); break;
+	}
+	return (size_t)(ctx->L + 1) * nch * per_chunk;
 }
 
 // sizes the analysis operand buffer for nf real fields in a precision, so that
