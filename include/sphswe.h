@@ -299,6 +299,17 @@ int sphswe_band_solve(sphswe_ctx *ctx, sphswe_band *band, const double *d_rhs, d
 int sphswe_autotune(sphswe_ctx *ctx);
 int sphswe_set_tuning(sphswe_ctx *ctx, int fft_nseq_r2c, int fft_nseq_c2r, int synth_mfma_min);
 int sphswe_get_tuning(const sphswe_ctx *ctx, int *fft_nseq_r2c, int *fft_nseq_c2r, int *synth_mfma_min, int *tuned);
+/* kernel shapes of the FP64 real Legendre transforms (no counterpart in the
+ * reference; every choice computes the same sums in another order):
+ *   synth_shape   vector-unit synthesis workgroups: 0 = 8 wavefronts x 3 rings
+ *                 per lane, fields in groups of three (default); 1 = 4 x 2, fields
+ *                 in groups of four and three;
+ *   anal_variant  0 = up to four fields on v_mfma_f64_4x4x4f64, more fields in
+ *                 groups of eight on v_mfma_f64_16x16x4f64 (default); 1 = always
+ *                 the latter.  The two kernels agree to FP64 round-off. 
+ * sphswe_autotune times both and keeps the faster. */
+int sphswe_set_legendre_tuning(sphswe_ctx *ctx, int synth_shape, int anal_variant);
+int sphswe_get_legendre_tuning(const sphswe_ctx *ctx, int *synth_shape, int *anal_variant);
 
 /* FP32 real Legendre syntheses / analyses (SPHSWE_PREC_FP32) of at least
  * synth_min_fields / anal_min_fields fields run as batched GEMMs on the FP32

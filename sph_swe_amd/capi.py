@@ -76,6 +76,8 @@ _SIGS = {
     "sphswe_autotune": [_vp],
     "sphswe_set_tuning": [_vp, _i, _i, _i],
     "sphswe_get_tuning": [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)],
+    "sphswe_set_legendre_tuning": [_vp, _i, _i],
+    "sphswe_get_legendre_tuning": [_vp, _ip, _ip],
     "sphswe_set_fp32_tables": [_vp, _i, _i],
     "sphswe_get_fp32_tables": [_vp, _ip, _ip],
     "sphswe_rexi_create": [_vp, ctypes.POINTER(RexiParams), ctypes.POINTER(_vp)],
@@ -314,6 +316,14 @@ class Context:
         ts, ta = self.fp32_tables()
         return {"fft_nseq_r2c": v[0].value, "fft_nseq_c2r": v[1].value, "synth_mfma_min": v[2].value,
                 "fp32_table_synth_min": ts, "fp32_table_anal_min": ta, "tuned": bool(v[3].value)}
+
+    def set_legendre_tuning(self, synth_shape=0, anal_variant=0):
+        self._check(self.lib.sphswe_set_legendre_tuning(self.h, synth_shape, anal_variant))
+
+    def legendre_tuning(self):
+        v = [_i(0) for _ in range(2)]
+        self._check(self.lib.sphswe_get_legendre_tuning(self.h, *[ctypes.byref(x) for x in v]))
+        return {"synth_shape": v[0].value, "anal_variant": v[1].value}
 
     def set_tuning(self, fft_nseq_r2c=0, fft_nseq_c2r=0, synth_mfma_min=0):
         self._check(self.lib.sphswe_set_tuning(self.h, fft_nseq_r2c, fft_nseq_c2r, synth_mfma_min))

@@ -38,7 +38,11 @@ struct sphswe_ctx {
 	double2 *d_coef = nullptr;    // recurrence (a_lm, a_lm*b_lm) [nlm], index of the real spectrum
 	double2 *d_seed = nullptr;    // (Pbar_s, Pbar_s+1) at each degree-segment start s [nseg_total * NJ]
 	int *d_segoff = nullptr;      // first segment of each m [L+2]
+	double2 *d_seed16 = nullptr;  // as d_seed at every 16-degree segment start (4x4x4 analysis)
+	int *d_segoff16 = nullptr;
 	int *d_unit_m = nullptr;      // m of each segment [nseg_total]
+	int2 *d_ablk = nullptr;       // (m, first degree) of each 64-degree block of an order
+	int nablk = 0;
 	int seg = 16;                 // degrees per segment
 	int nseg_total = 0;
 	int max_lds = 65536;          // LDS bytes a workgroup may allocate
@@ -46,6 +50,8 @@ struct sphswe_ctx {
 	int fft_nseq_r2c = 0;         // complex sequences per workgroup of the real FFTs
 	int fft_nseq_c2r = 0;
 	int synth_mfma_min = 0;       // FP64 real synthesis on the matrix cores from this many fields (0: never)
+	int synth_shape = 0;          // FP64 vector-unit synthesis: 0 = 8 wavefronts x 3 rings per lane, 1 = 4 x 2
+	int anal_variant = 0;         // FP64 real analysis: 0 = 4x4x4 kernel for <= 4 fields, 1 = 16x16x4 kernel always
 	int tab_min_synth = 0;        // FP32 real syntheses from stored tables from this many fields (0: never)
 	int tab_min_anal = 0;         // FP32 real analyses likewise
 	// FP32 associated-Legendre tables (built on first use, sphswe_legendre_tab.hip)

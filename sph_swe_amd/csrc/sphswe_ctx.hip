@@ -230,11 +230,28 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	std::vector<double2> seed;
 	// 32-degree segments (one MFMA analysis wavefront each)
 	ctx->seg = 32;
-	make_seeds(lmax, ctx->NJ, ctx->seg, ctx->ct, ctx->st, segoff, seed);
+	// seeds every 16 degrees (the fused analysis); those of the 32-degree
+	// segments are every other one of them
+	std::vector<int> segoff16;
+	std::vector<double2> seed16;
+	make_seeds(lmax, ctx->NJ, 16, ctx->ct, ctx->st, segoff16, seed16);
+	segoff.assign(lmax + 2, 0);
+	for (int m = 0; m <= lmax; m++) segoff[m + 1] = segoff[m] + (lmax - m + 32) / 32;
+	seed.resize((size_t)segoff[lmax + 1] * ctx->NJ);
+	for (int m = 0; m <= lmax; m++)
+		for (int k = 0; k < segoff[m + 1] - segoff[m]; k++)
+			std::copy(seed16.begin() + (size_t)(segoff16[m] + 2 * k) * ctx->NJ,
+				  seed16.begin() + (size_t)(segoff16[m] + 2 * k + 1) * ctx->NJ,
+				  seed.begin() + (size_t)(segoff[m] + k) * ctx->NJ);
 	ctx->nseg_total = segoff[lmax + 1];
 	std::vector<int> unit_m(ctx->nseg_total);
 	for (int m = 0; m <= lmax; m++)
 		for (int k = segoff[m]; k < segoff[m + 1]; k++) unit_m[k] = m;
+	// 64-degree blocks of every order, (m, first degree), for the 4x4x4 analysis
+	std::vector<int2> ablk;
+	for (int m = 0; m <= lmax; m++)
+		for (int d0 = m; d0 <= lmax; d0 += 64) ablk.push_back(make_int2(m, d0));
+	ctx->nablk = (int)ablk.size();
 	{
 		int v = 0;
 		if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess) ctx->max_lds = v;
@@ -253,8 +270,11 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	if ((rc = upload(ctx, &ctx->d_wg, w))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_coef, coef))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_seed, seed))) return fail(rc);
+	if ((rc = upload(ctx, &ctx->d_seed16, seed16))) return fail(rc);
+	if ((rc = upload(ctx, &ctx->d_segoff16, segoff16))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_segoff, segoff))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_unit_m, unit_m))) return fail(rc);
+	if ((rc = upload(ctx, &ctx->d_ablk, ablk))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_mofidx, mof))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_lofcidx, lofc))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_tw, tw))) return fail(rc);
@@ -281,7 +301,7 @@ extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 	ctx->rexis.clear();
 	ctx->bands.clear();
 	void *ptrs[] = {ctx->d_ct, ctx->d_st, ctx->d_sq, ctx->d_coslat, ctx->d_wg, ctx->d_coef,
-			ctx->d_seed, ctx->d_segoff, ctx->d_unit_m, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_escratch, ctx->d_flag};
+			ctx->d_seed, ctx->d_segoff, ctx->d_seed16, ctx->d_segoff16, ctx->d_unit_m, ctx->d_ablk, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_escratch, ctx->d_flag};
 	for (void *p : ptrs)
 		if (p) hipFree(p);
 	for (auto &kv : ctx->allocs) hipFree(kv.first);

@@ -34,14 +34,13 @@ namespace {
 // and spectral values are uniform over a wavefront (scalar loads); the partial
 // sums of the wavefronts of an order are reduced through LDS in a fixed order
 // (deterministic).
-constexpr int SY_RPL = 2;          // rings per lane
-constexpr int SY_WAVES = 4;        // wavefronts per workgroup
-constexpr int SY_RW = 64 * SY_RPL; // rings per workgroup
-
-template <int NS>
+// W wavefronts per workgroup, RPL rings per lane (64 RPL rings per
+// workgroup); the partial sums of the wavefronts of one order are added in a
+// fixed pairwise tree through LDS, at most W/2 wavefronts' partials at a time.
+template <int NS, int W, int RPL>
 __host__ __device__ inline size_t synth_lds_bytes()
 {
-	return sizeof(double) * (size_t)(SY_WAVES - 1) * NS * 4 * SY_RW;
+	return sizeof(double) * (size_t)(W / 2) * NS * 4 * 64 * RPL;
 }
 
 
@@ -49,7 +48,7 @@ __host__ __device__ inline size_t synth_lds_bytes()
 // this lane's rings.  Every segment restarts the recurrence from its stored
 // seeds; a seed that underflows only drops terms below 2^-981 (the recurrence
 // grows by less than 2^41 over a segment).
-template <int LAY, int NS>
+template <int LAY, int NS, int SY_RPL>
 __device__ __forceinline__ void synth_range(const double2 *__restrict__ spec, const Slot<LAY> (&sl)[NS], int am, int s0,
 					    int se, int so, const Geo &g, const int (&jcr)[SY_RPL], const double (&mu)[SY_RPL],
 					    double (&eR)[SY_RPL][NS], double (&eI)[SY_RPL][NS], double (&oR)[SY_RPL][NS],
@@ -147,13 +146,14 @@ __device__ __forceinline__ void synth_range(const double2 *__restrict__ spec, co
 	}
 }
 
-template <int LAY, int NS>
+template <int LAY, int NS, int SY_WAVES, int SY_RPL>
 __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__restrict__ spec,
 							     double2 *__restrict__ four, Geo g, int nfields,
 							     const double *__restrict__ oscale)
 {
+	constexpr int SY_RW = 64 * SY_RPL;
 	extern __shared__ __align__(16) unsigned char lds_raw[];
-	double *red = (double *)lds_raw;     // [wave - 1][NS*4][SY_RW]
+	double *red = (double *)lds_raw;     // [W / 2][NS*4][SY_RW]
 	const int L = g.L, NJ = g.NJ, seg = g.seg;
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 	const int ma = blockIdx.y, mb = L - (int)blockIdx.y;
@@ -184,39 +184,48 @@ __global__ __launch_bounds__(64 * SY_WAVES) void k_leg_synth(const double2 *__re
 #pragma unroll
 		for (int s = 0; s < NS; s++) eR[r][s] = eI[r][s] = oR[r][s] = oI[r][s] = 0.0;
 	if (sb > sa)
-		synth_range<LAY, NS>(spec, sl, am, __builtin_amdgcn_readfirstlane(am + sa * seg),
-				     __builtin_amdgcn_readfirstlane(min(am + sb * seg - 1, L)), g.segoff[am], g, jcr, mu, eR, eI,
-				     oR, oI);
-	// the first wavefront of each order reduces; the others write their partials
-	const bool leader = wi == 0;
-	if (!leader) {
-		double *rw = red + (size_t)(wave - 1) * NS * 4 * SY_RW;
+		synth_range<LAY, NS, SY_RPL>(spec, sl, am, __builtin_amdgcn_readfirstlane(am + sa * seg),
+					     __builtin_amdgcn_readfirstlane(min(am + sb * seg - 1, L)), g.segoff[am], g, jcr,
+					     mu, eR, eI, oR, oI);
+	// pairwise tree over the nw wavefronts of this order: in each round the
+	// upper half [h, n) writes its partials, the lower half adds them; every
+	// wavefront takes part in the same number of rounds (barriers)
+	constexpr int REG = NS * 4 * SY_RW;
+	const int roff = part ? kA / 2 : 0;   // first LDS region of this order
+	int n = nw;
+	for (int span = max(kA, SY_WAVES - kA); span > 1; span = (span + 1) / 2) {
+		const int h = (n + 1) / 2;
+		if (n > 1 && wi >= h && wi < n) {
+			double *rw = red + (size_t)(roff + wi - h) * REG;
 #pragma unroll
-		for (int r = 0; r < SY_RPL; r++)
+			for (int r = 0; r < SY_RPL; r++)
 #pragma unroll
-			for (int s = 0; s < NS; s++) {
-				const int o = r * 64 + lane;
-				rw[(4 * s + 0) * SY_RW + o] = eR[r][s];
-				rw[(4 * s + 1) * SY_RW + o] = eI[r][s];
-				rw[(4 * s + 2) * SY_RW + o] = oR[r][s];
-				rw[(4 * s + 3) * SY_RW + o] = oI[r][s];
-			}
+				for (int s = 0; s < NS; s++) {
+					const int o = r * 64 + lane;
+					rw[(4 * s + 0) * SY_RW + o] = eR[r][s];
+					rw[(4 * s + 1) * SY_RW + o] = eI[r][s];
+					rw[(4 * s + 2) * SY_RW + o] = oR[r][s];
+					rw[(4 * s + 3) * SY_RW + o] = oI[r][s];
+				}
+		}
+		__syncthreads();
+		if (n > 1 && wi < n - h) {
+			const double *rw = red + (size_t)(roff + wi) * REG;
+#pragma unroll
+			for (int r = 0; r < SY_RPL; r++)
+#pragma unroll
+				for (int s = 0; s < NS; s++) {
+					const int o = r * 64 + lane;
+					eR[r][s] += rw[(4 * s + 0) * SY_RW + o];
+					eI[r][s] += rw[(4 * s + 1) * SY_RW + o];
+					oR[r][s] += rw[(4 * s + 2) * SY_RW + o];
+					oI[r][s] += rw[(4 * s + 3) * SY_RW + o];
+				}
+		}
+		__syncthreads();
+		if (n > 1) n = h;
 	}
-	__syncthreads();
-	if (!leader) return;
-	for (int w = wave + 1; w < wave + nw; w++) {
-		const double *rw = red + (size_t)(w - 1) * NS * 4 * SY_RW;
-#pragma unroll
-		for (int r = 0; r < SY_RPL; r++)
-#pragma unroll
-			for (int s = 0; s < NS; s++) {
-				const int o = r * 64 + lane;
-				eR[r][s] += rw[(4 * s + 0) * SY_RW + o];
-				eI[r][s] += rw[(4 * s + 1) * SY_RW + o];
-				oR[r][s] += rw[(4 * s + 2) * SY_RW + o];
-				oI[r][s] += rw[(4 * s + 3) * SY_RW + o];
-			}
-	}
+	if (wi != 0) return;
 #pragma unroll
 	for (int r = 0; r < SY_RPL; r++) {
 		const int j = jr[r];
@@ -359,20 +368,246 @@ __global__ __launch_bounds__(64 * AN_WAVES) void k_leg_anal(const double *__rest
 	}
 }
 
+// ----------------------------------------------------------------------------
+// analysis on the 4x4x4 FP64 matrix instruction, no operand pass
+// ----------------------------------------------------------------------------
+
+// v_mfma_f64_4x4x4f64 computes four independent 4x4x4 products per wavefront
+// (it sustains a higher FP64 rate than the 16x16x4 form).  Lane layout
+// (checked on the device): lane = 16 k + 4 blk + x, block blk = (lane >> 2) & 3,
+//   A_blk[i][k] in lane 16 k + 4 blk + i,
+//   B_blk[k][n] in lane 16 k + 4 blk + n,
+//   D_blk[i][n] in lane 16 i + 4 blk + n.
+// A workgroup of AW wavefronts takes AW * 32 degrees of one order m (a block of
+// the table ablk) and a group of up to four real fields.  Wavefront w takes the
+// 32 degrees H = d0 + 32 w ...: lanes 0-31 run the recurrence for the first 16
+// degrees, lanes 32-63 for the next 16 (each from the seeds of its 16-degree
+// segment), both over the 32 rings of a chunk (lane & 31).  The tile
+// T[ring][row] holds Pbar with rows 16 h + (even degrees 0-7 | odd 8-15) of
+// half h.  Per 4-ring k-step t and row set s (= half), blocks blk are
+// (parity blk >> 1, degree quad blk & 1), and column group cg (two slots) is
+// one instruction:
+//   D[s][cg] += Pbar[4 degrees][4 rings] * E[4 rings][4 columns],
+// E[ring][col] = w_j (F_N + F_S) for col < 8, w_j (F_N - F_S) for col >= 8,
+// col = 8 parity + 2 slot + re/im, staged per chunk by the workgroup from the
+// Fourier rows (coalesced) into one of two LDS buffers.
+constexpr int A4_PS = 33;   // LDS row stride (doubles) of a tile row (one ring)
+constexpr int A4_ES = 17;   // LDS row stride of a staged E row
+
+template <int AW, int NSL>
+__global__ __launch_bounds__(64 * AW) void k_leg_anal4(const double2 *__restrict__ four, double2 *__restrict__ spec,
+							Geo g, const double2 *__restrict__ seed16,
+							const int *__restrict__ segoff16, const int2 *__restrict__ ablk,
+							int nblk, int nblk_pad, int group, int nfields)
+{
+	constexpr int NCG = NSL / 2;            // column groups of four columns (two slots)
+	constexpr int ES = 4 * NSL + 1;         // LDS row stride of a staged E row (padded)
+	constexpr int NST = (NSL * 32 + 64 * AW - 1) / (64 * AW);   // staging pairs per thread
+	__shared__ double T[AW][32][A4_PS];
+	__shared__ double Eb[2][32][ES];
+	__shared__ double2 Cf[AW][32];
+	const int lb = xcd_remap(blockIdx.x, nblk_pad);
+	if (lb >= nblk) return;   // whole workgroup
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+	const int2 bk = ablk[lb];
+	const int am = bk.x;
+	const int L = g.L, NJ = g.NJ, nlat = g.nlat;
+	const int H = bk.y + 32 * wave;             // first degree of this wavefront
+	const bool active = H <= L;                 // wavefront-uniform
+	const int rh = lane >> 5, rr = lane & 31;   // recurrence: half, ring of the chunk
+	const int h0 = H + 16 * rh;                 // first degree of this lane's half
+	const bool ract = h0 <= L;
+	const int useg = segoff16[am] + (min(h0, L) - am) / 16;
+	const int nch = (NJ + 31) / 32;
+	// MFMA lane roles
+	const int mk = lane >> 4, mblk = (lane >> 2) & 3, mx = lane & 3;
+	const int f0 = group * NSL;
+
+	// staging: pair = (slot, ring) of a chunk, NST pairs per thread
+	struct Stage {
+		double2 fn[NST], fs[NST];
+		double w[NST];
+	};
+	auto stage_load = [&](int c) __attribute__((always_inline)) {
+		Stage t;
+#pragma unroll
+		for (int k = 0; k < NST; k++) {
+			const int pr = threadIdx.x + k * 64 * AW;
+			const int sl = pr >> 5, r = pr & 31;
+			const int j = c * 32 + r, jc = min(j, NJ - 1);
+			const bool ok = pr < NSL * 32 && f0 + sl < nfields;
+			const double2 *Fs = four + ((long)(ok ? f0 + sl : 0) * (L + 1) + am) * nlat;
+			t.w[k] = (ok && j < NJ) ? g.wg[jc] : 0.0;
+			t.fn[k] = ok ? Fs[jc] : make_double2(0.0, 0.0);
+			t.fs[k] = ok ? Fs[nlat - 1 - jc] : make_double2(0.0, 0.0);
+		}
+		return t;
+	};
+	auto stage_store = [&](int buf, const Stage &t) __attribute__((always_inline)) {
+#pragma unroll
+		for (int k = 0; k < NST; k++) {
+			const int pr = threadIdx.x + k * 64 * AW;
+			if (pr >= NSL * 32) continue;
+			const int sl = pr >> 5, r = pr & 31;
+			double *e = Eb[buf][r];
+			e[2 * sl] = t.w[k] * (t.fn[k].x + t.fs[k].x);
+			e[2 * sl + 1] = t.w[k] * (t.fn[k].y + t.fs[k].y);
+			e[2 * NSL + 2 * sl] = t.w[k] * (t.fn[k].x - t.fs[k].x);
+			e[2 * NSL + 2 * sl + 1] = t.w[k] * (t.fn[k].y - t.fs[k].y);
+		}
+	};
+	double2 *cf = Cf[wave];
+	if (lane < 32) cf[lane] = g.coef[lm_index(L, min(H + lane, L), am)];
+	double (*Tw)[A4_PS] = T[wave];
+	// seed and mu of this lane's ring in chunk c, loaded one chunk ahead
+	struct RingIn {
+		double2 sd;
+		double mu;
+	};
+	auto ring_in = [&](int c) __attribute__((always_inline)) {
+		const int jc = min(c * 32 + rr, NJ - 1);
+		RingIn r;
+		r.mu = g.ct[jc];
+		r.sd = (active && ract) ? seed16[(long)useg * NJ + jc] : make_double2(0.0, 0.0);
+		return r;
+	};
+	// the tile rows of this lane's half: degree h0 + t is row 16 rh + (t odd ? 8 + t/2 : t/2)
+	auto make_tile = [&](const RingIn &ri) __attribute__((always_inline)) {
+		double *row = Tw[rr] + 16 * rh;
+		double p0 = ri.sd.x, p1 = ri.sd.y;
+		row[0] = p0;
+		row[8] = p1;
+#pragma unroll
+		for (int tb = 2; tb < 16; tb += 7) {
+			double2 cc[7];
+			double v[7];
+#pragma unroll
+			for (int k = 0; k < 7; k++) cc[k] = cf[16 * rh + tb + k];
+#pragma unroll
+			for (int k = 0; k < 7; k++) {
+				v[k] = rec_step(cc[k], ri.mu, p1, p0);
+				p0 = p1;
+				p1 = v[k];
+			}
+#pragma unroll
+			for (int k = 0; k < 7; k++) {
+				const int t = tb + k;
+				row[(t & 1) ? 8 + (t >> 1) : (t >> 1)] = v[k];
+			}
+		}
+	};
+	double acc[2][NCG];
+#pragma unroll
+	for (int s = 0; s < 2; s++)
+#pragma unroll
+		for (int cg = 0; cg < NCG; cg++) acc[s][cg] = 0.0;
+
+	// global loads run two chunks ahead of their use (HBM latency is thousands
+	// of cycles); the loop is unrolled by two so that the register sets
+	// alternate without copies (a copy would wait for the load in flight)
+	{
+		const Stage t0 = stage_load(0);
+		stage_store(0, t0);
+	}
+	const RingIn r0 = ring_in(0);
+	Stage sA = stage_load(min(1, nch - 1)), sB = stage_load(min(2, nch - 1));
+	RingIn rA = ring_in(min(1, nch - 1)), rB = ring_in(min(2, nch - 1));
+	wave_sync();   // the coefficients
+	if (active) make_tile(r0);
+	__syncthreads();
+	// chunk c: contract the tile and staged operands of c, make the tile of
+	// c + 1 from r_use, stage c + 1 from s_use, then load c + 3 into them
+	auto chunk = [&](int c, Stage &s_use, RingIn &r_use) __attribute__((always_inline)) {
+		const bool more = c + 1 < nch;
+		if (active) {
+			double a[8][2], b[8][NCG];
+#pragma unroll
+			for (int t = 0; t < 8; t++) {
+				const double *tr = Tw[4 * t + mk];
+				a[t][0] = tr[4 * mblk + mx];
+				a[t][1] = tr[16 + 4 * mblk + mx];
+				const double *er = Eb[c & 1][4 * t + mk] + 2 * NSL * (mblk >> 1) + mx;
+#pragma unroll
+				for (int cg = 0; cg < NCG; cg++) b[t][cg] = er[4 * cg];
+			}
+			wave_sync();   // the tile is free
+#pragma unroll
+			for (int t = 0; t < 8; t++)
+#pragma unroll
+				for (int s = 0; s < 2; s++)
+#pragma unroll
+					for (int cg = 0; cg < NCG; cg++)
+						acc[s][cg] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[t][s], b[t][cg], acc[s][cg], 0, 0, 0);
+			if (more) make_tile(r_use);
+		}
+		if (more) {
+			stage_store((c + 1) & 1, s_use);
+			__syncthreads();
+		}
+		s_use = stage_load(min(c + 3, nch - 1));
+		r_use = ring_in(min(c + 3, nch - 1));
+	};
+	for (int c = 0; c < nch; c += 2) {
+		chunk(c, sA, rA);
+		if (c + 1 < nch) chunk(c + 1, sB, rB);
+	}
+	if (!active) return;
+	// D[i][n] of block mblk in lane 16 i + 4 mblk + n: i = mk, n = mx; row
+	// 16 s + 4 mblk + i -> degree H + 16 s + 2 (4 (mblk & 1) + i) + (mblk >> 1)
+	const int par = mblk >> 1, q = 4 * (mblk & 1) + mk;
+	const long nlm = (long)(L + 1) * (L + 2) / 2;
+#pragma unroll
+	for (int cg = 0; cg < NCG; cg++) {
+		const int f = f0 + 2 * cg + (mx >> 1);
+		if (f >= nfields) continue;
+		double *sdo = (double *)spec + 2 * (f * nlm + lm_index(L, 0, am)) + (mx & 1);
+#pragma unroll
+		for (int s = 0; s < 2; s++) {
+			const int l = H + 16 * s + 2 * q + par;
+			if (l <= L) sdo[2 * (long)l] = acc[s][cg];
+		}
+	}
+}
+
+// groups of NSL fields, workgroups of AW wavefronts (32 AW degrees; the block
+// table has 64-degree blocks, so AW = 2)
+template <int AW, int NSL>
+int launch_anal4(sphswe_ctx *ctx, const Geo &g, const double2 *four, double2 *spec, int nf)
+{
+	static_assert(AW == 2, "block table of 64 degrees");
+	const int nblk = ctx->nablk, nblk_pad = (nblk + 7) & ~7;
+	const int2 *tab = ctx->d_ablk;
+	for (int gr = 0; gr < (nf + NSL - 1) / NSL; gr++) {
+		k_leg_anal4<AW, NSL><<<nblk_pad, 64 * AW, 0, ctx->stream>>>(four, spec, g, ctx->d_seed16, ctx->d_segoff16,
+									     tab, nblk, nblk_pad, gr, nf);
+		SPHSWE_LAUNCH_CHECK(ctx);
+	}
+	return SPHSWE_OK;
+}
+
+template <int LAY, int NS, int W, int RPL>
+int launch_synth_wr(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 *four, int nf, const double *oscale)
+{
+	const int per = (LAY != LAY_REAL) ? NS / 2 : NS;
+	const size_t lds = synth_lds_bytes<NS, W, RPL>();
+	if ((long)lds > ctx->max_lds) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "synthesis needs %zu bytes of LDS", lds);
+	if (lds > 65536)
+		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_leg_synth<LAY, NS, W, RPL>,
+						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	// one workgroup per (64 RPL rings, pair of orders (m, L - m), group of slots)
+	dim3 grid((ctx->NJ + 64 * RPL - 1) / (64 * RPL), ctx->L / 2 + 1, (nf + per - 1) / per);
+	k_leg_synth<LAY, NS, W, RPL><<<grid, 64 * W, lds, ctx->stream>>>(spec, four, g, nf, oscale);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return SPHSWE_OK;
+}
+
+// the workgroup shape chosen for the context: 8 wavefronts x 3 rings per lane
+// (the default, 0) or 4 x 2 (1)
 template <int LAY, int NS>
 int launch_synth(sphswe_ctx *ctx, const Geo &g, const double2 *spec, double2 *four, int nf, const double *oscale)
 {
-	const int per = (LAY != LAY_REAL) ? NS / 2 : NS;
-	const size_t lds = synth_lds_bytes<NS>();
-	if ((long)lds > ctx->max_lds) return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "synthesis needs %zu bytes of LDS", lds);
-	if (lds > 65536)
-		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_leg_synth<LAY, NS>,
-						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-	// one workgroup per (128 rings, pair of orders (m, L - m), group of slots)
-	dim3 grid((ctx->NJ + SY_RW - 1) / SY_RW, ctx->L / 2 + 1, (nf + per - 1) / per);
-	k_leg_synth<LAY, NS><<<grid, 64 * SY_WAVES, lds, ctx->stream>>>(spec, four, g, nf, oscale);
-	SPHSWE_LAUNCH_CHECK(ctx);
-	return SPHSWE_OK;
+	if (ctx->synth_shape == 1) return launch_synth_wr<LAY, NS, 4, 2>(ctx, g, spec, four, nf, oscale);
+	return launch_synth_wr<LAY, NS, 8, 3>(ctx, g, spec, four, nf, oscale);
 }
 
 template <int LAY>
@@ -409,14 +644,22 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 	// groups of 4 and 3 fields, no group of one where avoidable: a group
 	// reruns the recurrence, so a lone field costs nearly as much as four
 	int n4 = nf / 4, n3 = 0, n2 = 0, n1 = 0;
-	switch (nf % 4) {
-	case 1:
-		if (nf >= 9) { n4 -= 2; n3 = 3; } else if (nf == 5) { n4 = 0; n3 = 1; n2 = 1; } else n1 = 1;
-		break;
-	case 2:
-		if (nf >= 6) { n4 -= 1; n3 = 2; } else n2 = 1;
-		break;
-	case 3: n3 = 1; break;
+	if (ctx->synth_shape == 0) {
+		// the 8 x 3 shape: groups of three (its four-field form is slower)
+		n4 = 0;
+		n3 = nf / 3;
+		n2 = nf % 3 == 2;
+		n1 = nf % 3 == 1;
+	} else {
+		switch (nf % 4) {
+		case 1:
+			if (nf >= 9) { n4 -= 2; n3 = 3; } else if (nf == 5) { n4 = 0; n3 = 1; n2 = 1; } else n1 = 1;
+			break;
+		case 2:
+			if (nf >= 6) { n4 -= 1; n3 = 2; } else n2 = 1;
+			break;
+		case 3: n3 = 1; break;
+		}
 	}
 	const Geo g = geo_of(ctx);
 	int rc = SPHSWE_OK;
@@ -437,6 +680,10 @@ int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf,
 	if (prec == SPHSWE_PREC_FP32 && ctx->tab_min_anal > 0 && nf >= ctx->tab_min_anal && !ctx->ext_escratch)
 		return sphswe_leg_anal_tab(ctx, four, spec, nf);
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_mx(ctx, four, spec, nf, prec);
+	// up to four fields on the 4x4x4 kernel (default; it runs one group of four
+	// in about two thirds of the 16x16x4 kernel's time), more fields in groups of
+	// eight on the 16x16x4 kernel (which wins once its columns are full)
+	if (ctx->anal_variant == 0 && nf <= 4) return launch_anal4<2, 4>(ctx, geo_of(ctx), four, spec, nf);
 	return launch_anal<LAY_REAL>(ctx, geo_of(ctx), four, spec, nf);
 }
 

@@ -5,7 +5,10 @@
 //   - real FFTs: complex sequences per workgroup (rings per workgroup, LDS per
 //     workgroup), per direction;
 //   - FP64 real synthesis: the vector-unit kernel against the FP64 matrix-core
-//     kernel, as the number of fields from which the latter is used;
+//     kernel, as the number of fields from which the latter is used, and the
+//     workgroup shape of the vector-unit kernel;
+//   - FP64 real analysis of up to four fields: the 4x4x4 matrix-instruction
+//     kernel or the 16x16x4 kernel;
 //   - FP32 real synthesis and analysis: the stored-table GEMMs against the
 //     recurrence kernels, per direction, at 16 fields.
 #include "sphswe_common.h"
@@ -40,6 +43,26 @@ extern "C" int sphswe_set_tuning(sphswe_ctx *ctx, int fft_nseq_r2c, int fft_nseq
 	ctx->fft_nseq_c2r = fft_nseq_c2r;
 	ctx->synth_mfma_min = synth_mfma_min;
 	ctx->buf_gen++;   // a captured step may use the previous choices
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_set_legendre_tuning(sphswe_ctx *ctx, int synth_shape, int anal_variant)
+{
+	if (!ctx) return SPHSWE_ERR_ARG;
+	if (synth_shape < 0 || synth_shape > 1 || anal_variant < 0 || anal_variant > 1)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "legendre tuning: synthesis shape %d, analysis variant %d",
+				      synth_shape, anal_variant);
+	ctx->synth_shape = synth_shape;
+	ctx->anal_variant = anal_variant;
+	ctx->buf_gen++;
+	return SPHSWE_OK;
+}
+
+extern "C" int sphswe_get_legendre_tuning(const sphswe_ctx *ctx, int *synth_shape, int *anal_variant)
+{
+	if (!ctx) return SPHSWE_ERR_ARG;
+	if (synth_shape) *synth_shape = ctx->synth_shape;
+	if (anal_variant) *anal_variant = ctx->anal_variant;
 	return SPHSWE_OK;
 }
 
@@ -129,6 +152,24 @@ extern "C" int sphswe_autotune(sphswe_ctx *ctx)
 		else break;
 	}
 	ctx->synth_mfma_min = threshold;
+	// FP64 real Legendre kernel shapes: synthesis of 3 + 9 fields, analysis of
+	// 3 + 4 fields (a round trip; a SWE step)
+	for (int which = 0; which < 2 && !rc; which++) {
+		float t[2] = {0, 0};
+		for (int v = 0; v < 2 && !rc; v++) {
+			(which ? ctx->anal_variant : ctx->synth_shape) = v;
+			float a = 0, b = 0;
+			if (which == 0) {
+				rc = time_ms(ctx, REPS, [&]() { return sphswe_leg_synth(ctx, (const double2 *)spec, (double2 *)four, 3, SPHSWE_PREC_FP64); }, &a);
+				if (!rc) rc = time_ms(ctx, REPS, [&]() { return sphswe_leg_synth(ctx, (const double2 *)spec, (double2 *)four, 9, SPHSWE_PREC_FP64); }, &b);
+			} else {
+				rc = time_ms(ctx, REPS, [&]() { return sphswe_leg_anal(ctx, (const double2 *)four, (double2 *)spec, 3, SPHSWE_PREC_FP64); }, &a);
+				if (!rc) rc = time_ms(ctx, REPS, [&]() { return sphswe_leg_anal(ctx, (const double2 *)four, (double2 *)spec, 4, SPHSWE_PREC_FP64); }, &b);
+			}
+			t[v] = a + b;
+		}
+		(which ? ctx->anal_variant : ctx->synth_shape) = t[1] < t[0] ? 1 : 0;
+	}
 	// FP32 real transforms of NF fields: stored tables or the recurrence kernels,
 	// per direction; the tables are released when neither direction uses them
 	for (int dir = 0; dir < 2 && !rc; dir++) {
