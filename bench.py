@@ -433,19 +433,51 @@ def bench_ensemble(nsteps=200, nmem=64, L=127):
     return {"config": cfg, "runs": res}
 
 
-def cpu_baseline(L, ctx_nlat, ctx_nphi):
-    """The oracle (single-threaded C restatement) on a bounded sample: one
-    round trip of 3 fields at T511."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(L, ctx_nlat, ctx_nphi, runs=5):
+    """The oracle (the C restatement of the reference's SHT path, OpenMP over
+    rings and orders as SHTNS runs over all cores) on the host: the median of
+    `runs` timed SHT round trips with all the cores this process may use
+    (3 fields) and with one thread (1 field, scaled to 3), after one warm-up."""
     from oracle import oracle as O
     orc = O.Oracle(L, ctx_nlat, ctx_nphi)
     spec = seeded_spec(L, NFIELDS)
-    t0 = time.perf_counter()
-    for k in range(NFIELDS):
-        f = orc.synthesis(spec[k])
-        orc.analysis(f)
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "round_trips/s", "cores": 1, "kind": "port",
-            "sample": "1 round trip of 3 fields at T511 (768x1536), oracle/sphswe_oracle.c, %.1f s" % dt}
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+
+    def timed(nf):
+        t0 = time.perf_counter()
+        for k in range(nf):
+            orc.analysis(orc.synthesis(spec[k]))
+        return time.perf_counter() - t0
+
+    O.set_threads(cores)
+    timed(1)
+    multi = sorted(timed(NFIELDS) for _ in range(runs))[runs // 2]
+    O.set_threads(1)
+    single = sorted(timed(1) for _ in range(runs))[runs // 2] * NFIELDS
+    O.set_threads(cores)
+    model = _cpu_model()
+    return {"value": 1.0 / multi, "unit": "round_trips/s", "cores": cores, "kind": "port",
+            "cpu_model": model,
+            "sample": "median of %d round trips of 3 fields at T%d (%dx%d), oracle/sphswe_oracle.c with %d OpenMP "
+                      "threads, %.2f s each" % (runs, L, ctx_nphi, ctx_nlat, cores, multi),
+            "single_thread": {"value": 1.0 / single, "unit": "round_trips/s", "cores": 1,
+                              "sample": "median of %d one-field round trips x 3, %.2f s per 3-field round trip"
+                                        % (runs, single)}}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leg_anal_T511_3fields_hbm.json")
@@ -653,11 +685,12 @@ def main():
         "rexi_config": rexi_cfg,
         "mixed_rexi_T256_200poles": rexi_modes,
         "split_T1023_galewsky": split,
-        "roofline": {"bound": "mfma", "engine": "fp64 matrix cores (v_mfma_f64_16x16x4f64)",
-                     "kernel": "k_leg_eprep + k_leg_anal (Legendre analysis, 3 fields, T511)",
+        "roofline": {"bound": "mfma", "engine": "fp64 matrix cores (v_mfma_f64_4x4x4f64)",
+                     "kernel": "k_leg_anal4 (Legendre analysis, 3 fields, T511)",
                      "achieved": achieved_ana, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_ana / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "avg_ms": ms_ana, "flops_per_launch": flops,
+                     "legendre_tuning": ctx.legendre_tuning(),
                      "synthesis": {"kernel": "k_leg_synth (FP64 VALU)", "avg_ms": ms_syn, "achieved": achieved,
                                    "frac": achieved / FP64_PEAK_TFLOPS},
                      "fft_T511_9fields": dict(fft, bound="hbm", kernel="k_fft_real (reference layout)")},

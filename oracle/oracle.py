@@ -36,6 +36,7 @@ def _load():
     sig = {
         "orc_create": (vp, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
         "orc_destroy": (None, [vp]),
+        "orc_set_threads": (None, [ctypes.c_int]),
         "orc_auto_grid": (None, [ctypes.c_int, _ip, _ip]),
         "orc_get_grid": (None, [vp, _dp, _dp, _dp, _dp]),
         "orc_get_counters": (None, [vp, _lp]),
@@ -297,3 +298,8 @@ def ref_rexi(h, M, half=True):
 def ref_identities(L):
     out = subprocess.run([_REF_DUMP, "ident", str(L)], check=True, capture_output=True, text=True).stdout
     return np.array([[float(x) for x in line.split()] for line in out.strip().splitlines()])
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (the CPU baseline times 1 and all)."""
+    lib().orc_set_threads(int(n))

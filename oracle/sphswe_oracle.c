@@ -47,6 +47,9 @@
  * reference's solver headers cannot be built without SHTNS.
  */
 #define _GNU_SOURCE
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <complex.h>
 #include <math.h>
 #include <stdio.h>
@@ -314,33 +317,52 @@ void orc_fft(ocfg *c, const double *in_ri, double *out_ri, int sign)
 static inline int lm_real(const ocfg *c, int l, int m) { return (m * (2 * c->L - m + 1)) / 2 + l; }
 static inline int lm_cplx(int l, int m) { return l * l + l + m; }
 
+/* threads of the OpenMP loops below (the rings of the FFT stage, the orders of
+ * the Legendre stage; every output is written by one thread in the serial
+ * order, so results do not depend on the thread count).  The reference runs
+ * SHTNS with OpenMP over all cores (SPHConfig.hpp:332). */
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+	omp_set_num_threads(n > 0 ? n : 1);
+#else
+	(void)n;
+#endif
+}
+
 /* real analysis: spat[nphi*nlat] -> spec[nlm] */
 void orc_sht_analysis(ocfg *c, const double *spat, cplx *spec)
 {
 	int L = c->L, nlat = c->nlat, nphi = c->nphi;
 	cplx *F = (cplx *)malloc(sizeof(cplx) * (size_t)(L + 1) * nlat);
-	cplx *x = (cplx *)malloc(sizeof(cplx) * nphi);
-	cplx *y = (cplx *)malloc(sizeof(cplx) * nphi);
-	cplx *tmp = (cplx *)malloc(sizeof(cplx) * 4 * nphi);
-	double *P = (double *)malloc(sizeof(double) * (L + 1));
 	double norm = 2.0 * M_PI / nphi;
-	for (int j = 0; j < nlat; j++) {
-		for (int i = 0; i < nphi; i++) x[i] = spat[(size_t)i * nlat + j];
-		fft_rec(nphi, c->tw, nphi, x, 1, y, -1, tmp);
-		for (int m = 0; m <= L; m++) F[(size_t)m * nlat + j] = y[m] * norm;
-		F[j] = creal(F[j]);
-	}
-	for (int m = 0; m <= L; m++) {
-		cplx *q = spec + lm_real(c, m, m) - m;   /* q[l] */
-		for (int l = m; l <= L; l++) q[l] = 0;
+#pragma omp parallel
+	{
+		cplx *x = (cplx *)malloc(sizeof(cplx) * nphi);
+		cplx *y = (cplx *)malloc(sizeof(cplx) * nphi);
+		cplx *tmp = (cplx *)malloc(sizeof(cplx) * 4 * nphi);
+		double *P = (double *)malloc(sizeof(double) * (L + 1));
+#pragma omp for schedule(static)
 		for (int j = 0; j < nlat; j++) {
-			const double *Pc = plm_col(c, m, j, P);
-			cplx f = F[(size_t)m * nlat + j] * c->w[j];
-			for (int l = m; l <= L; l++) q[l] += Pc[l] * f;
+			for (int i = 0; i < nphi; i++) x[i] = spat[(size_t)i * nlat + j];
+			fft_rec(nphi, c->tw, nphi, x, 1, y, -1, tmp);
+			for (int m = 0; m <= L; m++) F[(size_t)m * nlat + j] = y[m] * norm;
+			F[j] = creal(F[j]);
 		}
+#pragma omp for schedule(dynamic, 4)
+		for (int m = 0; m <= L; m++) {
+			cplx *q = spec + lm_real(c, m, m) - m;   /* q[l] */
+			for (int l = m; l <= L; l++) q[l] = 0;
+			for (int j = 0; j < nlat; j++) {
+				const double *Pc = plm_col(c, m, j, P);
+				cplx f = F[(size_t)m * nlat + j] * c->w[j];
+				for (int l = m; l <= L; l++) q[l] += Pc[l] * f;
+			}
+		}
+		free(x); free(y); free(tmp); free(P);
 	}
 	c->n_anal++;
-	free(F); free(x); free(y); free(tmp); free(P);
+	free(F);
 }
 
 /* real synthesis: spec[nlm] -> spat[nphi*nlat] */
@@ -348,33 +370,39 @@ void orc_sht_synthesis(ocfg *c, const cplx *spec, double *spat)
 {
 	int L = c->L, nlat = c->nlat, nphi = c->nphi;
 	cplx *G = (cplx *)malloc(sizeof(cplx) * (size_t)(L + 1) * nlat);
-	cplx *x = (cplx *)malloc(sizeof(cplx) * nphi);
-	cplx *y = (cplx *)malloc(sizeof(cplx) * nphi);
-	cplx *tmp = (cplx *)malloc(sizeof(cplx) * 4 * nphi);
-	double *P = (double *)malloc(sizeof(double) * (L + 1));
-	for (int m = 0; m <= L; m++) {
-		const cplx *q = spec + lm_real(c, m, m) - m;
+#pragma omp parallel
+	{
+		cplx *x = (cplx *)malloc(sizeof(cplx) * nphi);
+		cplx *y = (cplx *)malloc(sizeof(cplx) * nphi);
+		cplx *tmp = (cplx *)malloc(sizeof(cplx) * 4 * nphi);
+		double *P = (double *)malloc(sizeof(double) * (L + 1));
+#pragma omp for schedule(dynamic, 4)
+		for (int m = 0; m <= L; m++) {
+			const cplx *q = spec + lm_real(c, m, m) - m;
+			for (int j = 0; j < nlat; j++) {
+				const double *Pc = plm_col(c, m, j, P);
+				cplx g = 0;
+				if (m == 0) for (int l = 0; l <= L; l++) g += creal(q[l]) * Pc[l];
+				else for (int l = m; l <= L; l++) g += q[l] * Pc[l];
+				G[(size_t)m * nlat + j] = g;
+			}
+		}
+#pragma omp for schedule(static)
 		for (int j = 0; j < nlat; j++) {
-			const double *Pc = plm_col(c, m, j, P);
-			cplx g = 0;
-			if (m == 0) for (int l = 0; l <= L; l++) g += creal(q[l]) * Pc[l];
-			else for (int l = m; l <= L; l++) g += q[l] * Pc[l];
-			G[(size_t)m * nlat + j] = g;
+			for (int i = 0; i < nphi; i++) x[i] = 0;
+			x[0] = creal(G[j]);
+			for (int m = 1; m <= L; m++) {
+				cplx g = G[(size_t)m * nlat + j];
+				x[m] += g;
+				x[nphi - m] += conj(g);
+			}
+			fft_rec(nphi, c->tw, nphi, x, 1, y, +1, tmp);
+			for (int i = 0; i < nphi; i++) spat[(size_t)i * nlat + j] = creal(y[i]);
 		}
-	}
-	for (int j = 0; j < nlat; j++) {
-		for (int i = 0; i < nphi; i++) x[i] = 0;
-		x[0] = creal(G[j]);
-		for (int m = 1; m <= L; m++) {
-			cplx g = G[(size_t)m * nlat + j];
-			x[m] += g;
-			x[nphi - m] += conj(g);
-		}
-		fft_rec(nphi, c->tw, nphi, x, 1, y, +1, tmp);
-		for (int i = 0; i < nphi; i++) spat[(size_t)i * nlat + j] = creal(y[i]);
+		free(x); free(y); free(tmp); free(P);
 	}
 	c->n_synth++;
-	free(G); free(x); free(y); free(tmp); free(P);
+	free(G);
 }
 
 /* complex analysis: spat cplx[nphi*nlat] -> spec[(L+1)^2] */
