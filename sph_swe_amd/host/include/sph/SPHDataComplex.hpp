@@ -106,6 +106,20 @@ public:
 		return out;
 	}
 
+	// largest |real| and |imaginary| difference to i over the grid
+	// (SPHDataComplex.hpp:744-771), -1 for an empty grid; on the host
+	double spat_reduce_error_max(const SPHDataComplex &i) const
+	{
+		spat_download();
+		i.spat_download();
+		double e = -1.0;
+		for (std::size_t k = 0; k < data_spat.size(); k++) {
+			const cplx d = data_spat[k] - i.data_spat[k];
+			e = std::max(e, std::max(std::abs(d.real()), std::abs(d.imag())));
+		}
+		return e;
+	}
+
 	SPHDataComplex spec_solve_helmholtz(const cplx &i_a, const cplx &i_b, double r) const
 	{
 		request_data_spectral();
@@ -260,5 +274,8 @@ private:
 		spat_upload();
 	}
 };
+
+inline SPHDataComplex operator*(const std::complex<double> &v, const SPHDataComplex &i) { return i * v; }
+inline SPHDataComplex operator*(double v, const SPHDataComplex &i) { return i * v; }
 
 #endif

@@ -36,8 +36,8 @@ public:
 	}
 
 	void solver_component_scalar_phi(const T &s) { add(SPHSWE_BAND_SCALAR, s, 1.0); }
-	void solver_component_mu_phi(const T &s) { add(SPHSWE_BAND_MU, s, 1.0); }
-	void solver_component_one_minus_mu_mu_diff_mu_phi(const T &s) { add(SPHSWE_BAND_ONE_MINUS_MU2_DMU, s, 1.0); }
+	void solver_component_mu_phi(const T &s = T(1.0)) { add(SPHSWE_BAND_MU, s, 1.0); }
+	void solver_component_one_minus_mu_mu_diff_mu_phi(const T &s = T(1.0)) { add(SPHSWE_BAND_ONE_MINUS_MU2_DMU, s, 1.0); }
 	void solver_component_rexi_z1(const T &s, double r) { add(SPHSWE_BAND_REXI_Z1 + 0, s, r); }
 	void solver_component_rexi_z2(const T &s, double r) { add(SPHSWE_BAND_REXI_Z1 + 1, s, r); }
 	void solver_component_rexi_z3(const T &s, double r) { add(SPHSWE_BAND_REXI_Z1 + 2, s, r); }

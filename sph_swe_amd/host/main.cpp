@@ -1,5 +1,5 @@
 // Command-line entry of the shallow-water program (src/main.cpp:33-262): the
-// same flags, program P0 (AppSWE).  Two letters are additions of this build:
+// same flags, programs P0 (AppSWE) and P4 (AppTestSPHSolversComplex).  Two letters are additions of this build:
 // G<0|1> (one library call per time step, default 1) and Z<file> (write the
 // final (h, u, v) spectra as raw complex doubles).
 #include <cstdio>
@@ -7,6 +7,7 @@
 #include <iostream>
 
 #include "AppSWE.hpp"
+#include "AppTestSPHSolverComplex.hpp"
 #include "SimVars.hpp"
 #include "sph/SPHConfig.hpp"
 
@@ -82,6 +83,14 @@ int main(int argc, const char *argv[])
 		app.run();
 		return 0;
 	}
-	std::cerr << "Program " << simVars.program_id << " (self-test programs P1-P5) is not available in this build" << std::endl;
+	if (simVars.program_id == 4) {
+		std::cout << "************************************************" << std::endl;
+		std::cout << "AppTestSPHSolversComplex" << std::endl;
+		std::cout << "************************************************" << std::endl;
+		AppTestSPHSolversComplex test_solvers;
+		test_solvers.run(&sphConfig);
+		return 0;
+	}
+	std::cerr << "Program " << simVars.program_id << " is not available in this build" << std::endl;
 	return 1;
 }

@@ -1,7 +1,8 @@
 """The C++ host program (host/main.cpp with the SPHConfig / SPHData /
 SPHOperators / TimesteppingRK / REXI / SWERexiSPH / AppSWE classes over the C
 ABI) against the oracle: RK4 and REXI runs through both the one-call-per-step
-path (G1) and the reference's class-by-class loop (G0)."""
+path (G1) and the reference's class-by-class loop (G0); program P4, the
+reference's known-answer tests of the complex banded solver."""
 import os
 import subprocess
 
@@ -109,6 +110,18 @@ def test_app_rexi_vs_oracle(tmp_path, fused, omega):
         assert e < 1e-11
 
 
+def test_app_p4_solver_known_answers(tmp_path):
+    # AppTestSPHSolverComplex.hpp:29-517 at T32: for the 76 poles of
+    # REXI::setup(0.2, 64), scalar, mu, (1-mu^2) d/dmu, z1..z8 and Helmholtz
+    # solves through the GPU band solver reproduce the Gaussian bump to 1e-10
+    out = _run(["T32", "P4"], str(tmp_path))
+    errs = [float(line.rsplit(":", 1)[1]) for line in out.splitlines() if "||| Error:" in line]
+    print("P4 largest error", max(errs))
+    assert len(errs) == 76 * 12
+    assert max(errs) < 1e-10
+    assert "P4: all 76 poles within" in out
+
+
 def test_app_reports_missing_programs(tmp_path):
-    r = subprocess.run([APP, "T16", "P4"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    r = subprocess.run([APP, "T16", "P5"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "not available" in r.stderr
