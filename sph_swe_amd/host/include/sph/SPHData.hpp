@@ -235,7 +235,18 @@ public:
 	double spat_reduce_min() const { double mn, mx, ab; minmax(mn, mx, ab); return mn; }
 	double spat_reduce_max() const { double mn, mx, ab; minmax(mn, mx, ab); return mx; }
 	double spat_reduce_abs_max() const { double mn, mx, ab; minmax(mn, mx, ab); return ab; }
-	double spat_reduce_error_max(const SPHData &i) const { return (*this - i).spat_reduce_abs_max(); }
+	// largest grid-point difference to i, -1 for an empty grid
+	// (SPHData.hpp:800-821): pointwise on the physical values, on the host
+	double spat_reduce_error_max(const SPHData &i) const
+	{
+		request_data_physical();
+		i.request_data_physical();
+		spat_download();
+		i.spat_download();
+		double e = -1.0;
+		for (std::size_t k = 0; k < data_spat.size(); k++) e = std::max(e, std::abs(data_spat[k] - i.data_spat[k]));
+		return e;
+	}
 
 	void spec_print(int i_precision = 8) const
 	{

@@ -1,5 +1,6 @@
 // SPHOperators over the C ABI (src/include/sph/SPHOperators.hpp): spectral band
-// operators (diff_lon :30-54, spec_one_minus_mu_squared_diff_lat_mu :234-260,
+// operators (diff_lon :30-54, diff_lat_mu :65-78, diff_lat_phi :86-92,
+// spec_one_minus_mu_squared_diff_lat_mu :234-260,
 // mu :268-297, mu2 :303-333, laplace :482-498) and the composite operators with
 // their projection points (grad_lon :100-119, grad_lat :339-388, div_lon
 // :397-402, div_lat :411-439, vort :507-513, div :523-529) and the
@@ -33,6 +34,13 @@ public:
 		SPHData o = metric(i, SPHSWE_SPAT_MUL_ST);
 		return metric(grad_lat(o), SPHSWE_SPAT_DIV_ST);
 	}
+	// d/dmu: (1 - mu^2) d/dmu, then division by 1 - mu^2 = cos^2 phi (:65-78)
+	SPHData diff_lat_mu(const SPHData &i) const
+	{
+		return metric(spec_one_minus_mu_squared_diff_lat_mu(i), SPHSWE_SPAT_DIV_ST2);
+	}
+	// d/dphi (:86-92)
+	SPHData diff_lat_phi(const SPHData &i) const { return grad_lat(i); }
 	// grad_lon(v) - grad_lat(u), without the metric term of the true vorticity
 	SPHData vort(const SPHData &u, const SPHData &v) const { return grad_lon(v) - grad_lat(u); }
 	SPHData div(const SPHData &u, const SPHData &v) const { return div_lon(u) + div_lat(v); }

@@ -1,5 +1,6 @@
 // Command-line entry of the shallow-water program (src/main.cpp:33-262): the
-// same flags, programs P0 (AppSWE) and P4 (AppTestSPHSolversComplex).  Two letters are additions of this build:
+// same flags, programs P0 (AppSWE), P1 (AppTestSPHOperators), P4 (AppTestSPHSolversComplex)
+// and P5 (AppOutputSphericalHarmonics).  Two letters have no counterpart in the reference:
 // G<0|1> (one library call per time step, default 1) and Z<file> (write the
 // final (h, u, v) spectra as raw complex doubles).
 #include <cstdio>
@@ -7,6 +8,8 @@
 #include <iostream>
 
 #include "AppSWE.hpp"
+#include "AppOutputSphericalHarmonics.hpp"
+#include "AppTestSPHOperators.hpp"
 #include "AppTestSPHSolverComplex.hpp"
 #include "SimVars.hpp"
 #include "sph/SPHConfig.hpp"
@@ -81,6 +84,22 @@ int main(int argc, const char *argv[])
 		std::cout << "************************************************" << std::endl;
 		AppSWE app(&sphConfig, simVars);
 		app.run();
+		return 0;
+	}
+	if (simVars.program_id == 1) {
+		std::cout << "************************************************" << std::endl;
+		std::cout << "AppTestSPHOperators" << std::endl;
+		std::cout << "************************************************" << std::endl;
+		AppTestSPHOperators test_operators;
+		test_operators.run(&sphConfig);
+		return 0;
+	}
+	if (simVars.program_id == 5) {
+		std::cout << "************************************************" << std::endl;
+		std::cout << "AppOutputSphericalHarmonics" << std::endl;
+		std::cout << "************************************************" << std::endl;
+		AppOutputSphericalHarmonics output_spherical_harmonics;
+		output_spherical_harmonics.run(&sphConfig);
 		return 0;
 	}
 	if (simVars.program_id == 4) {

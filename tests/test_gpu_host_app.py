@@ -1,8 +1,9 @@
 """The C++ host program (host/main.cpp with the SPHConfig / SPHData /
 SPHOperators / TimesteppingRK / REXI / SWERexiSPH / AppSWE classes over the C
 ABI) against the oracle: RK4 and REXI runs through both the one-call-per-step
-path (G1) and the reference's class-by-class loop (G0); program P4, the
-reference's known-answer tests of the complex banded solver."""
+path (G1) and the reference's class-by-class loop (G0); programs P1 (real
+operators against analytic derivatives), P4 (the known-answer tests of the
+complex banded solver) and P5 (one CSV per spherical harmonic)."""
 import os
 import subprocess
 
@@ -122,6 +123,90 @@ def test_app_p4_solver_known_answers(tmp_path):
     assert "P4: all 76 poles within" in out
 
 
+def _gauss_div(lon, mu):
+    """The Gaussian bump of the self-test programs (centre (pi/5, pi/4),
+    f = exp(-10 d^2)) with div_lat f = d(cos(phi) f)/dphi / cos(phi) and
+    div_lon f = grad_lon f = (df/dlam) / cos(phi), grad_lat f = df/dphi."""
+    phi = np.arcsin(mu)
+    lon0, phi0 = np.pi / 5, np.pi / 4
+    c = np.clip(np.sin(phi) * np.sin(phi0) + np.cos(phi) * np.cos(phi0) * np.cos(lon - lon0), -1, 1)
+    d = np.arccos(c)
+    s = np.sqrt(1 - c * c)
+    q = np.where(s > 0, d / np.where(s > 0, s, 1), 1.0)
+    f = np.exp(-10 * d * d)
+    dlam = 20 * q * f * (-np.cos(phi) * np.cos(phi0) * np.sin(lon - lon0))
+    dphi = 20 * q * f * (np.cos(phi) * np.sin(phi0) - np.sin(phi) * np.cos(phi0) * np.cos(lon - lon0))
+    return f, dphi - np.tan(phi) * f, dlam / np.cos(phi), dphi
+
+
+def test_app_p1_operators_vs_analytic(tmp_path):
+    # AppTestSPHOperators.hpp:25-520 at T64: every real operator on the GPU
+    # against the analytic derivatives of the Gaussian bump (the reference only
+    # prints the errors).  Where the operator is exact on band-limited input the
+    # error is the bump's spectral tail (exp(-l^2 / 40) at l = 64, times the
+    # derivative); div_lat multiplies by cos(phi) before its projection, which
+    # is not band-limited where the bump reaches the pole (f = 2e-3 there), so
+    # DIV LAT and the sums of two operator results (DIVERGENCE, VORTICITY,
+    # div.grad; projected by SPHData's spectral + and -) are checked against the
+    # same errors computed by the oracle's operators
+    import re
+    L = 64
+    out = _run(["T%d" % L, "P1"], str(tmp_path))
+    errs = {}
+    for line in out.splitlines():
+        mt = re.match(r"^(.*?)[: -]+max error: ([-+0-9.eE]+)$", line) or re.match(r"^ERROR (.*): ([-+0-9.eE]+)$", line)
+        if mt:
+            errs[mt.group(1).strip()] = float(mt.group(2))
+    print(errs)
+    assert len(errs) == 18, errs
+    nlat, nphi = capi.grid_auto(L)
+    orc = O.Oracle(L, nlat, nphi)
+    lon = np.repeat(orc.lon(), nlat).reshape(nphi, nlat)
+    mu = np.tile(np.asarray(orc.ct), nphi).reshape(nphi, nlat)
+    f, div_mu, div_lam, dphi = _gauss_div(lon, mu)
+    f = f.ravel()
+    grad_lon = orc.phys_op(0, f)
+    grad_lat = orc.phys_op(1, f)
+    # SPHData's + and - work on the spectra (SPHData.hpp:231-314): a sum of two
+    # operator results is projected before the pointwise comparison
+    proj = lambda x: orc.synthesis(orc.analysis(x))
+    inexact = {
+        "TEST DIV LAT": np.max(np.abs(orc.phys_op(2, f) - div_mu.ravel())),
+        "TEST DIVERGENCE": np.max(np.abs(proj(grad_lon + orc.phys_op(2, f)) - (div_mu + div_lam).ravel())),
+        "TEST VORTICITY": np.max(np.abs(proj(grad_lon - grad_lat) - (div_lam - dphi).ravel())),
+        "TEST LAPLACE (div.grad vs. sph laplace)":
+            np.max(np.abs(proj(orc.phys_op(0, grad_lon) + orc.phys_op(2, grad_lat)) - orc.phys_op(3, f))),
+    }
+    for name, e in errs.items():
+        if name in inexact:
+            print(name, "app", e, "oracle", inexact[name])
+            assert abs(e - inexact[name]) < 1e-9 + 1e-6 * inexact[name], name
+            assert e < 0.2
+        else:
+            assert 0 <= e < 1e-10, (name, e)
+
+
+def test_app_p5_writes_every_harmonic(tmp_path):
+    # AppOutputSphericalHarmonics.hpp: one CSV per (n, m); the grid values of
+    # a unit coefficient are Y_n^0, or 2 Re Y_n^m for m > 0 (orthonormal,
+    # Condon-Shortley phase), written with 8 significant digits
+    from scipy.special import sph_harm_y
+    L = 8
+    _run(["T%d" % L, "P5"], str(tmp_path))
+    names = sorted(f for f in os.listdir(tmp_path) if f.startswith("SPH_n"))
+    assert len(names) == (L + 1) * (L + 2) // 2
+    for n, m in [(0, 0), (2, 1), (5, 3), (8, 8)]:
+        with open(tmp_path / ("SPH_n%d_m%d.csv" % (n, m))) as f:
+            lines = f.read().splitlines()
+        assert lines[0].startswith("#TI") and lines[1] == "#TX Longitude" and lines[2] == "#TY Latitude"
+        lon = np.radians(np.array(lines[3].split("\t")[1:], float))
+        rows = np.array([r.split("\t") for r in lines[4:]], float)
+        lat = np.radians(rows[:, 0])
+        Y = sph_harm_y(n, m, (np.pi / 2 - lat)[:, None], lon[None, :])
+        ref = Y.real if m == 0 else 2 * Y.real
+        assert np.max(np.abs(rows[:, 1:] - ref)) < 1e-7 * max(1.0, np.max(np.abs(ref))), (n, m)
+
+
 def test_app_reports_missing_programs(tmp_path):
-    r = subprocess.run([APP, "T16", "P5"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    r = subprocess.run([APP, "T16", "P9"], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "not available" in r.stderr

@@ -87,3 +87,50 @@ def test_gaussian_derivatives_match_finite_differences(gauss):
     assert np.max(np.abs(v[:, 9] - div_mu)) < tol
     # the derivatives are not trivially small where the bump is
     assert np.max(np.abs(d_lon)) > 0.1 and np.max(np.abs(d_phi)) > 0.1
+
+
+SPH_DRIVER = r"""
+#include <cstdio>
+#include "benchmarks/SphereTestSolutions_SPH.hpp"
+int main() {
+	int n, m;
+	double lon, mu;
+	while (std::scanf("%d %d %lf %lf", &n, &m, &lon, &mu) == 4) {
+		SphereTestSolutions_SPH t(n, m);
+		double f, d;
+		t.test_function__grid_gaussian(lon, mu, f);
+		t.correct_result_diff_mu__grid_gaussian(lon, mu, d);
+		std::printf("%.17g %.17g\n", f, d);
+	}
+}
+"""
+
+
+def test_sph_test_solution_normalisation_and_derivative(tmp_path):
+    # host/include/benchmarks/SphereTestSolutions_SPH.hpp: the norm on the
+    # sphere (Gauss-Legendre in mu, exact for these polynomial degrees), the
+    # closed form the reference lists for n = 2, m = 1, and (1 - mu^2) d/dmu
+    # against central differences
+    src, exe = tmp_path / "s.cpp", tmp_path / "s"
+    src.write_text(SPH_DRIVER)
+    subprocess.run(["g++", "-O1", "-I", HOST_INC, str(src), "-o", str(exe)], check=True)
+
+    def ev(n, m, lon, mu):
+        inp = "".join("%d %d %.17g %.17g\n" % (n, m, a, b) for a, b in zip(np.ravel(lon), np.ravel(mu)))
+        out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout
+        return np.array([[float(x) for x in line.split()] for line in out.splitlines()])
+
+    x, w = np.polynomial.legendre.leggauss(24)
+    lon = np.linspace(0, 2 * np.pi, 48, endpoint=False)
+    for n, m in [(0, 0), (1, 0), (2, 1), (3, 3), (6, 4)]:
+        v = ev(n, m, np.repeat(lon, len(x)), np.tile(x, len(lon)))[:, 0].reshape(len(lon), len(x))
+        norm = np.sum((v ** 2) * w[None, :]) * (2 * np.pi / len(lon))
+        # cos(m lon) / sqrt(2 pi) carries half the norm of e^{i m lon} for m > 0
+        assert abs(norm - (1.0 if m == 0 else 0.5)) < 1e-12, (n, m, norm)
+    mu = np.linspace(-0.95, 0.95, 21)
+    v = ev(2, 1, np.zeros_like(mu), mu)
+    assert np.max(np.abs(v[:, 0] - (-mu / 2 * np.sqrt(15 * (1 - mu * mu)) / np.sqrt(2 * np.pi)))) < 1e-15
+    h = 1e-6
+    for n, m in [(2, 1), (5, 3), (7, 0)]:
+        d = (ev(n, m, 0.3 + 0 * mu, mu + h)[:, 0] - ev(n, m, 0.3 + 0 * mu, mu - h)[:, 0]) / (2 * h)
+        assert np.max(np.abs(ev(n, m, 0.3 + 0 * mu, mu)[:, 1] - (1 - mu * mu) * d)) < 1e-8, (n, m)
