@@ -415,9 +415,50 @@ struct Dft<16> {
 	}
 };
 
+// DFT of R = A B as B-point DFTs over n1 of x[A n1 + n2], twiddles
+// w_R^(n2 k1), A-point DFTs over n2; X[k1 + B k2].  R = 6 (A = 2) and R = 12
+// (A = 4) with B = 3; the twiddles are multiples of 30 degrees.
+template <int A>
+struct DftX3 {
+	__device__ static void run(double2 *v, int sign)
+	{
+		constexpr int R = 3 * A;
+		constexpr double c30 = 0.86602540378443864676;
+		// (cos, sin) of 30 e degrees, e = 0..6
+		constexpr double cw[7] = {1.0, c30, 0.5, 0.0, -0.5, -c30, -1.0};
+		constexpr double sw[7] = {0.0, 0.5, c30, 1.0, c30, 0.5, 0.0};
+		double2 y[A][3];
+#pragma unroll
+		for (int n2 = 0; n2 < A; n2++) {
+			double2 t[3] = {v[n2], v[A + n2], v[2 * A + n2]};
+			Dft<3>::run(t, sign);
+#pragma unroll
+			for (int k1 = 0; k1 < 3; k1++) {
+				const int e = (n2 * k1) * (12 / R);   // in units of 30 degrees
+				y[n2][k1] = e == 0 ? t[k1] : cmul(t[k1], make_double2(cw[e], sign * sw[e]));
+			}
+		}
+#pragma unroll
+		for (int k1 = 0; k1 < 3; k1++) {
+			double2 t[A];
+#pragma unroll
+			for (int n2 = 0; n2 < A; n2++) t[n2] = y[n2][k1];
+			Dft<A>::run(t, sign);
+#pragma unroll
+			for (int k2 = 0; k2 < A; k2++) v[k1 + 3 * k2] = t[k2];
+		}
+	}
+};
+template <>
+struct Dft<6> : DftX3<2> {};
+template <>
+struct Dft<12> : DftX3<4> {};
+
+// radix of the next pass for a remaining length n: 16 while it divides, a
+// last 12 or 6 in one pass, then 8, 4, 2, 3, 5, 7
 constexpr int first_radix(int n)
 {
-	return n % 16 == 0 ? 16 : n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : n % 5 == 0 ? 5 : 7;
+	return n % 16 == 0 ? 16 : n == 12 ? 12 : n == 6 ? 6 : n % 8 == 0 ? 8 : n % 4 == 0 ? 4 : n % 2 == 0 ? 2 : n % 3 == 0 ? 3 : n % 5 == 0 ? 5 : 7;
 }
 template <int N>
 struct CPlan {
@@ -605,6 +646,187 @@ __global__ __launch_bounds__(256) void k_fft_real(const void *__restrict__ in, v
 	}
 }
 
+// ---- reference layout, wide ring tiles ----------------------------------------
+// A workgroup owns 2 NSEQ adjacent rings; ring pair s is the complex sequence
+// z_s[i] = f(i, j0 + 2s) + i f(i, j0 + 2s + 1), which is one 16-byte load or
+// store at [i * nlat + j0 + 2s], so a row of the tile is 32 NSEQ contiguous
+// bytes.  The sequences live in one LDS buffer; every pass reads its inputs
+// into registers, waits at a barrier and writes its outputs in place, so the
+// tile is twice as wide as with two buffers.  The sequence stride is padded to
+// 8 / NSEQ (mod 8) elements beyond the every-16 padding, so the 8 lanes of a
+// 128-byte LDS access that hold one row's ring pairs (and the next rows) fall
+// on distinct banks.
+template <int N, int NSEQ>
+struct WPad {
+	static constexpr int base = N + N / 16;
+	static constexpr int want = NSEQ >= 8 ? 1 : 8 / NSEQ;
+	static constexpr int len = base + ((want - base % 8) + 8) % 8;
+	__device__ static __forceinline__ int at(int s, int i) { return s * len + i + (i >> 4); }
+};
+
+template <int N, int NSEQ, int NT, int R, int Ns, int SIGN>
+__device__ __forceinline__ void pass_ip(double2 *__restrict__ buf, const double2 *__restrict__ tw)
+{
+	constexpr int NB = N / R, TOT = NSEQ * NB, TWS = N / (Ns * R);
+	constexpr int IT = (TOT + NT - 1) / NT;
+	double2 v[IT][R];
+#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int idx = (int)threadIdx.x + it * NT;
+		if (TOT % NT != 0 && idx >= TOT) break;
+		const int sq = idx / NB, b = idx % NB;
+		const int k = b % Ns;
+#pragma unroll
+		for (int t = 0; t < R; t++) v[it][t] = buf[WPad<N, NSEQ>::at(sq, b + t * NB)];
+		if (Ns > 1) {
+#pragma unroll
+			for (int t = 1; t < R; t++) {
+				double2 w = tw[t * k * TWS];
+				if (SIGN > 0) w.y = -w.y;
+				v[it][t] = cmul(v[it][t], w);
+			}
+		}
+		Dft<R>::run(v[it], SIGN);
+	}
+	__syncthreads();
+#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int idx = (int)threadIdx.x + it * NT;
+		if (TOT % NT != 0 && idx >= TOT) break;
+		const int sq = idx / NB, b = idx % NB;
+		const int k = b % Ns, jb = b / Ns;
+		const int o = jb * Ns * R + k;
+#pragma unroll
+		for (int t = 0; t < R; t++) buf[WPad<N, NSEQ>::at(sq, o + t * Ns)] = v[it][t];
+	}
+	__syncthreads();
+}
+
+template <int N, int NSEQ, int NT, int SIGN, int P>
+__device__ __forceinline__ void passes_ip(double2 *buf, const double2 *__restrict__ tw)
+{
+	if constexpr (P < CPlan<N>::npass()) {
+		pass_ip<N, NSEQ, NT, CPlan<N>::radix(P), CPlan<N>::ns(P), SIGN>(buf, tw);
+		passes_ip<N, NSEQ, NT, SIGN, P + 1>(buf, tw);
+	}
+}
+
+// MODE_R2C: nf real fields [i * nlat + j] -> Fourier rows (L + 1, nlat);
+// MODE_C2R: back.  nlat is even (checked by the launcher).
+template <int MODE, int N, int NSEQ, int NT>
+__global__ __launch_bounds__(NT) void k_fft_wide(const void *__restrict__ in, void *__restrict__ out,
+						 const double2 *__restrict__ tw, int nlat, int L, int nblk, int nblk_pad,
+						 double norm)
+{
+	constexpr int RB = 2 * NSEQ;
+	typedef WPad<N, NSEQ> W;
+	extern __shared__ double2 sm[];
+	const int lb = xcd_remap(blockIdx.x, nblk_pad);
+	if (lb >= nblk) return;
+	const int f = blockIdx.y;
+	const size_t nspat = (size_t)nlat * N;
+	const int tid = threadIdx.x;
+	const int j0 = lb * RB;
+	const int h = nlat >> 1;   // ring pairs per longitude
+	if (MODE == MODE_R2C) {
+		const double2 *src = (const double2 *)((const double *)in + f * nspat);
+		constexpr int TOT = N * NSEQ;
+		constexpr int KW = (TOT + NT - 1) / NT < 16 ? (TOT + NT - 1) / NT : 16;
+		for (int b = 0; b < TOT; b += KW * NT) {
+			double2 v[KW];
+#pragma unroll
+			for (int u = 0; u < KW; u++) {
+				const int idx = min(b + u * NT + tid, TOT - 1);
+				const int i = idx / NSEQ, sq = idx % NSEQ;
+				v[u] = src[(size_t)i * h + min((j0 >> 1) + sq, h - 1)];
+			}
+#pragma unroll
+			for (int u = 0; u < KW; u++) {
+				const int idx = b + u * NT + tid;
+				if (TOT % (KW * NT) != 0 && idx >= TOT) break;
+				const int i = idx / NSEQ, sq = idx % NSEQ;
+				sm[W::at(sq, i)] = (j0 + 2 * sq < nlat) ? v[u] : make_double2(0, 0);
+			}
+		}
+		__syncthreads();
+		passes_ip<N, NSEQ, NT, -1, 0>(sm, tw);
+		double2 *dst = (double2 *)out + (size_t)f * (L + 1) * nlat;
+		for (int idx = tid; idx < (L + 1) * NSEQ; idx += NT) {
+			const int m = idx / NSEQ, sq = idx % NSEQ;
+			const int j = j0 + 2 * sq;
+			if (j >= nlat) continue;
+			const double2 Z = sm[W::at(sq, m)];
+			const double2 Zc = cconj(sm[W::at(sq, m == 0 ? 0 : N - m)]);
+			double2 a = make_double2(0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y));
+			double2 c = make_double2(0.5 * (Z.y - Zc.y), -0.5 * (Z.x - Zc.x));
+			if (m == 0) a.y = c.y = 0.0;
+			dst[(size_t)m * nlat + j] = make_double2(a.x * norm, a.y * norm);
+			dst[(size_t)m * nlat + j + 1] = make_double2(c.x * norm, c.y * norm);
+		}
+	} else {
+		const double2 *src = (const double2 *)in + (size_t)f * (L + 1) * nlat;
+		// the orders above L are zero
+		for (int idx = tid; idx < NSEQ * (N - 2 * L - 1); idx += NT) {
+			const int sq = idx / (N - 2 * L - 1), k = L + 1 + idx % (N - 2 * L - 1);
+			sm[W::at(sq, k)] = make_double2(0, 0);
+		}
+		// orders 0..L of the ring pair: z[m] = A + iB and z[N - m] = conj(A) + i conj(B)
+		const int ng = (L + 1) * NSEQ;
+		for (int b = 0; b < ng; b += KB * NT) {
+			double2 va[KB], vb[KB];
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				const int idx = min(b + u * NT + tid, ng - 1);
+				const int m = idx / NSEQ, sq = idx % NSEQ;
+				const size_t o = (size_t)m * nlat + min(j0 + 2 * sq, nlat - 2);
+				va[u] = src[o];
+				vb[u] = src[o + 1];
+			}
+#pragma unroll
+			for (int u = 0; u < KB; u++) {
+				const int idx = b + u * NT + tid;
+				if (idx >= ng) break;
+				const int m = idx / NSEQ, sq = idx % NSEQ;
+				double2 A = va[u], B = vb[u];
+				if (j0 + 2 * sq >= nlat) A = B = make_double2(0, 0);
+				if (m == 0) {
+					sm[W::at(sq, 0)] = make_double2(A.x, B.x);
+				} else {
+					sm[W::at(sq, m)] = make_double2(A.x - B.y, A.y + B.x);
+					sm[W::at(sq, N - m)] = make_double2(A.x + B.y, -A.y + B.x);
+				}
+			}
+		}
+		__syncthreads();
+		passes_ip<N, NSEQ, NT, +1, 0>(sm, tw);
+		double2 *dst = (double2 *)((double *)out + f * nspat);
+		constexpr int TOT = N * NSEQ;
+		for (int idx = tid; idx < TOT; idx += NT) {
+			const int i = idx / NSEQ, sq = idx % NSEQ;
+			if (j0 + 2 * sq < nlat) dst[(size_t)i * h + (j0 >> 1) + sq] = sm[W::at(sq, i)];
+		}
+	}
+}
+
+template <int MODE, int N, int NSEQ>
+int launch_real_wide(sphswe_ctx *ctx, const void *in, void *out, int nf)
+{
+	constexpr int RB = 2 * NSEQ;
+	constexpr int NT = NSEQ * N >= 6144 ? 512 : 256;
+	const int nblk = (ctx->nlat + RB - 1) / RB;
+	const int nblk_pad = (nblk + 7) / 8 * 8;
+	const size_t lds = (size_t)NSEQ * WPad<N, NSEQ>::len * sizeof(double2);
+	static_assert(NSEQ * WPad<N, NSEQ>::len * sizeof(double2) <= 160 * 1024, "tile exceeds the LDS");
+	if (lds > 65536)
+		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_wide<MODE, N, NSEQ, NT>,
+						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	dim3 grid(nblk_pad, nf);
+	k_fft_wide<MODE, N, NSEQ, NT><<<grid, NT, lds, ctx->stream>>>(in, out, ctx->d_tw, ctx->nlat, ctx->L, nblk, nblk_pad,
+								       2.0 * M_PI / N);
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return SPHSWE_OK;
+}
+
 template <int MODE, int N, int NSEQ, bool RM>
 int launch_real_ct(sphswe_ctx *ctx, const void *in, void *out, int nf)
 {
@@ -628,6 +850,32 @@ int launch_real_ct(sphswe_ctx *ctx, const void *in, void *out, int nf)
 template <int MODE, bool RM = false>
 int launch_real_fixed(sphswe_ctx *ctx, const void *in, void *out, int nf, int nseq)
 {
+	if (!RM && ctx->nlat % 2 == 0 && 2 * ctx->L + 1 <= ctx->plan.n) {
+		switch (ctx->plan.n) {
+		case 96:
+			if (nseq == 16) return launch_real_wide<MODE, 96, 16>(ctx, in, out, nf);
+			return launch_real_wide<MODE, 96, 8>(ctx, in, out, nf);
+		case 196:
+			if (nseq == 4) return launch_real_wide<MODE, 196, 4>(ctx, in, out, nf);
+			return launch_real_wide<MODE, 196, 8>(ctx, in, out, nf);
+		case 384:
+			if (nseq == 2) return launch_real_wide<MODE, 384, 2>(ctx, in, out, nf);
+			if (nseq == 8) return launch_real_wide<MODE, 384, 8>(ctx, in, out, nf);
+			return launch_real_wide<MODE, 384, 4>(ctx, in, out, nf);
+		case 784:
+			if (nseq == 1) return launch_real_wide<MODE, 784, 1>(ctx, in, out, nf);
+			if (nseq == 4) return launch_real_wide<MODE, 784, 4>(ctx, in, out, nf);
+			return launch_real_wide<MODE, 784, 2>(ctx, in, out, nf);
+		case 1536:
+			if (nseq == 1) return launch_real_wide<MODE, 1536, 1>(ctx, in, out, nf);
+			if (nseq == 4) return launch_real_wide<MODE, 1536, 4>(ctx, in, out, nf);
+			return launch_real_wide<MODE, 1536, 2>(ctx, in, out, nf);
+		case 3072:
+			if (nseq == 2) return launch_real_wide<MODE, 3072, 2>(ctx, in, out, nf);
+			return launch_real_wide<MODE, 3072, 1>(ctx, in, out, nf);
+		}
+		return -1;
+	}
 	switch (ctx->plan.n) {
 	case 96:
 		if (nseq == 16) return launch_real_ct<MODE, 96, 16, RM>(ctx, in, out, nf);
@@ -735,7 +983,8 @@ bool sphswe_fft_has_rm(const sphswe_ctx *ctx)
 // grid (empty for other sizes), for the autotuner
 int sphswe_fft_nseq_candidates(const sphswe_ctx *ctx, int *out, int max)
 {
-	static const int c96[] = {8, 16}, c196[] = {4, 8}, c384[] = {2, 4, 8}, c784[] = {1, 2, 4}, c1536[] = {1, 2};
+	static const int c96[] = {8, 16}, c196[] = {4, 8}, c384[] = {2, 4, 8}, c784[] = {1, 2, 4}, c1536[] = {1, 2, 4},
+			 c3072[] = {1, 2};
 	const int *c = nullptr;
 	int n = 0;
 	switch (ctx->plan.n) {
@@ -743,7 +992,8 @@ int sphswe_fft_nseq_candidates(const sphswe_ctx *ctx, int *out, int max)
 	case 196: c = c196; n = 2; break;
 	case 384: c = c384; n = 3; break;
 	case 784: c = c784; n = 3; break;
-	case 1536: c = c1536; n = 2; break;
+	case 1536: c = c1536; n = 3; break;
+	case 3072: c = c3072; n = 2; break;
 	}
 	for (int i = 0; i < n && i < max; i++) out[i] = c[i];
 	return n < max ? n : max;
