@@ -115,8 +115,20 @@ struct DftOdd {
 		for (int q = 0; q < R; q++) v[q] = out[q];
 	}
 };
+// radix 3: X0 = x0 + (x1 + x2), X1,2 = x0 - (x1 + x2) / 2 +- sign i sqrt(3)/2 (x1 - x2)
 template <>
-struct Dft<3> : DftOdd<3> {};
+struct Dft<3> {
+	__device__ static void run(double2 *v, int sign)
+	{
+		constexpr double s3 = 0.86602540378443864676;
+		const double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+		const double2 m1 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+		const double2 m2 = make_double2(-sign * s3 * t2.y, sign * s3 * t2.x);
+		v[0] = cadd(v[0], t1);
+		v[1] = cadd(m1, m2);
+		v[2] = csub(m1, m2);
+	}
+};
 template <>
 struct Dft<5> : DftOdd<5> {};
 template <>
@@ -711,15 +723,37 @@ __device__ __forceinline__ void passes_ip(double2 *buf, const double2 *__restric
 	}
 }
 
+template <int N, int NSEQ, int NT, int SIGN, int P, int PEND>
+__device__ __forceinline__ void passes_ip_range(double2 *buf, const double2 *__restrict__ tw)
+{
+	if constexpr (P < PEND) {
+		pass_ip<N, NSEQ, NT, CPlan<N>::radix(P), CPlan<N>::ns(P), SIGN>(buf, tw);
+		passes_ip_range<N, NSEQ, NT, SIGN, P + 1, PEND>(buf, tw);
+	}
+}
+
 // MODE_R2C: nf real fields [i * nlat + j] -> Fourier rows (L + 1, nlat);
-// MODE_C2R: back.  nlat is even (checked by the launcher).
+// MODE_C2R: back.  nlat is even and N > 2 L (checked by the launcher).
+// The first radix pass (Ns = 1) runs on values loaded from HBM and writes
+// the LDS tile; the middle passes run in place; the last pass (Ns = N / R)
+// reads the tile and writes HBM.  In the r2c direction a thread takes the
+// last-pass butterflies k and Ns - k (0 and Ns / 2 together), whose outputs
+// hold both Z[m] and Z[N - m] of every order m they produce, so the split
+// into the two rings of a pair needs no further exchange.
 template <int MODE, int N, int NSEQ, int NT>
 __global__ __launch_bounds__(NT) void k_fft_wide(const void *__restrict__ in, void *__restrict__ out,
 						 const double2 *__restrict__ tw, int nlat, int L, int nblk, int nblk_pad,
 						 double norm)
 {
-	constexpr int RB = 2 * NSEQ;
 	typedef WPad<N, NSEQ> W;
+	typedef CPlan<N> PL;
+	constexpr int RB = 2 * NSEQ;
+	constexpr int NP = PL::npass();
+	static_assert(NP >= 2, "two radix passes at least");
+	constexpr int R0 = PL::radix(0), NB0 = N / R0;
+	constexpr int RL = PL::radix(NP - 1), NSL = PL::ns(NP - 1);
+	static_assert(NSL * RL == N && NSL % 2 == 0, "last pass");
+	constexpr int SIGN = MODE == MODE_R2C ? -1 : 1;
 	extern __shared__ double2 sm[];
 	const int lb = xcd_remap(blockIdx.x, nblk_pad);
 	if (lb >= nblk) return;
@@ -728,82 +762,108 @@ __global__ __launch_bounds__(NT) void k_fft_wide(const void *__restrict__ in, vo
 	const int tid = threadIdx.x;
 	const int j0 = lb * RB;
 	const int h = nlat >> 1;   // ring pairs per longitude
-	if (MODE == MODE_R2C) {
-		const double2 *src = (const double2 *)((const double *)in + f * nspat);
-		constexpr int TOT = N * NSEQ;
-		constexpr int KW = (TOT + NT - 1) / NT < 16 ? (TOT + NT - 1) / NT : 16;
-		for (int b = 0; b < TOT; b += KW * NT) {
-			double2 v[KW];
+	const int p0 = j0 >> 1;    // first ring pair of the tile
+
+	// ---- first pass from HBM ----
+	for (int it = tid; it < NSEQ * NB0; it += NT) {
+		const int b = it / NSEQ, sq = it % NSEQ;
+		const bool live = j0 + 2 * sq < nlat;
+		double2 v[R0];
+		if (MODE == MODE_R2C) {
+			const double2 *src = (const double2 *)((const double *)in + f * nspat);
 #pragma unroll
-			for (int u = 0; u < KW; u++) {
-				const int idx = min(b + u * NT + tid, TOT - 1);
-				const int i = idx / NSEQ, sq = idx % NSEQ;
-				v[u] = src[(size_t)i * h + min((j0 >> 1) + sq, h - 1)];
-			}
+			for (int t = 0; t < R0; t++)
+				v[t] = live ? src[(size_t)(b + t * NB0) * h + p0 + sq] : make_double2(0, 0);
+		} else {
+			// z[k] = A + iB (k <= L), conj(A) + i conj(B) of order N - k (k >= N - L)
+			const double2 *src = (const double2 *)in + (size_t)f * (L + 1) * nlat;
+			double2 A[R0], B[R0];
 #pragma unroll
-			for (int u = 0; u < KW; u++) {
-				const int idx = b + u * NT + tid;
-				if (TOT % (KW * NT) != 0 && idx >= TOT) break;
-				const int i = idx / NSEQ, sq = idx % NSEQ;
-				sm[W::at(sq, i)] = (j0 + 2 * sq < nlat) ? v[u] : make_double2(0, 0);
-			}
-		}
-		__syncthreads();
-		passes_ip<N, NSEQ, NT, -1, 0>(sm, tw);
-		double2 *dst = (double2 *)out + (size_t)f * (L + 1) * nlat;
-		for (int idx = tid; idx < (L + 1) * NSEQ; idx += NT) {
-			const int m = idx / NSEQ, sq = idx % NSEQ;
-			const int j = j0 + 2 * sq;
-			if (j >= nlat) continue;
-			const double2 Z = sm[W::at(sq, m)];
-			const double2 Zc = cconj(sm[W::at(sq, m == 0 ? 0 : N - m)]);
-			double2 a = make_double2(0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y));
-			double2 c = make_double2(0.5 * (Z.y - Zc.y), -0.5 * (Z.x - Zc.x));
-			if (m == 0) a.y = c.y = 0.0;
-			dst[(size_t)m * nlat + j] = make_double2(a.x * norm, a.y * norm);
-			dst[(size_t)m * nlat + j + 1] = make_double2(c.x * norm, c.y * norm);
-		}
-	} else {
-		const double2 *src = (const double2 *)in + (size_t)f * (L + 1) * nlat;
-		// the orders above L are zero
-		for (int idx = tid; idx < NSEQ * (N - 2 * L - 1); idx += NT) {
-			const int sq = idx / (N - 2 * L - 1), k = L + 1 + idx % (N - 2 * L - 1);
-			sm[W::at(sq, k)] = make_double2(0, 0);
-		}
-		// orders 0..L of the ring pair: z[m] = A + iB and z[N - m] = conj(A) + i conj(B)
-		const int ng = (L + 1) * NSEQ;
-		for (int b = 0; b < ng; b += KB * NT) {
-			double2 va[KB], vb[KB];
-#pragma unroll
-			for (int u = 0; u < KB; u++) {
-				const int idx = min(b + u * NT + tid, ng - 1);
-				const int m = idx / NSEQ, sq = idx % NSEQ;
-				const size_t o = (size_t)m * nlat + min(j0 + 2 * sq, nlat - 2);
-				va[u] = src[o];
-				vb[u] = src[o + 1];
-			}
-#pragma unroll
-			for (int u = 0; u < KB; u++) {
-				const int idx = b + u * NT + tid;
-				if (idx >= ng) break;
-				const int m = idx / NSEQ, sq = idx % NSEQ;
-				double2 A = va[u], B = vb[u];
-				if (j0 + 2 * sq >= nlat) A = B = make_double2(0, 0);
-				if (m == 0) {
-					sm[W::at(sq, 0)] = make_double2(A.x, B.x);
+			for (int t = 0; t < R0; t++) {
+				const int k = b + t * NB0;
+				const int m = k <= L ? k : (k >= N - L ? N - k : -1);
+				if (live && m >= 0) {
+					const size_t o = (size_t)m * nlat + j0 + 2 * sq;
+					A[t] = src[o];
+					B[t] = src[o + 1];
 				} else {
-					sm[W::at(sq, m)] = make_double2(A.x - B.y, A.y + B.x);
-					sm[W::at(sq, N - m)] = make_double2(A.x + B.y, -A.y + B.x);
+					A[t] = B[t] = make_double2(0, 0);
 				}
 			}
+#pragma unroll
+			for (int t = 0; t < R0; t++) {
+				const int k = b + t * NB0;
+				if (k == 0) v[t] = make_double2(A[t].x, B[t].x);
+				else if (k <= L) v[t] = make_double2(A[t].x - B[t].y, A[t].y + B[t].x);
+				else v[t] = make_double2(A[t].x + B[t].y, -A[t].y + B[t].x);
+			}
 		}
-		__syncthreads();
-		passes_ip<N, NSEQ, NT, +1, 0>(sm, tw);
+		Dft<R0>::run(v, SIGN);
+#pragma unroll
+		for (int t = 0; t < R0; t++) sm[W::at(sq, b * R0 + t)] = v[t];
+	}
+	__syncthreads();
+	passes_ip_range<N, NSEQ, NT, SIGN, 1, NP - 1>(sm, tw);
+
+	// ---- last pass to HBM ----
+	if (MODE == MODE_R2C) {
+		double2 *dst = (double2 *)out + (size_t)f * (L + 1) * nlat;
+		constexpr int NH = NSL / 2;
+		const double hn = 0.5 * norm;
+		for (int it = tid; it < NSEQ * NH; it += NT) {
+			const int pp = it / NSEQ, sq = it % NSEQ;
+			const int j = j0 + 2 * sq;
+			const int k1 = pp, k2 = pp == 0 ? NH : NSL - pp;
+			double2 z1[RL], z2[RL];
+#pragma unroll
+			for (int t = 0; t < RL; t++) {
+				z1[t] = sm[W::at(sq, k1 + t * NSL)];
+				z2[t] = sm[W::at(sq, k2 + t * NSL)];
+			}
+#pragma unroll
+			for (int t = 1; t < RL; t++) {
+				z1[t] = cmul(z1[t], tw[t * k1]);
+				z2[t] = cmul(z2[t], tw[t * k2]);
+			}
+			Dft<RL>::run(z1, SIGN);
+			Dft<RL>::run(z2, SIGN);
+			if (j >= nlat) continue;
+			// orders m = k + t NSL of butterflies k1 and k2, with their partners
+			auto emit = [&](int m, double2 Z, double2 P) __attribute__((always_inline)) {
+				if (m > L) return;
+				const double2 Zc = cconj(P);
+				double2 a = make_double2(hn * (Z.x + Zc.x), hn * (Z.y + Zc.y));
+				double2 c = make_double2(hn * (Z.y - Zc.y), -hn * (Z.x - Zc.x));
+				if (m == 0) a.y = c.y = 0.0;
+				dst[(size_t)m * nlat + j] = a;
+				dst[(size_t)m * nlat + j + 1] = c;
+			};
+#pragma unroll
+			for (int t = 0; t < RL; t++) {
+				const double2 q1 = pp == 0 ? z1[(RL - t) % RL] : z2[RL - 1 - t];
+				const double2 q2 = pp == 0 ? z2[RL - 1 - t] : z1[RL - 1 - t];
+				emit(k1 + t * NSL, z1[t], q1);
+				emit(k2 + t * NSL, z2[t], q2);
+			}
+		}
+	} else {
 		double2 *dst = (double2 *)((double *)out + f * nspat);
-		constexpr int TOT = N * NSEQ;
-		for (int idx = tid; idx < TOT; idx += NT) {
-			const int i = idx / NSEQ, sq = idx % NSEQ;
-			if (j0 + 2 * sq < nlat) dst[(size_t)i * h + (j0 >> 1) + sq] = sm[W::at(sq, i)];
+		for (int it = tid; it < NSEQ * NSL; it += NT) {
+			const int k = it / NSEQ, sq = it % NSEQ;
+			double2 v[RL];
+#pragma unroll
+			for (int t = 0; t < RL; t++) v[t] = sm[W::at(sq, k + t * NSL)];
+#pragma unroll
+			for (int t = 1; t < RL; t++) {
+				double2 w = tw[t * k];
+				w.y = -w.y;
+				v[t] = cmul(v[t], w);
+			}
+			Dft<RL>::run(v, SIGN);
+			if (j0 + 2 * sq < nlat) {
+#pragma unroll
+				for (int t = 0; t < RL; t++) dst[(size_t)(k + t * NSL) * h + p0 + sq] = v[t];
+			}
 		}
 	}
 }
@@ -871,8 +931,8 @@ int launch_real_fixed(sphswe_ctx *ctx, const void *in, void *out, int nf, int ns
 			if (nseq == 4) return launch_real_wide<MODE, 1536, 4>(ctx, in, out, nf);
 			return launch_real_wide<MODE, 1536, 2>(ctx, in, out, nf);
 		case 3072:
-			if (nseq == 2) return launch_real_wide<MODE, 3072, 2>(ctx, in, out, nf);
-			return launch_real_wide<MODE, 3072, 1>(ctx, in, out, nf);
+			if (nseq == 1) return launch_real_wide<MODE, 3072, 1>(ctx, in, out, nf);
+			return launch_real_wide<MODE, 3072, 2>(ctx, in, out, nf);
 		}
 		return -1;
 	}
