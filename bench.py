@@ -481,15 +481,24 @@ def cpu_baseline(L, ctx_nlat, ctx_nphi, runs=5):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leg_anal_T511_3fields_hbm.json")
+FFT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fft_T511_9fields_hbm.json")
 
 
 def pmc_kernels():
     """Counter-collection workload: 5 Legendre syntheses and 5 analyses (3 fields,
-    T511) and 5 launches of a copy-shaped kernel of known size (k_axpby with y = 0:
-    64 MiB read, 64 MiB written) that calibrates FETCH_SIZE / WRITE_SIZE."""
+    T511), 5 real FFTs each way of 9 fields (the reference layout, default tile
+    width) and 5 launches of a copy-shaped kernel of known size (k_axpby with
+    y = 0: 64 MiB read, 64 MiB written) that calibrates FETCH_SIZE / WRITE_SIZE."""
     ctx = capi.Context(L_BENCH)
     spec = ctx.to_device(seeded_spec(ctx.L, NFIELDS, seed=7))
     four = ctx.empty(NFIELDS * (ctx.L + 1) * ctx.nlat * 16)
+    nff = 9
+    fspat = ctx.to_device(np.random.default_rng(8).standard_normal(nff * ctx.nspat))
+    ffour = ctx.empty(nff * (ctx.L + 1) * ctx.nlat * 16)
+    for _ in range(5):
+        ctx._check(ctx.lib.sphswe_fft_analysis(ctx.h, fspat.ptr, ffour.ptr, nff))
+    for _ in range(5):
+        ctx._check(ctx.lib.sphswe_fft_synthesis(ctx.h, ffour.ptr, fspat.ptr, nff))
     ncopy = 1 << 22
     src = ctx.to_device(np.ones(ncopy, np.complex128))
     dst = ctx.empty(ncopy * 16)
@@ -504,13 +513,12 @@ def pmc_kernels():
     print("pmc kernels done", flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per Legendre analysis (operand pass + contraction) from the
-    committed counter summary (FETCH_SIZE and WRITE_SIZE scaled by the copy
-    calibration), or None."""
+def measured_traffic(path=TRAFFIC_FILE, key="bytes_per_launch"):
+    """HBM bytes per launch from a committed counter summary (FETCH_SIZE and
+    WRITE_SIZE scaled by the copy calibration), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
-            return json.load(f)["bytes_per_launch"]
+        with open(path) as f:
+            return json.load(f)[key]
     except (OSError, ValueError, KeyError):
         return None
 
@@ -693,7 +701,9 @@ def main():
                      "legendre_tuning": ctx.legendre_tuning(),
                      "synthesis": {"kernel": "k_leg_synth (FP64 VALU)", "avg_ms": ms_syn, "achieved": achieved,
                                    "frac": achieved / FP64_PEAK_TFLOPS},
-                     "fft_T511_9fields": dict(fft, bound="hbm", kernel="k_fft_real (reference layout)")},
+                     "fft_T511_9fields": dict(fft, bound="hbm", kernel="k_fft_wide (reference layout)",
+                                              traffic={"r2c": measured_traffic(FFT_TRAFFIC_FILE, "r2c_bytes_per_launch"),
+                                                       "c2r": measured_traffic(FFT_TRAFFIC_FILE, "c2r_bytes_per_launch")})},
         "cpu_baseline": cpu,
     }
     ctx.close()
