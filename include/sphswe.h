@@ -138,6 +138,15 @@ int sphswe_spec_op_cplx(sphswe_ctx *ctx, int op, const double *d_in, double *d_o
 /* out = a*x + b*y over n complex entries (a, b complex); y may be NULL */
 int sphswe_spec_axpby(sphswe_ctx *ctx, size_t n, double a_re, double a_im, const double *d_x,
 		      double b_re, double b_im, const double *d_y, double *d_out);
+/* nfields real spectra of src_ctx's truncation copied into dst_ctx's: orders
+ * and degrees beyond the smaller truncation are dropped or zero-filled
+ * (SPHData::spec_copyToDifferentModes, SPHData.hpp:108-187, used with
+ * SPHConfig::setupAdditionalModes, SPHConfig.hpp:403-417, for the extended
+ * REXI modes of AppSWE.hpp:338-348,434-445,524-537).  Both contexts on one
+ * device; the copy runs on dst_ctx's stream after the work queued on
+ * src_ctx's stream. */
+int sphswe_spec_copy_modes(sphswe_ctx *src_ctx, const double *d_src, sphswe_ctx *dst_ctx, double *d_dst,
+			   int nfields);
 /* spectral (0,0) += v*sqrt(4 pi) (SPHData.hpp:427-440) */
 int sphswe_spec_add_scalar(sphswe_ctx *ctx, double *d_spec, double v_re, double v_im);
 

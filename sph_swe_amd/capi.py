@@ -66,6 +66,7 @@ _SIGS = {
     "sphswe_spec_op": [_vp, _i, _vp, _vp, _i, _d, _d, _d, _d],
     "sphswe_spec_op_cplx": [_vp, _i, _vp, _vp, _i, _d, _d, _d, _d, _d],
     "sphswe_spec_axpby": [_vp, _sz, _d, _d, _vp, _d, _d, _vp, _vp],
+    "sphswe_spec_copy_modes": [_vp, _vp, _vp, _vp, _i],
     "sphswe_spec_add_scalar": [_vp, _vp, _d, _d],
     "sphswe_spat_op": [_vp, _i, _vp, _vp, _vp, _d, _i],
     "sphswe_spat_op_cplx": [_vp, _i, _vp, _vp, _vp, _d, _i],
@@ -269,6 +270,16 @@ class Context:
         else:
             self._check(self.lib.sphswe_spec_op(self.h, op, d_in.ptr, d_out.ptr, nf, s.real, s.imag, p0, p1))
         return self.to_host(d_out, np.complex128, spec.shape)
+
+    def copy_modes_to(self, dst_ctx, spec):
+        """Real spectra of this context's truncation (nf, nlm) as spectra of
+        dst_ctx's truncation (SPHData::spec_copyToDifferentModes)."""
+        spec2 = np.ascontiguousarray(spec, dtype=np.complex128).reshape(-1, self.nlm)
+        nf = spec2.shape[0]
+        d_in = self.to_device(spec2)
+        d_out = dst_ctx.empty(nf * dst_ctx.nlm * 16)
+        dst_ctx._check(self.lib.sphswe_spec_copy_modes(self.h, d_in.ptr, dst_ctx.h, d_out.ptr, nf))
+        return dst_ctx.to_host(d_out, np.complex128, (nf, dst_ctx.nlm))
 
     def spat_op(self, op, a, b=None, s=1.0, cplx=False):
         dt = np.complex128 if cplx else np.float64

@@ -191,6 +191,46 @@ extern "C" int sphswe_spec_axpby(sphswe_ctx *ctx, size_t n, double a_re, double 
 	return SPHSWE_OK;
 }
 
+namespace {
+// real spectra between truncations Ls and Ld (m-major, m (2L - m + 1) / 2 + l):
+// block y = order m of the destination, threads over its degrees; orders and
+// degrees above Ls are zero
+__global__ void k_copy_modes(const double2 *__restrict__ src, double2 *__restrict__ dst, int Ls, int Ld, long nlm_s,
+			     long nlm_d, int nfields)
+{
+	const int m = blockIdx.y;
+	const long ob_d = (long)m * (2 * Ld - m + 1) / 2, ob_s = (long)m * (2 * Ls - m + 1) / 2;
+	for (int l = m + blockIdx.x * blockDim.x + threadIdx.x; l <= Ld; l += gridDim.x * blockDim.x)
+		for (int f = 0; f < nfields; f++)
+			dst[f * nlm_d + ob_d + l] = (m <= Ls && l <= Ls) ? src[f * nlm_s + ob_s + l] : make_double2(0, 0);
+}
+}  // namespace
+
+extern "C" int sphswe_spec_copy_modes(sphswe_ctx *src_ctx, const double *d_src, sphswe_ctx *dst_ctx, double *d_dst,
+				      int nfields)
+{
+	if (!src_ctx || !dst_ctx || !d_src || !d_dst || nfields < 0) return SPHSWE_ERR_ARG;
+	if (src_ctx->device != dst_ctx->device)
+		return sphswe_set_err(dst_ctx, SPHSWE_ERR_ARG, "spec_copy_modes: the contexts are on devices %d and %d",
+				      src_ctx->device, dst_ctx->device);
+	if (nfields == 0) return SPHSWE_OK;
+	if (src_ctx != dst_ctx) {
+		// the copy runs on the destination stream, after the work queued on the source stream
+		hipEvent_t ev;
+		SPHSWE_HIP(dst_ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+		hipError_t e = hipEventRecord(ev, src_ctx->stream);
+		if (e == hipSuccess) e = hipStreamWaitEvent(dst_ctx->stream, ev, 0);
+		hipEventDestroy(ev);
+		if (e != hipSuccess) return sphswe_set_err(dst_ctx, SPHSWE_ERR_HIP, "spec_copy_modes: %s", hipGetErrorString(e));
+	}
+	const int Ld = dst_ctx->L;
+	dim3 grid((Ld + 1 + 255) / 256, Ld + 1);
+	k_copy_modes<<<grid, 256, 0, dst_ctx->stream>>>((const double2 *)d_src, (double2 *)d_dst, src_ctx->L, Ld,
+						       src_ctx->nlm, dst_ctx->nlm, nfields);
+	SPHSWE_LAUNCH_CHECK(dst_ctx);
+	return SPHSWE_OK;
+}
+
 extern "C" int sphswe_spec_add_scalar(sphswe_ctx *ctx, double *d_spec, double v_re, double v_im)
 {
 	if (!ctx || !d_spec) return SPHSWE_ERR_ARG;

@@ -107,10 +107,9 @@ public:
 			}
 			simVars.use_nonlinear_equations = 0;
 		}
-		if ((simVars.use_robert_functions && simVars.use_rexi && !simVars.gpu_fused_step) ||
-		    simVars.rexi_use_extended_modes) {
-			std::cerr << "Robert functions with the class-by-class REXI loop (G0) and extended REXI modes "
-				     "are not available in this build" << std::endl;
+		if (simVars.use_robert_functions && simVars.use_rexi && !simVars.gpu_fused_step) {
+			std::cerr << "Robert functions with the class-by-class REXI loop (G0) are not available in this build"
+				  << std::endl;
 			exit(1);
 		}
 
@@ -285,6 +284,17 @@ private:
 
 	void run_rexi(bool with_coriolis)
 	{
+		// X > 0: the REXI terms work in a configuration with X more modes
+		// (AppSWE.hpp:338-351); the state is copied into it and back every step
+		SPHConfig sphConfigRexiAddedModes;
+		SPHConfig *sphConfigRexi = sphConfig;
+		const int X = simVars.rexi_use_extended_modes;
+		if (X > 0) {
+			sphConfigRexiAddedModes.setupAdditionalModes(sphConfig, X, X);
+			sphConfigRexi = &sphConfigRexiAddedModes;
+			std::cout << "REXI with " << X << " additional modes: T" << sphConfigRexi->spec_n_max << ", "
+				  << sphConfigRexi->spat_num_lon << " x " << sphConfigRexi->spat_num_lat << std::endl;
+		}
 		rexi.setup(simVars.rexi_h, simVars.rexi_M, 0, simVars.rexi_use_half_poles);
 		std::cout << "REXI poles: " << rexi.alpha.size() << std::endl;
 		const double dt = simVars.timecontrol.current_timestep_size;
@@ -292,42 +302,80 @@ private:
 		std::vector<SWERexiSPH> terms;
 		double *d_state = nullptr;
 		if (simVars.gpu_fused_step) {
+			// the implicit viscosity is diagonal in the degree, so applying it in
+			// the larger configuration before the copy back is the same as after
 			sphswe_rexi_params p = {simVars.earth_radius, simVars.gravitation, with_coriolis ? simVars.coriolis_omega : 0.0,
 						simVars.h0, simVars.viscosity2, dt, simVars.rexi_h, simVars.rexi_M,
 						simVars.rexi_use_half_poles ? 1 : 0, SPHSWE_PREC_FP64, SPHSWE_PREC_FP64, 0,
 						simVars.use_robert_functions ? 1 : 0};
-			sphConfig->check(sphswe_rexi_create(sphConfig->gpu, &p, &batched), "sphswe_rexi_create");
-			sphConfig->check(sphswe_malloc(sphConfig->gpu, 3 * sizeof(std::complex<double>) * sphConfig->spec_num_elems,
-						       (void **)&d_state), "state");
+			sphConfigRexi->check(sphswe_rexi_create(sphConfigRexi->gpu, &p, &batched), "sphswe_rexi_create");
+			sphConfigRexi->check(sphswe_malloc(sphConfigRexi->gpu,
+							   3 * sizeof(std::complex<double>) * sphConfigRexi->spec_num_elems,
+							   (void **)&d_state), "state");
 		} else {
 			terms.resize(rexi.alpha.size());
 			for (std::size_t i = 0; i < rexi.alpha.size(); i++)
-				terms[i].setup(sphConfig, rexi.alpha[i], rexi.beta_re[i], simVars.earth_radius, simVars.coriolis_omega,
-					       simVars.h0 * simVars.gravitation, dt, with_coriolis);
+				terms[i].setup(sphConfigRexi, rexi.alpha[i], rexi.beta_re[i], simVars.earth_radius,
+					       simVars.coriolis_omega, simVars.h0 * simVars.gravitation, dt, with_coriolis);
 		}
-		SPHData tmp_phi(sphConfig), tmp_u(sphConfig), tmp_v(sphConfig);
-		SPHData acc_phi(sphConfig), acc_u(sphConfig), acc_v(sphConfig);
+		SPHData tmp_phi(sphConfigRexi), tmp_u(sphConfigRexi), tmp_v(sphConfigRexi);
+		SPHData acc_phi(sphConfigRexi), acc_u(sphConfigRexi), acc_v(sphConfigRexi);
 		simVars.timecontrol.current_simulation_time = 0;
 		while (simVars.timecontrol.current_simulation_time < simVars.timecontrol.max_simulation_time) {
 			output_if_due();
 			if (batched) {
-				pack(d_state);
-				sphConfig->check(sphswe_rexi_step(sphConfig->gpu, batched, d_state, 1), "sphswe_rexi_step");
-				unpack(d_state);
+				if (X > 0) {
+					SPHData *p[3] = {&prog_h, &prog_u, &prog_v};
+					const std::size_t n = sphConfigRexi->spec_num_elems;
+					for (int k = 0; k < 3; k++) {
+						p[k]->request_data_spectral();
+						sphConfig->check(sphswe_spec_copy_modes(sphConfig->gpu, p[k]->d_spec, sphConfigRexi->gpu,
+											(double *)((std::complex<double> *)d_state + k * n), 1),
+								 "copy to the REXI modes");
+					}
+					sphConfigRexi->check(sphswe_rexi_step(sphConfigRexi->gpu, batched, d_state, 1), "sphswe_rexi_step");
+					for (int k = 0; k < 3; k++) {
+						sphConfig->check(sphswe_spec_copy_modes(sphConfigRexi->gpu,
+											(const double *)((std::complex<double> *)d_state + k * n),
+											sphConfig->gpu, p[k]->d_spec, 1),
+								 "copy from the REXI modes");
+						p[k]->data_spec_valid = true;
+						p[k]->data_spat_valid = false;
+					}
+				} else {
+					pack(d_state);
+					sphConfig->check(sphswe_rexi_step(sphConfig->gpu, batched, d_state, 1), "sphswe_rexi_step");
+					unpack(d_state);
+				}
 			} else {
-				SPHData phi = prog_h * simVars.gravitation;
+				SPHData phi(sphConfigRexi), u(sphConfigRexi), v(sphConfigRexi);
+				if (X == 0) {
+					phi = prog_h * simVars.gravitation;
+					u = prog_u;
+					v = prog_v;
+				} else {
+					(prog_h * simVars.gravitation).spec_copyToDifferentModes(phi);
+					prog_u.spec_copyToDifferentModes(u);
+					prog_v.spec_copyToDifferentModes(v);
+				}
 				acc_phi.spec_set_zero();
 				acc_u.spec_set_zero();
 				acc_v.spec_set_zero();
 				for (auto &t : terms) {
-					t.solve(phi, prog_u, prog_v, tmp_phi, tmp_u, tmp_v, opComplex);
+					t.solve(phi, u, v, tmp_phi, tmp_u, tmp_v, opComplex);
 					acc_phi += tmp_phi;
 					acc_u += tmp_u;
 					acc_v += tmp_v;
 				}
-				prog_h = acc_phi * (1.0 / simVars.gravitation);
-				prog_u = acc_u;
-				prog_v = acc_v;
+				if (X == 0) {
+					prog_h = acc_phi * (1.0 / simVars.gravitation);
+					prog_u = acc_u;
+					prog_v = acc_v;
+				} else {
+					(acc_phi * (1.0 / simVars.gravitation)).spec_copyToDifferentModes(prog_h);
+					acc_u.spec_copyToDifferentModes(prog_u);
+					acc_v.spec_copyToDifferentModes(prog_v);
+				}
 				if (simVars.viscosity2 != 0) {
 					const double scalar = simVars.viscosity2 * dt, r = simVars.earth_radius;
 					prog_h = prog_h.spec_solve_helmholtz(1.0, -scalar, r);
@@ -342,7 +390,7 @@ private:
 		write_output();
 		std::cout << std::endl;
 		if (batched) sphswe_rexi_destroy(batched);
-		if (d_state) sphswe_free(sphConfig->gpu, d_state);
+		if (d_state) sphswe_free(sphConfigRexi->gpu, d_state);
 	}
 
 	void dump_state()

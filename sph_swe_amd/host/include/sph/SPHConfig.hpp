@@ -1,6 +1,7 @@
 // SPHConfig over the C ABI: one GPU context per configuration.
 // Mirrors src/include/sph/SPHConfig.hpp (fields :20-105, index maps :106-156,
-// setup :324-352, setupAutoPhysicalSpace :366-399, shutdown :421-446).
+// setup :324-352, setupAutoPhysicalSpace :366-399, setupAdditionalModes
+// :403-417, shutdown :421-446).
 #ifndef SPHSWE_HOST_SPHCONFIG_HPP
 #define SPHSWE_HOST_SPHCONFIG_HPP
 
@@ -69,6 +70,19 @@ public:
 		sphswe_get_grid(gpu, lat_gaussian.data(), lat_cogaussian.data(), gauss_weights.data());
 		lat.assign(nlat, 0.0);
 		for (int j = 0; j < nlat; j++) lat[j] = M_PI * 0.5 - std::acos(lat_gaussian[j]);   // SPHConfig.hpp:303-319
+	}
+
+	// a configuration with more modes than i (SPHConfig.hpp:403-417): the
+	// automatic grid of the larger truncation; this configuration must be empty
+	void setupAdditionalModes(SPHConfig *i, int i_additional_modes_longitude, int i_additional_modes_latitude)
+	{
+		if (gpu) {
+			std::cerr << "SPHConfig::setupAdditionalModes: configuration already set up" << std::endl;
+			exit(1);
+		}
+		device = i->device;
+		setupAutoPhysicalSpace(i->spec_m_max + i_additional_modes_longitude, i->spec_n_max + i_additional_modes_latitude,
+				       &spat_num_lon, &spat_num_lat);
 	}
 
 	void setupAutoPhysicalSpace(int mmax, int nmax, int *nphi, int *nlat)

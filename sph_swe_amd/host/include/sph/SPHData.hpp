@@ -148,6 +148,21 @@ public:
 		return out;
 	}
 
+	// the spectrum in o's truncation (SPHData.hpp:108-187): orders and degrees
+	// beyond the smaller truncation are dropped or zero-filled
+	const SPHData &spec_copyToDifferentModes(SPHData &o) const
+	{
+		if (o.sphConfig == sphConfig) {
+			std::cerr << "spec_copyToDifferentModes: same configuration" << std::endl;
+			exit(1);
+		}
+		request_data_spectral();
+		sphConfig->check(sphswe_spec_copy_modes(gpu(), d_spec, o.gpu(), o.d_spec, 1), "spec_copyToDifferentModes");
+		o.data_spec_valid = true;
+		o.data_spat_valid = false;
+		return *this;
+	}
+
 	// ---- host-side element access -----------------------------------------------
 	void spec_download() const
 	{

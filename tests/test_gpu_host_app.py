@@ -111,6 +111,45 @@ def test_app_rexi_vs_oracle(tmp_path, fused, omega):
         assert e < 1e-11
 
 
+def _copy_modes(spec, L_src, L_dst):
+    """SPHData::spec_copyToDifferentModes on m-major real spectra (numpy)."""
+    out = np.zeros((L_dst + 1) * (L_dst + 2) // 2, complex)
+    for m in range(min(L_src, L_dst) + 1):
+        n = min(L_src, L_dst) - m + 1
+        od, os_ = m * (2 * L_dst - m + 1) // 2 + m, m * (2 * L_src - m + 1) // 2 + m
+        out[od:od + n] = spec[os_:os_ + n]
+    return out
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_app_rexi_extended_modes_vs_oracle(tmp_path, fused):
+    # X4 (AppSWE.hpp:338-351,434-445,524-537): each step copies the state into
+    # the T(L+4) configuration, runs the REXI terms there and copies back; the
+    # oracle does the same with its own REXI step at T(L+4)
+    L, X, dt, nsteps, M = 31, 4, 3600.0, 2, 8
+    nlat, nphi = capi.grid_auto(L)
+    orc = O.Oracle(L, nlat, nphi)
+    nl2, np2 = capi.grid_auto(L + X)
+    orc2 = O.Oracle(L + X, nl2, np2)
+    dump = str(tmp_path / "state.bin")
+    _run(["T%d" % L, "P0", "Q4", "M%d" % M, "H0.2", "I1", "X%d" % X, "R%g" % dt, "E%g" % (dt * nsteps), "O1e12",
+          "V100000", "G%d" % fused, "Z" + dump], str(tmp_path))
+    out = _load(dump, orc.nlm)
+    x = [orc.analysis(_dam(orc)), np.zeros(orc.nlm, complex), np.zeros(orc.nlm, complex)]
+    prm = dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, visc=1e5, rexi_h=0.2, rexi_M=M, half=1, dt=dt)
+    for _ in range(nsteps):
+        y = orc2.swe_rexi_run(prm, *[_copy_modes(f, L, L + X) for f in x], 1)
+        x = [_copy_modes(f, L + X, L) for f in y]
+    plain = orc.swe_rexi_run(prm, orc.analysis(_dam(orc)), np.zeros(orc.nlm, complex), np.zeros(orc.nlm, complex),
+                             nsteps)
+    for k in range(3):
+        e = rel_l2(out[k], x[k])
+        print("rexi X%d" % X, fused, "huv"[k], e, "vs plain", rel_l2(x[k], plain[k]))
+        assert e < 1e-11
+    # the extra modes change the result (else the test would not see them)
+    assert rel_l2(x[2], plain[2]) > 1e-8
+
+
 def test_app_p4_solver_known_answers(tmp_path):
     # AppTestSPHSolverComplex.hpp:29-517 at T32: for the 76 poles of
     # REXI::setup(0.2, 64), scalar, mu, (1-mu^2) d/dmu, z1..z8 and Helmholtz

@@ -108,3 +108,21 @@ def test_t511_round_trip_three_fields():
     orc = O.Oracle(511)
     assert rel_l2(f[0], orc.synthesis(qs[0])) < 1e-12
     ctx.close()
+
+
+def test_copy_modes_between_truncations():
+    # sphswe_spec_copy_modes (SPHData::spec_copyToDifferentModes) against numpy:
+    # zero-fill into a larger truncation, drop into a smaller one
+    from tests.test_gpu_host_app import _copy_modes
+    a, b = capi.Context(31), capi.Context(40)
+    qa = np.stack([random_real_spec(31, seed=70 + s) for s in range(2)])
+    qb = np.stack([random_real_spec(40, seed=80 + s) for s in range(3)])
+    up = a.copy_modes_to(b, qa)
+    down = b.copy_modes_to(a, qb)
+    for k in range(2):
+        assert np.array_equal(up[k], _copy_modes(qa[k], 31, 40))
+        assert np.array_equal(b.copy_modes_to(a, up[k:k + 1])[0], qa[k])
+    for k in range(3):
+        assert np.array_equal(down[k], _copy_modes(qb[k], 40, 31))
+    a.close()
+    b.close()
