@@ -141,6 +141,143 @@ __device__ inline double2 ldf(const float2 *p)
 	const float2 v = *p;
 	return make_double2(v.x, v.y);
 }
+template <typename FT>
+__device__ inline FT stf(double2 v);
+template <>
+__device__ inline double2 stf<double2>(double2 v) { return v; }
+template <>
+__device__ inline float2 stf<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+
+// LU factorisation with partial pivoting of the REXI bands on the device, one
+// thread per (pole, m, parity): the zgbtf2 steps that zgbsv_ runs on the band
+// (BandedMatrixSolver.hpp:416-427; pivot = the largest |re| + |im|, interchanges
+// applied to the current and later columns only) on the compact index of one
+// parity class, rows j = 2c + par of the m block, which couple only among
+// themselves.  The band is A(alpha) = alpha^D I + sum_k s_k(alpha) K_k with the
+// pole-independent pieces K_k at the even offsets of every row:
+//   phi (KC = 2, D = 4): K0, K2, K1, K-1 with s = 1, alpha^2, alpha, 1 / alpha;
+//   winds (KC = 1, D = 2): K0 with s = 1.
+// The window holds KC + 1 rows from the pivot column over 2 KC + 1 columns.
+// Writes the parity-compact factors (rexi_nec(2 KC) entries per row: 1 / U(j,j),
+// U(j-2e, j), L(j+2e, j)) and the even pivot offsets; *info = the smallest
+// 1-based row (within its block) whose pivot is zero, as zgbtf2's INFO.
+template <int KC, typename FT>
+__global__ __launch_bounds__(64) void k_rexi_factor(const double2 *__restrict__ alpha, const int *__restrict__ moff,
+						    const double2 *__restrict__ K, FT *__restrict__ fac,
+						    unsigned char *__restrict__ piv, int *__restrict__ info, int L, int P)
+{
+	constexpr int NK = KC == 2 ? 4 : 1;   // pieces
+	constexpr int WD = 2 * KC + 1;        // entries per row of a piece (offsets -2KC .. 2KC, even)
+	constexpr int NC = 1 + 2 * KC + KC;   // = rexi_nec(2 KC)
+	const int pole = blockIdx.x * 64 + threadIdx.x;
+	if (pole >= P) return;
+	const int par = blockIdx.y & 1;
+	const int m = m_of(blockIdx.y >> 1);
+	const int N = L + 1 - abs(m);
+	const int Np = (N - par + 1) >> 1;
+	if (Np <= 0) return;
+	const size_t r0 = moff[m + L];
+	const double2 al = alpha[pole];
+	double2 sc[NK];
+	double2 diag;
+	if (KC == 2) {
+		const double2 a2 = cm(al, al);
+		sc[0] = make_double2(1, 0);
+		sc[NK > 1 ? 1 : 0] = a2;
+		sc[NK > 2 ? 2 : 0] = al;
+		sc[NK > 3 ? 3 : 0] = cinv(al);
+		diag = cm(a2, a2);
+	} else {
+		sc[0] = make_double2(1, 0);
+		diag = cm(al, al);
+	}
+	// entry (compact row cr, compact offset e) of A(alpha), zero outside the block
+	auto entry = [&](int cr, int e) __attribute__((always_inline)) -> double2 {
+		const int cc = cr + e;
+		if (cr >= Np || cc < 0 || cc >= Np) return make_double2(0, 0);
+		const size_t q = r0 + 2 * cr + par;
+		double2 v = e == 0 ? diag : make_double2(0, 0);
+#pragma unroll
+		for (int k = 0; k < NK; k++) v = ca(v, cm(sc[k], K[(q * NK + k) * WD + KC + e]));
+		return v;
+	};
+	auto slot = [&](int cr, int e) __attribute__((always_inline)) -> FT * {
+		return fac + ((r0 + 2 * cr + par) * NC + e) * (size_t)P + pole;
+	};
+	double2 w[KC + 1][2 * KC + 1];
+#pragma unroll
+	for (int i = 0; i <= KC; i++)
+#pragma unroll
+		for (int d = 0; d <= 2 * KC; d++) w[i][d] = (d - i >= -KC && d - i <= KC) ? entry(i, d - i) : make_double2(0, 0);
+	for (int c = 0; c < Np; c++) {
+		const int km = min(KC, Np - 1 - c);
+		int ip = 0;
+		double amax = -1.0;
+#pragma unroll
+		for (int i = 0; i <= KC; i++) {
+			const double a = fabs(w[i][0].x) + fabs(w[i][0].y);
+			if (i <= km && a > amax) {
+				amax = a;
+				ip = i;
+			}
+		}
+		// interchange rows 0 and ip of the window (selects on registers)
+#pragma unroll
+		for (int i = 1; i <= KC; i++) {
+			const bool sw = ip == i;
+#pragma unroll
+			for (int d = 0; d <= 2 * KC; d++) {
+				const double2 a = w[0][d], b = w[i][d];
+				w[0][d] = make_double2(sw ? b.x : a.x, sw ? b.y : a.y);
+				w[i][d] = make_double2(sw ? a.x : b.x, sw ? a.y : b.y);
+			}
+		}
+		const bool nz = w[0][0].x != 0.0 || w[0][0].y != 0.0;
+		if (!nz) atomicMin(info, 2 * c + par + 1);
+		const double2 rp = nz ? cinv(w[0][0]) : make_double2(0, 0);
+		*slot(c, 0) = stf<FT>(rp);
+		// U(c, c + e) belongs to column c + e: entry e of that row
+#pragma unroll
+		for (int e = 1; e <= 2 * KC; e++)
+			if (c + e < Np) *slot(c + e, e) = stf<FT>(w[0][e]);
+#pragma unroll
+		for (int i = 1; i <= KC; i++) {
+			const double2 l = (i <= km && nz) ? cm(w[i][0], rp) : make_double2(0, 0);
+			*slot(c, 2 * KC + i) = stf<FT>(l);
+			if (i <= km && nz)
+#pragma unroll
+				for (int d = 1; d <= 2 * KC; d++) w[i][d] = cb(w[i][d], cm(l, w[0][d]));
+		}
+		piv[(r0 + 2 * c + par) * (size_t)P + pole] = (unsigned char)(2 * ip);
+		// next pivot column: shift the window, the row c + 1 + KC enters
+#pragma unroll
+		for (int i = 0; i < KC; i++) {
+#pragma unroll
+			for (int d = 0; d < 2 * KC; d++) w[i][d] = w[i + 1][d + 1];
+			w[i][2 * KC] = make_double2(0, 0);
+		}
+#pragma unroll
+		for (int d = 0; d <= 2 * KC; d++) w[KC][d] = entry(c + 1 + KC, d - KC);
+	}
+}
+
+// without rotation the phi system is diagonal: 1 / (alpha^2 + (Phi / r^2) n (n + 1))
+// (SWERexiSPH.hpp:230-231), entry 0 of every row
+template <typename FT>
+__global__ void k_rexi_diag(const double2 *__restrict__ alpha, const int *__restrict__ moff, FT *__restrict__ fac,
+			    int L, int P, int NC, double g_over_r2)
+{
+	const int m = m_of(blockIdx.y);
+	const int N = L + 1 - abs(m);
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= (long)N * P) return;
+	const int j = (int)(t / P), pole = (int)(t % P);
+	const int n = abs(m) + j;
+	const double2 al = alpha[pole];
+	double2 d = cm(al, al);
+	d.x += g_over_r2 * (double)n * ((double)n + 1.0);
+	fac[((size_t)(moff[m + L] + j) * NC) * P + pole] = stf<FT>(cinv(d));
+}
 
 // right-hand side of the phi system of one pole at degree n, order m
 // (SWERexiSPH.hpp:162-195, with the pole-independent fields in the basis B)
@@ -1045,125 +1182,52 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	std::vector<int> moff(2 * L + 1);
 	for (int m = -L; m <= L; m++) moff[m + L] = ncompact(L, std::abs(m), m);
 
-	// assemble and factor the band matrices of every pole (SWERexiSPH::setup,
-	// SWERexiSPH.hpp:104-129), in parallel over poles.  Every component couples
-	// degrees of equal parity (offsets 0, +-2, +-4), so the LU with partial
-	// pivoting keeps exact zeros at odd offsets and even pivot offsets; only
-	// the even entries are kept (checked here).
+	// the band matrices of every pole (SWERexiSPH::setup, SWERexiSPH.hpp:104-129)
+	// grouped by their power of alpha,
+	//   phi:   A(alpha) = alpha^4 I + K0 + alpha^2 K2 + alpha K1 + K-1 / alpha,
+	//   winds: alpha^2 I + W^2 mu^2,
+	// with the pole-independent pieces assembled once on the host; the pivoted LU
+	// of every (pole, m) runs on the device (k_rexi_factor).  Every component
+	// couples degrees of equal parity (offsets 0, +-2, +-4), so each m block splits
+	// into two independent parity classes and only the even offsets are kept
+	// (checked here on the pieces).
 	const double rr = prm->earth_radius, W = 2.0 * prm->coriolis_omega, G = prm->h0 * prm->gravitation;
-	const int NEp = 1 + 8 + 4, NEv = 1 + 4 + 2;           // full LU entries per row (halo 4, 2)
-	const int NCp = rexi_nec(4), NCv = rexi_nec(2);       // kept entries per row
-	std::vector<double2> hfphi, hfvel;
-	std::vector<unsigned char> hpphi, hpvel;
-	int info_all = 0, parity_bad = 0;
-	// full entry of compact entry c for halo h
-	auto full_of = [](int c, int h) { return c == 0 ? 0 : (c <= h ? 2 * c : 2 * h + 2 * (c - h)); };
-	if (r->coriolis) {
-		hfphi.assign(r->rows * NCp * P, make_double2(0, 0));
-		hfvel.assign(r->rows * NCv * P, make_double2(0, 0));
-		hpphi.assign(r->rows * P, 0);
-		hpvel.assign(r->rows * P, 0);
-		unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-		std::vector<std::thread> th;
-		std::vector<int> infos(nth, 0), bad(nth, 0);
-		auto pack = [&](const std::vector<cplx> &fac, const std::vector<unsigned char> &piv, int NE, int NC, int h,
-				size_t r0, int pp, std::vector<double2> &hf, std::vector<unsigned char> &hp, int &badv) {
-			for (size_t j = 0; j < piv.size(); j++) {
-				for (int e = 1; e < NE; e++)
-					if (e != 0 && (e <= 2 * h ? (e & 1) : ((e - 2 * h) & 1)) && fac[j * NE + e] != cplx(0, 0)) badv = 1;
-				if (piv[j] & 1) badv = 1;
-				for (int c = 0; c < NC; c++) {
-					const cplx v = fac[j * NE + full_of(c, h)];
-					hf[((r0 + j) * NC + c) * P + pp] = make_double2(v.real(), v.imag());
-				}
-				hp[(r0 + j) * P + pp] = piv[j];
-			}
-		};
-		for (unsigned t = 0; t < nth; t++)
-			th.emplace_back([&, t]() {
-				std::vector<cplx> fac;
-				std::vector<unsigned char> piv;
-				for (int pp = (int)t; pp < P; pp += (int)nth) {
-					const cplx a = alc[pp];
-					BandRows Bp(L, 4);
-					band_add(Bp, 11, (a * a) * (a * a), rr);
-					band_add(Bp, 12, 2.0 * W * W * a * a, rr);
-					band_add(Bp, 13, (W * W) * (W * W), rr);
-					// z4..z6, or their Robert forms (SWERexiSPHRobert.hpp setup)
-					band_add(Bp, r->robert ? 19 : 14, -G * a * W, rr);
-					band_add(Bp, r->robert ? 20 : 15, G / a * W * W * W, rr);
-					band_add(Bp, r->robert ? 21 : 16, G * 2.0 * W * W, rr);
-					band_add(Bp, 17, -G * a * a, rr);
-					band_add(Bp, 18, -G * W * W, rr);
-					BandRows Bv(L, 2);
-					band_add(Bv, 11, a * a, rr);
-					band_add(Bv, 12, W * W, rr);
-					for (int m = -L; m <= L; m++) {
-						const size_t r0 = moff[m + L];
-						int info = band_factor_block(Bp, m, fac, piv);
-						if (info && !infos[t]) infos[t] = info;
-						pack(fac, piv, NEp, NCp, 4, r0, pp, hfphi, hpphi, bad[t]);
-						info = band_factor_block(Bv, m, fac, piv);
-						if (info && !infos[t]) infos[t] = info;
-						pack(fac, piv, NEv, NCv, 2, r0, pp, hfvel, hpvel, bad[t]);
-					}
-				}
-			});
-		for (auto &x : th) x.join();
-		for (int v : infos)
-			if (v && !info_all) info_all = v;
-		for (int v : bad) parity_bad |= v;
-	} else {
-		// no Coriolis: diagonal Helmholtz 1 / (alpha^2 + (Phi/r^2) n(n+1)) (SWERexiSPH.hpp:230-231)
-		hfphi.assign(r->rows * NCp * P, make_double2(0, 0));
-		for (int m = -L; m <= L; m++)
-			for (int n = std::abs(m); n <= L; n++)
-				for (int pp = 0; pp < P; pp++) {
-					const cplx a2 = alc[pp] * alc[pp];
-					const cplx bb = cplx(-G, 0) / (rr * rr);
-					const cplx d = a2 + (-bb * (double)n * ((double)n + 1.0));
-					const cplx inv = 1.0 / d;
-					hfphi[((moff[m + L] + n - std::abs(m)) * NCp) * P + pp] = make_double2(inv.real(), inv.imag());
-				}
-	}
-	if (info_all) {
-		delete r;
-		return sphswe_set_err(ctx, SPHSWE_ERR_SINGULAR, "zgbsv-equivalent factorisation: INFO = %d", info_all);
-	}
-	if (parity_bad) {
-		delete r;
-		return sphswe_set_err(ctx, SPHSWE_ERR_INTERNAL, "REXI band factors: nonzero entry at an odd offset");
-	}
+	const int NCp = rexi_nec(4), NCv = rexi_nec(2);       // kept factor entries per row
 	r->fac32 = prm->band_precision == SPHSWE_PREC_FP32;
 	r->refine = prm->refine && r->coriolis;   // without rotation the solves are diagonal
-	// refinement: A(alpha) = alpha^4 I + K0 + alpha^2 K2 + alpha K1 + K-1 / alpha for
-	// phi and alpha^2 I + W^2 mu^2 for the winds, the pole-independent pieces at the
-	// even offsets of every row (the components of SWERexiSPH::setup grouped by
-	// their power of alpha)
 	std::vector<double2> hK, hKv;
-	if (r->refine) {
+	if (r->coriolis) {
 		BandRows K0(L, 4), K2(L, 4), K1(L, 4), Km1(L, 4), Kv(L, 2);
 		band_add(K0, 13, (W * W) * (W * W), rr);
-		band_add(K0, 16, G * 2.0 * W * W, rr);
+		band_add(K0, r->robert ? 21 : 16, G * 2.0 * W * W, rr);
 		band_add(K0, 18, -G * W * W, rr);
 		band_add(K2, 12, 2.0 * W * W, rr);
 		band_add(K2, 17, -G, rr);
-		band_add(K1, 14, -G * W, rr);
-		band_add(Km1, 15, G * W * W * W, rr);
+		band_add(K1, r->robert ? 19 : 14, -G * W, rr);
+		band_add(Km1, r->robert ? 20 : 15, G * W * W * W, rr);
 		band_add(Kv, 12, W * W, rr);
 		hK.resize(r->rows * 20);
 		hKv.resize(r->rows * 3);
 		const BandRows *kinds[4] = {&K0, &K2, &K1, &Km1};
+		int parity_bad = 0;
 		for (size_t q = 0; q < r->rows; q++) {
-			for (int k = 0; k < 4; k++)
+			for (int k = 0; k < 4; k++) {
 				for (int e = 0; e < 5; e++) {
 					const cplx v = kinds[k]->data[q * 9 + 2 * e];
 					hK[q * 20 + k * 5 + e] = make_double2(v.real(), v.imag());
 				}
+				for (int e = 0; e < 4; e++)
+					if (kinds[k]->data[q * 9 + 2 * e + 1] != cplx(0, 0)) parity_bad = 1;
+			}
 			for (int e = 0; e < 3; e++) {
 				const cplx v = Kv.data[q * 5 + 2 * e];
 				hKv[q * 3 + e] = make_double2(v.real(), v.imag());
 			}
+			if (Kv.data[q * 5 + 1] != cplx(0, 0) || Kv.data[q * 5 + 3] != cplx(0, 0)) parity_bad = 1;
+		}
+		if (parity_bad) {
+			delete r;
+			return sphswe_set_err(ctx, SPHSWE_ERR_INTERNAL, "REXI bands: nonzero entry at an odd offset");
 		}
 	}
 	std::vector<double> osc(ctx->nlat);
@@ -1179,25 +1243,25 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 			return sphswe_set_err(ctx, SPHSWE_ERR_HIP, "rexi: upload failed");
 		return SPHSWE_OK;
 	};
-	// factors to the device, in FP64 or rounded to FP32
-	auto up_fac = [&](void **d, const std::vector<double2> &h) -> int {
+	// factor storage on the device, FP64 or FP32, zero-filled
+	auto alloc_fac = [&](void **d, size_t n) -> int {
 		int rc2;
 		if (!r->fac32) {
 			double2 *p2 = nullptr;
-			if ((rc2 = dev_alloc(ctx, &p2, h.size()))) return rc2;
+			if ((rc2 = dev_alloc(ctx, &p2, n))) return rc2;
 			*d = p2;
-			return up(p2, h.data(), sizeof(double2) * h.size());
+		} else {
+			float2 *p2 = nullptr;
+			if ((rc2 = dev_alloc(ctx, &p2, n))) return rc2;
+			*d = p2;
 		}
-		std::vector<float2> h32(h.size());
-		for (size_t i = 0; i < h.size(); i++) h32[i] = make_float2((float)h[i].x, (float)h[i].y);
-		float2 *p2 = nullptr;
-		if ((rc2 = dev_alloc(ctx, &p2, h32.size()))) return rc2;
-		*d = p2;
-		return up(p2, h32.data(), sizeof(float2) * h32.size());
+		if (hipMemsetAsync(*d, 0, n * (r->fac32 ? sizeof(float2) : sizeof(double2)), ctx->stream) != hipSuccess)
+			return sphswe_set_err(ctx, SPHSWE_ERR_HIP, "rexi: clearing the factors failed");
+		return SPHSWE_OK;
 	};
 	const size_t nsp = (size_t)ctx->nspat;
 	if ((rc = dev_alloc(ctx, &r->d_alpha, P)) || (rc = dev_alloc(ctx, &r->d_beta, P)) ||
-	    (rc = dev_alloc(ctx, &r->d_moff, 2 * L + 1)) || (rc = up_fac(&r->d_fphi, hfphi)) ||
+	    (rc = dev_alloc(ctx, &r->d_moff, 2 * L + 1)) || (rc = alloc_fac(&r->d_fphi, r->rows * NCp * P)) ||
 	    (rc = dev_alloc(ctx, &r->d_oscale, ctx->nlat)) ||
 	    (rc = dev_alloc(ctx, &r->d_X, r->rows * 2 * P)) || (rc = dev_alloc(ctx, &r->d_G, r->rows * 2 * P)) ||
 	    (rc = dev_alloc(ctx, &r->d_Y, r->rows * 2 * P)) ||
@@ -1215,19 +1279,69 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 		sphswe_rexi_destroy(r);
 		return rc;
 	}
+	// the LU factors of every (pole, m) on the device
+	int *d_info = nullptr;
+	if ((rc = dev_alloc(ctx, &d_info, 1))) {
+		sphswe_rexi_destroy(r);
+		return rc;
+	}
+	const int no_info = 0x7fffffff;
+	if (hipMemcpy(d_info, &no_info, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+		hipFree(d_info);
+		sphswe_rexi_destroy(r);
+		return sphswe_set_err(ctx, SPHSWE_ERR_HIP, "rexi: upload failed");
+	}
 	if (r->coriolis) {
-		if ((rc = up_fac(&r->d_fvel, hfvel)) || (rc = dev_alloc(ctx, &r->d_pphi, hpphi.size())) ||
-		    (rc = dev_alloc(ctx, &r->d_pvel, hpvel.size())) ||
-		    (rc = up(r->d_pphi, hpphi.data(), hpphi.size())) || (rc = up(r->d_pvel, hpvel.data(), hpvel.size()))) {
+		if ((rc = alloc_fac(&r->d_fvel, r->rows * NCv * P)) || (rc = dev_alloc(ctx, &r->d_pphi, r->rows * P)) ||
+		    (rc = dev_alloc(ctx, &r->d_pvel, r->rows * P)) || (rc = dev_alloc(ctx, &r->d_kphi, hK.size())) ||
+		    (rc = dev_alloc(ctx, &r->d_kvel, hKv.size())) ||
+		    (rc = up(r->d_kphi, hK.data(), sizeof(double2) * hK.size())) ||
+		    (rc = up(r->d_kvel, hKv.data(), sizeof(double2) * hKv.size()))) {
+			hipFree(d_info);
 			sphswe_rexi_destroy(r);
 			return rc;
 		}
+		dim3 fg(r->ngroups, 2 * (2 * L + 1));
+		if (r->fac32) {
+			k_rexi_factor<2, float2><<<fg, 64, 0, ctx->stream>>>(r->d_alpha, r->d_moff, r->d_kphi, (float2 *)r->d_fphi,
+									   r->d_pphi, d_info, L, P);
+			k_rexi_factor<1, float2><<<fg, 64, 0, ctx->stream>>>(r->d_alpha, r->d_moff, r->d_kvel, (float2 *)r->d_fvel,
+									   r->d_pvel, d_info, L, P);
+		} else {
+			k_rexi_factor<2, double2><<<fg, 64, 0, ctx->stream>>>(r->d_alpha, r->d_moff, r->d_kphi, (double2 *)r->d_fphi,
+									    r->d_pphi, d_info, L, P);
+			k_rexi_factor<1, double2><<<fg, 64, 0, ctx->stream>>>(r->d_alpha, r->d_moff, r->d_kvel, (double2 *)r->d_fvel,
+									    r->d_pvel, d_info, L, P);
+		}
+	} else {
+		dim3 dg((unsigned)(((long)(L + 1) * P + 255) / 256), 2 * L + 1);
+		if (r->fac32)
+			k_rexi_diag<float2><<<dg, 256, 0, ctx->stream>>>(r->d_alpha, r->d_moff, (float2 *)r->d_fphi, L, P, NCp,
+									 G / (rr * rr));
+		else
+			k_rexi_diag<double2><<<dg, 256, 0, ctx->stream>>>(r->d_alpha, r->d_moff, (double2 *)r->d_fphi, L, P, NCp,
+									  G / (rr * rr));
 	}
-	if (r->refine && ((rc = dev_alloc(ctx, &r->d_kphi, hK.size())) || (rc = dev_alloc(ctx, &r->d_kvel, hKv.size())) ||
-			  (rc = up(r->d_kphi, hK.data(), sizeof(double2) * hK.size())) ||
-			  (rc = up(r->d_kvel, hKv.data(), sizeof(double2) * hKv.size())))) {
-		sphswe_rexi_destroy(r);
-		return rc;
+	{
+		hipError_t e = hipGetLastError();
+		int info = no_info;
+		if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+		if (e == hipSuccess) e = hipMemcpy(&info, d_info, sizeof(int), hipMemcpyDeviceToHost);
+		hipFree(d_info);
+		if (e != hipSuccess) {
+			sphswe_rexi_destroy(r);
+			return sphswe_set_err(ctx, SPHSWE_ERR_HIP, "rexi: band factorisation: %s", hipGetErrorString(e));
+		}
+		if (info != no_info) {
+			sphswe_rexi_destroy(r);
+			return sphswe_set_err(ctx, SPHSWE_ERR_SINGULAR, "zgbsv-equivalent factorisation: INFO = %d", info);
+		}
+	}
+	// the pieces stay for the refinement sweep only
+	if (r->coriolis && !r->refine) {
+		hipFree(r->d_kphi);
+		hipFree(r->d_kvel);
+		r->d_kphi = r->d_kvel = nullptr;
 	}
 	// Robert functions: the gradients of phi are the spectral i m phi and D' phi
 	// that the phi sweep writes, so there is no gradient operator
