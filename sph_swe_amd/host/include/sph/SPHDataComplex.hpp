@@ -120,6 +120,18 @@ public:
 		return e;
 	}
 
+	// the field projected on the spectral modes and back (synthesis, analysis)
+	SPHDataComplex spec_truncate() const
+	{
+		request_data_spectral();
+		SPHDataComplex out(sphConfig);
+		sphConfig->check(sphswe_sht_synthesis_cplx(gpu(), d_spec, out.d_spat, 1, SPHSWE_PREC_FP64), "spec_truncate");
+		sphConfig->check(sphswe_sht_analysis_cplx(gpu(), out.d_spat, out.d_spec, 1, SPHSWE_PREC_FP64), "spec_truncate");
+		out.data_spec_valid = true;
+		out.data_spat_valid = false;
+		return out;
+	}
+
 	SPHDataComplex spec_solve_helmholtz(const cplx &i_a, const cplx &i_b, double r) const
 	{
 		request_data_spectral();

@@ -1,5 +1,6 @@
 // SPHOperatorsComplex over the C ABI (src/include/sph/SPHOperatorsComplex.hpp:
-// diff_lon :27-60, spec_one_minus_mu_squared_diff_lat_mu :119-153, mu :155-187,
+// diff_lon :27-60, diff_lat_mu :62-77, diff_lat_phi :87-92,
+// spec_one_minus_mu_squared_diff_lat_mu :119-153, mu :155-187,
 // mu2 :189-222, grad_lon :100-117, grad_lat :224-282, div_lon :284-298,
 // div_lat :300-336 (division by cos(lat[j])), laplace :454-478, vort :480-494,
 // div :513-528).
@@ -25,6 +26,12 @@ public:
 	{
 		return metric(spec_one_minus_mu_squared_diff_lat_mu(i), SPHSWE_SPAT_DIV_COSPHI);
 	}
+	// d/dmu: (1 - mu^2) d/dmu, then division by 1 - mu^2
+	SPHDataComplex diff_lat_mu(const SPHDataComplex &i) const
+	{
+		return metric(spec_one_minus_mu_squared_diff_lat_mu(i), SPHSWE_SPAT_DIV_ST2);
+	}
+	SPHDataComplex diff_lat_phi(const SPHDataComplex &i) const { return grad_lat(i); }
 	SPHDataComplex div_lon(const SPHDataComplex &i) const { return grad_lon(i); }
 	SPHDataComplex div_lat(const SPHDataComplex &i) const
 	{

@@ -1,6 +1,7 @@
 // Command-line entry of the shallow-water program (src/main.cpp:33-262): the
-// same flags, programs P0 (AppSWE), P1 (AppTestSPHOperators), P4 (AppTestSPHSolversComplex)
-// and P5 (AppOutputSphericalHarmonics).  Two letters have no counterpart in the reference:
+// same flags and programs: P0 (AppSWE), P1 (AppTestSPHOperators), P2
+// (AppTestSPHOperatorsComplex), P3 (AppTestSPHSolvers), P4
+// (AppTestSPHSolversComplex) and P5 (AppOutputSphericalHarmonics).  Two letters have no counterpart in the reference:
 // G<0|1> (one library call per time step, default 1) and Z<file> (write the
 // final (h, u, v) spectra as raw complex doubles).
 #include <cstdio>
@@ -10,6 +11,8 @@
 #include "AppSWE.hpp"
 #include "AppOutputSphericalHarmonics.hpp"
 #include "AppTestSPHOperators.hpp"
+#include "AppTestSPHOperatorsComplex.hpp"
+#include "AppTestSPHSolver.hpp"
 #include "AppTestSPHSolverComplex.hpp"
 #include "SimVars.hpp"
 #include "sph/SPHConfig.hpp"
@@ -92,6 +95,22 @@ int main(int argc, const char *argv[])
 		std::cout << "************************************************" << std::endl;
 		AppTestSPHOperators test_operators;
 		test_operators.run(&sphConfig);
+		return 0;
+	}
+	if (simVars.program_id == 2) {
+		std::cout << "************************************************" << std::endl;
+		std::cout << "AppTestSPHOperatorsComplex" << std::endl;
+		std::cout << "************************************************" << std::endl;
+		AppTestSPHOperatorsComplex test_operators;
+		test_operators.run(&sphConfig);
+		return 0;
+	}
+	if (simVars.program_id == 3) {
+		std::cout << "************************************************" << std::endl;
+		std::cout << "AppTestSPHSolvers" << std::endl;
+		std::cout << "************************************************" << std::endl;
+		AppTestSPHSolvers test_solvers;
+		test_solvers.run(&sphConfig);
 		return 0;
 	}
 	if (simVars.program_id == 5) {

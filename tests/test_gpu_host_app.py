@@ -1,9 +1,10 @@
 """The C++ host program (host/main.cpp with the SPHConfig / SPHData /
 SPHOperators / TimesteppingRK / REXI / SWERexiSPH / AppSWE classes over the C
 ABI) against the oracle: RK4 and REXI runs through both the one-call-per-step
-path (G1) and the reference's class-by-class loop (G0); programs P1 (real
-operators against analytic derivatives), P4 (the known-answer tests of the
-complex banded solver) and P5 (one CSV per spherical harmonic)."""
+path (G1) and the reference's class-by-class loop (G0); the self-test
+programs P1 (real operators against analytic derivatives), P2 (complex
+operators), P3 (the banded solver on real spectra), P4 (the known-answer tests
+of the complex banded solver) and P5 (one CSV per spherical harmonic)."""
 import os
 import subprocess
 
@@ -188,14 +189,8 @@ def test_app_p1_operators_vs_analytic(tmp_path):
     # DIV LAT and the sums of two operator results (DIVERGENCE, VORTICITY,
     # div.grad; projected by SPHData's spectral + and -) are checked against the
     # same errors computed by the oracle's operators
-    import re
     L = 64
-    out = _run(["T%d" % L, "P1"], str(tmp_path))
-    errs = {}
-    for line in out.splitlines():
-        mt = re.match(r"^(.*?)[: -]+max error: ([-+0-9.eE]+)$", line) or re.match(r"^ERROR (.*): ([-+0-9.eE]+)$", line)
-        if mt:
-            errs[mt.group(1).strip()] = float(mt.group(2))
+    errs = _p_errors(_run(["T%d" % L, "P1"], str(tmp_path)))
     print(errs)
     assert len(errs) == 18, errs
     nlat, nphi = capi.grid_auto(L)
@@ -223,6 +218,57 @@ def test_app_p1_operators_vs_analytic(tmp_path):
             assert e < 0.2
         else:
             assert 0 <= e < 1e-10, (name, e)
+
+
+def _p_errors(out):
+    import re
+    errs = {}
+    for line in out.splitlines():
+        mt = (re.match(r"^(.*?)[: -]+max error: ([-+0-9.eE]+)$", line) or re.match(r"^ERROR (.*): ([-+0-9.eE]+)$", line)
+              or re.match(r"^(Test .*) \|\|\| Error: ([-+0-9.eE]+)$", line))
+        if mt:
+            errs[mt.group(1).strip()] = float(mt.group(2))
+    return errs
+
+
+def test_app_p2_complex_operators_vs_analytic(tmp_path):
+    # AppTestSPHOperatorsComplex.hpp:26-606 at T64: the complex operators on a
+    # real-valued bump; the exact cases to 1e-10, div_lat and div.grad equal the
+    # errors of the real operators computed by the oracle (the complex analysis
+    # of a real field has the real analysis' coefficients)
+    L = 64
+    errs = _p_errors(_run(["T%d" % L, "P2"], str(tmp_path)))
+    print(errs)
+    assert len(errs) == 17, errs
+    nlat, nphi = capi.grid_auto(L)
+    orc = O.Oracle(L, nlat, nphi)
+    lon = np.repeat(orc.lon(), nlat).reshape(nphi, nlat)
+    mu = np.tile(np.asarray(orc.ct), nphi).reshape(nphi, nlat)
+    f, div_mu, _, _ = _gauss_div(lon, mu)
+    f = f.ravel()
+    proj = lambda x: orc.synthesis(orc.analysis(x))
+    inexact = {
+        "TEST DIV LAT": np.max(np.abs(orc.phys_op(2, f) - div_mu.ravel())),
+        "TEST LAPLACE": np.max(np.abs(proj(orc.phys_op(0, orc.phys_op(0, f)) + orc.phys_op(2, orc.phys_op(1, f)))
+                                      - orc.phys_op(3, f))),
+    }
+    for name, e in errs.items():
+        if name in inexact:
+            print(name, "app", e, "oracle", inexact[name])
+            assert abs(e - inexact[name]) < 1e-9 + 1e-6 * inexact[name], name
+        else:
+            assert 0 <= e < 1e-10, (name, e)
+
+
+def test_app_p3_real_solver_known_answers(tmp_path):
+    # AppTestSPHSolver.hpp:34-482 at T32: the banded solver on real spectra
+    # (alpha = 3 and the REXI scalars), each solution against the bump
+    L = 32
+    errs = _p_errors(_run(["T%d" % L, "P3"], str(tmp_path)))
+    print(errs)
+    assert len(errs) == 11, errs
+    for name, e in errs.items():
+        assert 0 <= e < 1e-10, (name, e)   # measured <= 2.4e-11 (z3)
 
 
 def test_app_p5_writes_every_harmonic(tmp_path):
