@@ -107,11 +107,6 @@ public:
 			}
 			simVars.use_nonlinear_equations = 0;
 		}
-		if (simVars.use_robert_functions && simVars.use_rexi && !simVars.gpu_fused_step) {
-			std::cerr << "Robert functions with the class-by-class REXI loop (G0) are not available in this build"
-				  << std::endl;
-			exit(1);
-		}
 
 		switch (simVars.benchmark_scenario_id) {
 		case 0: setup_initial_conditions_gaussian(); break;
@@ -300,6 +295,7 @@ private:
 		const double dt = simVars.timecontrol.current_timestep_size;
 		sphswe_rexi *batched = nullptr;
 		std::vector<SWERexiSPH> terms;
+		std::vector<SWERexiSPHRobert> terms_robert;   // Robert-function velocities (AppSWE.hpp:372-390)
 		double *d_state = nullptr;
 		if (simVars.gpu_fused_step) {
 			// the implicit viscosity is diagonal in the degree, so applying it in
@@ -313,10 +309,16 @@ private:
 							   3 * sizeof(std::complex<double>) * sphConfigRexi->spec_num_elems,
 							   (void **)&d_state), "state");
 		} else {
-			terms.resize(rexi.alpha.size());
-			for (std::size_t i = 0; i < rexi.alpha.size(); i++)
-				terms[i].setup(sphConfigRexi, rexi.alpha[i], rexi.beta_re[i], simVars.earth_radius,
-					       simVars.coriolis_omega, simVars.h0 * simVars.gravitation, dt, with_coriolis);
+			if (simVars.use_robert_functions) terms_robert.resize(rexi.alpha.size());
+			else terms.resize(rexi.alpha.size());
+			for (std::size_t i = 0; i < rexi.alpha.size(); i++) {
+				if (simVars.use_robert_functions)
+					terms_robert[i].setup(sphConfigRexi, rexi.alpha[i], rexi.beta_re[i], simVars.earth_radius,
+							      simVars.coriolis_omega, simVars.h0 * simVars.gravitation, dt, with_coriolis);
+				else
+					terms[i].setup(sphConfigRexi, rexi.alpha[i], rexi.beta_re[i], simVars.earth_radius,
+						       simVars.coriolis_omega, simVars.h0 * simVars.gravitation, dt, with_coriolis);
+			}
 		}
 		SPHData tmp_phi(sphConfigRexi), tmp_u(sphConfigRexi), tmp_v(sphConfigRexi);
 		SPHData acc_phi(sphConfigRexi), acc_u(sphConfigRexi), acc_v(sphConfigRexi);
@@ -361,12 +363,14 @@ private:
 				acc_phi.spec_set_zero();
 				acc_u.spec_set_zero();
 				acc_v.spec_set_zero();
-				for (auto &t : terms) {
+				auto add_term = [&](auto &t) {
 					t.solve(phi, u, v, tmp_phi, tmp_u, tmp_v, opComplex);
 					acc_phi += tmp_phi;
 					acc_u += tmp_u;
 					acc_v += tmp_v;
-				}
+				};
+				for (auto &t : terms) add_term(t);
+				for (auto &t : terms_robert) add_term(t);
 				if (X == 0) {
 					prog_h = acc_phi * (1.0 / simVars.gravitation);
 					prog_u = acc_u;

@@ -3,28 +3,32 @@
 //   (alpha / dt + L)(phi, u, v) = (phi0, u0, v0) / dt,  return Re(beta / dt (phi, u, v)),
 // with the per-m band solves on the device.  The state carries the geopotential
 // phi = g h.  The Coriolis forcing product covers the whole grid.
+// SWERexiSPHRobert (src/SWERexiSPHRobert.hpp:68-258) is the same term for
+// Robert-function velocities: z4..z6robert bands, robert_div / robert_vort
+// right-hand sides and the spectral Robert gradients.
 #ifndef SPHSWE_HOST_SWEREXISPH_HPP
 #define SPHSWE_HOST_SWEREXISPH_HPP
 
 #include "sph/SPHData.hpp"
 #include "sph/SPHOperatorsComplex.hpp"
 
-class SWERexiSPH
+template <int ROBERT>
+class SWERexiSPHTerm
 {
 	SPHConfig *sphConfig = nullptr;
 	sphswe_rexi *term = nullptr;
 	double *d_io = nullptr;       // (phi, u, v) real spectra
 
 public:
-	SWERexiSPH() {}
-	SWERexiSPH(const SWERexiSPH &) = delete;
-	SWERexiSPH &operator=(const SWERexiSPH &) = delete;
-	SWERexiSPH(SWERexiSPH &&o) noexcept : sphConfig(o.sphConfig), term(o.term), d_io(o.d_io)
+	SWERexiSPHTerm() {}
+	SWERexiSPHTerm(const SWERexiSPHTerm &) = delete;
+	SWERexiSPHTerm &operator=(const SWERexiSPHTerm &) = delete;
+	SWERexiSPHTerm(SWERexiSPHTerm &&o) noexcept : sphConfig(o.sphConfig), term(o.term), d_io(o.d_io)
 	{
 		o.term = nullptr;
 		o.d_io = nullptr;
 	}
-	~SWERexiSPH() { release(); }
+	~SWERexiSPHTerm() { release(); }
 
 	void setup(SPHConfig *i_sphConfig, const std::complex<double> &i_alpha, const std::complex<double> &i_beta,
 		   double i_radius, double i_coriolis_omega, double i_avg_geopotential, double i_timestep_size,
@@ -34,7 +38,8 @@ public:
 		sphConfig = i_sphConfig;
 		// the term sees only Phi = g h0: pass it as h0 with g = 1
 		sphswe_rexi_params p = {i_radius, 1.0, i_include_coriolis_effect ? i_coriolis_omega : 0.0,
-					i_avg_geopotential, 0.0, i_timestep_size, 0.2, 1, 1};
+					i_avg_geopotential, 0.0, i_timestep_size, 0.2, 1, 1, SPHSWE_PREC_FP64, SPHSWE_PREC_FP64, 0,
+					ROBERT};
 		const double a[2] = {i_alpha.real(), i_alpha.imag()}, b[2] = {i_beta.real(), i_beta.imag()};
 		sphConfig->check(sphswe_rexi_create_poles(sphConfig->gpu, &p, 1, a, b, &term), "SWERexiSPH::setup");
 		sphConfig->check(sphswe_malloc(sphConfig->gpu, 3 * sizeof(std::complex<double>) * sphConfig->spec_num_elems,
@@ -68,5 +73,8 @@ private:
 		d_io = nullptr;
 	}
 };
+
+typedef SWERexiSPHTerm<0> SWERexiSPH;
+typedef SWERexiSPHTerm<1> SWERexiSPHRobert;
 
 #endif

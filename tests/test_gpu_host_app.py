@@ -76,21 +76,20 @@ def test_app_rk4_robert_functions_vs_oracle(tmp_path, fused):
         e = rel_l2(out[k], ref[name])
         print("rk4 robert", fused, name, e)
         assert e < 1e-10
-    # REXI with Robert functions: the one-call step (G1) matches the oracle's
-    # SWERexiSPHRobert; the class-by-class loop (G0) is refused, as documented
+    # REXI with Robert functions: the one-call step (G1) and the class-by-class
+    # loop of SWERexiSPHRobert terms (G0) match the oracle's SWERexiSPHRobert
     M, dtr = 8, 3600.0
     _run(["T%d" % L, "P0", "Q4", "N0", "S1", "M%d" % M, "H0.2", "I1", "R%g" % dtr, "E%g" % (2 * dtr), "O1e12", "V100000",
-          "G1", "Z" + dump], str(tmp_path))
+          "G%d" % fused, "Z" + dump], str(tmp_path))
     out = _load(dump, orc.nlm)
     hs = orc.analysis(h)
     zs = np.zeros_like(hs)
     ref = orc.swe_rexi_run(dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, visc=1e5, rexi_h=0.2, rexi_M=M, half=1, dt=dtr,
                                 robert=1), hs, zs, zs, 2)
     for k in range(3):
-        assert rel_l2(out[k], ref[k]) < 1e-11, k
-    r = subprocess.run([APP, "T%d" % L, "P0", "N0", "S1", "M8", "R3600", "E3600", "O1e12", "G0"], cwd=str(tmp_path),
-                       capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0
+        e = rel_l2(out[k], ref[k])
+        print("rexi robert", fused, "huv"[k], e)
+        assert e < 1e-11, k
 
 
 @pytest.mark.parametrize("fused,omega", [(1, OMEGA), (0, OMEGA), (0, 0.0)])
