@@ -17,6 +17,7 @@
 #include "sph/SPHOperators.hpp"
 #include "sph/SPHOperatorsComplex.hpp"
 #include "sweet/TimesteppingRK.hpp"
+#include "benchmarks/BenchmarkGalewsky.hpp"
 
 class AppSWE
 {
@@ -110,6 +111,15 @@ public:
 
 		switch (simVars.benchmark_scenario_id) {
 		case 0: setup_initial_conditions_gaussian(); break;
+		case 1: {
+			// the Galewsky jet and bump (AppSWE.hpp:199-207)
+			BenchmarkGalewsky galewsky(simVars);
+			galewsky.setup_initial_h(prog_h);
+			galewsky.setup_initial_h_add_bump(prog_h);
+			galewsky.setup_initial_u(prog_u);
+			galewsky.setup_initial_v(prog_v);
+			break;
+		}
 		case 2: case 3:
 			simVars.h0 = 1;
 			simVars.gravitation = 1;

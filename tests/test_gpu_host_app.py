@@ -150,6 +150,33 @@ def test_app_rexi_extended_modes_vs_oracle(tmp_path, fused):
     assert rel_l2(x[2], plain[2]) > 1e-8
 
 
+def test_app_galewsky_initial_state(tmp_path):
+    # scenario Q1 (AppSWE.hpp:199-207): the balanced jet (pinned on the CPU in
+    # tests/test_host_galewsky.py) plus the bump, analysed on the GPU; against
+    # the same grid values computed here (the jet of sph_swe_amd.workloads with
+    # the bump in numpy) and the oracle's analysis
+    from sph_swe_amd import workloads
+    L = 63
+    nlat, nphi = capi.grid_auto(L)
+    orc = O.Oracle(L, nlat, nphi)
+    ctx = capi.Context(L)
+    jet = workloads.galewsky_state(ctx, seed=None)
+    hj = ctx.synthesis(jet[0:1])[0].reshape(nphi, nlat)
+    ctx.close()
+    lam = orc.lon()[:, None]
+    phi = np.arcsin(np.asarray(orc.ct))[None, :]
+    bump = 120.0 * np.cos(phi) * np.exp(-((lam - np.pi) * 3.0) ** 2) * np.exp(-((np.pi / 4 - phi) * 15.0) ** 2)
+    dump = str(tmp_path / "state.bin")
+    _run(["T%d" % L, "P0", "Q1", "N1", "R1", "E0", "O1e12", "G1", "Z" + dump], str(tmp_path))
+    out = _load(dump, orc.nlm)
+    ref_h = orc.analysis((hj + bump).ravel())
+    e = rel_l2(out[0], ref_h)
+    print("galewsky h", e, "u", rel_l2(out[1], jet[1]))
+    assert e < 1e-11
+    assert rel_l2(out[1], jet[1]) < 1e-12
+    assert np.max(np.abs(out[2])) == 0.0
+
+
 def test_app_p4_solver_known_answers(tmp_path):
     # AppTestSPHSolverComplex.hpp:29-517 at T32: for the 76 poles of
     # REXI::setup(0.2, 64), scalar, mu, (1-mu^2) d/dmu, z1..z8 and Helmholtz
