@@ -52,10 +52,20 @@ def test_app_rk4_vs_oracle(tmp_path, fused):
         e = rel_l2(out[k], ref[name])
         print("rk4", fused, name, e)
         assert e < 1e-10
-    # the reference's output files: initial and final state, four fields each
+    # the reference's output files: initial and final state, four fields each;
+    # the final height file holds the grid values of the final spectrum (8
+    # significant digits), rows south to north after the longitude header
     assert len([f for f in os.listdir(tmp_path) if f.endswith(".csv")]) == 8
-    with open(tmp_path / sorted(f for f in os.listdir(tmp_path) if f.startswith("prog_h"))[0]) as f:
+    names = sorted(f for f in os.listdir(tmp_path) if f.startswith("prog_h"))
+    with open(tmp_path / names[0]) as f:
         assert f.readline().startswith("#TI")
+    with open(tmp_path / names[-1]) as f:
+        lines = f.read().splitlines()
+    rows = np.array([r.split("\t") for r in lines[4:]], float)[::-1]
+    lat = rows[:, 0]
+    assert np.allclose(lat, np.degrees(np.arcsin(np.asarray(orc.ct))), rtol=1e-7, atol=1e-6)
+    grid = orc.synthesis(out[0]).reshape(orc.nphi, orc.nlat).T   # [ring][longitude]
+    assert np.max(np.abs(rows[:, 1:] - grid)) < 1e-7 * np.max(np.abs(grid))
 
 
 @pytest.mark.parametrize("fused", [1, 0])
