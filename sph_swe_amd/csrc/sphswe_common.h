@@ -125,9 +125,6 @@ int sphswe_fft_r2c(sphswe_ctx *ctx, const double *spat, double2 *four, int nf);
 int sphswe_fft_nseq_candidates(const sphswe_ctx *ctx, int *out, int max);
 int sphswe_leg_synth_f64_mfma(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf);
 int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf);
-int sphswe_fft_r2c_rm(sphswe_ctx *ctx, const double *spat, double2 *four, int nf);
-int sphswe_fft_c2r_rm(sphswe_ctx *ctx, const double2 *four, double *spat, int nf);
-bool sphswe_fft_has_rm(const sphswe_ctx *ctx);
 int sphswe_fft_c2c_fwd(sphswe_ctx *ctx, const double2 *spat, double2 *four, int nf);
 int sphswe_fft_c2c_bwd(sphswe_ctx *ctx, const double2 *four, double2 *spat, int nf);
 int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);

@@ -1,11 +1,13 @@
-// Longitude FFTs of the SHT (the FFTW part of SHTNS).  One workgroup holds
-// `nseq` complex sequences of length nphi in LDS and runs a mixed radix
-// (4, 2, 3, 5, 7) Stockham autosort FFT on them, ping-ponging between two LDS
-// buffers.  Real rings are packed two per complex sequence
-// (z = f_a + i f_b), so a real r2c/c2r costs half a complex FFT.
-// The physical layout is theta-contiguous ([i*nlat + j]); a workgroup reads
-// 2*nseq (real) or nseq (complex) adjacent rings, and consecutive workgroups
-// are gathered on one XCD so the 128-byte lines they share stay in its L2.
+// Longitude FFTs of the SHT (the FFTW part of SHTNS) on the theta-contiguous
+// physical layout ([i*nlat + j]).  Real rings are packed two per complex
+// sequence (z = f_a + i f_b), so a real r2c/c2r costs half a complex FFT.
+//   - k_fft_wide: the real transforms at compile-time sizes (the grids of the
+//     usual truncations), a tile of adjacent ring pairs per workgroup in one
+//     in-place LDS buffer, first and last radix pass against HBM;
+//   - k_fft: any size and the complex transforms, runtime mixed radix
+//     (4, 2, 3, 5, 7) Stockham passes ping-ponging between two LDS buffers.
+// Consecutive workgroups are gathered on one XCD so the 128-byte lines they
+// share stay in its L2.
 #include "sphswe_common.h"
 
 namespace {
@@ -501,163 +503,6 @@ struct CPlan {
 	}
 };
 
-// padded LDS index: one spare element every 16 complex values
-template <int N>
-struct CPad {
-	static constexpr int len = N + N / 16 + 1;
-	__device__ static __forceinline__ int at(int s, int i) { return s * len + i + (i >> 4); }
-};
-
-template <int N, int NSEQ, int R, int Ns, int SIGN>
-__device__ __forceinline__ void pass_ct(const double2 *__restrict__ src, double2 *__restrict__ dst,
-					const double2 *__restrict__ tw)
-{
-	constexpr int NB = N / R, TOT = NSEQ * NB, TWS = N / (Ns * R);
-	constexpr int IT = (TOT + 255) / 256;
-#pragma unroll
-	for (int it = 0; it < IT; it++) {
-		const int idx = (int)threadIdx.x + it * 256;
-		if (TOT % 256 != 0 && idx >= TOT) break;
-		const int sq = idx / NB, b = idx % NB;
-		const int k = b % Ns, jb = b / Ns;
-		double2 v[R];
-#pragma unroll
-		for (int t = 0; t < R; t++) v[t] = src[CPad<N>::at(sq, b + t * NB)];
-		if (Ns > 1) {
-#pragma unroll
-			for (int t = 1; t < R; t++) {
-				double2 w = tw[t * k * TWS];
-				if (SIGN > 0) w.y = -w.y;
-				v[t] = cmul(v[t], w);
-			}
-		}
-		Dft<R>::run(v, SIGN);
-		const int o = jb * Ns * R + k;
-#pragma unroll
-		for (int t = 0; t < R; t++) dst[CPad<N>::at(sq, o + t * Ns)] = v[t];
-	}
-}
-
-template <int N, int NSEQ, int SIGN, int P>
-struct PassesCT {
-	__device__ static __forceinline__ double2 *run(double2 *a, double2 *b, const double2 *__restrict__ tw)
-	{
-		if constexpr (P == CPlan<N>::npass()) {
-			return a;
-		} else {
-			pass_ct<N, NSEQ, CPlan<N>::radix(P), CPlan<N>::ns(P), SIGN>(a, b, tw);
-			__syncthreads();
-			return PassesCT<N, NSEQ, SIGN, P + 1>::run(b, a, tw);
-		}
-	}
-};
-
-// MODE_R2C: nf real fields -> Fourier rows (L + 1, nlat); MODE_C2R: back.
-// Workgroup = 2 NSEQ adjacent rings packed into NSEQ complex sequences.
-// RM: the physical fields are ring-major, [j * N + i] (the SWE stepper's
-// internal layout: every ring is one contiguous row), instead of the
-// reference's [i * nlat + j].
-template <int MODE, int N, int NSEQ, bool RM>
-__global__ __launch_bounds__(256) void k_fft_real(const void *__restrict__ in, void *__restrict__ out,
-						  const double2 *__restrict__ tw, int nlat, int L, int nblk, int nblk_pad,
-						  double norm)
-{
-	constexpr int RB = 2 * NSEQ;
-	extern __shared__ double2 sm[];
-	const int lb = xcd_remap(blockIdx.x, nblk_pad);
-	if (lb >= nblk) return;
-	const int f = blockIdx.y;
-	const size_t nspat = (size_t)nlat * N;
-	double2 *bufA = sm;
-	double2 *bufB = sm + (size_t)NSEQ * CPad<N>::len;
-	const int tid = threadIdx.x;
-	const int j0 = lb * RB;
-	if (MODE == MODE_R2C) {
-		const double *src = (const double *)in + f * nspat;
-		double *lds = (double *)bufA;
-		constexpr int TOT = N * RB;
-		for (int b = 0; b < TOT; b += KB * 256) {
-			double v[KB];
-#pragma unroll
-			for (int u = 0; u < KB; u++) {
-				const int idx = min(b + u * 256 + tid, TOT - 1);
-				const int i = RM ? idx % N : idx / RB, r = RM ? idx / N : idx % RB;
-				v[u] = RM ? src[(size_t)min(j0 + r, nlat - 1) * N + i] : src[(size_t)i * nlat + min(j0 + r, nlat - 1)];
-			}
-#pragma unroll
-			for (int u = 0; u < KB; u++) {
-				const int idx = b + u * 256 + tid;
-				if (TOT % (KB * 256) != 0 && idx >= TOT) break;
-				const int i = RM ? idx % N : idx / RB, r = RM ? idx / N : idx % RB;
-				lds[2 * CPad<N>::at(r >> 1, i) + (r & 1)] = (j0 + r < nlat) ? v[u] : 0.0;
-			}
-		}
-		__syncthreads();
-		const double2 *res = PassesCT<N, NSEQ, -1, 0>::run(bufA, bufB, tw);
-		double2 *dst = (double2 *)out + (size_t)f * (L + 1) * nlat;
-		for (int idx = tid; idx < (L + 1) * RB; idx += 256) {
-			const int m = idx / RB, r = idx % RB;
-			const int j = j0 + r;
-			if (j >= nlat) continue;
-			const int sq = r >> 1;
-			const double2 Z = res[CPad<N>::at(sq, m)];
-			const double2 Zc = cconj(res[CPad<N>::at(sq, m == 0 ? 0 : N - m)]);
-			double2 v;
-			if ((r & 1) == 0) v = make_double2(0.5 * (Z.x + Zc.x), 0.5 * (Z.y + Zc.y));
-			else v = make_double2(0.5 * (Z.y - Zc.y), -0.5 * (Z.x - Zc.x));
-			if (m == 0) v.y = 0.0;
-			dst[(size_t)m * nlat + j] = make_double2(v.x * norm, v.y * norm);
-		}
-	} else {
-		const double2 *src = (const double2 *)in + (size_t)f * (L + 1) * nlat;
-		// stage G[s][m][part] in bufB (2 (L + 1) <= N + 1 entries per sequence)
-		const int ng = (L + 1) * RB;
-		for (int b = 0; b < ng; b += KB * 256) {
-			double2 v[KB];
-#pragma unroll
-			for (int u = 0; u < KB; u++) {
-				const int idx = min(b + u * 256 + tid, ng - 1);
-				const int m = idx / RB, r = idx % RB;
-				v[u] = src[(size_t)m * nlat + min(j0 + r, nlat - 1)];
-			}
-#pragma unroll
-			for (int u = 0; u < KB; u++) {
-				const int idx = b + u * 256 + tid;
-				if (idx >= ng) break;
-				const int m = idx / RB, r = idx % RB;
-				bufB[((r >> 1) * (L + 1) + m) * 2 + (r & 1)] = (j0 + r < nlat) ? v[u] : make_double2(0, 0);
-			}
-		}
-		__syncthreads();
-		for (int idx = tid; idx < NSEQ * N; idx += 256) {
-			const int sq = idx / N, k = idx % N;
-			double2 z = make_double2(0, 0);
-			if (k == 0) {
-				const double2 A = bufB[(sq * (L + 1)) * 2], B = bufB[(sq * (L + 1)) * 2 + 1];
-				z = make_double2(A.x, B.x);
-			} else if (k <= L) {
-				const double2 A = bufB[(sq * (L + 1) + k) * 2], B = bufB[(sq * (L + 1) + k) * 2 + 1];
-				z = make_double2(A.x - B.y, A.y + B.x);
-			} else if (k >= N - L) {
-				const int m = N - k;
-				const double2 A = bufB[(sq * (L + 1) + m) * 2], B = bufB[(sq * (L + 1) + m) * 2 + 1];
-				z = make_double2(A.x + B.y, -A.y + B.x);
-			}
-			bufA[CPad<N>::at(sq, k)] = z;
-		}
-		__syncthreads();
-		const double2 *res = PassesCT<N, NSEQ, +1, 0>::run(bufA, bufB, tw);
-		double *dst = (double *)out + f * nspat;
-		const double *rl = (const double *)res;
-		constexpr int TOT = N * RB;
-		for (int idx = tid; idx < TOT; idx += 256) {
-			const int i = RM ? idx % N : idx / RB, r = RM ? idx / N : idx % RB;
-			const int j = j0 + r;
-			if (j < nlat) dst[RM ? (size_t)j * N + i : (size_t)i * nlat + j] = rl[2 * CPad<N>::at(r >> 1, i) + (r & 1)];
-		}
-	}
-}
-
 // ---- reference layout, wide ring tiles ----------------------------------------
 // A workgroup owns 2 NSEQ adjacent rings; ring pair s is the complex sequence
 // z_s[i] = f(i, j0 + 2s) + i f(i, j0 + 2s + 1), which is one 16-byte load or
@@ -887,74 +732,35 @@ int launch_real_wide(sphswe_ctx *ctx, const void *in, void *out, int nf)
 	return SPHSWE_OK;
 }
 
-template <int MODE, int N, int NSEQ, bool RM>
-int launch_real_ct(sphswe_ctx *ctx, const void *in, void *out, int nf)
-{
-	constexpr int RB = 2 * NSEQ;
-	const int nblk = (ctx->nlat + RB - 1) / RB;
-	const int nblk_pad = (nblk + 7) / 8 * 8;
-	const size_t lds = 2 * (size_t)NSEQ * CPad<N>::len * sizeof(double2);
-	if (lds > 65536)
-		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_real<MODE, N, NSEQ, RM>,
-						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-	dim3 grid(nblk_pad, nf);
-	k_fft_real<MODE, N, NSEQ, RM><<<grid, 256, lds, ctx->stream>>>(in, out, ctx->d_tw, ctx->nlat, ctx->L, nblk, nblk_pad,
-								    2.0 * M_PI / N);
-	SPHSWE_LAUNCH_CHECK(ctx);
-	return SPHSWE_OK;
-}
-
 // the compile-time kernel for the grid sizes of the usual truncations, or -1.
 // nseq: complex sequences per workgroup (2 nseq rings); 0 takes the default,
 // a value without an instantiation for this size falls back to it
-template <int MODE, bool RM = false>
+template <int MODE>
 int launch_real_fixed(sphswe_ctx *ctx, const void *in, void *out, int nf, int nseq)
 {
-	if (!RM && ctx->nlat % 2 == 0 && 2 * ctx->L + 1 <= ctx->plan.n) {
-		switch (ctx->plan.n) {
-		case 96:
-			if (nseq == 16) return launch_real_wide<MODE, 96, 16>(ctx, in, out, nf);
-			return launch_real_wide<MODE, 96, 8>(ctx, in, out, nf);
-		case 196:
-			if (nseq == 4) return launch_real_wide<MODE, 196, 4>(ctx, in, out, nf);
-			return launch_real_wide<MODE, 196, 8>(ctx, in, out, nf);
-		case 384:
-			if (nseq == 2) return launch_real_wide<MODE, 384, 2>(ctx, in, out, nf);
-			if (nseq == 8) return launch_real_wide<MODE, 384, 8>(ctx, in, out, nf);
-			return launch_real_wide<MODE, 384, 4>(ctx, in, out, nf);
-		case 784:
-			if (nseq == 1) return launch_real_wide<MODE, 784, 1>(ctx, in, out, nf);
-			if (nseq == 4) return launch_real_wide<MODE, 784, 4>(ctx, in, out, nf);
-			return launch_real_wide<MODE, 784, 2>(ctx, in, out, nf);
-		case 1536:
-			if (nseq == 1) return launch_real_wide<MODE, 1536, 1>(ctx, in, out, nf);
-			if (nseq == 4) return launch_real_wide<MODE, 1536, 4>(ctx, in, out, nf);
-			return launch_real_wide<MODE, 1536, 2>(ctx, in, out, nf);
-		case 3072:
-			if (nseq == 1) return launch_real_wide<MODE, 3072, 1>(ctx, in, out, nf);
-			return launch_real_wide<MODE, 3072, 2>(ctx, in, out, nf);
-		}
-		return -1;
-	}
+	if (ctx->nlat % 2 != 0 || 2 * ctx->L + 1 > ctx->plan.n) return -1;
 	switch (ctx->plan.n) {
 	case 96:
-		if (nseq == 16) return launch_real_ct<MODE, 96, 16, RM>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 96, 8, RM>(ctx, in, out, nf);
+		if (nseq == 16) return launch_real_wide<MODE, 96, 16>(ctx, in, out, nf);
+		return launch_real_wide<MODE, 96, 8>(ctx, in, out, nf);
 	case 196:
-		if (nseq == 4) return launch_real_ct<MODE, 196, 4, RM>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 196, 8, RM>(ctx, in, out, nf);
+		if (nseq == 4) return launch_real_wide<MODE, 196, 4>(ctx, in, out, nf);
+		return launch_real_wide<MODE, 196, 8>(ctx, in, out, nf);
 	case 384:
-		if (nseq == 2) return launch_real_ct<MODE, 384, 2, RM>(ctx, in, out, nf);
-		if (nseq == 8) return launch_real_ct<MODE, 384, 8, RM>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 384, 4, RM>(ctx, in, out, nf);
+		if (nseq == 2) return launch_real_wide<MODE, 384, 2>(ctx, in, out, nf);
+		if (nseq == 8) return launch_real_wide<MODE, 384, 8>(ctx, in, out, nf);
+		return launch_real_wide<MODE, 384, 4>(ctx, in, out, nf);
 	case 784:
-		if (nseq == 1) return launch_real_ct<MODE, 784, 1, RM>(ctx, in, out, nf);
-		if (nseq == 4) return launch_real_ct<MODE, 784, 4, RM>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 784, 2, RM>(ctx, in, out, nf);
+		if (nseq == 1) return launch_real_wide<MODE, 784, 1>(ctx, in, out, nf);
+		if (nseq == 4) return launch_real_wide<MODE, 784, 4>(ctx, in, out, nf);
+		return launch_real_wide<MODE, 784, 2>(ctx, in, out, nf);
 	case 1536:
-		if (nseq == 2) return launch_real_ct<MODE, 1536, 2, RM>(ctx, in, out, nf);
-		return launch_real_ct<MODE, 1536, 1, RM>(ctx, in, out, nf);
-	case 3072: return launch_real_ct<MODE, 3072, 1, RM>(ctx, in, out, nf);
+		if (nseq == 1) return launch_real_wide<MODE, 1536, 1>(ctx, in, out, nf);
+		if (nseq == 4) return launch_real_wide<MODE, 1536, 4>(ctx, in, out, nf);
+		return launch_real_wide<MODE, 1536, 2>(ctx, in, out, nf);
+	case 3072:
+		if (nseq == 1) return launch_real_wide<MODE, 3072, 1>(ctx, in, out, nf);
+		return launch_real_wide<MODE, 3072, 2>(ctx, in, out, nf);
 	}
 	return -1;
 }
@@ -1018,27 +824,6 @@ int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf)
 	const int rc = launch_real_fixed<MODE_C2R>(ctx, four, spat, nf, ctx->fft_nseq_c2r);
 	return rc >= 0 ? rc : launch<MODE_C2R>(ctx, four, spat, nf);
 }
-// ring-major physical fields ([j * nphi + i]), for buffers internal to the
-// library; -1 when the grid size has no compile-time kernel (the caller then
-// keeps the reference's layout)
-int sphswe_fft_r2c_rm(sphswe_ctx *ctx, const double *spat, double2 *four, int nf)
-{
-	if (nf <= 0) return SPHSWE_OK;
-	return launch_real_fixed<MODE_R2C, true>(ctx, spat, four, nf, ctx->fft_nseq_r2c);
-}
-int sphswe_fft_c2r_rm(sphswe_ctx *ctx, const double2 *four, double *spat, int nf)
-{
-	if (nf <= 0) return SPHSWE_OK;
-	return launch_real_fixed<MODE_C2R, true>(ctx, four, spat, nf, ctx->fft_nseq_c2r);
-}
-bool sphswe_fft_has_rm(const sphswe_ctx *ctx)
-{
-	switch (ctx->plan.n) {
-	case 96: case 196: case 384: case 784: case 1536: case 3072: return true;
-	}
-	return false;
-}
-
 // the sequences-per-workgroup candidates of the compile-time real FFT of this
 // grid (empty for other sizes), for the autotuner
 int sphswe_fft_nseq_candidates(const sphswe_ctx *ctx, int *out, int max)

@@ -67,14 +67,14 @@ __global__ void k_prep9(const double2 *__restrict__ X, double2 *__restrict__ S9,
 
 // pointwise level 0 -> 4 fields per member
 __global__ void k_phys1(const double *__restrict__ P9, double *__restrict__ P4, const double *__restrict__ ct,
-			const double *__restrict__ st, const double *__restrict__ sq, int nlat, int nphi, int rm, long nspat,
+			const double *__restrict__ st, const double *__restrict__ sq, int nlat, long nspat,
 			int nmem, double ga, double two_omega, double ia, int nonlinear, double h0)
 {
 	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
 	if (i >= nspat * nmem) return;
 	int mem = (int)(i / nspat);
 	long k = i - (long)mem * nspat;
-	int j = rm ? (int)(k / nphi) : (int)(k % nlat);   // ring of point k
+	int j = (int)(k % nlat);   // ring of point k ([i * nlat + j])
 	const double *p = P9 + (long)mem * 9 * nspat + k;
 	double H = p[0], U = p[nspat], V = p[2 * nspat];
 	double Hl = p[3 * nspat], Hm = p[4 * nspat];
@@ -135,13 +135,13 @@ __global__ void k_prep2(const double2 *__restrict__ S4, double2 *__restrict__ S2
 
 // pointwise level 2: h tendency in physical space
 __global__ void k_phys2(const double *__restrict__ P2, double *__restrict__ P1, const double *__restrict__ st,
-			const double *__restrict__ sq, int nlat, int nphi, int rm, long nspat, int nmem, double scale)
+			const double *__restrict__ sq, int nlat, long nspat, int nmem, double scale)
 {
 	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
 	if (i >= nspat * nmem) return;
 	int mem = (int)(i / nspat);
 	long k = i - (long)mem * nspat;
-	int j = rm ? (int)(k / nphi) : (int)(k % nlat);
+	int j = (int)(k % nlat);
 	const double *p = P2 + (long)mem * 2 * nspat + k;
 	double c = sq[j];
 	double dlon = p[0] / c;
@@ -283,23 +283,18 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 	const int L = ctx->L, nlat = ctx->nlat;
 	hipStream_t st = ctx->stream;
 	int rc;
-	// the physical buffers are internal: ring-major where the grid has a
-	// compile-time FFT (every ring one contiguous row), else the reference's layout
-	const int rm = sphswe_fft_has_rm(ctx) ? 1 : 0;
-	auto c2r = [&](const double2 *f, double *ph, int n) {
-		return rm ? sphswe_fft_c2r_rm(ctx, f, ph, n) : sphswe_fft_c2r(ctx, f, ph, n);
-	};
-	auto r2c = [&](const double *ph, double2 *f, int n) {
-		return rm ? sphswe_fft_r2c_rm(ctx, ph, f, n) : sphswe_fft_r2c(ctx, ph, f, n);
-	};
+	// the physical buffers keep the reference's layout, whose wide-tile FFTs
+	// run at about twice the rate of a ring-major two-buffer kernel
+	auto c2r = [&](const double2 *f, double *ph, int n) { return sphswe_fft_c2r(ctx, f, ph, n); };
+	auto r2c = [&](const double *ph, double2 *f, int n) { return sphswe_fft_r2c(ctx, ph, f, n); };
 	const double2 *Y = X;
 	double2 *YN = b.Ya;
 	for (int s = 0; s < 4; s++) {
 		k_prep9<<<g1(nlm * nm), 256, 0, st>>>(Y, b.S9, ctx->d_mofidx, L, (int)nlm, nm);
 		if ((rc = sphswe_leg_synth(ctx, b.S9, b.F, 9 * nm, prec))) return rc;
 		if ((rc = c2r(b.F, b.P9, 9 * nm))) return rc;
-		k_phys1<<<g1(nspat * nm), 256, 0, st>>>(b.P9, b.P4, ctx->d_ct, ctx->d_st, ctx->d_sq, nlat, ctx->nphi, rm,
-							 nspat, nm, ga, two_omega, ia, mode, mode == 3 ? p->h0 / a : p->h0);
+		k_phys1<<<g1(nspat * nm), 256, 0, st>>>(b.P9, b.P4, ctx->d_ct, ctx->d_st, ctx->d_sq, nlat, nspat, nm, ga,
+							 two_omega, ia, mode, mode == 3 ? p->h0 / a : p->h0);
 		if ((rc = r2c(b.P4, b.F, 4 * nm))) return rc;
 		if ((rc = sphswe_leg_anal(ctx, b.F, b.S4, 4 * nm, prec))) return rc;
 		// the h tendency: field 0 of each member's four (Robert functions: complete
@@ -310,8 +305,8 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 			k_prep2<<<g1(nlm * nm), 256, 0, st>>>(b.S4, b.S2, ctx->d_mofidx, L, (int)nlm, nm);
 			if ((rc = sphswe_leg_synth(ctx, b.S2, b.F, 2 * nm, prec))) return rc;
 			if ((rc = c2r(b.F, b.P2, 2 * nm))) return rc;
-			k_phys2<<<g1(nspat * nm), 256, 0, st>>>(b.P2, b.P1, ctx->d_st, ctx->d_sq, nlat, ctx->nphi, rm, nspat,
-								 nm, p->nonlinear ? ia : p->h0 / a);
+			k_phys2<<<g1(nspat * nm), 256, 0, st>>>(b.P2, b.P1, ctx->d_st, ctx->d_sq, nlat, nspat, nm,
+								 p->nonlinear ? ia : p->h0 / a);
 			if ((rc = r2c(b.P1, b.F, nm))) return rc;
 			if ((rc = sphswe_leg_anal(ctx, b.F, b.TH, nm, prec))) return rc;
 			TH = b.TH;
