@@ -40,19 +40,27 @@ enum {
 };
 
 /* precision of the Legendre contraction; the associated-Legendre recurrence is
- * always FP64 and the Fourier stage always FP64.  The reduced modes run the
- * contraction on the matrix cores with FP32 accumulation:
+ * always FP64.  The reduced modes run the contraction on the matrix cores with
+ * FP32 accumulation:
  *   FP32    operands rounded to FP32;
  *   BF16    operands split into bf16 hi + lo, products hi*hi + hi*lo + lo*hi;
  *   BF16X3  operands split into bf16 hi + mid + lo (exact for FP32 values),
- *           the six products of total order <= 2.
- * The real and complex transforms take every mode; the pole-layout transforms
- * inside the REXI step are FP64. */
+ *           the six products of total order <= 2;
+ *   MIXED   as BF16, with the associated-Legendre values generated once per
+ *           context by the FP64 recurrence and stored as bf16 hi/lo tables
+ *           (two tables, 441 MB at T511, allocated on first use), contracted on
+ *           the double-K bf16 matrix cores; real transforms only.  In the SWE
+ *           stepper this mode also keeps the Fourier rows, FFTs and physical
+ *           fields in FP32 (the state and the RK4 combinations stay FP64).
+ * The Fourier stage of the transform entry points is FP64 in every mode.  The
+ * real and complex transforms take the modes FP64 .. BF16X3; the pole-layout
+ * transforms inside the REXI step are FP64. */
 enum {
 	SPHSWE_PREC_FP64 = 0,
 	SPHSWE_PREC_FP32 = 1,
 	SPHSWE_PREC_BF16 = 2,
-	SPHSWE_PREC_BF16X3 = 3
+	SPHSWE_PREC_BF16X3 = 3,
+	SPHSWE_PREC_MIXED = 4
 };
 
 /* ---- configuration (replaces shtns_create / shtns_set_grid{,_auto} and

@@ -14,8 +14,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libsphswe.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sphswe.h")
 
 OK, ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_SINGULAR, ERR_NAN = range(6)
-PREC_FP64, PREC_FP32, PREC_BF16, PREC_BF16X3 = 0, 1, 2, 3
-PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16, "bf16x3": PREC_BF16X3}
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_BF16X3, PREC_MIXED = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16": PREC_BF16, "bf16x3": PREC_BF16X3, "mixed": PREC_MIXED}
 # spectral operators (sphswe_spec_op / _cplx)
 OP_IDENTITY, OP_DIFF_LON, OP_ONE_MINUS_MU2_DMU, OP_MU, OP_MU2, OP_LAPLACE, OP_HELMHOLTZ = range(7)
 # physical operators (sphswe_spat_op / _cplx)

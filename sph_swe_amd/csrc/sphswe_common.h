@@ -60,10 +60,24 @@ struct sphswe_ctx {
 	long *d_tab_off = nullptr;    // offset of (m, parity), index 2m + parity
 	int2 *d_tab_tiles_s = nullptr, *d_tab_tiles_a = nullptr;
 	int tab_ntiles_s = 0, tab_ntiles_a = 0;
+	// bf16 hi/lo associated-Legendre tables in MFMA operand order (built on
+	// first use, sphswe_legendre_bt.hip)
+	uint4 *d_bt_s = nullptr;      // synthesis fragments (rows rings, contraction degrees)
+	uint4 *d_bt_a = nullptr;      // analysis fragments (rows degrees, contraction rings)
+	int *d_bt_soff = nullptr;     // first fragment of (m, parity), index 2m + parity
+	int *d_bt_aoff = nullptr;
+	int *d_bt_kboff = nullptr;    // first 32-degree block of (m, parity) among all blocks
+	int2 *d_bt_kbl = nullptr;     // (2m + parity, block) of every block
+	int bt_nkb = 0;               // number of blocks
+	int2 *d_bt_mkb = nullptr;     // (m, 64-degree block) units of the operand split
+	int bt_nmkb = 0;
+	int bt_tmax = 0;              // most degree tiles of an order pair (m, L - m)
+	size_t bt_bytes = 0;
 	int tuned = 0;
 	int *d_mofidx = nullptr;      // m of real spectral index [nlm]
 	int *d_lofcidx = nullptr;     // l of complex spectral index [(L+1)^2]
 	double2 *d_tw = nullptr;      // e^{-2 pi i k / nphi} [nphi]
+	float2 *d_tw32 = nullptr;     // the same rounded to FP32 (the FP32 transforms of the mixed form)
 	FftPlan plan;
 
 	// scratch (grown on demand, never shrunk)
@@ -125,6 +139,10 @@ int sphswe_fft_r2c(sphswe_ctx *ctx, const double *spat, double2 *four, int nf);
 int sphswe_fft_nseq_candidates(const sphswe_ctx *ctx, int *out, int max);
 int sphswe_leg_synth_f64_mfma(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf);
 int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf);
+// FP32 real transforms between ring-major FP32 fields and FP32 Fourier rows
+int sphswe_fft_r2c32(sphswe_ctx *ctx, const float *spat, float2 *four, int nf);
+int sphswe_fft_c2r32(sphswe_ctx *ctx, const float2 *four, float *spat, int nf);
+bool sphswe_fft32_available(const sphswe_ctx *ctx);
 int sphswe_fft_c2c_fwd(sphswe_ctx *ctx, const double2 *spat, double2 *four, int nf);
 int sphswe_fft_c2c_bwd(sphswe_ctx *ctx, const double2 *four, double2 *spat, int nf);
 int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec);
@@ -145,6 +163,25 @@ int sphswe_leg_synth_cplx_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four
 int sphswe_leg_tables(sphswe_ctx *ctx);
 void sphswe_leg_tables_free(sphswe_ctx *ctx);
 int sphswe_leg_synth_tab(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf);
+int sphswe_leg_bt_tables(sphswe_ctx *ctx);
+void sphswe_leg_bt_tables_free(sphswe_ctx *ctx);
+int sphswe_leg_synth_bt(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf);
+int sphswe_leg_anal_bt(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf);
+// the pieces of the table form: pre-split column operands (synthesis SB from
+// FP64 spectra, analysis G from Fourier rows) and the contractions on them
+size_t sphswe_bt_sb_bytes(const sphswe_ctx *ctx, int nf);
+size_t sphswe_bt_g_bytes(const sphswe_ctx *ctx, int nf);
+int sphswe_bt_sprep(sphswe_ctx *ctx, const double2 *spec, void *SB, int nf);
+// the SWE levels' operands straight from the spectra: 9 fields per member from
+// (h, u, v) (the fields, i m and D' of each), 2 per member from the level-1
+// analyses (i m of the first, D' of the second of each member's four)
+int sphswe_bt_sprep_swe9(sphswe_ctx *ctx, const double2 *X, void *SB, int nmem);
+int sphswe_bt_sprep_swe2(sphswe_ctx *ctx, const double2 *S4, void *SB, int nmem);
+int sphswe_bt_gprep(sphswe_ctx *ctx, const double2 *four, void *G, int nf);
+int sphswe_bt_gprep32(sphswe_ctx *ctx, const float2 *four, void *G, int nf);
+int sphswe_bt_synth(sphswe_ctx *ctx, const void *SB, double2 *four, int nf);
+int sphswe_bt_synth32(sphswe_ctx *ctx, const void *SB, float2 *four, int nf);
+int sphswe_bt_anal(sphswe_ctx *ctx, const void *G, double2 *spec, int nf);
 int sphswe_leg_anal_tab(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf);
 int sphswe_leg_synth_pole_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,
 			     const double *oscale, int prec);
@@ -164,6 +201,13 @@ __host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_doub
 __host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b)
+{
+	return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 
 // real spectral index (SPHConfig.hpp:106-116)
 __host__ __device__ inline int lm_index(int L, int l, int m) { return (m * (2 * L - m + 1)) / 2 + l; }

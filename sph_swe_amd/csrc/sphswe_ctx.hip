@@ -278,6 +278,11 @@ extern "C" int sphswe_create(int lmax, int mmax, int nlat, int nphi, int device,
 	if ((rc = upload(ctx, &ctx->d_mofidx, mof))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_lofcidx, lofc))) return fail(rc);
 	if ((rc = upload(ctx, &ctx->d_tw, tw))) return fail(rc);
+	{
+		std::vector<float2> tw32(tw.size());
+		for (size_t k = 0; k < tw.size(); k++) tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
+		if ((rc = upload(ctx, &ctx->d_tw32, tw32))) return fail(rc);
+	}
 	if (hipMalloc((void **)&ctx->d_flag, 64 * sizeof(double)) != hipSuccess)
 		return fail(sphswe_set_err(ctx, SPHSWE_ERR_NOMEM, "hipMalloc flag"));
 	if ((rc = sphswe_noalloc_init(ctx))) return fail(rc);
@@ -294,6 +299,7 @@ extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 	if (ctx->stream) hipStreamSynchronize(ctx->stream);
 	sphswe_swe_release(ctx);
 	sphswe_leg_tables_free(ctx);
+	sphswe_leg_bt_tables_free(ctx);
 	// objects still alive on this context keep their own memory and are freed
 	// by their destroy calls; they no longer refer to the context
 	for (sphswe_rexi *r : ctx->rexis) sphswe_rexi_detach(r);
@@ -301,7 +307,7 @@ extern "C" int sphswe_destroy(sphswe_ctx *ctx)
 	ctx->rexis.clear();
 	ctx->bands.clear();
 	void *ptrs[] = {ctx->d_ct, ctx->d_st, ctx->d_sq, ctx->d_coslat, ctx->d_wg, ctx->d_coef,
-			ctx->d_seed, ctx->d_segoff, ctx->d_seed16, ctx->d_segoff16, ctx->d_unit_m, ctx->d_ablk, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_scratch, ctx->d_escratch, ctx->d_flag};
+			ctx->d_seed, ctx->d_segoff, ctx->d_seed16, ctx->d_segoff16, ctx->d_unit_m, ctx->d_ablk, ctx->d_mofidx, ctx->d_lofcidx, ctx->d_tw, ctx->d_tw32, ctx->d_scratch, ctx->d_escratch, ctx->d_flag};
 	for (void *p : ptrs)
 		if (p) hipFree(p);
 	for (auto &kv : ctx->allocs) hipFree(kv.first);

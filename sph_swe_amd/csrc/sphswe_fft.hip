@@ -50,9 +50,31 @@ struct FftArgs {
 __device__ __forceinline__ int lidx(int s, int i, int NP) { return s * NP + i + (i >> 3); }
 __host__ __device__ inline int padded_len(int N) { return N + (N >> 3) + 1; }
 
-__device__ inline double2 mul_i(double2 a, int sign)  // sign * i * a
+// the butterflies below are written once for double2 and float2; RealOf<C>
+// is the scalar type, constants are rounded to it once
+template <typename C>
+struct RealOf;
+template <>
+struct RealOf<double2> {
+	typedef double T;
+};
+template <>
+struct RealOf<float2> {
+	typedef float T;
+};
+template <typename C>
+__device__ __forceinline__ C cmk(typename RealOf<C>::T re, typename RealOf<C>::T im)
 {
-	return sign > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+	C z;
+	z.x = re;
+	z.y = im;
+	return z;
+}
+
+template <typename C>
+__device__ inline C mul_i(C a, int sign)  // sign * i * a
+{
+	return sign > 0 ? cmk<C>(-a.y, a.x) : cmk<C>(a.y, -a.x);
 }
 
 template <int R>
@@ -60,16 +82,23 @@ struct Dft;
 
 template <>
 struct Dft<2> {
-	__device__ static void run(double2 *v, int) { double2 a = v[0], b = v[1]; v[0] = cadd(a, b); v[1] = csub(a, b); }
+	template <typename C>
+	__device__ static void run(C *v, int)
+	{
+		C a = v[0], b = v[1];
+		v[0] = cadd(a, b);
+		v[1] = csub(a, b);
+	}
 };
 
 template <>
 struct Dft<4> {
-	__device__ static void run(double2 *v, int sign)
+	template <typename C>
+	__device__ static void run(C *v, int sign)
 	{
-		double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-		double2 a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
-		double2 ia3 = mul_i(a3, sign);   // sign*i*a3
+		C a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+		C a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
+		C ia3 = mul_i(a3, sign);   // sign*i*a3
 		v[0] = cadd(a0, a2);
 		v[2] = csub(a0, a2);
 		v[1] = cadd(a1, ia3);
@@ -98,16 +127,18 @@ struct OddTab<7> {
 
 template <int R>
 struct DftOdd {
-	__device__ static void run(double2 *v, int sign)
+	template <typename C>
+	__device__ static void run(C *v, int sign)
 	{
-		double2 out[R];
+		typedef typename RealOf<C>::T Rl;
+		C out[R];
 #pragma unroll
 		for (int q = 0; q < R; q++) {
-			double2 acc = v[0];
+			C acc = v[0];
 #pragma unroll
 			for (int t = 1; t < R; t++) {
 				const int e = (q * t) % R;
-				const double wr = OddTab<R>::c[e], wi = sign * OddTab<R>::s[e];
+				const Rl wr = (Rl)OddTab<R>::c[e], wi = (Rl)(sign * OddTab<R>::s[e]);
 				acc.x += v[t].x * wr - v[t].y * wi;
 				acc.y += v[t].x * wi + v[t].y * wr;
 			}
@@ -120,12 +151,14 @@ struct DftOdd {
 // radix 3: X0 = x0 + (x1 + x2), X1,2 = x0 - (x1 + x2) / 2 +- sign i sqrt(3)/2 (x1 - x2)
 template <>
 struct Dft<3> {
-	__device__ static void run(double2 *v, int sign)
+	template <typename C>
+	__device__ static void run(C *v, int sign)
 	{
-		constexpr double s3 = 0.86602540378443864676;
-		const double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-		const double2 m1 = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
-		const double2 m2 = make_double2(-sign * s3 * t2.y, sign * s3 * t2.x);
+		typedef typename RealOf<C>::T Rl;
+		constexpr Rl s3 = (Rl)0.86602540378443864676, hf = (Rl)0.5;
+		const C t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+		const C m1 = cmk<C>(v[0].x - hf * t1.x, v[0].y - hf * t1.y);
+		const C m2 = cmk<C>(-sign * s3 * t2.y, sign * s3 * t2.x);
 		v[0] = cadd(v[0], t1);
 		v[1] = cadd(m1, m2);
 		v[2] = csub(m1, m2);
@@ -370,18 +403,20 @@ __global__ __launch_bounds__(256) void k_fft(const void *__restrict__ in, void *
 // DFT of 8: radix 2 then two DFTs of 4, w8^n = exp(sign i 2 pi n / 8)
 template <>
 struct Dft<8> {
-	__device__ static void run(double2 *v, int sign)
+	template <typename C>
+	__device__ static void run(C *v, int sign)
 	{
-		constexpr double h = 0.70710678118654752440;
-		double2 a[4], b[4];
+		typedef typename RealOf<C>::T Rl;
+		constexpr Rl h = (Rl)0.70710678118654752440;
+		C a[4], b[4];
 #pragma unroll
 		for (int n = 0; n < 4; n++) {
 			a[n] = cadd(v[n], v[n + 4]);
 			b[n] = csub(v[n], v[n + 4]);
 		}
-		b[1] = cmul(b[1], make_double2(h, sign * h));
+		b[1] = cmul(b[1], cmk<C>(h, sign * h));
 		b[2] = mul_i(b[2], sign);
-		b[3] = cmul(b[3], make_double2(-h, sign * h));
+		b[3] = cmul(b[3], cmk<C>(-h, sign * h));
 		Dft<4>::run(a, sign);
 		Dft<4>::run(b, sign);
 #pragma unroll
@@ -396,20 +431,22 @@ struct Dft<8> {
 // w16^(n2 k1), DFTs of 4 over n2; X[k1 + 4 k2]
 template <>
 struct Dft<16> {
-	__device__ static void run(double2 *v, int sign)
+	template <typename C>
+	__device__ static void run(C *v, int sign)
 	{
-		constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+		typedef typename RealOf<C>::T Rl;
+		constexpr Rl c1 = (Rl)0.92387953251128675613, s1 = (Rl)0.38268343236508977173, h = (Rl)0.70710678118654752440;
 		// w16^e for e = n2 k1 in {0, 1, 2, 3, 4, 6, 9}: (cos, sin) of 2 pi e / 16
-		double2 y[4][4];
+		C y[4][4];
 #pragma unroll
 		for (int n2 = 0; n2 < 4; n2++) {
-			double2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+			C t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
 			Dft<4>::run(t, sign);
 #pragma unroll
 			for (int k1 = 0; k1 < 4; k1++) y[n2][k1] = t[k1];
 		}
-		const double2 w1 = make_double2(c1, sign * s1), w2 = make_double2(h, sign * h), w3 = make_double2(s1, sign * c1);
-		const double2 w6 = make_double2(-h, sign * h), w9 = make_double2(-c1, -sign * s1);
+		const C w1 = cmk<C>(c1, sign * s1), w2 = cmk<C>(h, sign * h), w3 = cmk<C>(s1, sign * c1);
+		const C w6 = cmk<C>(-h, sign * h), w9 = cmk<C>(-c1, -sign * s1);
 		y[1][1] = cmul(y[1][1], w1);
 		y[1][2] = cmul(y[1][2], w2);
 		y[1][3] = cmul(y[1][3], w3);
@@ -421,7 +458,7 @@ struct Dft<16> {
 		y[3][3] = cmul(y[3][3], w9);
 #pragma unroll
 		for (int k1 = 0; k1 < 4; k1++) {
-			double2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+			C t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
 			Dft<4>::run(t, sign);
 #pragma unroll
 			for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = t[k2];
@@ -434,27 +471,29 @@ struct Dft<16> {
 // (A = 4) with B = 3; the twiddles are multiples of 30 degrees.
 template <int A>
 struct DftX3 {
-	__device__ static void run(double2 *v, int sign)
+	template <typename C>
+	__device__ static void run(C *v, int sign)
 	{
+		typedef typename RealOf<C>::T Rl;
 		constexpr int R = 3 * A;
 		constexpr double c30 = 0.86602540378443864676;
 		// (cos, sin) of 30 e degrees, e = 0..6
 		constexpr double cw[7] = {1.0, c30, 0.5, 0.0, -0.5, -c30, -1.0};
 		constexpr double sw[7] = {0.0, 0.5, c30, 1.0, c30, 0.5, 0.0};
-		double2 y[A][3];
+		C y[A][3];
 #pragma unroll
 		for (int n2 = 0; n2 < A; n2++) {
-			double2 t[3] = {v[n2], v[A + n2], v[2 * A + n2]};
+			C t[3] = {v[n2], v[A + n2], v[2 * A + n2]};
 			Dft<3>::run(t, sign);
 #pragma unroll
 			for (int k1 = 0; k1 < 3; k1++) {
 				const int e = (n2 * k1) * (12 / R);   // in units of 30 degrees
-				y[n2][k1] = e == 0 ? t[k1] : cmul(t[k1], make_double2(cw[e], sign * sw[e]));
+				y[n2][k1] = e == 0 ? t[k1] : cmul(t[k1], cmk<C>((Rl)cw[e], (Rl)(sign * sw[e])));
 			}
 		}
 #pragma unroll
 		for (int k1 = 0; k1 < 3; k1++) {
-			double2 t[A];
+			C t[A];
 #pragma unroll
 			for (int n2 = 0; n2 < A; n2++) t[n2] = y[n2][k1];
 			Dft<A>::run(t, sign);
@@ -521,12 +560,12 @@ struct WPad {
 	__device__ static __forceinline__ int at(int s, int i) { return s * len + i + (i >> 4); }
 };
 
-template <int N, int NSEQ, int NT, int R, int Ns, int SIGN>
-__device__ __forceinline__ void pass_ip(double2 *__restrict__ buf, const double2 *__restrict__ tw)
+template <int N, int NSEQ, int NT, int R, int Ns, int SIGN, typename C>
+__device__ __forceinline__ void pass_ip(C *__restrict__ buf, const C *__restrict__ tw)
 {
 	constexpr int NB = N / R, TOT = NSEQ * NB, TWS = N / (Ns * R);
 	constexpr int IT = (TOT + NT - 1) / NT;
-	double2 v[IT][R];
+	C v[IT][R];
 #pragma unroll
 	for (int it = 0; it < IT; it++) {
 		const int idx = (int)threadIdx.x + it * NT;
@@ -538,7 +577,7 @@ __device__ __forceinline__ void pass_ip(double2 *__restrict__ buf, const double2
 		if (Ns > 1) {
 #pragma unroll
 			for (int t = 1; t < R; t++) {
-				double2 w = tw[t * k * TWS];
+				C w = tw[t * k * TWS];
 				if (SIGN > 0) w.y = -w.y;
 				v[it][t] = cmul(v[it][t], w);
 			}
@@ -559,8 +598,8 @@ __device__ __forceinline__ void pass_ip(double2 *__restrict__ buf, const double2
 	__syncthreads();
 }
 
-template <int N, int NSEQ, int NT, int SIGN, int P>
-__device__ __forceinline__ void passes_ip(double2 *buf, const double2 *__restrict__ tw)
+template <int N, int NSEQ, int NT, int SIGN, int P, typename C>
+__device__ __forceinline__ void passes_ip(C *buf, const C *__restrict__ tw)
 {
 	if constexpr (P < CPlan<N>::npass()) {
 		pass_ip<N, NSEQ, NT, CPlan<N>::radix(P), CPlan<N>::ns(P), SIGN>(buf, tw);
@@ -568,8 +607,8 @@ __device__ __forceinline__ void passes_ip(double2 *buf, const double2 *__restric
 	}
 }
 
-template <int N, int NSEQ, int NT, int SIGN, int P, int PEND>
-__device__ __forceinline__ void passes_ip_range(double2 *buf, const double2 *__restrict__ tw)
+template <int N, int NSEQ, int NT, int SIGN, int P, int PEND, typename C>
+__device__ __forceinline__ void passes_ip_range(C *buf, const C *__restrict__ tw)
 {
 	if constexpr (P < PEND) {
 		pass_ip<N, NSEQ, NT, CPlan<N>::radix(P), CPlan<N>::ns(P), SIGN>(buf, tw);
@@ -713,6 +752,185 @@ __global__ __launch_bounds__(NT) void k_fft_wide(const void *__restrict__ in, vo
 	}
 }
 
+// ---- FP32, ring-major internal fields ---------------------------------------
+// The mixed-precision SWE stepper keeps its physical fields in FP32, ring-major
+// (float[f][j][N], every ring one contiguous row), and its Fourier rows in FP32
+// (float2[f][L + 1][nlat], the layout above).  A workgroup owns 2 NSEQ adjacent
+// rings; sequence s is z_s[i] = f(j0 + 2s, i) + i f(j0 + 2s + 1, i).  The
+// physical side is read and written a ring row at a time (consecutive lanes,
+// consecutive longitudes); the Fourier side as NSEQ ring pairs of one order,
+// 16 NSEQ contiguous bytes.  Passes, radices and the split into the two rings
+// of a pair are those of k_fft_wide, in FP32 with FP32 twiddles.
+template <int MODE, int N, int NSEQ, int NT>
+__global__ __launch_bounds__(NT) void k_fft_rm32(const void *__restrict__ in, void *__restrict__ out,
+						 const float2 *__restrict__ tw, int nlat, int L, int nblk, int nblk_pad,
+						 float norm)
+{
+	typedef WPad<N, NSEQ> W;
+	typedef CPlan<N> PL;
+	constexpr int RB = 2 * NSEQ;
+	constexpr int NP = PL::npass();
+	static_assert(NP >= 2, "two radix passes at least");
+	constexpr int R0 = PL::radix(0), NB0 = N / R0;
+	constexpr int RL = PL::radix(NP - 1), NSL = PL::ns(NP - 1);
+	static_assert(NSL * RL == N && NSL % 2 == 0, "last pass");
+	constexpr int SIGN = MODE == MODE_R2C ? -1 : 1;
+	extern __shared__ float2 smf[];
+	const int lb = xcd_remap(blockIdx.x, nblk_pad);
+	if (lb >= nblk) return;
+	const int f = blockIdx.y;
+	const size_t nspat = (size_t)nlat * N;
+	const int tid = threadIdx.x;
+	const int j0 = lb * RB;
+
+	// ---- first pass from HBM ----
+	for (int it = tid; it < NSEQ * NB0; it += NT) {
+		const int b = it % NB0, sq = it / NB0;
+		const int j = j0 + 2 * sq;
+		const bool live = j < nlat;
+		float2 v[R0];
+		if (MODE == MODE_R2C) {
+			const float *ra = (const float *)in + f * nspat + (size_t)(live ? j : 0) * N;
+			const float *rb = ra + N;
+#pragma unroll
+			for (int t = 0; t < R0; t++)
+				v[t] = live ? make_float2(ra[b + t * NB0], rb[b + t * NB0]) : make_float2(0.f, 0.f);
+		} else {
+			// z[k] = A + iB (k <= L), conj(A) + i conj(B) of order N - k (k >= N - L)
+			const float2 *src = (const float2 *)in + (size_t)f * (L + 1) * nlat;
+			float2 A[R0], B[R0];
+#pragma unroll
+			for (int t = 0; t < R0; t++) {
+				const int k = b + t * NB0;
+				const int m = k <= L ? k : (k >= N - L ? N - k : -1);
+				if (live && m >= 0) {
+					const float4 ab = *(const float4 *)(src + (size_t)m * nlat + j);
+					A[t] = make_float2(ab.x, ab.y);
+					B[t] = make_float2(ab.z, ab.w);
+				} else {
+					A[t] = B[t] = make_float2(0.f, 0.f);
+				}
+			}
+#pragma unroll
+			for (int t = 0; t < R0; t++) {
+				const int k = b + t * NB0;
+				if (k == 0) v[t] = make_float2(A[t].x, B[t].x);
+				else if (k <= L) v[t] = make_float2(A[t].x - B[t].y, A[t].y + B[t].x);
+				else v[t] = make_float2(A[t].x + B[t].y, -A[t].y + B[t].x);
+			}
+		}
+		Dft<R0>::run(v, SIGN);
+#pragma unroll
+		for (int t = 0; t < R0; t++) smf[W::at(sq, b * R0 + t)] = v[t];
+	}
+	__syncthreads();
+	passes_ip_range<N, NSEQ, NT, SIGN, 1, NP - 1>(smf, tw);
+
+	// ---- last pass to HBM ----
+	if (MODE == MODE_R2C) {
+		float2 *dst = (float2 *)out + (size_t)f * (L + 1) * nlat;
+		constexpr int NH = NSL / 2;
+		const float hn = 0.5f * norm;
+		for (int it = tid; it < NSEQ * NH; it += NT) {
+			const int pp = it / NSEQ, sq = it % NSEQ;
+			const int j = j0 + 2 * sq;
+			const int k1 = pp, k2 = pp == 0 ? NH : NSL - pp;
+			float2 z1[RL], z2[RL];
+#pragma unroll
+			for (int t = 0; t < RL; t++) {
+				z1[t] = smf[W::at(sq, k1 + t * NSL)];
+				z2[t] = smf[W::at(sq, k2 + t * NSL)];
+			}
+#pragma unroll
+			for (int t = 1; t < RL; t++) {
+				z1[t] = cmul(z1[t], tw[t * k1]);
+				z2[t] = cmul(z2[t], tw[t * k2]);
+			}
+			Dft<RL>::run(z1, SIGN);
+			Dft<RL>::run(z2, SIGN);
+			if (j >= nlat) continue;
+			// orders m = k + t NSL of butterflies k1 and k2, with their partners
+			auto emit = [&](int m, float2 Z, float2 P) __attribute__((always_inline)) {
+				if (m > L) return;
+				const float2 Zc = cconj(P);
+				float2 a = make_float2(hn * (Z.x + Zc.x), hn * (Z.y + Zc.y));
+				float2 c = make_float2(hn * (Z.y - Zc.y), -hn * (Z.x - Zc.x));
+				if (m == 0) a.y = c.y = 0.f;
+				*(float4 *)(dst + (size_t)m * nlat + j) = make_float4(a.x, a.y, c.x, c.y);
+			};
+#pragma unroll
+			for (int t = 0; t < RL; t++) {
+				const float2 q1 = pp == 0 ? z1[(RL - t) % RL] : z2[RL - 1 - t];
+				const float2 q2 = pp == 0 ? z2[RL - 1 - t] : z1[RL - 1 - t];
+				emit(k1 + t * NSL, z1[t], q1);
+				emit(k2 + t * NSL, z2[t], q2);
+			}
+		}
+	} else {
+		float *dst = (float *)out + f * nspat;
+		for (int it = tid; it < NSEQ * NSL; it += NT) {
+			const int k = it % NSL, sq = it / NSL;
+			const int j = j0 + 2 * sq;
+			float2 v[RL];
+#pragma unroll
+			for (int t = 0; t < RL; t++) v[t] = smf[W::at(sq, k + t * NSL)];
+#pragma unroll
+			for (int t = 1; t < RL; t++) {
+				float2 w = tw[t * k];
+				w.y = -w.y;
+				v[t] = cmul(v[t], w);
+			}
+			Dft<RL>::run(v, SIGN);
+			if (j < nlat) {
+				float *ra = dst + (size_t)j * N, *rb = ra + N;
+#pragma unroll
+				for (int t = 0; t < RL; t++) {
+					ra[k + t * NSL] = v[t].x;
+					rb[k + t * NSL] = v[t].y;
+				}
+			}
+		}
+	}
+}
+
+template <int MODE, int N, int NSEQ>
+int launch_rm32(sphswe_ctx *ctx, const void *in, void *out, int nf)
+{
+	constexpr int RB = 2 * NSEQ;
+	constexpr int NT = 256;
+	const int nblk = (ctx->nlat + RB - 1) / RB;
+	const int nblk_pad = (nblk + 7) / 8 * 8;
+	const size_t lds = (size_t)NSEQ * WPad<N, NSEQ>::len * sizeof(float2);
+	static_assert(NSEQ * WPad<N, NSEQ>::len * sizeof(float2) <= 160 * 1024, "tile exceeds the LDS");
+	if (lds > 65536)
+		SPHSWE_HIP(ctx, hipFuncSetAttribute((const void *)k_fft_rm32<MODE, N, NSEQ, NT>,
+						    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+	dim3 grid(nblk_pad, nf);
+	k_fft_rm32<MODE, N, NSEQ, NT><<<grid, NT, lds, ctx->stream>>>(in, out, ctx->d_tw32, ctx->nlat, ctx->L, nblk,
+								       nblk_pad, (float)(2.0 * M_PI / N));
+	SPHSWE_LAUNCH_CHECK(ctx);
+	return SPHSWE_OK;
+}
+
+// the FP32 ring-major transform for the grid sizes of the usual truncations,
+// or an error for other sizes
+template <int MODE>
+int launch_rm32_fixed(sphswe_ctx *ctx, const void *in, void *out, int nf)
+{
+	if (ctx->nlat % 2 != 0 || 2 * ctx->L + 1 > ctx->plan.n)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "FP32 FFT: grid %d x %d", ctx->nlat, ctx->plan.n);
+	if (!ctx->d_tw32) return sphswe_set_err(ctx, SPHSWE_ERR_INTERNAL, "FP32 FFT: no FP32 twiddles");
+	switch (ctx->plan.n) {
+	case 96: return launch_rm32<MODE, 96, 4>(ctx, in, out, nf);
+	case 196: return launch_rm32<MODE, 196, 4>(ctx, in, out, nf);
+	case 384: return launch_rm32<MODE, 384, 4>(ctx, in, out, nf);
+	case 784: return launch_rm32<MODE, 784, 4>(ctx, in, out, nf);
+	case 1536: return launch_rm32<MODE, 1536, 4>(ctx, in, out, nf);
+	case 3072: return launch_rm32<MODE, 3072, 4>(ctx, in, out, nf);
+	}
+	return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "FP32 FFT: no compile-time transform of length %d", ctx->plan.n);
+}
+
 template <int MODE, int N, int NSEQ>
 int launch_real_wide(sphswe_ctx *ctx, const void *in, void *out, int nf)
 {
@@ -817,6 +1035,25 @@ int sphswe_fft_r2c(sphswe_ctx *ctx, const double *spat, double2 *four, int nf)
 	if (nf <= 0) return SPHSWE_OK;
 	const int rc = launch_real_fixed<MODE_R2C>(ctx, spat, four, nf, ctx->fft_nseq_r2c);
 	return rc >= 0 ? rc : launch<MODE_R2C>(ctx, spat, four, nf);
+}
+int sphswe_fft_r2c32(sphswe_ctx *ctx, const float *spat, float2 *four, int nf)
+{
+	if (nf <= 0) return SPHSWE_OK;
+	return launch_rm32_fixed<MODE_R2C>(ctx, spat, four, nf);
+}
+int sphswe_fft_c2r32(sphswe_ctx *ctx, const float2 *four, float *spat, int nf)
+{
+	if (nf <= 0) return SPHSWE_OK;
+	return launch_rm32_fixed<MODE_C2R>(ctx, four, spat, nf);
+}
+// whether the FP32 transforms have a compile-time kernel for this grid
+bool sphswe_fft32_available(const sphswe_ctx *ctx)
+{
+	if (ctx->nlat % 2 != 0 || 2 * ctx->L + 1 > ctx->plan.n) return false;
+	switch (ctx->plan.n) {
+	case 96: case 196: case 384: case 784: case 1536: case 3072: return true;
+	}
+	return false;
 }
 int sphswe_fft_c2r(sphswe_ctx *ctx, const double2 *four, double *spat, int nf)
 {

@@ -623,9 +623,11 @@ int launch_anal(sphswe_ctx *ctx, const Geo &g, const double2 *four, double2 *spe
 	});
 }
 
-int check_prec(sphswe_ctx *ctx, int prec)
+// real transforms take every mode, the complex and pole-layout ones FP64 .. BF16X3
+int check_prec(sphswe_ctx *ctx, int prec, bool real = false)
 {
-	if (prec < SPHSWE_PREC_FP64 || prec > SPHSWE_PREC_BF16X3)
+	const int top = real ? SPHSWE_PREC_MIXED : SPHSWE_PREC_BF16X3;
+	if (prec < SPHSWE_PREC_FP64 || prec > top)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "precision %d is not available for this transform", prec);
 	return SPHSWE_OK;
 }
@@ -634,7 +636,8 @@ int check_prec(sphswe_ctx *ctx, int prec)
 
 int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int prec)
 {
-	if (int rc = check_prec(ctx, prec)) return rc;
+	if (int rc = check_prec(ctx, prec, true)) return rc;
+	if (prec == SPHSWE_PREC_MIXED) return sphswe_leg_synth_bt(ctx, spec, four, nf);
 	if (prec == SPHSWE_PREC_FP32 && ctx->tab_min_synth > 0 && nf >= ctx->tab_min_synth)
 		return sphswe_leg_synth_tab(ctx, spec, four, nf);
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_synth_mx(ctx, spec, four, nf, prec);
@@ -676,7 +679,8 @@ int sphswe_leg_synth(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf
 
 int sphswe_leg_anal(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf, int prec)
 {
-	if (int rc = check_prec(ctx, prec)) return rc;
+	if (int rc = check_prec(ctx, prec, true)) return rc;
+	if (prec == SPHSWE_PREC_MIXED) return sphswe_leg_anal_bt(ctx, four, spec, nf);
 	if (prec == SPHSWE_PREC_FP32 && ctx->tab_min_anal > 0 && nf >= ctx->tab_min_anal && !ctx->ext_escratch)
 		return sphswe_leg_anal_tab(ctx, four, spec, nf);
 	if (prec != SPHSWE_PREC_FP64) return sphswe_leg_anal_mx(ctx, four, spec, nf, prec);
@@ -750,7 +754,9 @@ size_t sphswe_leg_anal_group_bytes(const sphswe_ctx *ctx, int prec)
 // later analyses allocate nothing (needed before a stream capture)
 int sphswe_leg_anal_reserve(sphswe_ctx *ctx, int nf, int prec)
 {
-	if (int rc = check_prec(ctx, prec)) return rc;
+	if (int rc = check_prec(ctx, prec, true)) return rc;
+	// the table form needs no operand buffer, only its tables
+	if (prec == SPHSWE_PREC_MIXED) return sphswe_leg_bt_tables(ctx);
 	const int nch = (ctx->NJ + 63) / 64;
 	size_t per_chunk = 0;
 	switch (prec) {

@@ -149,6 +149,61 @@ __global__ void k_phys2(const double *__restrict__ P2, double *__restrict__ P1, 
 	P1[(long)mem * nspat + k] = -(dlon + dlat) * scale;
 }
 
+// ---- the mixed form (SPHSWE_PREC_MIXED): FP32 ring-major physical fields ----
+// The same pointwise terms as k_phys1 / k_phys2 in FP32 on fields laid out
+// [member][field][ring j][longitude], so that ring j of point k is k / nphi.
+__global__ void k_phys1_mx(const float *__restrict__ P9, float *__restrict__ P4, const double *__restrict__ ct,
+			   const double *__restrict__ st, const double *__restrict__ sq, int nphi, long nspat, int nmem,
+			   float ga, float two_omega, float ia, int nonlinear, float h0)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= nspat * nmem) return;
+	int mem = (int)(i / nspat);
+	long k = i - (long)mem * nspat;
+	int j = (int)(k / nphi);
+	const float *p = P9 + (long)mem * 9 * nspat + k;
+	float H = p[0], U = p[nspat], V = p[2 * nspat];
+	float Hl = p[3 * nspat], Hm = p[4 * nspat];
+	float Ul = p[5 * nspat], Um = p[6 * nspat], Vl = p[7 * nspat], Vm = p[8 * nspat];
+	const float c = (float)sq[j], mu = (float)ct[j], sj = (float)st[j];
+	float *o = P4 + (long)mem * 4 * nspat + k;
+	if (nonlinear == 1) {
+		o[0] = U * H;
+		o[nspat] = (V * H) * sj;
+		o[2 * nspat] = -ga * (Hl / c) + two_omega * (V * mu) - ((Ul / c) * U + (Um / c) * V) * ia;
+		o[3 * nspat] = -ga * (Hm / c) - two_omega * (U * mu) - ((Vl / c) * U + (Vm / c) * V) * ia;
+	} else if (nonlinear == 3) {
+		o[0] = -((Ul + Vm) / (sj * sj)) * h0;
+		o[nspat] = 0.f;
+		o[2 * nspat] = -ga * Hl + two_omega * (V * mu);
+		o[3 * nspat] = -ga * Hm - two_omega * (U * mu);
+	} else if (nonlinear == 2) {
+		const float Hp = H - h0;
+		o[0] = U * Hp;
+		o[nspat] = (V * Hp) * sj;
+		o[2 * nspat] = -((Ul / c) * U + (Um / c) * V) * ia;
+		o[3 * nspat] = -((Vl / c) * U + (Vm / c) * V) * ia;
+	} else {
+		o[0] = U;
+		o[nspat] = V * sj;
+		o[2 * nspat] = -ga * (Hl / c) + two_omega * (V * mu);
+		o[3 * nspat] = -ga * (Hm / c) - two_omega * (U * mu);
+	}
+}
+
+__global__ void k_phys2_mx(const float *__restrict__ P2, float *__restrict__ P1, const double *__restrict__ st,
+			   const double *__restrict__ sq, int nphi, long nspat, int nmem, float scale)
+{
+	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= nspat * nmem) return;
+	int mem = (int)(i / nspat);
+	long k = i - (long)mem * nspat;
+	int j = (int)(k / nphi);
+	const float *p = P2 + (long)mem * 2 * nspat + k;
+	const float c = (float)sq[j];
+	P1[(long)mem * nspat + k] = -(p[0] / c + (p[nspat] / c) / (float)st[j]) * scale;
+}
+
 // tendencies k = (T_h, T_u, T_v) + nu/a^2 laplace(Y) and the RK4 bookkeeping
 //   stage < 3: acc (=|+=) b_s k ; Ynext = X + dt*(a_{s+1} k)
 //   stage 3:   X += dt*(acc + b_3 k)
@@ -208,6 +263,11 @@ inline int g1(long n) { return (int)((n + 255) / 256); }
 struct SweBufs {
 	double2 *Ya, *ACC, *Yb, *S9, *S4, *S2, *TH, *F;
 	double *P9, *P4, *P2, *P1;
+	// the mixed form: pre-split synthesis and analysis operands, FP32 Fourier
+	// rows and FP32 ring-major physical fields
+	void *SB, *G;
+	float2 *F32;
+	float *Q9, *Q4, *Q2, *Q1;
 };
 
 // one captured step per context, rebuilt when its arguments change
@@ -220,16 +280,33 @@ struct SweGraph {
 };
 std::unordered_map<const sphswe_ctx *, SweGraph> g_graph;
 
-int swe_buffers(sphswe_ctx *ctx, int nmem, SweBufs &b)
+int swe_buffers(sphswe_ctx *ctx, int nmem, int prec, SweBufs &b)
 {
 	const long nlm = ctx->nlm, nspat = ctx->nspat;
 	const long nfour = (long)(ctx->L + 1) * ctx->nlat;
-	// workspace: Y (3 nlm), ACC (3), YN (3), S9 (9), S4 (4), S2 (2), TH (1) spectral;
-	// P9 (9), P4 (4), P2 (2), P1 (1) physical; Fourier for 9 fields
-	const size_t spec_e = (size_t)nlm * nmem * (3 + 3 + 3 + 9 + 4 + 2 + 1);
-	const size_t spat_e = (size_t)nspat * nmem * (9 + 4 + 2 + 1);
-	const size_t four_e = (size_t)nfour * nmem * 9;
-	const size_t bytes = spec_e * 16 + spat_e * 8 + four_e * 16 + 1024;
+	const bool mixed = prec == SPHSWE_PREC_MIXED;
+	b = SweBufs{};
+	size_t bytes;
+	size_t sb_b = 0, g_b = 0;
+	if (mixed) {
+		if (!sphswe_fft32_available(ctx))
+			return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: the mixed form has no FP32 FFT for %d longitudes",
+					      ctx->nphi);
+		if (int rc = sphswe_leg_bt_tables(ctx)) return rc;
+		// Y, ACC, YN (3 nlm each), S4 (4), TH (1) spectral in FP64; operands of
+		// 9 and 4 fields; FP32 Fourier rows of 9 fields; FP32 fields 9 + 4 + 2 + 1
+		sb_b = (sphswe_bt_sb_bytes(ctx, 9 * nmem) + 255) & ~(size_t)255;
+		g_b = (sphswe_bt_g_bytes(ctx, 4 * nmem) + 255) & ~(size_t)255;
+		bytes = (size_t)nlm * nmem * (3 + 3 + 3 + 4 + 1) * 16 + sb_b + g_b + (size_t)nfour * nmem * 9 * 8 +
+			(size_t)nspat * nmem * (9 + 4 + 2 + 1) * 4 + 1024;
+	} else {
+		// workspace: Y (3 nlm), ACC (3), YN (3), S9 (9), S4 (4), S2 (2), TH (1) spectral;
+		// P9 (9), P4 (4), P2 (2), P1 (1) physical; Fourier for 9 fields
+		const size_t spec_e = (size_t)nlm * nmem * (3 + 3 + 3 + 9 + 4 + 2 + 1);
+		const size_t spat_e = (size_t)nspat * nmem * (9 + 4 + 2 + 1);
+		const size_t four_e = (size_t)nfour * nmem * 9;
+		bytes = spec_e * 16 + spat_e * 8 + four_e * 16 + 1024;
+	}
 	SweWs &ws = g_ws[ctx];
 	if (ws.bytes < bytes) {
 		if (ws.base) {
@@ -247,6 +324,27 @@ int swe_buffers(sphswe_ctx *ctx, int nmem, SweBufs &b)
 		if (hipMalloc(&ws.base, bytes) != hipSuccess)
 			return sphswe_set_err(ctx, SPHSWE_ERR_NOMEM, "swe workspace of %zu bytes", bytes);
 		ws.bytes = bytes;
+	}
+	if (mixed) {
+		char *c = (char *)ws.base;
+		auto take = [&](size_t n) {
+			char *r = c;
+			c += (n + 255) & ~(size_t)255;
+			return (void *)r;
+		};
+		b.Ya = (double2 *)take((size_t)3 * nlm * nmem * 16);
+		b.ACC = (double2 *)take((size_t)3 * nlm * nmem * 16);
+		b.Yb = (double2 *)take((size_t)3 * nlm * nmem * 16);
+		b.S4 = (double2 *)take((size_t)4 * nlm * nmem * 16);
+		b.TH = (double2 *)take((size_t)nlm * nmem * 16);
+		b.SB = take(sb_b);
+		b.G = take(g_b);
+		b.F32 = (float2 *)take((size_t)9 * nfour * nmem * 8);
+		b.Q9 = (float *)take((size_t)9 * nspat * nmem * 4);
+		b.Q4 = (float *)take((size_t)4 * nspat * nmem * 4);
+		b.Q2 = (float *)take((size_t)2 * nspat * nmem * 4);
+		b.Q1 = (float *)take((size_t)nspat * nmem * 4);
+		return SPHSWE_OK;
 	}
 	double2 *sp = (double2 *)ws.base;
 	b.Ya = sp; sp += 3 * nlm * nmem;
@@ -323,9 +421,63 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 	return SPHSWE_OK;
 }
 
+// one RK4 step in the mixed form: the Legendre contractions on the bf16 matrix
+// cores from the stored tables (operands split into bf16 hi + lo), Fourier rows,
+// longitude FFTs and pointwise terms in FP32, the state and the stage
+// combinations in FP64.  The projection levels are those of swe_step.
+int swe_step_mixed(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, const SweBufs &b)
+{
+	const long nlm = ctx->nlm, nspat = ctx->nspat;
+	const double a = p->earth_radius, g = p->gravitation;
+	const double ga = g / a, ia = 1.0 / a, two_omega = 2.0 * p->coriolis_omega;
+	const double visc = p->viscosity / (a * a);
+	const double dt = p->dt;
+	const double bcoef[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+	const double anext[3] = {0.5, 0.5, 1.0};
+	const int mode = (p->nonlinear == 0 && p->robert_functions) ? 3 : p->nonlinear;
+	const int L = ctx->L, nphi = ctx->nphi;
+	hipStream_t st = ctx->stream;
+	int rc;
+	const double2 *Y = X;
+	double2 *YN = b.Ya;
+	for (int s = 0; s < 4; s++) {
+		if ((rc = sphswe_bt_sprep_swe9(ctx, Y, b.SB, nm))) return rc;
+		if ((rc = sphswe_bt_synth32(ctx, b.SB, b.F32, 9 * nm))) return rc;
+		if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q9, 9 * nm))) return rc;
+		k_phys1_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q9, b.Q4, ctx->d_ct, ctx->d_st, ctx->d_sq, nphi, nspat, nm,
+							   (float)ga, (float)two_omega, (float)ia, mode,
+							   (float)(mode == 3 ? p->h0 / a : p->h0));
+		if ((rc = sphswe_fft_r2c32(ctx, b.Q4, b.F32, 4 * nm))) return rc;
+		if ((rc = sphswe_bt_gprep32(ctx, b.F32, b.G, 4 * nm))) return rc;
+		if ((rc = sphswe_bt_anal(ctx, b.G, b.S4, 4 * nm))) return rc;
+		const double2 *TH = b.S4;
+		int th_stride = 4;
+		if (mode != 3) {
+			if ((rc = sphswe_bt_sprep_swe2(ctx, b.S4, b.SB, nm))) return rc;
+			if ((rc = sphswe_bt_synth32(ctx, b.SB, b.F32, 2 * nm))) return rc;
+			if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q2, 2 * nm))) return rc;
+			k_phys2_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q2, b.Q1, ctx->d_st, ctx->d_sq, nphi, nspat, nm,
+								   (float)(p->nonlinear ? ia : p->h0 / a));
+			if ((rc = sphswe_fft_r2c32(ctx, b.Q1, b.F32, nm))) return rc;
+			if ((rc = sphswe_bt_gprep32(ctx, b.F32, b.G, nm))) return rc;
+			if ((rc = sphswe_bt_anal(ctx, b.G, b.TH, nm))) return rc;
+			TH = b.TH;
+			th_stride = 1;
+		}
+		k_stage<<<g1(nlm * nm), 256, 0, st>>>(TH, th_stride, b.S4, Y, X, b.ACC, YN, ctx->d_mofidx, L, (int)nlm, nm,
+						       visc, s, bcoef[s], s < 3 ? anext[s] : 0.0, dt);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		if (s < 3) {
+			Y = YN;
+			YN = (YN == b.Ya) ? b.Yb : b.Ya;
+		}
+	}
+	return SPHSWE_OK;
+}
+
 int check_params(sphswe_ctx *ctx, const sphswe_swe_params *p)
 {
-	if (p->precision < SPHSWE_PREC_FP64 || p->precision > SPHSWE_PREC_BF16X3)
+	if (p->precision < SPHSWE_PREC_FP64 || p->precision > SPHSWE_PREC_MIXED)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: precision %d not available", p->precision);
 	if (p->nonlinear < 0 || p->nonlinear > 2)
 		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: nonlinear must be 0, 1 or 2");
@@ -357,9 +509,11 @@ extern "C" int sphswe_swe_rk4(sphswe_ctx *ctx, const sphswe_swe_params *p, doubl
 	if (!ctx || !p || !d_state || nmem < 1 || nsteps < 0) return SPHSWE_ERR_ARG;
 	if (int rc = check_params(ctx, p)) return rc;
 	SweBufs b;
-	if (int rc = swe_buffers(ctx, nmem, b)) return rc;
+	if (int rc = swe_buffers(ctx, nmem, p->precision, b)) return rc;
 	for (int n = 0; n < nsteps; n++)
-		if (int rc = swe_step(ctx, p, (double2 *)d_state, nmem, b)) return rc;
+		if (int rc = p->precision == SPHSWE_PREC_MIXED ? swe_step_mixed(ctx, p, (double2 *)d_state, nmem, b)
+							       : swe_step(ctx, p, (double2 *)d_state, nmem, b))
+			return rc;
 	return SPHSWE_OK;
 }
 
@@ -371,7 +525,7 @@ extern "C" int sphswe_swe_rk4_graph(sphswe_ctx *ctx, const sphswe_swe_params *p,
 	// every buffer the step uses exists before the check below, so that a
 	// regrowth it causes is seen by the generation test
 	SweBufs b;
-	if (int rc = swe_buffers(ctx, nmem, b)) return rc;
+	if (int rc = swe_buffers(ctx, nmem, p->precision, b)) return rc;
 	if (int rc = sphswe_leg_anal_reserve(ctx, 4 * nmem, p->precision)) return rc;
 	if (p->precision == SPHSWE_PREC_FP32 && (ctx->tab_min_synth > 0 || ctx->tab_min_anal > 0))
 		if (int rc = sphswe_leg_tables(ctx)) return rc;
@@ -385,7 +539,8 @@ extern "C" int sphswe_swe_rk4_graph(sphswe_ctx *ctx, const sphswe_swe_params *p,
 		SPHSWE_HIP(ctx, hipStreamSynchronize(ctx->stream));
 		hipGraph_t graph;
 		SPHSWE_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-		const int rc = swe_step(ctx, p, (double2 *)d_state, nmem, b);
+		const int rc = p->precision == SPHSWE_PREC_MIXED ? swe_step_mixed(ctx, p, (double2 *)d_state, nmem, b)
+								   : swe_step(ctx, p, (double2 *)d_state, nmem, b);
 		hipGraph_t captured = nullptr;
 		const hipError_t ec = hipStreamEndCapture(ctx->stream, &captured);
 		if (rc) {
