@@ -27,8 +27,10 @@
 // (e.g. div_lon(hu) + div_lat(hv)) are formed here in physical space before
 // one analysis; the analysis is linear, so the two agree to round-off.
 #include "sphswe_common.h"
+#include "sphswe_fft_blocks.h"
 
 #include <cstring>
+#include <type_traits>
 
 namespace {
 
@@ -204,6 +206,243 @@ __global__ void k_phys2_mx(const float *__restrict__ P2, float *__restrict__ P1,
 	P1[(long)mem * nspat + k] = -(p[0] / c + (p[nspat] / c) / (float)st[j]) * scale;
 }
 
+// The mixed form's physical stages without physical fields in HBM: one
+// workgroup per (member, ring pair j, j + 1) runs the inverse FFTs of the
+// level's input Fourier rows (9 at level 0, 2 at level 2), which leave each
+// thread the longitudes k + NT w + t NSL of its last-pass butterflies in
+// registers, forms the pointwise terms there (those of k_phys1_mx /
+// k_phys2_mx), and runs the forward FFTs of the outputs (4, 1) from LDS to the
+// output Fourier rows.  The two rings are the real and imaginary parts of one
+// complex sequence.  Fields go through the passes FI (inputs) or FO (outputs)
+// at a time as the sequences of one tile; one at a time measured fastest (at
+// T511 three inputs per round need twice the registers and run 1.9x slower),
+// and the next field's Fourier rows are loaded while the current one is
+// transformed.  Passes, radices and normalisation are those of the FFT kernels
+// with radices of 8 first (at N = 1536, 8 8 8 3 keeps 192 of the 256 threads
+// busy in the first passes, 16 16 6 only 96: 527 against 563 us for 16
+// members); lengths whose last pass has fewer butterflies than a wavefront
+// take the separate kernels.
+template <int N, int NTMAX = 256>
+struct FusedPlan {
+	static constexpr int RULE = 1;
+	typedef CPlan<N, RULE> PL;
+	static constexpr int NP = PL::npass();
+	static constexpr int R0 = PL::radix(0);
+	static constexpr int RL = PL::radix(NP - 1), NSL = PL::ns(NP - 1);
+	static constexpr int NT = NSL < NTMAX ? NSL : NTMAX;   // threads per workgroup
+	static constexpr int NW = NSL / NT;                 // last-pass butterflies per thread and field
+	static constexpr int NV = NW * RL;                  // longitudes held per thread
+	static constexpr bool ok = NP >= 2 && NSL % 64 == 0 && NSL % NT == 0 && NV <= 8;
+};
+
+template <int LEV, int MODE, int N>
+__global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
+	const float2 *__restrict__ Fin, float2 *__restrict__ Fout, const float2 *__restrict__ twg,
+	const double *__restrict__ ct, const double *__restrict__ st, const double *__restrict__ sq, int nlat, int L,
+	int nblk, int nblk_pad, float ga, float two_omega, float ia, float h0, float scale)
+{
+	constexpr int mode = MODE;
+	typedef FusedPlan<N> FP;
+	constexpr int NIN = LEV == 0 ? 9 : 2, NOUT = LEV == 0 ? 4 : 1;
+	constexpr int FI = 1, FO = 1;   // fields per round, in and out
+	typedef WPad<N, FI> WI;
+	typedef WPad<N, FO> WO;
+	constexpr int NT = FP::NT, NP = FP::NP, RL = FP::RL, NSL = FP::NSL, NW = FP::NW, NV = FP::NV;
+	constexpr int SMN = FI * WI::len > FO * WO::len ? FI * WI::len : FO * WO::len;
+	__shared__ float2 sm[SMN];
+	// the twiddles in LDS: read there, they are not hoisted out of the
+	// unrolled field loop and kept in registers
+	__shared__ float2 tws[N];
+	const int lb = xcd_remap(blockIdx.x, nblk_pad);
+	if (lb >= nblk) return;
+	const int npair = nlat >> 1;
+	const int mem = lb / npair, j = 2 * (lb - mem * npair);
+	const int tid = threadIdx.x;
+	for (int k = tid; k < N; k += NT) tws[k] = twg[k];
+	const long frow = (long)(L + 1) * nlat;   // one field's Fourier rows
+
+	// Each input field's values (ring j in x, ring j + 1 in y, at this thread's
+	// longitudes) are folded into the outputs O as the field arrives, so that
+	// only U, V and the outputs stay live.  Level 0, in field order H, U, V,
+	// i m H, D'H, i m U, D'U, i m V, D'V:
+	//   mode 1  O0 = U H, O1 = V H st, O2 = -ga Hl/c + 2 Omega V mu - (Ul/c U + Um/c V)/a,
+	//           O3 = -ga Hm/c - 2 Omega U mu - (Vl/c U + Vm/c V)/a;
+	//   mode 0  O0 = U, O1 = V st, O2, O3 without the advection terms;
+	//   mode 2  O0 = U (H - h0), O1 = V (H - h0) st, O2, O3 the advection terms only;
+	//   mode 3  O0 = -(Ul + Vm) h0 / st^2, O1 = 0, O2 = -ga Hl + 2 Omega V mu,
+	//           O3 = -ga Hm - 2 Omega U mu;
+	// level 2: O0 = -(P0/c + P1/(c st)) scale.
+	float rc_[2], rmu[2], rst[2];
+#pragma unroll
+	for (int r = 0; r < 2; r++) {
+		rc_[r] = (float)sq[j + r];
+		rmu[r] = (float)ct[j + r];
+		rst[r] = (float)st[j + r];
+	}
+	float Hs[NV][2], Us[NV][2], Vs[NV][2], O[NOUT][NV][2];
+	auto fold = [&](int q, int e, int r, float x) __attribute__((always_inline)) {
+		const float c = rc_[r], mu = rmu[r], sj = rst[r];
+		if (LEV == 1) {
+			if (q == 0) O[0][e][r] = x / c;
+			else O[0][e][r] = -(O[0][e][r] + (x / c) / sj) * scale;
+			return;
+		}
+		switch (q) {
+		case 0: Hs[e][r] = mode == 2 ? x - h0 : x; break;
+		case 1:
+			Us[e][r] = x;
+			O[0][e][r] = (mode == 1 || mode == 2) ? x * Hs[e][r] : x;
+			break;
+		case 2:
+			Vs[e][r] = x;
+			O[1][e][r] = mode == 3 ? 0.f : (mode == 0 ? x * sj : (x * Hs[e][r]) * sj);
+			O[2][e][r] = mode == 2 ? 0.f : two_omega * (x * mu);
+			O[3][e][r] = mode == 2 ? 0.f : -(two_omega * (Us[e][r] * mu));
+			break;
+		case 3:
+			if (mode != 2) O[2][e][r] = -ga * (mode == 3 ? x : x / c) + O[2][e][r];
+			break;
+		case 4:
+			if (mode != 2) O[3][e][r] = -ga * (mode == 3 ? x : x / c) + O[3][e][r];
+			break;
+		case 5:
+			if (mode == 3) O[0][e][r] = x;
+			else if (mode != 0) O[2][e][r] -= ((x / c) * Us[e][r]) * ia;
+			break;
+		case 6:
+			if (mode == 1 || mode == 2) O[2][e][r] -= ((x / c) * Vs[e][r]) * ia;
+			break;
+		case 7:
+			if (mode == 1 || mode == 2) O[3][e][r] -= ((x / c) * Us[e][r]) * ia;
+			break;
+		case 8:
+			if (mode == 3) O[0][e][r] = -((O[0][e][r] + x) / (sj * sj)) * h0;
+			else if (mode != 0) O[3][e][r] -= ((x / c) * Vs[e][r]) * ia;
+			break;
+		}
+	};
+
+	// The Fourier rows of the next round are loaded one round ahead: thread
+	// tid holds orders m = tid + NT u of each field, both rings (16 bytes); the
+	// sequence z[m] = A + iB, z[N - m] = conj(A) + i conj(B) (0 between L and
+	// N - L) is written to LDS in natural order and every pass runs from there.
+	constexpr int NMU = (N / 2 + NT) / NT;   // orders per thread: m <= L < N / 2
+	float4 ab[FI][NMU];
+	auto fetch = [&](int q0) __attribute__((always_inline)) {
+#pragma unroll
+		for (int f = 0; f < FI; f++) {
+			const float2 *src = Fin + (long)(mem * NIN + q0 + f) * frow;
+#pragma unroll
+			for (int u = 0; u < NMU; u++) {
+				const int m = tid + u * NT;
+				ab[f][u] = (q0 + f < NIN && m <= L) ? *(const float4 *)(src + (size_t)m * nlat + j)
+								    : make_float4(0.f, 0.f, 0.f, 0.f);
+			}
+		}
+	};
+	fetch(0);
+	// unrolled: every register array is indexed by compile-time numbers only
+#pragma unroll
+	for (int q0 = 0; q0 < NIN; q0 += FI) {
+#pragma unroll
+		for (int f = 0; f < FI; f++) {
+#pragma unroll
+			for (int u = 0; u < NMU; u++) {
+				const int m = tid + u * NT;
+				const float4 v = ab[f][u];
+				if (m == 0) {
+					sm[WI::at(f, 0)] = make_float2(v.x, v.z);
+				} else if (m <= L) {
+					sm[WI::at(f, m)] = make_float2(v.x - v.w, v.y + v.z);
+					sm[WI::at(f, N - m)] = make_float2(v.x + v.w, -v.y + v.z);
+				}
+			}
+			for (int k = L + 1 + tid; k < N - L; k += NT) sm[WI::at(f, k)] = make_float2(0.f, 0.f);
+		}
+		if (q0 + FI < NIN) fetch(q0 + FI);
+		__syncthreads();
+		passes_ip_range<N, FI, NT, +1, 0, NP - 1, float2, FP::RULE>(sm, tws);
+		// last pass into registers, folded into the outputs
+#pragma unroll
+		for (int f = 0; f < FI; f++) {
+			if (q0 + f >= NIN) break;
+#pragma unroll
+			for (int w = 0; w < NW; w++) {
+				const int k = tid + w * NT;
+				float2 v[RL];
+#pragma unroll
+				for (int t = 0; t < RL; t++) v[t] = sm[WI::at(f, k + t * NSL)];
+#pragma unroll
+				for (int t = 1; t < RL; t++) {
+					float2 tw = tws[t * k];
+					tw.y = -tw.y;
+					v[t] = cmul(v[t], tw);
+				}
+				Dft<RL>::run(v, +1);
+#pragma unroll
+				for (int t = 0; t < RL; t++) {
+					fold(q0 + f, w * RL + t, 0, v[t].x);
+					fold(q0 + f, w * RL + t, 1, v[t].y);
+				}
+			}
+		}
+		__syncthreads();
+	}
+
+	// the outputs, FO per round: natural order into LDS, forward passes, last
+	// pass to the Fourier rows (butterflies k and NSL - k give both Z[m] and
+	// Z[N - m])
+	const float hn = (float)(M_PI / N);   // half the analysis normalisation 2 pi / N
+	constexpr int NH = NSL / 2;
+#pragma unroll
+	for (int o0 = 0; o0 < NOUT; o0 += FO) {
+#pragma unroll
+	for (int o = 0; o < FO; o++)
+#pragma unroll
+		for (int w = 0; w < NW; w++)
+#pragma unroll
+			for (int t = 0; t < RL; t++)
+				sm[WO::at(o, tid + w * NT + t * NSL)] =
+					make_float2(O[o0 + o][w * RL + t][0], O[o0 + o][w * RL + t][1]);
+	__syncthreads();
+	passes_ip_range<N, FO, NT, -1, 0, NP - 1, float2, FP::RULE>(sm, tws);
+	for (int it = tid; it < FO * NH; it += NT) {
+		const int o = it / NH, pp = it - o * NH;
+		float2 *dst = Fout + (long)(mem * NOUT + o0 + o) * frow;
+		const int k1 = pp, k2 = pp == 0 ? NH : NSL - pp;
+		float2 z1[RL], z2[RL];
+#pragma unroll
+		for (int t = 0; t < RL; t++) {
+			z1[t] = sm[WO::at(o, k1 + t * NSL)];
+			z2[t] = sm[WO::at(o, k2 + t * NSL)];
+		}
+#pragma unroll
+		for (int t = 1; t < RL; t++) {
+			z1[t] = cmul(z1[t], tws[t * k1]);
+			z2[t] = cmul(z2[t], tws[t * k2]);
+		}
+		Dft<RL>::run(z1, -1);
+		Dft<RL>::run(z2, -1);
+		auto emit = [&](int m, float2 Z, float2 Q) __attribute__((always_inline)) {
+			if (m > L) return;
+			const float2 Zc = cconj(Q);
+			float2 a = make_float2(hn * (Z.x + Zc.x), hn * (Z.y + Zc.y));
+			float2 c = make_float2(hn * (Z.y - Zc.y), -hn * (Z.x - Zc.x));
+			if (m == 0) a.y = c.y = 0.f;
+			*(float4 *)(dst + (size_t)m * nlat + j) = make_float4(a.x, a.y, c.x, c.y);
+		};
+#pragma unroll
+		for (int t = 0; t < RL; t++) {
+			const float2 q1 = pp == 0 ? z1[(RL - t) % RL] : z2[RL - 1 - t];
+			const float2 q2 = pp == 0 ? z2[RL - 1 - t] : z1[RL - 1 - t];
+			emit(k1 + t * NSL, z1[t], q1);
+			emit(k2 + t * NSL, z2[t], q2);
+		}
+	}
+	__syncthreads();
+	}
+}
+
 // tendencies k = (T_h, T_u, T_v) + nu/a^2 laplace(Y) and the RK4 bookkeeping
 //   stage < 3: acc (=|+=) b_s k ; Ynext = X + dt*(a_{s+1} k)
 //   stage 3:   X += dt*(acc + b_3 k)
@@ -294,10 +533,11 @@ int swe_buffers(sphswe_ctx *ctx, int nmem, int prec, SweBufs &b)
 					      ctx->nphi);
 		if (int rc = sphswe_leg_bt_tables(ctx)) return rc;
 		// Y, ACC, YN (3 nlm each), S4 (4), TH (1) spectral in FP64; operands of
-		// 9 and 4 fields; FP32 Fourier rows of 9 fields; FP32 fields 9 + 4 + 2 + 1
+		// 9 and 4 fields; FP32 Fourier rows of 9 + 4 fields; FP32 fields 9 + 4 + 2 + 1
+		// (the last only for grids without a fused physical stage)
 		sb_b = (sphswe_bt_sb_bytes(ctx, 9 * nmem) + 255) & ~(size_t)255;
 		g_b = (sphswe_bt_g_bytes(ctx, 4 * nmem) + 255) & ~(size_t)255;
-		bytes = (size_t)nlm * nmem * (3 + 3 + 3 + 4 + 1) * 16 + sb_b + g_b + (size_t)nfour * nmem * 9 * 8 +
+		bytes = (size_t)nlm * nmem * (3 + 3 + 3 + 4 + 1) * 16 + sb_b + g_b + (size_t)nfour * nmem * 13 * 8 +
 			(size_t)nspat * nmem * (9 + 4 + 2 + 1) * 4 + 1024;
 	} else {
 		// workspace: Y (3 nlm), ACC (3), YN (3), S9 (9), S4 (4), S2 (2), TH (1) spectral;
@@ -339,7 +579,7 @@ int swe_buffers(sphswe_ctx *ctx, int nmem, int prec, SweBufs &b)
 		b.TH = (double2 *)take((size_t)nlm * nmem * 16);
 		b.SB = take(sb_b);
 		b.G = take(g_b);
-		b.F32 = (float2 *)take((size_t)9 * nfour * nmem * 8);
+		b.F32 = (float2 *)take((size_t)13 * nfour * nmem * 8);   // 9 synthesised + 4 analysed fields
 		b.Q9 = (float *)take((size_t)9 * nspat * nmem * 4);
 		b.Q4 = (float *)take((size_t)4 * nspat * nmem * 4);
 		b.Q2 = (float *)take((size_t)2 * nspat * nmem * 4);
@@ -421,6 +661,39 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 	return SPHSWE_OK;
 }
 
+// the fused physical stage of level LEV for the context's longitude count, or
+// 1 when there is no fused kernel for it (the caller runs the separate kernels)
+template <int LEV>
+int phys_fft_mx(sphswe_ctx *ctx, const float2 *Fin, float2 *Fout, int nm, float ga, float two_omega, float ia,
+		int mode, float h0, float scale)
+{
+	const int nblk = nm * (ctx->nlat / 2);
+	const int nblk_pad = (nblk + 7) / 8 * 8;
+	auto by_n = [&](auto md) -> int {
+		constexpr int M = decltype(md)::value;
+		auto go = [&](auto kern, int nt) -> int {
+			kern<<<nblk_pad, nt, 0, ctx->stream>>>(Fin, Fout, ctx->d_tw32, ctx->d_ct, ctx->d_st, ctx->d_sq,
+							       ctx->nlat, ctx->L, nblk, nblk_pad, ga, two_omega, ia, h0, scale);
+			SPHSWE_LAUNCH_CHECK(ctx);
+			return SPHSWE_OK;
+		};
+		switch (ctx->plan.n) {
+		case 384: return go(k_phys_fft_mx<LEV, M, 384>, FusedPlan<384>::NT);
+		case 1536: return go(k_phys_fft_mx<LEV, M, 1536>, FusedPlan<1536>::NT);
+		}
+		return 1;
+	};
+	if (ctx->nlat % 2 != 0 || 2 * ctx->L + 1 > ctx->plan.n) return 1;
+	if (LEV == 1) return by_n(std::integral_constant<int, 0>{});   // level 2 has one form
+	switch (mode) {
+	case 0: return by_n(std::integral_constant<int, 0>{});
+	case 1: return by_n(std::integral_constant<int, 1>{});
+	case 2: return by_n(std::integral_constant<int, 2>{});
+	default: return by_n(std::integral_constant<int, 3>{});
+	}
+}
+static_assert(FusedPlan<384>::ok && FusedPlan<1536>::ok, "fused plans");
+
 // one RK4 step in the mixed form: the Legendre contractions on the bf16 matrix
 // cores from the stored tables (operands split into bf16 hi + lo), Fourier rows,
 // longitude FFTs and pointwise terms in FP32, the state and the stage
@@ -436,6 +709,7 @@ int swe_step_mixed(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int 
 	const double anext[3] = {0.5, 0.5, 1.0};
 	const int mode = (p->nonlinear == 0 && p->robert_functions) ? 3 : p->nonlinear;
 	const int L = ctx->L, nphi = ctx->nphi;
+	const long nfour = (long)(L + 1) * ctx->nlat;
 	hipStream_t st = ctx->stream;
 	int rc;
 	const double2 *Y = X;
@@ -443,23 +717,35 @@ int swe_step_mixed(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int 
 	for (int s = 0; s < 4; s++) {
 		if ((rc = sphswe_bt_sprep_swe9(ctx, Y, b.SB, nm))) return rc;
 		if ((rc = sphswe_bt_synth32(ctx, b.SB, b.F32, 9 * nm))) return rc;
-		if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q9, 9 * nm))) return rc;
-		k_phys1_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q9, b.Q4, ctx->d_ct, ctx->d_st, ctx->d_sq, nphi, nspat, nm,
-							   (float)ga, (float)two_omega, (float)ia, mode,
-							   (float)(mode == 3 ? p->h0 / a : p->h0));
-		if ((rc = sphswe_fft_r2c32(ctx, b.Q4, b.F32, 4 * nm))) return rc;
-		if ((rc = sphswe_bt_gprep32(ctx, b.F32, b.G, 4 * nm))) return rc;
+		const float h0f = (float)(mode == 3 ? p->h0 / a : p->h0);
+		float2 *F4 = b.F32 + (long)9 * nfour * nm;   // level-1 rows after the 9 synthesised fields
+		rc = phys_fft_mx<0>(ctx, b.F32, F4, nm, (float)ga, (float)two_omega, (float)ia, mode, h0f, 0.f);
+		if (rc == 1) {
+			if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q9, 9 * nm))) return rc;
+			k_phys1_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q9, b.Q4, ctx->d_ct, ctx->d_st, ctx->d_sq, nphi, nspat,
+								   nm, (float)ga, (float)two_omega, (float)ia, mode, h0f);
+			if ((rc = sphswe_fft_r2c32(ctx, b.Q4, F4, 4 * nm))) return rc;
+		} else if (rc) {
+			return rc;
+		}
+		if ((rc = sphswe_bt_gprep32(ctx, F4, b.G, 4 * nm))) return rc;
 		if ((rc = sphswe_bt_anal(ctx, b.G, b.S4, 4 * nm))) return rc;
 		const double2 *TH = b.S4;
 		int th_stride = 4;
 		if (mode != 3) {
 			if ((rc = sphswe_bt_sprep_swe2(ctx, b.S4, b.SB, nm))) return rc;
 			if ((rc = sphswe_bt_synth32(ctx, b.SB, b.F32, 2 * nm))) return rc;
-			if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q2, 2 * nm))) return rc;
-			k_phys2_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q2, b.Q1, ctx->d_st, ctx->d_sq, nphi, nspat, nm,
-								   (float)(p->nonlinear ? ia : p->h0 / a));
-			if ((rc = sphswe_fft_r2c32(ctx, b.Q1, b.F32, nm))) return rc;
-			if ((rc = sphswe_bt_gprep32(ctx, b.F32, b.G, nm))) return rc;
+			const float sc = (float)(p->nonlinear ? ia : p->h0 / a);
+			rc = phys_fft_mx<1>(ctx, b.F32, F4, nm, 0.f, 0.f, 0.f, mode, 0.f, sc);
+			if (rc == 1) {
+				if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q2, 2 * nm))) return rc;
+				k_phys2_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q2, b.Q1, ctx->d_st, ctx->d_sq, nphi, nspat, nm,
+									   sc);
+				if ((rc = sphswe_fft_r2c32(ctx, b.Q1, F4, nm))) return rc;
+			} else if (rc) {
+				return rc;
+			}
+			if ((rc = sphswe_bt_gprep32(ctx, F4, b.G, nm))) return rc;
 			if ((rc = sphswe_bt_anal(ctx, b.G, b.TH, nm))) return rc;
 			TH = b.TH;
 			th_stride = 1;
