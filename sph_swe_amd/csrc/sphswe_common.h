@@ -71,6 +71,7 @@ struct sphswe_ctx {
 	int bt_nkb = 0;               // number of blocks
 	int2 *d_bt_mkb = nullptr;     // (m, 64-degree block) units of the operand split
 	int bt_nmkb = 0;
+	double2 *d_bt_dmu = nullptr;  // (1 - mu^2) d/dmu band per real spectral index: (on l - 1, on l + 1)
 	int bt_tmax = 0;              // most degree tiles of an order pair (m, L - m)
 	size_t bt_bytes = 0;
 	int tuned = 0;

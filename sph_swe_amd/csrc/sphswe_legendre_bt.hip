@@ -54,15 +54,15 @@ __device__ __forceinline__ f4v mfma_hl(uint4 ah, uint4 al, uint4 bh, uint4 bl, f
 	return acc;
 }
 
-// hi and lo bf16 terms of eight values, packed as 16-byte lane operands
+// hi and lo bf16 terms of eight values, packed as 16-byte lane operands; the
+// conversions round to nearest even (v_cvt_pk_bf16_f32, two values each)
 __device__ __forceinline__ void split8(const float (&v)[8], uint4 &hi, uint4 &lo)
 {
-	s8v h, l;
+	bf8v h, l;
 #pragma unroll
 	for (int e = 0; e < 8; e++) {
-		const short a = bf16_rn(v[e]);
-		h[e] = a;
-		l[e] = bf16_rn(v[e] - bf16_f(a));
+		h[e] = (__bf16)v[e];
+		l[e] = (__bf16)(v[e] - (float)h[e]);
 	}
 	hi = __builtin_bit_cast(uint4, h);
 	lo = __builtin_bit_cast(uint4, l);
@@ -122,6 +122,20 @@ template <>
 __device__ __forceinline__ float2 mk2<float2>(float a, float b) { return make_float2(a, b); }
 template <>
 __device__ __forceinline__ double2 mk2<double2>(float a, float b) { return make_double2(a, b); }
+// two consecutive complex values (an even index of a row): one 16-byte store in FP32
+template <typename FT>
+__device__ __forceinline__ void st2(FT *p, float a, float b, float c, float d);
+template <>
+__device__ __forceinline__ void st2<float2>(float2 *p, float a, float b, float c, float d)
+{
+	*(float4 *)p = make_float4(a, b, c, d);
+}
+template <>
+__device__ __forceinline__ void st2<double2>(double2 *p, float a, float b, float c, float d)
+{
+	p[0] = make_double2(a, b);
+	p[1] = make_double2(c, d);
+}
 
 // ---- column operands ---------------------------------------------------------
 // Both contractions take their column operands pre-split into bf16 hi/lo in
@@ -153,6 +167,7 @@ template <int RCP>
 struct Recipe {
 	static constexpr int FO = RCP == BT_RCP_SWE9 ? 9 : RCP == BT_RCP_SWE2 ? 2 : 1;
 	static constexpr int FS = RCP == BT_RCP_SWE9 ? 3 : RCP == BT_RCP_SWE2 ? 4 : 1;
+	static constexpr int FU = RCP == BT_RCP_SWE2 ? 2 : FS;   // the sources used: the first FU of FS
 	__device__ static void of(int q, int &s, int &op)
 	{
 		if (RCP == BT_RCP_SWE9) {
@@ -168,46 +183,78 @@ struct Recipe {
 	}
 };
 
+// A workgroup takes one (order m, 64-degree block) unit and MB members: it
+// copies the source fields' degrees d0 - 1 .. d0 + 64 of the unit into LDS
+// (consecutive lanes, consecutive degrees), then one thread per (output field,
+// 16-degree group) applies the operator from there and writes its lanes.
 template <int RCP>
-__global__ __launch_bounds__(256) void k_bt_sprep(const double2 *__restrict__ spec, uint4 *__restrict__ SB,
-						  const int2 *__restrict__ mkb, const int *__restrict__ kboff, int L, int nmem,
-						  int NCT)
+struct SprepShape {
+	static constexpr int MB = RCP == BT_RCP_SWE9 ? 8 : 16;   // members per workgroup
+	static constexpr int NSRC = MB * Recipe<RCP>::FU;        // source fields staged
+	static constexpr int NOUT = MB * Recipe<RCP>::FO;        // output fields written
+	static constexpr int NT = 4 * NOUT < 64 ? 64 : (4 * NOUT + 63) / 64 * 64;
+	static constexpr int DW = 66;                             // staged degrees per field
+};
+
+template <int RCP>
+__global__ __launch_bounds__(SprepShape<RCP>::NT) void k_bt_sprep(const double2 *__restrict__ spec,
+								   uint4 *__restrict__ SB, const int2 *__restrict__ mkb,
+								   const int *__restrict__ kboff,
+								   const double2 *__restrict__ dmu, int L, int nmem, int NCT)
 {
 	typedef Recipe<RCP> RC;
-	const int t = blockIdx.y * blockDim.x + threadIdx.x;
-	const int fo = t >> 2, g = t & 3;
-	if (fo >= nmem * RC::FO) return;
-	const int mem = fo / RC::FO, qd = fo - mem * RC::FO;
-	int sf, op;
-	RC::of(qd, sf, op);
+	typedef SprepShape<RCP> SH;
+	__shared__ double2 xs[SH::NSRC][SH::DW];
 	const int2 e = mkb[blockIdx.x];
 	const int m = e.x, kb = e.y;
+	const int mem0 = blockIdx.y * SH::MB;
 	const long nlm = (long)(L + 1) * (L + 2) / 2;
-	const double2 *x = spec + (long)(mem * RC::FS + sf) * nlm + lm_index(L, m, m);
-	const int d0 = 64 * kb + 16 * g;   // l - m of the first degree
 	const int dmax = L - m;
+	const int db = 64 * kb - 1;   // l - m of staged degree 0
+	// all loads in flight before the first LDS store
+	constexpr int NIT = (SH::NSRC * SH::DW + SH::NT - 1) / SH::NT;
+	double2 v[NIT];
+#pragma unroll
+	for (int it = 0; it < NIT; it++) {
+		const int i = threadIdx.x + it * SH::NT;
+		const int f = i / SH::DW, dd = i - f * SH::DW;
+		const int d = db + dd;
+		const int ms = f / RC::FU, fs = f - ms * RC::FU;   // member, source of the member
+		const bool ok = i < SH::NSRC * SH::DW && mem0 + ms < nmem && d >= 0 && d <= dmax;
+		v[it] = ok ? spec[(long)((mem0 + ms) * RC::FS + fs) * nlm + lm_index(L, m, m) + d] : make_double2(0.0, 0.0);
+	}
+#pragma unroll
+	for (int it = 0; it < NIT; it++) {
+		const int i = threadIdx.x + it * SH::NT;
+		if (i < SH::NSRC * SH::DW) (&xs[0][0])[i] = v[it];
+	}
+	__syncthreads();
+	const int t = threadIdx.x;
+	const int fl = t >> 2, g = t & 3;   // output field within the workgroup, 16-degree group
+	if (fl >= SH::NOUT) return;
+	const int fo = mem0 * RC::FO + fl;
+	if (fo >= nmem * RC::FO) return;
+	const int ml = fl / RC::FO, qd = fl - ml * RC::FO;
+	int sf, op;
+	RC::of(qd, sf, op);
+	const double2 *x = xs[ml * RC::FU + sf] + 1 + 16 * g;   // x[i] = degree d0 + i, d0 = 64 kb + 16 g
+	const int d0 = 64 * kb + 16 * g;
 	float re[2][8], im[2][8];
 	if (op == BT_OP_DMU) {
-		double2 v[18];   // degrees d0 - 1 .. d0 + 16
-#pragma unroll
-		for (int i = 0; i < 18; i++) {
-			const int d = d0 - 1 + i;
-			v[i] = (d >= 0 && d <= dmax) ? x[d] : make_double2(0.0, 0.0);
-		}
+		const double2 *cf = dmu + lm_index(L, m, m) + d0;   // (cm, cp) of each degree
 #pragma unroll
 		for (int i = 0; i < 16; i++) {
-			const double l = m + d0 + i, dm = m;
-			const double cm = (-l + 1.0) * idR(l - 1, dm), cp = (l + 2.0) * idS(l + 1, dm);
 			const bool ok = d0 + i <= dmax;
-			re[i & 1][i >> 1] = ok ? (float)(cm * v[i].x + cp * v[i + 2].x) : 0.f;
-			im[i & 1][i >> 1] = ok ? (float)(cm * v[i].y + cp * v[i + 2].y) : 0.f;
+			const double2 c = ok ? cf[i] : make_double2(0.0, 0.0);
+			const double2 a = x[i - 1], b = x[i + 1];
+			re[i & 1][i >> 1] = (float)(c.x * a.x + c.y * b.x);
+			im[i & 1][i >> 1] = (float)(c.x * a.y + c.y * b.y);
 		}
 	} else {
 		const double dm = op == BT_OP_IM ? m : 0.0;
 #pragma unroll
 		for (int i = 0; i < 16; i++) {
-			const int d = d0 + i;
-			const double2 v = d <= dmax ? x[d] : make_double2(0.0, 0.0);
+			const double2 v = x[i];   // zero past the order's last degree
 			re[i & 1][i >> 1] = (float)(op == BT_OP_IM ? -dm * v.y : v.x);
 			im[i & 1][i >> 1] = (float)(op == BT_OP_IM ? dm * v.x : v.y);
 		}
@@ -226,51 +273,54 @@ __global__ __launch_bounds__(256) void k_bt_sprep(const double2 *__restrict__ sp
 	}
 }
 
-// one thread per (order, ring block kb, field, 8-ring group g)
+// A workgroup takes (order m, ring block kb) and the 8 fields of one column
+// fragment: one thread per (field, ring) reads both hemispheres' Fourier
+// values (consecutive lanes, consecutive rings) and forms w_j (F_N +- F_S);
+// one thread per (field, 8-ring group, quantity) then gathers its 8 values
+// through LDS and writes their hi and lo lanes.
 template <typename FT>
 __global__ __launch_bounds__(256) void k_bt_gprep(const FT *__restrict__ four, const double *__restrict__ wg,
 						  uint4 *__restrict__ G, int L, int nlat, int NJ, int nf, int NCT)
 {
+	__shared__ float gv[8][4][33];   // [field][er, ei, or, oi][ring]
 	const int NKR = (NJ + 31) >> 5;
 	const int mkb = blockIdx.x;
 	const int m = mkb / NKR;
-	const int t = blockIdx.y * blockDim.x + threadIdx.x;
-	const int f = t >> 2, g = t & 3;
+	const int ct = blockIdx.y;
+	const int tid = threadIdx.x;
+	{
+		const int fl = tid >> 5, r = tid & 31;
+		const int f = ct * 8 + fl;
+		const int j = (mkb - m * NKR) * 32 + r;
+		float er = 0.f, ei = 0.f, orr = 0.f, oi = 0.f;
+		if (f < nf && j < NJ) {
+			const FT *row = four + ((long)f * (L + 1) + m) * nlat;
+			const FT a = row[j], b = row[nlat - 1 - j];
+			const double w = wg[j];
+			er = (float)(w * ((double)a.x + (double)b.x));
+			ei = (float)(w * ((double)a.y + (double)b.y));
+			orr = (float)(w * ((double)a.x - (double)b.x));
+			oi = (float)(w * ((double)a.y - (double)b.y));
+		}
+		gv[fl][0][r] = er;
+		gv[fl][1][r] = ei;
+		gv[fl][2][r] = orr;
+		gv[fl][3][r] = oi;
+	}
+	__syncthreads();
+	if (tid >= 128) return;
+	const int fl = tid >> 4, g = (tid >> 2) & 3, qn = tid & 3;   // quantity 2 p + re/im
+	const int f = ct * 8 + fl;
 	if (f >= nf) return;
-	const FT *row = four + ((long)f * (L + 1) + m) * nlat;
-	const int j0 = (mkb - m * NKR) * 32 + 8 * g;
-	FT a[8], b[8];
-	double w[8];
+	float v[8];
 #pragma unroll
-	for (int i = 0; i < 8; i++) {
-		const int j = j0 + i;
-		const bool ok = j < NJ;
-		a[i] = ok ? row[j] : FT{};
-		b[i] = ok ? row[nlat - 1 - j] : FT{};
-		w[i] = ok ? wg[j] : 0.0;
-	}
-	float er[8], ei[8], orr[8], oi[8];
-#pragma unroll
-	for (int i = 0; i < 8; i++) {
-		er[i] = (float)(w[i] * ((double)a[i].x + (double)b[i].x));
-		ei[i] = (float)(w[i] * ((double)a[i].y + (double)b[i].y));
-		orr[i] = (float)(w[i] * ((double)a[i].x - (double)b[i].x));
-		oi[i] = (float)(w[i] * ((double)a[i].y - (double)b[i].y));
-	}
+	for (int i = 0; i < 8; i++) v[i] = gv[fl][qn][8 * g + i];
 	uint4 h, l;
-	uint4 *o = G + ((long)mkb * NCT + (f >> 3)) * 256 + 2 * (f & 7) + 16 * g;
-	split8(er, h, l);
+	split8(v, h, l);
+	const int p = qn >> 1, ri = qn & 1;
+	uint4 *o = G + (((long)mkb * NCT + ct) * 2 + p) * 128 + 2 * fl + ri + 16 * g;
 	o[0] = h;
 	o[64] = l;
-	split8(ei, h, l);
-	o[1] = h;
-	o[65] = l;
-	split8(orr, h, l);
-	o[128] = h;
-	o[192] = l;
-	split8(oi, h, l);
-	o[129] = h;
-	o[193] = l;
 }
 
 // blocks of one order pair (all its ring and column groups) on one XCD, in
@@ -405,15 +455,24 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 					const int f = (cg * 16 * CT + 16 * c + n) >> 1;
 					if (f >= nf || rt >= NJt) continue;
 					FT *row = four + ((long)f * (L + 1) + m) * nlat;
+					// rings j0 and j0 + 1 (elements 2 odd, 2 odd + 1, selected
+					// without indexing by a lane value); both hemispheres' pairs
+					// are adjacent in the row
+					const int j0 = rt * 16 + 4 * g + 2 * odd;
+					float nr[2], ni[2], sr[2], si[2];
 #pragma unroll
 					for (int h = 0; h < 2; h++) {
-						const int j = rt * 16 + 4 * g + 2 * odd + h;
-						if (j >= NJ) continue;
-						// element 2 odd + h, selected without indexing by a lane value
-						const float nre = odd ? pn[2 + h] : fn[h], nim = odd ? fn[2 + h] : pn[h];
-						const float sre = odd ? ps[2 + h] : fs[h], sim = odd ? fs[2 + h] : ps[h];
-						row[j] = mk2<FT>(nre, nim);
-						row[nlat - 1 - j] = mk2<FT>(sre, sim);
+						nr[h] = odd ? pn[2 + h] : fn[h];
+						ni[h] = odd ? fn[2 + h] : pn[h];
+						sr[h] = odd ? ps[2 + h] : fs[h];
+						si[h] = odd ? fs[2 + h] : ps[h];
+					}
+					if (j0 + 1 < NJ) {
+						st2<FT>(row + j0, nr[0], ni[0], nr[1], ni[1]);
+						st2<FT>(row + nlat - 2 - j0, sr[1], si[1], sr[0], si[0]);
+					} else if (j0 < NJ) {
+						row[j0] = mk2<FT>(nr[0], ni[0]);
+						row[nlat - 1 - j0] = mk2<FT>(sr[0], si[0]);
 					}
 				}
 			}
@@ -649,11 +708,11 @@ template <int RCP>
 int bt_sprep(sphswe_ctx *ctx, const double2 *spec, void *SB, int nmem)
 {
 	if (nmem <= 0) return SPHSWE_OK;
+	typedef SprepShape<RCP> SH;
 	const int nfo = nmem * Recipe<RCP>::FO;
-	const int nt = std::min(256, (4 * nfo + 63) / 64 * 64);
-	dim3 grid(ctx->bt_nmkb, (4 * nfo + nt - 1) / nt);
-	k_bt_sprep<RCP><<<grid, nt, 0, ctx->stream>>>(spec, (uint4 *)SB, ctx->d_bt_mkb, ctx->d_bt_kboff, ctx->L, nmem,
-						      (nfo + 7) / 8);
+	dim3 grid(ctx->bt_nmkb, (nmem + SH::MB - 1) / SH::MB);
+	k_bt_sprep<RCP><<<grid, SH::NT, 0, ctx->stream>>>(spec, (uint4 *)SB, ctx->d_bt_mkb, ctx->d_bt_kboff, ctx->d_bt_dmu,
+							  ctx->L, nmem, (nfo + 7) / 8);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }
@@ -674,10 +733,9 @@ template <typename FT>
 int bt_gprep(sphswe_ctx *ctx, const FT *four, void *G, int nf)
 {
 	if (nf <= 0) return SPHSWE_OK;
-	const int nt = std::min(256, (4 * nf + 63) / 64 * 64);
-	dim3 grid((ctx->L + 1) * ((ctx->NJ + 31) / 32), (4 * nf + nt - 1) / nt);
-	k_bt_gprep<FT><<<grid, nt, 0, ctx->stream>>>(four, ctx->d_wg, (uint4 *)G, ctx->L, ctx->nlat, ctx->NJ, nf,
-						     (nf + 7) / 8);
+	const int NCT = (nf + 7) / 8;
+	dim3 grid((ctx->L + 1) * ((ctx->NJ + 31) / 32), NCT);
+	k_bt_gprep<FT><<<grid, 256, 0, ctx->stream>>>(four, ctx->d_wg, (uint4 *)G, ctx->L, ctx->nlat, ctx->NJ, nf, NCT);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }
@@ -690,8 +748,7 @@ int bt_synth_split(sphswe_ctx *ctx, const void *SB, FT *four, int nf)
 	if (nf <= 0) return SPHSWE_OK;
 	const int NCT = (nf + 7) / 8;
 	const uint4 *sb = (const uint4 *)SB;
-	static const int cts = getenv("SPHSWE_BT_CTS") ? atoi(getenv("SPHSWE_BT_CTS")) : BT_CTS;   // TEMP
-	switch (bt_ct(NCT, cts)) {
+	switch (bt_ct(NCT, BT_CTS)) {
 	case 1: return launch_bt_synth<1, FT>(ctx, sb, four, nf, NCT);
 	case 2: return launch_bt_synth<2, FT>(ctx, sb, four, nf, NCT);
 	case 3: return launch_bt_synth<3, FT>(ctx, sb, four, nf, NCT);
@@ -710,8 +767,7 @@ int sphswe_bt_anal(sphswe_ctx *ctx, const void *Gv, double2 *spec, int nf)
 	const uint4 *G = (const uint4 *)Gv;
 	// (tiles per wavefront, wavefronts): every degree tile of the fullest order pair
 	const int tm = ctx->bt_tmax;
-	static const int cta = getenv("SPHSWE_BT_CTA") ? atoi(getenv("SPHSWE_BT_CTA")) : BT_CTA;   // TEMP
-	const int ct = bt_ct(NCT, cta);
+	const int ct = bt_ct(NCT, BT_CTA);
 	if (tm <= 16) return ct == 1 ? launch_bt_anal<1, 1, 16>(ctx, G, spec, nf, NCT) : launch_bt_anal<2, 1, 16>(ctx, G, spec, nf, NCT);
 	if (tm <= 32) return ct == 1 ? launch_bt_anal<1, 2, 16>(ctx, G, spec, nf, NCT) : launch_bt_anal<2, 2, 16>(ctx, G, spec, nf, NCT);
 	if (tm <= 36) return ct == 1 ? launch_bt_anal<1, 3, 12>(ctx, G, spec, nf, NCT) : launch_bt_anal<2, 3, 12>(ctx, G, spec, nf, NCT);
@@ -757,6 +813,12 @@ int sphswe_leg_bt_tables(sphswe_ctx *ctx)
 			kbo[2 * m + p] = (int)kbl.size();
 			for (int kb = 0; kb < (bt_np(L, m, p) + 31) / 32; kb++) kbl.push_back(make_int2(2 * m + p, kb));
 		}
+	// the band of D' = (1 - mu^2) d/dmu per degree: (1 - l) R(l - 1, m) on x_{l-1}
+	// and (l + 2) S(l + 1, m) on x_{l+1} (SPHOperators.hpp:234-260)
+	std::vector<double2> dmu((size_t)(L + 1) * (L + 2) / 2);
+	for (int m = 0; m <= L; m++)
+		for (int l = m; l <= L; l++)
+			dmu[lm_index(L, l, m)] = make_double2((-(double)l + 1.0) * idR(l - 1, m), ((double)l + 2.0) * idS(l + 1, m));
 	// the (m, 64-degree block) units of the operand split
 	std::vector<int2> mkb;
 	for (int m = 0; m <= L; m++)
@@ -764,8 +826,9 @@ int sphswe_leg_bt_tables(sphswe_ctx *ctx)
 	uint4 *s = nullptr, *a = nullptr;
 	int *dso = nullptr, *dao = nullptr, *dkbo = nullptr;
 	int2 *dkbl = nullptr, *dmkb = nullptr;
+	double2 *ddmu = nullptr;
 	auto fail = [&](const char *what) {
-		for (void *p : {(void *)s, (void *)a, (void *)dso, (void *)dao, (void *)dkbo, (void *)dkbl, (void *)dmkb})
+		for (void *p : {(void *)s, (void *)a, (void *)dso, (void *)dao, (void *)dkbo, (void *)dkbl, (void *)dmkb, (void *)ddmu})
 			if (p) hipFree(p);
 		return sphswe_set_err(ctx, SPHSWE_ERR_NOMEM, "bf16 Legendre tables: %s", what);
 	};
@@ -776,11 +839,13 @@ int sphswe_leg_bt_tables(sphswe_ctx *ctx)
 	if (hipMalloc((void **)&dkbo, sizeof(int) * kbo.size()) != hipSuccess) return fail("allocation");
 	if (hipMalloc((void **)&dkbl, sizeof(int2) * kbl.size()) != hipSuccess) return fail("allocation");
 	if (hipMalloc((void **)&dmkb, sizeof(int2) * mkb.size()) != hipSuccess) return fail("allocation");
+	if (hipMalloc((void **)&ddmu, sizeof(double2) * dmu.size()) != hipSuccess) return fail("allocation");
 	if (hipMemcpy(dso, so.data(), sizeof(int) * so.size(), hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemcpy(dao, ao.data(), sizeof(int) * ao.size(), hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemcpy(dkbo, kbo.data(), sizeof(int) * kbo.size(), hipMemcpyHostToDevice) != hipSuccess ||
 	    hipMemcpy(dkbl, kbl.data(), sizeof(int2) * kbl.size(), hipMemcpyHostToDevice) != hipSuccess ||
-	    hipMemcpy(dmkb, mkb.data(), sizeof(int2) * mkb.size(), hipMemcpyHostToDevice) != hipSuccess)
+	    hipMemcpy(dmkb, mkb.data(), sizeof(int2) * mkb.size(), hipMemcpyHostToDevice) != hipSuccess ||
+	    hipMemcpy(ddmu, dmu.data(), sizeof(double2) * dmu.size(), hipMemcpyHostToDevice) != hipSuccess)
 		return fail("upload");
 	if (hipMemsetAsync(s, 0, (size_t)ns * 2048, ctx->stream) != hipSuccess ||
 	    hipMemsetAsync(a, 0, (size_t)na * 2048, ctx->stream) != hipSuccess)
@@ -797,6 +862,7 @@ int sphswe_leg_bt_tables(sphswe_ctx *ctx)
 	ctx->bt_nkb = (int)kbl.size();
 	ctx->d_bt_mkb = dmkb;
 	ctx->bt_nmkb = (int)mkb.size();
+	ctx->d_bt_dmu = ddmu;
 	ctx->bt_tmax = tmax;
 	ctx->bt_bytes = (size_t)(ns + na) * 2048;
 	return SPHSWE_OK;
@@ -806,9 +872,10 @@ void sphswe_leg_bt_tables_free(sphswe_ctx *ctx)
 {
 	ctx->buf_gen++;   // a captured step may read the tables
 	for (void *p : {(void *)ctx->d_bt_s, (void *)ctx->d_bt_a, (void *)ctx->d_bt_soff, (void *)ctx->d_bt_aoff,
-			(void *)ctx->d_bt_kboff, (void *)ctx->d_bt_kbl, (void *)ctx->d_bt_mkb})
+			(void *)ctx->d_bt_kboff, (void *)ctx->d_bt_kbl, (void *)ctx->d_bt_mkb, (void *)ctx->d_bt_dmu})
 		if (p) hipFree(p);
 	ctx->d_bt_mkb = nullptr;
+	ctx->d_bt_dmu = nullptr;
 	ctx->bt_nmkb = 0;
 	ctx->d_bt_s = ctx->d_bt_a = nullptr;
 	ctx->d_bt_soff = ctx->d_bt_aoff = ctx->d_bt_kboff = nullptr;
