@@ -5,8 +5,10 @@
 A step is one SHT round trip (synthesis + analysis) of 3 fields (phi, u, v) at
 T511 on the 768 x 1536 Gaussian grid, FP64, spectral inputs already resident
 in HBM, seeded coefficients with power ~ l^-3.  The JSON line also carries the
-same round trip with the Legendre contractions in FP32, BF16 (hi/lo) and BF16
-(hi/mid/lo) with per-field errors, the nonlinear SWE RK4 rate at T511
+same round trip with the Legendre contractions in FP32, BF16 (hi/lo), BF16
+(hi/mid/lo) and the mixed form (bf16 hi/lo products from stored tables; in the
+SWE stepper also FP32 Fourier rows, FFTs and pointwise terms) with per-field
+errors, the nonlinear SWE RK4 rate at T511
 (Gaussian dam, dt = 0.001 a / nlat = 8.296 s) in FP64 and in each reduced mode
 with the geopotential error after 100 steps,
 the linear REXI rate at T256 (200 poles, dt = 3600 s, seeded (phi, vort, div)
@@ -109,6 +111,41 @@ def bench_legendre_kernel(ctx, reps=20):
     return ms, ms_a, flops
 
 
+def bench_bt_contractions(ctx, reps=10):
+    """The mixed form's Legendre contractions at the batched T511 widths
+    (144-field synthesis, 64-field analysis: 16 members at level 0 and 1),
+    through the FP64-row entry points: the operand split and the GEMM on the
+    stored bf16 hi/lo tables, average time with HIP events.  achieved counts the
+    bf16 MFMA work issued, three products per FP64-equivalent multiply-add
+    (2 * 3 * nlm * nlat/2 * 2 * nf), against the 2.5 PFLOP/s dense bf16 peak."""
+    lib, h = ctx.lib, ctx.h
+    out = {}
+    for name, nf, synth in (("synthesis_144", 144, True), ("analysis_64", 64, False)):
+        spec = ctx.empty(nf * ctx.nlm * 16)
+        four = ctx.empty(nf * (ctx.L + 1) * ctx.nlat * 16)
+        ctx._check(lib.sphswe_fill_zero(h, spec.ptr, nf * ctx.nlm * 16))
+        ctx._check(lib.sphswe_fill_zero(h, four.ptr, nf * (ctx.L + 1) * ctx.nlat * 16))
+        if synth:
+            fn = lambda: lib.sphswe_legendre_synthesis(h, spec.ptr, four.ptr, nf, capi.PREC_MIXED)  # noqa: E731
+        else:
+            fn = lambda: lib.sphswe_legendre_analysis(h, four.ptr, spec.ptr, nf, capi.PREC_MIXED)  # noqa: E731
+        for _ in range(3):
+            ctx._check(fn())
+        ctx.mark(2)
+        for _ in range(reps):
+            ctx._check(fn())
+        ctx.mark(3)
+        ms = ctx.elapsed_ms(2, 3) / reps
+        fl = 4.0 * ctx.nlm * (ctx.nlat // 2) * nf
+        out[name] = {"avg_ms": ms, "fp64_equivalent_tflops": fl / (ms * 1e-3) / 1e12,
+                     "achieved": 3 * fl / (ms * 1e-3) / 1e12, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": 3 * fl / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS}
+        spec.free()
+        four.free()
+    return dict(out, engine="bf16 matrix cores (v_mfma_f32_16x16x32_bf16), hi/lo operands, 3 products",
+                includes="the operand split pass from FP64 spectra / Fourier rows, FP64 Fourier rows")
+
+
 def bench_fft_kernels(ctx, nf=9, reps=20):
     """Longitude FFTs of nf fields at the context's grid (the reference's
     [i * nlat + j] layout, through the C ABI): average launch time with HIP
@@ -194,7 +231,8 @@ def bench_rexi(steps, L=256, M=188):
     return steps / (ms / 1000.0), cfg
 
 
-MODES = ("fp32", "bf16", "bf16x3")
+MODES = ("fp32", "bf16", "bf16x3", "mixed")
+BF16_PEAK_TFLOPS = 2500.0
 
 # REXI precision forms: (precision of the per-pole gradient operator, storage
 # precision of the band LU factors)
@@ -656,6 +694,7 @@ def main():
     achieved = flops / (ms_syn * 1e-3) / 1e12
     achieved_ana = flops / (ms_ana * 1e-3) / 1e12
     mixed_sht = bench_sht_modes(ctx, max(args.steps // 4, 10), 3)
+    bt = bench_bt_contractions(ctx)
     mixed_swe = bench_swe_modes(ctx) if args.swe_steps > 0 else None
     batched_swe = (bench_swe_batched(ctx, nmem=args.batched_members)
                    if args.swe_steps > 0 and args.batched_members > 0 else None)
@@ -701,6 +740,7 @@ def main():
                      "legendre_tuning": ctx.legendre_tuning(),
                      "synthesis": {"kernel": "k_leg_synth (FP64 VALU)", "avg_ms": ms_syn, "achieved": achieved,
                                    "frac": achieved / FP64_PEAK_TFLOPS},
+                     "bf16_tables_T511": bt,
                      "fft_T511_9fields": dict(fft, bound="hbm", kernel="k_fft_wide (reference layout)",
                                               traffic={"r2c": measured_traffic(FFT_TRAFFIC_FILE, "r2c_bytes_per_launch"),
                                                        "c2r": measured_traffic(FFT_TRAFFIC_FILE, "c2r_bytes_per_launch")})},
