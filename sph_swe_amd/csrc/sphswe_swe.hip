@@ -167,15 +167,16 @@ __global__ void k_phys1_mx(const float *__restrict__ P9, float *__restrict__ P4,
 	float H = p[0], U = p[nspat], V = p[2 * nspat];
 	float Hl = p[3 * nspat], Hm = p[4 * nspat];
 	float Ul = p[5 * nspat], Um = p[6 * nspat], Vl = p[7 * nspat], Vm = p[8 * nspat];
-	const float c = (float)sq[j], mu = (float)ct[j], sj = (float)st[j];
+	// the metric divisions as products with reciprocals rounded once from FP64
+	const float ic = (float)(1.0 / sq[j]), mu = (float)ct[j], sj = (float)st[j];
 	float *o = P4 + (long)mem * 4 * nspat + k;
 	if (nonlinear == 1) {
 		o[0] = U * H;
 		o[nspat] = (V * H) * sj;
-		o[2 * nspat] = -ga * (Hl / c) + two_omega * (V * mu) - ((Ul / c) * U + (Um / c) * V) * ia;
-		o[3 * nspat] = -ga * (Hm / c) - two_omega * (U * mu) - ((Vl / c) * U + (Vm / c) * V) * ia;
+		o[2 * nspat] = -ga * (Hl * ic) + two_omega * (V * mu) - ((Ul * ic) * U + (Um * ic) * V) * ia;
+		o[3 * nspat] = -ga * (Hm * ic) - two_omega * (U * mu) - ((Vl * ic) * U + (Vm * ic) * V) * ia;
 	} else if (nonlinear == 3) {
-		o[0] = -((Ul + Vm) / (sj * sj)) * h0;
+		o[0] = -((Ul + Vm) * (float)(1.0 / (st[j] * st[j]))) * h0;
 		o[nspat] = 0.f;
 		o[2 * nspat] = -ga * Hl + two_omega * (V * mu);
 		o[3 * nspat] = -ga * Hm - two_omega * (U * mu);
@@ -183,13 +184,13 @@ __global__ void k_phys1_mx(const float *__restrict__ P9, float *__restrict__ P4,
 		const float Hp = H - h0;
 		o[0] = U * Hp;
 		o[nspat] = (V * Hp) * sj;
-		o[2 * nspat] = -((Ul / c) * U + (Um / c) * V) * ia;
-		o[3 * nspat] = -((Vl / c) * U + (Vm / c) * V) * ia;
+		o[2 * nspat] = -((Ul * ic) * U + (Um * ic) * V) * ia;
+		o[3 * nspat] = -((Vl * ic) * U + (Vm * ic) * V) * ia;
 	} else {
 		o[0] = U;
 		o[nspat] = V * sj;
-		o[2 * nspat] = -ga * (Hl / c) + two_omega * (V * mu);
-		o[3 * nspat] = -ga * (Hm / c) - two_omega * (U * mu);
+		o[2 * nspat] = -ga * (Hl * ic) + two_omega * (V * mu);
+		o[3 * nspat] = -ga * (Hm * ic) - two_omega * (U * mu);
 	}
 }
 
@@ -202,8 +203,8 @@ __global__ void k_phys2_mx(const float *__restrict__ P2, float *__restrict__ P1,
 	long k = i - (long)mem * nspat;
 	int j = (int)(k / nphi);
 	const float *p = P2 + (long)mem * 2 * nspat + k;
-	const float c = (float)sq[j];
-	P1[(long)mem * nspat + k] = -(p[0] / c + (p[nspat] / c) / (float)st[j]) * scale;
+	const float ic = (float)(1.0 / sq[j]);
+	P1[(long)mem * nspat + k] = -(p[0] * ic + (p[nspat] * ic) * (float)(1.0 / st[j])) * scale;
 }
 
 // The mixed form's physical stages without physical fields in HBM: one
@@ -272,19 +273,23 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	//   mode 3  O0 = -(Ul + Vm) h0 / st^2, O1 = 0, O2 = -ga Hl + 2 Omega V mu,
 	//           O3 = -ga Hm - 2 Omega U mu;
 	// level 2: O0 = -(P0/c + P1/(c st)) scale.
-	float rc_[2], rmu[2], rst[2];
+	// per ring: 1 / sqrt(1 - mu^2), mu, cos phi and the reciprocals of cos phi
+	// and cos^2 phi, rounded once from FP64 (the terms multiply by them)
+	float ric[2], rmu[2], rst[2], ris[2], ris2[2];
 #pragma unroll
 	for (int r = 0; r < 2; r++) {
-		rc_[r] = (float)sq[j + r];
+		ric[r] = (float)(1.0 / sq[j + r]);
 		rmu[r] = (float)ct[j + r];
 		rst[r] = (float)st[j + r];
+		ris[r] = (float)(1.0 / st[j + r]);
+		ris2[r] = (float)(1.0 / (st[j + r] * st[j + r]));
 	}
 	float Hs[NV][2], Us[NV][2], Vs[NV][2], O[NOUT][NV][2];
 	auto fold = [&](int q, int e, int r, float x) __attribute__((always_inline)) {
-		const float c = rc_[r], mu = rmu[r], sj = rst[r];
+		const float ic = ric[r], mu = rmu[r], sj = rst[r];
 		if (LEV == 1) {
-			if (q == 0) O[0][e][r] = x / c;
-			else O[0][e][r] = -(O[0][e][r] + (x / c) / sj) * scale;
+			if (q == 0) O[0][e][r] = x * ic;
+			else O[0][e][r] = -(O[0][e][r] + (x * ic) * ris[r]) * scale;
 			return;
 		}
 		switch (q) {
@@ -300,24 +305,24 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 			O[3][e][r] = mode == 2 ? 0.f : -(two_omega * (Us[e][r] * mu));
 			break;
 		case 3:
-			if (mode != 2) O[2][e][r] = -ga * (mode == 3 ? x : x / c) + O[2][e][r];
+			if (mode != 2) O[2][e][r] = -ga * (mode == 3 ? x : x * ic) + O[2][e][r];
 			break;
 		case 4:
-			if (mode != 2) O[3][e][r] = -ga * (mode == 3 ? x : x / c) + O[3][e][r];
+			if (mode != 2) O[3][e][r] = -ga * (mode == 3 ? x : x * ic) + O[3][e][r];
 			break;
 		case 5:
 			if (mode == 3) O[0][e][r] = x;
-			else if (mode != 0) O[2][e][r] -= ((x / c) * Us[e][r]) * ia;
+			else if (mode != 0) O[2][e][r] -= ((x * ic) * Us[e][r]) * ia;
 			break;
 		case 6:
-			if (mode == 1 || mode == 2) O[2][e][r] -= ((x / c) * Vs[e][r]) * ia;
+			if (mode == 1 || mode == 2) O[2][e][r] -= ((x * ic) * Vs[e][r]) * ia;
 			break;
 		case 7:
-			if (mode == 1 || mode == 2) O[3][e][r] -= ((x / c) * Us[e][r]) * ia;
+			if (mode == 1 || mode == 2) O[3][e][r] -= ((x * ic) * Us[e][r]) * ia;
 			break;
 		case 8:
-			if (mode == 3) O[0][e][r] = -((O[0][e][r] + x) / (sj * sj)) * h0;
-			else if (mode != 0) O[3][e][r] -= ((x / c) * Vs[e][r]) * ia;
+			if (mode == 3) O[0][e][r] = -((O[0][e][r] + x) * ris2[r]) * h0;
+			else if (mode != 0) O[3][e][r] -= ((x * ic) * Vs[e][r]) * ia;
 			break;
 		}
 	};
