@@ -339,6 +339,49 @@ __device__ __forceinline__ void pass_ip(C *__restrict__ buf, const C *__restrict
 	__syncthreads();
 }
 
+// the same pass from one LDS buffer into another: one barrier per pass, twice
+// the LDS of the in-place form
+template <int N, int NSEQ, int NT, int R, int Ns, int SIGN, typename C>
+__device__ __forceinline__ void pass_pp(const C *__restrict__ src, C *__restrict__ dst, const C *__restrict__ tw)
+{
+	constexpr int NB = N / R, TOT = NSEQ * NB, TWS = N / (Ns * R);
+	constexpr int IT = (TOT + NT - 1) / NT;
+#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int idx = (int)threadIdx.x + it * NT;
+		if (TOT % NT != 0 && idx >= TOT) break;
+		const int sq = idx / NB, b = idx % NB;
+		const int k = b % Ns, jb = b / Ns;
+		C v[R];
+#pragma unroll
+		for (int t = 0; t < R; t++) v[t] = src[WPad<N, NSEQ>::at(sq, b + t * NB)];
+		if (Ns > 1) {
+#pragma unroll
+			for (int t = 1; t < R; t++) {
+				C w = tw[t * k * TWS];
+				if (SIGN > 0) w.y = -w.y;
+				v[t] = cmul(v[t], w);
+			}
+		}
+		Dft<R>::run(v, SIGN);
+		const int o = jb * Ns * R + k;
+#pragma unroll
+		for (int t = 0; t < R; t++) dst[WPad<N, NSEQ>::at(sq, o + t * Ns)] = v[t];
+	}
+	__syncthreads();
+}
+
+// passes P .. PEND - 1 alternating between buffers a and b, starting from a;
+// the result is in a when PEND - P is even, in b otherwise
+template <int N, int NSEQ, int NT, int SIGN, int P, int PEND, typename C, int RULE = 0>
+__device__ __forceinline__ void passes_pp_range(C *a, C *b, const C *__restrict__ tw)
+{
+	if constexpr (P < PEND) {
+		pass_pp<N, NSEQ, NT, CPlan<N, RULE>::radix(P), CPlan<N, RULE>::ns(P), SIGN>(a, b, tw);
+		passes_pp_range<N, NSEQ, NT, SIGN, P + 1, PEND, C, RULE>(b, a, tw);
+	}
+}
+
 template <int N, int NSEQ, int NT, int SIGN, int P, typename C>
 __device__ __forceinline__ void passes_ip(C *buf, const C *__restrict__ tw)
 {

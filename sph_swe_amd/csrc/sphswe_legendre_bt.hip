@@ -116,6 +116,13 @@ __global__ __launch_bounds__(64) void k_bt_gen(const double2 *__restrict__ coef,
 	}
 }
 
+// the value of lane n ^ 1 (the other lane of a (re, im) column pair): a DPP
+// quad permute [1, 0, 3, 2], without a trip through LDS
+__device__ __forceinline__ float xor1(float v)
+{
+	return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xb1, 0xf, 0xf, false));
+}
+
 template <typename FT>
 __device__ __forceinline__ FT mk2(float a, float b);
 template <>
@@ -345,7 +352,7 @@ __device__ __forceinline__ int bt_block(int nblk)
 // of an order the odd-parity wavefronts hand their sums to the even-parity
 // ones through the (drained) LDS buffers, which form F_N = E_0 + E_1 and
 // F_S = E_0 - E_1 and write both hemispheres.
-template <int CT, typename FT>
+template <int CT, typename FT, int DA = 2>
 __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ TS, const int *__restrict__ soff,
 						     const int *__restrict__ kboff, const uint4 *__restrict__ SB,
 						     FT *__restrict__ four, int L, int nlat, int NJ, int nf, int NCT, int nrg,
@@ -449,8 +456,8 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 					}
 #pragma unroll
 					for (int e = 0; e < 4; e++) {
-						pn[e] = __shfl_xor(fn[e], 1);
-						ps[e] = __shfl_xor(fs[e], 1);
+						pn[e] = xor1(fn[e]);
+						ps[e] = xor1(fs[e]);
 					}
 					const int f = (cg * 16 * CT + 16 * c + n) >> 1;
 					if (f >= nf || rt >= NJt) continue;
@@ -486,17 +493,20 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 	uint4 xa[4][2], ya[4][2];
 	uint4 sv[NSTG];
 	sload(0, sv);
-	aload(0, xa);
+	if (DA == 2) aload(0, xa);
 	sstore(0, sv);
 	__syncthreads();
-	auto step = [&](int t, const uint4 (&a)[4][2], uint4 (&na)[4][2]) {
+	// DA = 2: the table fragments of step t + 1 are loaded during step t;
+	// DA = 1: at the start of step t (fewer registers, more workgroups per CU)
+	auto step = [&](int t, uint4 (&a)[4][2], uint4 (&na)[4][2]) {
 		const bool o = t >= T0;
 		const int kb = o ? t - T0 : t;
 		const int buf = t & 1;
 		const bool more = t + 1 < T;
+		if (DA == 1) aload(t, a);
 		if (more) {
 			sload(t + 1, sv);
-			aload(t + 1, na);
+			if (DA == 2) aload(t + 1, na);
 		}
 		if (kb < (o ? O1.nkw : O0.nkw)) {
 #pragma unroll
@@ -645,7 +655,7 @@ __global__ __launch_bounds__(64 * AW) void k_bt_anal(const uint4 *__restrict__ T
 #pragma unroll
 			for (int e = 0; e < 4; e++) v[e] = acc[s][c][e];
 #pragma unroll
-			for (int e = 0; e < 4; e++) pv[e] = __shfl_xor(v[e], 1);
+			for (int e = 0; e < 4; e++) pv[e] = xor1(v[e]);
 			const int f = (cg * 16 * CT + 16 * c + n) >> 1;
 			if (f >= nf) continue;
 			double2 *qo = spec + (long)f * nlm + lm_index(L, m, m) + p;
@@ -684,13 +694,13 @@ int launch_bt_synth(sphswe_ctx *ctx, const uint4 *SB, FT *four, int nf, int NCT)
 	return SPHSWE_OK;
 }
 
-template <int CT, int RA, int AW>
+template <int CT, int RA, int AW, int D = BT_DA>
 int launch_bt_anal(sphswe_ctx *ctx, const uint4 *G, double2 *spec, int nf, int NCT)
 {
 	const int ncg = (NCT + CT - 1) / CT;
 	const int nblk = ((ctx->L + 2) / 2) * ncg;
-	k_bt_anal<CT, RA, AW, BT_DA><<<(nblk + 7) / 8 * 8, 64 * AW, 0, ctx->stream>>>(ctx->d_bt_a, ctx->d_bt_aoff, G, spec,
-										       ctx->L, ctx->NJ, nf, NCT, ncg);
+	k_bt_anal<CT, RA, AW, D><<<(nblk + 7) / 8 * 8, 64 * AW, 0, ctx->stream>>>(ctx->d_bt_a, ctx->d_bt_aoff, G, spec,
+										   ctx->L, ctx->NJ, nf, NCT, ncg);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }

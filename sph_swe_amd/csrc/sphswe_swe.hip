@@ -250,7 +250,10 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	typedef WPad<N, FO> WO;
 	constexpr int NT = FP::NT, NP = FP::NP, RL = FP::RL, NSL = FP::NSL, NW = FP::NW, NV = FP::NV;
 	constexpr int SMN = FI * WI::len > FO * WO::len ? FI * WI::len : FO * WO::len;
-	__shared__ float2 sm[SMN];
+	// two buffers: the passes alternate between them (one barrier each); the
+	// results of NP - 1 passes from sm are in smr
+	__shared__ float2 sm[SMN], sm2[SMN];
+	float2 *const smr = (NP - 1) % 2 ? sm2 : sm;
 	// the twiddles in LDS: read there, they are not hoisted out of the
 	// unrolled field loop and kept in registers
 	__shared__ float2 tws[N];
@@ -366,7 +369,7 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 		}
 		if (q0 + FI < NIN) fetch(q0 + FI);
 		__syncthreads();
-		passes_ip_range<N, FI, NT, +1, 0, NP - 1, float2, FP::RULE>(sm, tws);
+		passes_pp_range<N, FI, NT, +1, 0, NP - 1, float2, FP::RULE>(sm, sm2, tws);
 		// last pass into registers, folded into the outputs
 #pragma unroll
 		for (int f = 0; f < FI; f++) {
@@ -376,7 +379,7 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 				const int k = tid + w * NT;
 				float2 v[RL];
 #pragma unroll
-				for (int t = 0; t < RL; t++) v[t] = sm[WI::at(f, k + t * NSL)];
+				for (int t = 0; t < RL; t++) v[t] = smr[WI::at(f, k + t * NSL)];
 #pragma unroll
 				for (int t = 1; t < RL; t++) {
 					float2 tw = tws[t * k];
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 				sm[WO::at(o, tid + w * NT + t * NSL)] =
 					make_float2(O[o0 + o][w * RL + t][0], O[o0 + o][w * RL + t][1]);
 	__syncthreads();
-	passes_ip_range<N, FO, NT, -1, 0, NP - 1, float2, FP::RULE>(sm, tws);
+	passes_pp_range<N, FO, NT, -1, 0, NP - 1, float2, FP::RULE>(sm, sm2, tws);
 	for (int it = tid; it < FO * NH; it += NT) {
 		const int o = it / NH, pp = it - o * NH;
 		float2 *dst = Fout + (long)(mem * NOUT + o0 + o) * frow;
@@ -418,8 +421,8 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 		float2 z1[RL], z2[RL];
 #pragma unroll
 		for (int t = 0; t < RL; t++) {
-			z1[t] = sm[WO::at(o, k1 + t * NSL)];
-			z2[t] = sm[WO::at(o, k2 + t * NSL)];
+			z1[t] = smr[WO::at(o, k1 + t * NSL)];
+			z2[t] = smr[WO::at(o, k2 + t * NSL)];
 		}
 #pragma unroll
 		for (int t = 1; t < RL; t++) {
