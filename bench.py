@@ -551,6 +551,36 @@ def pmc_kernels():
     print("pmc kernels done", flush=True)
 
 
+def mixed_swe_kernels(nmem=16, nsteps=3):
+    """Trace and counter workload of the mixed form: nmem T511 members (the
+    batched leg's Gaussian dams) stepped eagerly (no captured graph) for one
+    warm-up and nsteps RK4 steps in SPHSWE_PREC_MIXED, then 5 launches of the
+    copy-shaped calibration kernel (k_axpby, 64 MiB read, 64 MiB written)."""
+    ctx = capi.Context(L_BENCH)
+    dt = 0.0005 * 6.37122e6 / ctx.nlat
+    rng = np.random.default_rng(4242)
+    lon = np.arange(ctx.nphi) / ctx.nphi * 2 * np.pi
+    x0 = []
+    for _ in range(nmem):
+        h = gaussian_dam(lon, ctx.ct, lat_c=np.pi / 4 + rng.uniform(-0.2, 0.2), lon_c=np.pi + rng.uniform(-0.2, 0.2),
+                         amp=rng.uniform(300, 700))
+        z = np.zeros_like(h)
+        x0.append(ctx.analysis(np.stack([h, z, z])))
+    state = ctx.to_device(np.concatenate(x0))
+    prm = capi.swe_params(dt=dt, precision=capi.PREC_MIXED)
+    ctx.swe_rk4(prm, state, nmem, 1)
+    ctx.sync()
+    ctx.swe_rk4(prm, state, nmem, nsteps)
+    ncopy = 1 << 22
+    src = ctx.to_device(np.ones(ncopy, np.complex128))
+    dst = ctx.empty(ncopy * 16)
+    for _ in range(5):
+        ctx._check(ctx.lib.sphswe_spec_axpby(ctx.h, ncopy, 1.0, 0.0, src.ptr, 0.0, 0.0, None, dst.ptr))
+    ctx.sync()
+    ctx.close()
+    print("mixed swe kernels done", flush=True)
+
+
 def measured_traffic(path=TRAFFIC_FILE, key="bytes_per_launch"):
     """HBM bytes per launch from a committed counter summary (FETCH_SIZE and
     WRITE_SIZE scaled by the copy calibration), or None."""
@@ -672,6 +702,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in launch shapes)")
     ap.add_argument("--pmc-kernels", action="store_true",
                     help="only launch the kernels used for counter collection")
+    ap.add_argument("--mixed-swe-kernels", action="store_true",
+                    help="only step the 16-member T511 mixed form eagerly (kernel trace and counter workload)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
@@ -679,6 +711,9 @@ def main():
         sys.exit(2)
     if args.pmc_kernels:
         pmc_kernels()
+        return
+    if args.mixed_swe_kernels:
+        mixed_swe_kernels()
         return
 
     ctx = capi.Context(L_BENCH)
