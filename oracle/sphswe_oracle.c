@@ -180,9 +180,12 @@ ocfg *orc_create(int L, int nlat, int nphi)
 	return c;
 }
 
+static void fdata_release(ocfg *c);
+
 void orc_destroy(ocfg *c)
 {
 	if (!c) return;
+	fdata_release(c);
 	free(c->ct); free(c->st); free(c->w); free(c->lat); free(c->tw);
 	free(c->ptab); free(c->pofs);
 	free(c);
@@ -989,6 +992,17 @@ static void make_fdata(ocfg *c)
 	for (int i = 0; i < c->nphi; i++)
 		for (int j = 0; j < c->nlat; j++) g_fdata->spat[(size_t)i * c->nlat + j] = c->ct[j];
 	g_fdata->spat_valid = 1;
+}
+
+/* the cached mu field belongs to one configuration: drop it with that
+ * configuration, so that a later one allocated at the same address never
+ * finds a field of another grid */
+static void fdata_release(ocfg *c)
+{
+	if (g_fdata && g_fdata->c == c) {
+		field_free(g_fdata);
+		g_fdata = NULL;
+	}
 }
 
 /*

@@ -229,3 +229,19 @@ def test_vorticity_divergence_diagnostics_equal_analytic():
     assert rel_l2(r["vort"], orc.analysis(vort.reshape(-1))) < 1e-8
     assert rel_l2(r["div"], orc.analysis(div.reshape(-1))) < 1e-8
     assert rel_l2(r["phi"], G * orc.analysis(h.reshape(-1))) < 1e-14
+
+
+def test_rotation_field_is_not_reused_across_configurations():
+    # the oracle caches the mu field of the Coriolis term per configuration; a
+    # configuration freed and followed by another of a different grid (which
+    # the allocator often places at the same address) must not see the old one
+    prm = dict(a=A_EARTH, g=G, omega=OMEGA, h0=H0, visc=0.0, nonlinear=0, dt=0.0)
+    x = [random_real_spec(31, s, power=-2.0) for s in (11, 12, 13)]
+    ref = O.Oracle(31).swe_tendency(prm, *x)
+    for Lo in (16, 24, 40):
+        other = O.Oracle(Lo)
+        other.swe_tendency(prm, *[random_real_spec(Lo, s) for s in (1, 2, 3)])
+        del other
+        got = O.Oracle(31).swe_tendency(prm, *x)
+        for k in range(3):
+            assert rel_l2(got[k], ref[k]) < 1e-13
