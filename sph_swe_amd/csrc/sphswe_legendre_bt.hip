@@ -38,7 +38,7 @@ namespace {
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef short s8v __attribute__((ext_vector_type(8)));
 
-constexpr int BT_CTS = 4;   // synthesis: most column fragments per workgroup
+constexpr int BT_CTS = 5;   // synthesis: most column fragments per workgroup
 constexpr int BT_CTA = 2;   // analysis: most column fragments per workgroup
 constexpr int BT_SEG = 32;  // degrees per recurrence segment (seed spacing)
 
@@ -191,31 +191,39 @@ struct Recipe {
 };
 
 // A workgroup takes one (order m, 64-degree block) unit and MB members: it
-// copies the source fields' degrees d0 - 1 .. d0 + 64 of the unit into LDS
-// (consecutive lanes, consecutive degrees), then one thread per (output field,
-// 16-degree group) applies the operator from there and writes its lanes.
+// copies the source fields' degrees d0 - 1 .. d0 + 64 of the unit, and the D'
+// band of its degrees, into LDS (consecutive lanes, consecutive degrees), then
+// eight threads per output field apply the operator from there: lane s of a
+// wavefront takes field 8 w + (s >> 1) & 7, the real (s & 1 = 0) or imaginary
+// part, and the 16 degrees 16 (s >> 4) .. + 15 of the unit, and writes 16-byte
+// slot s of a fragment's hi and lo halves, so that every store of a wavefront
+// covers 1 KB contiguously.  The register budget keeps WPE wavefronts per SIMD
+// (three workgroups of the level-0 recipe per CU).
 template <int RCP>
 struct SprepShape {
 	static constexpr int MB = RCP == BT_RCP_SWE9 ? 8 : 16;   // members per workgroup
 	static constexpr int NSRC = MB * Recipe<RCP>::FU;        // source fields staged
 	static constexpr int NOUT = MB * Recipe<RCP>::FO;        // output fields written
-	static constexpr int NT = 4 * NOUT < 64 ? 64 : (4 * NOUT + 63) / 64 * 64;
+	static constexpr int NT = 8 * NOUT;
 	static constexpr int DW = 66;                             // staged degrees per field
+	static constexpr int WPE = RCP == BT_RCP_SWE9 ? 7 : 4;
+	static_assert(NOUT % 8 == 0 && NT <= 1024, "fragment-aligned output fields");
 };
 
 template <int RCP>
-__global__ __launch_bounds__(SprepShape<RCP>::NT) void k_bt_sprep(const double2 *__restrict__ spec,
-								   uint4 *__restrict__ SB, const int2 *__restrict__ mkb,
-								   const int *__restrict__ kboff,
-								   const double2 *__restrict__ dmu, int L, int nmem, int NCT)
+__global__ __launch_bounds__(SprepShape<RCP>::NT) __attribute__((amdgpu_waves_per_eu(SprepShape<RCP>::WPE)))
+void k_bt_sprep(const double2 *__restrict__ spec, uint4 *__restrict__ SB, const int2 *__restrict__ mkb,
+		const int *__restrict__ kboff, const double2 *__restrict__ dmu, int L, int nmem, int NCT)
 {
 	typedef Recipe<RCP> RC;
 	typedef SprepShape<RCP> SH;
 	__shared__ double2 xs[SH::NSRC][SH::DW];
+	__shared__ double2 cs[64];   // (cm, cp) of D' for the unit's degrees
 	const int2 e = mkb[blockIdx.x];
 	const int m = e.x, kb = e.y;
 	const int mem0 = blockIdx.y * SH::MB;
 	const long nlm = (long)(L + 1) * (L + 2) / 2;
+	const long lm0 = lm_index(L, m, m);
 	const int dmax = L - m;
 	const int db = 64 * kb - 1;   // l - m of staged degree 0
 	// all loads in flight before the first LDS store
@@ -228,55 +236,50 @@ __global__ __launch_bounds__(SprepShape<RCP>::NT) void k_bt_sprep(const double2 
 		const int d = db + dd;
 		const int ms = f / RC::FU, fs = f - ms * RC::FU;   // member, source of the member
 		const bool ok = i < SH::NSRC * SH::DW && mem0 + ms < nmem && d >= 0 && d <= dmax;
-		v[it] = ok ? spec[(long)((mem0 + ms) * RC::FS + fs) * nlm + lm_index(L, m, m) + d] : make_double2(0.0, 0.0);
+		v[it] = ok ? spec[(long)((mem0 + ms) * RC::FS + fs) * nlm + lm0 + d] : make_double2(0.0, 0.0);
 	}
+	double2 cv = make_double2(0.0, 0.0);
+	if (threadIdx.x < 64 && 64 * kb + (int)threadIdx.x <= dmax) cv = dmu[lm0 + 64 * kb + threadIdx.x];
 #pragma unroll
 	for (int it = 0; it < NIT; it++) {
 		const int i = threadIdx.x + it * SH::NT;
 		if (i < SH::NSRC * SH::DW) (&xs[0][0])[i] = v[it];
 	}
+	if (threadIdx.x < 64) cs[threadIdx.x] = cv;
 	__syncthreads();
-	const int t = threadIdx.x;
-	const int fl = t >> 2, g = t & 3;   // output field within the workgroup, 16-degree group
-	if (fl >= SH::NOUT) return;
+	const int t = threadIdx.x, sl = t & 63;
+	const int fl = 8 * (t >> 6) + ((sl >> 1) & 7), ri = sl & 1, g = sl >> 4;
 	const int fo = mem0 * RC::FO + fl;
 	if (fo >= nmem * RC::FO) return;
 	const int ml = fl / RC::FO, qd = fl - ml * RC::FO;
 	int sf, op;
 	RC::of(qd, sf, op);
-	const double2 *x = xs[ml * RC::FU + sf] + 1 + 16 * g;   // x[i] = degree d0 + i, d0 = 64 kb + 16 g
-	const int d0 = 64 * kb + 16 * g;
-	float re[2][8], im[2][8];
+	// x[i] = degree 16 g + i of the unit (x[-1] and x[16] are the neighbours)
+	const double2 *x = xs[ml * RC::FU + sf] + 1 + 16 * g;
+	const double2 *c = cs + 16 * g;
+	float q[2][8];
 	if (op == BT_OP_DMU) {
-		const double2 *cf = dmu + lm_index(L, m, m) + d0;   // (cm, cp) of each degree
 #pragma unroll
 		for (int i = 0; i < 16; i++) {
-			const bool ok = d0 + i <= dmax;
-			const double2 c = ok ? cf[i] : make_double2(0.0, 0.0);
-			const double2 a = x[i - 1], b = x[i + 1];
-			re[i & 1][i >> 1] = (float)(c.x * a.x + c.y * b.x);
-			im[i & 1][i >> 1] = (float)(c.x * a.y + c.y * b.y);
+			const double2 cc = c[i], a = x[i - 1], b = x[i + 1];
+			q[i & 1][i >> 1] = (float)(ri ? cc.x * a.y + cc.y * b.y : cc.x * a.x + cc.y * b.x);
 		}
 	} else {
 		const double dm = op == BT_OP_IM ? m : 0.0;
 #pragma unroll
 		for (int i = 0; i < 16; i++) {
-			const double2 v = x[i];   // zero past the order's last degree
-			re[i & 1][i >> 1] = (float)(op == BT_OP_IM ? -dm * v.y : v.x);
-			im[i & 1][i >> 1] = (float)(op == BT_OP_IM ? dm * v.x : v.y);
+			const double2 w = x[i];   // zero past the order's last degree
+			q[i & 1][i >> 1] = (float)(op == BT_OP_IM ? (ri ? dm * w.x : -dm * w.y) : (ri ? w.y : w.x));
 		}
 	}
 #pragma unroll
 	for (int p = 0; p < 2; p++) {
 		if (kb >= (bt_np(L, m, p) + 31) >> 5) continue;
 		uint4 h, l;
-		uint4 *o = SB + ((long)(kboff[2 * m + p] + kb) * NCT + (fo >> 3)) * 128 + 2 * (fo & 7) + 16 * g;
-		split8(re[p], h, l);
+		uint4 *o = SB + ((long)(kboff[2 * m + p] + kb) * NCT + (fo >> 3)) * 128 + sl;
+		split8(q[p], h, l);
 		o[0] = h;
 		o[64] = l;
-		split8(im[p], h, l);
-		o[1] = h;
-		o[65] = l;
 	}
 }
 
@@ -340,23 +343,23 @@ __device__ __forceinline__ int bt_block(int nblk)
 }
 
 // ---- synthesis ----------------------------------------------------------------
-// Workgroup (pair q, ring group rg of 8 ring tiles = 128 rings, column group
-// cg of CT fragments) of four wavefronts: wavefront (parity p, ring half h)
-// takes parity p of ring tiles 8 rg + 4 h + r, r = 0..3, for all CT column
-// fragments (16 CT accumulator registers).  The 32-degree blocks of both
+// Workgroup (pair q, ring group rg of 2 RT ring tiles = 32 RT rings, column
+// group cg of CT fragments) of four wavefronts: wavefront (parity p, ring half
+// h) takes parity p of ring tiles 2 RT rg + RT h + r, r < RT, for all CT
+// column fragments (4 RT CT accumulator registers).  The 32-degree blocks of both
 // orders of the pair form one sequence of steps.  Per step the workgroup copies
 // the column fragments of both parities into one of two LDS buffers while the
-// wavefronts contract the other: each wavefront loads its four table fragments
+// wavefronts contract the other: each wavefront loads its RT table fragments
 // straight from HBM (each is used by this wavefront only) one step ahead and
-// issues 3 MFMAs per (ring tile, column fragment), 12 CT per step.  At the end
+// issues 3 MFMAs per (ring tile, column fragment), 3 RT CT per step.  At the end
 // of an order the odd-parity wavefronts hand their sums to the even-parity
 // ones through the (drained) LDS buffers, which form F_N = E_0 + E_1 and
 // F_S = E_0 - E_1 and write both hemispheres.
-template <int CT, typename FT, int DA = 2>
-__global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ TS, const int *__restrict__ soff,
-						     const int *__restrict__ kboff, const uint4 *__restrict__ SB,
-						     FT *__restrict__ four, int L, int nlat, int NJ, int nf, int NCT, int nrg,
-						     int ncg)
+template <int CT, typename FT, int DA = 2, int RT = 4, int WPE = 2>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_bt_synth(const uint4 *__restrict__ TS, const int *__restrict__ soff, const int *__restrict__ kboff,
+		const uint4 *__restrict__ SB, FT *__restrict__ four, int L, int nlat, int NJ, int nf, int NCT, int nrg,
+		int ncg)
 {
 	__shared__ uint4 Bs[2][2][CT][128];   // [buf][parity][ct][hi/lo, lane]
 	constexpr int NSTG = CT;              // 16-byte copies per thread per step (256 CT in all)
@@ -367,7 +370,7 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	const int wp = wave & 1, wh = wave >> 1;
 	const int NJt = (NJ + 15) >> 4;
-	const int rt0 = rg * 8 + wh * 4;
+	const int rt0 = rg * 2 * RT + wh * RT;
 	// the two orders of the pair (L even: the middle order once); per-order
 	// values are selected by the order index, never held in indexed arrays
 	// (which would live in scratch memory)
@@ -407,23 +410,23 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 #pragma unroll
 		for (int i = 0; i < NSTG; i++) (&Bs[buf][0][0][0])[tid + i * 256] = v[i];
 	};
-	auto aload = [&](int t, uint4 (&a)[4][2]) {
+	auto aload = [&](int t, uint4 (&a)[RT][2]) {
 		const bool o = t >= T0;
 		const int kb = o ? t - T0 : t;
 		const int nkw = o ? O1.nkw : O0.nkw;
 		const long fa = o ? O1.fa : O0.fa;
 		if (kb >= nkw) return;
 #pragma unroll
-		for (int r = 0; r < 4; r++) {
+		for (int r = 0; r < RT; r++) {
 			if (rt0 + r >= NJt) continue;
 			const long fr = fa + (long)(rt0 + r) * nkw + kb;
 			a[r][0] = TS[fr * 128 + lane];
 			a[r][1] = TS[fr * 128 + 64 + lane];
 		}
 	};
-	f4v acc[4][CT];
+	f4v acc[RT][CT];
 #pragma unroll
-	for (int r = 0; r < 4; r++)
+	for (int r = 0; r < RT; r++)
 #pragma unroll
 		for (int c = 0; c < CT; c++) acc[r][c] = f4v{0.f, 0.f, 0.f, 0.f};
 	// end of order m: odd-parity sums through LDS, then F_N, F_S from the
@@ -431,23 +434,23 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 	// even lane of a (re, im) pair writes rings 4 g, 4 g + 1, the odd lane
 	// rings 4 g + 2, 4 g + 3
 	auto emit = [&](int m) {
-		f4v *X = (f4v *)&Bs[0][0][0][0];   // [h][r][c][lane]: 2 * 4 * CT * 64 * 16 bytes = the LDS of Bs
+		f4v *X = (f4v *)&Bs[0][0][0][0];   // [h][r][c][lane]: 2 * RT * CT * 64 * 16 bytes, within the LDS of Bs
 		__syncthreads();
 		if (wp) {
 #pragma unroll
-			for (int r = 0; r < 4; r++)
+			for (int r = 0; r < RT; r++)
 #pragma unroll
-				for (int c = 0; c < CT; c++) X[((wh * 4 + r) * CT + c) * 64 + lane] = acc[r][c];
+				for (int c = 0; c < CT; c++) X[((wh * RT + r) * CT + c) * 64 + lane] = acc[r][c];
 		}
 		__syncthreads();
 		if (!wp) {
 			const int n = lane & 15, g = lane >> 4, odd = n & 1;
 #pragma unroll
-			for (int r = 0; r < 4; r++) {
+			for (int r = 0; r < RT; r++) {
 				const int rt = rt0 + r;
 #pragma unroll
 				for (int c = 0; c < CT; c++) {
-					const f4v e1 = X[((wh * 4 + r) * CT + c) * 64 + lane];
+					const f4v e1 = X[((wh * RT + r) * CT + c) * 64 + lane];
 					float fn[4], fs[4], pn[4], ps[4];
 #pragma unroll
 					for (int e = 0; e < 4; e++) {
@@ -485,12 +488,12 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 			}
 		}
 #pragma unroll
-		for (int r = 0; r < 4; r++)
+		for (int r = 0; r < RT; r++)
 #pragma unroll
 			for (int c = 0; c < CT; c++) acc[r][c] = f4v{0.f, 0.f, 0.f, 0.f};
 		__syncthreads();
 	};
-	uint4 xa[4][2], ya[4][2];
+	uint4 xa[RT][2], ya[RT][2];
 	uint4 sv[NSTG];
 	sload(0, sv);
 	if (DA == 2) aload(0, xa);
@@ -498,7 +501,7 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 	__syncthreads();
 	// DA = 2: the table fragments of step t + 1 are loaded during step t;
 	// DA = 1: at the start of step t (fewer registers, more workgroups per CU)
-	auto step = [&](int t, uint4 (&a)[4][2], uint4 (&na)[4][2]) {
+	auto step = [&](int t, uint4 (&a)[RT][2], uint4 (&na)[RT][2]) {
 		const bool o = t >= T0;
 		const int kb = o ? t - T0 : t;
 		const int buf = t & 1;
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 			for (int c = 0; c < CT; c++) {
 				const uint4 bh = Bs[buf][wp][c][lane], bl = Bs[buf][wp][c][64 + lane];
 #pragma unroll
-				for (int r = 0; r < 4; r++) acc[r][c] = mfma_hl(a[r][0], a[r][1], bh, bl, acc[r][c]);
+				for (int r = 0; r < RT; r++) acc[r][c] = mfma_hl(a[r][0], a[r][1], bh, bl, acc[r][c]);
 			}
 		}
 		if (t == T0 - 1 || t == T - 1) emit(o ? O1.m : O0.m);
@@ -527,24 +530,25 @@ __global__ __launch_bounds__(256, 2) void k_bt_synth(const uint4 *__restrict__ T
 }
 
 // ---- analysis -----------------------------------------------------------------
-// Workgroup (pair q, column group cg) of AW wavefronts.  The degree tiles of
-// the pair, ordered (order, parity, tile), are dealt out in runs of RA
-// consecutive tiles per wavefront; the contraction runs over 32-ring blocks.
-// Per block the workgroup copies the column fragments of both orders and
-// parities into one of two LDS buffers; a wavefront reloads them only where
-// its (order, parity) changes.  Table fragments and staged column fragments
-// are loaded D blocks ahead through register queues.
+// Workgroup (pair q, column group cg, share hs of SPL) of AW wavefronts.  The
+// degree tiles of the pair, ordered (order, parity, tile), are dealt out in
+// runs of RA consecutive tiles per wavefront, AW RA tiles per share; the
+// contraction runs over 32-ring blocks.  Per block the workgroup copies the
+// column fragments of the orders and parities its tiles touch into one of two
+// LDS buffers; a wavefront reloads them only where its (order, parity)
+// changes.  Table fragments and staged column fragments are loaded D blocks
+// ahead through register queues.
 template <int CT, int RA, int AW, int D>
 __global__ __launch_bounds__(64 * AW) void k_bt_anal(const uint4 *__restrict__ TA, const int *__restrict__ aoff,
 						     const uint4 *__restrict__ G, double2 *__restrict__ spec, int L, int NJ,
-						     int nf, int NCT, int ncg)
+						     int nf, int NCT, int ncg, int SPL)
 {
 	__shared__ uint4 Bs[2][2][2][CT][128];   // [buf][order][parity][ct][hi/lo, lane]
 	constexpr int NSTG = (512 * CT + 64 * AW - 1) / (64 * AW);
-	const int nblk = ((L + 2) / 2) * ncg;
+	const int nblk = ((L + 2) / 2) * ncg * SPL;
 	const int lb = bt_block(nblk);
 	if (lb >= nblk) return;
-	const int cg = lb % ncg, q = lb / ncg;
+	const int hs = lb % SPL, cg = (lb / SPL) % ncg, q = lb / (SPL * ncg);
 	const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 	const int NKR = (NJ + 31) >> 5;
 	const long nlm = (long)(L + 1) * (L + 2) / 2;
@@ -559,7 +563,7 @@ __global__ __launch_bounds__(64 * AW) void k_bt_anal(const uint4 *__restrict__ T
 	bool tv[RA];
 #pragma unroll
 	for (int s = 0; s < RA; s++) {
-		const int t = wave * RA + s;
+		const int t = (hs * AW + wave) * RA + s;
 		const int grp = (t >= c0) + (t >= c1) + (t >= c2);
 		const int base = grp == 0 ? 0 : grp == 1 ? c0 : grp == 2 ? c1 : c2;
 		tv[s] = t < c3;
@@ -568,6 +572,11 @@ __global__ __launch_bounds__(64 * AW) void k_bt_anal(const uint4 *__restrict__ T
 		const int m = (tg[s] >> 1) ? m1 : q;
 		tfr[s] = tv[s] ? aoff[2 * m + (tg[s] & 1)] + tdt[s] * NKR : 0;
 	}
+	// the (order, parity) groups 2 o + p that this workgroup's tiles touch:
+	// only their column fragments are staged
+	const int tw0 = hs * AW * RA, tw1 = min(tw0 + AW * RA, c3) - 1;
+	const int glo = (tw0 >= c0) + (tw0 >= c1) + (tw0 >= c2);
+	const int ghi = (tw1 >= c0) + (tw1 >= c1) + (tw1 >= c2);
 	auto sload = [&](int kb, uint4 (&v)[NSTG]) {
 #pragma unroll
 		for (int i = 0; i < NSTG; i++) {
@@ -575,7 +584,8 @@ __global__ __launch_bounds__(64 * AW) void k_bt_anal(const uint4 *__restrict__ T
 			// idx = ((o * CT + ct) * 2 + p) * 128 + w in G order; LDS order [o][p][ct]
 			const int o = idx / (256 * CT), r = idx % (256 * CT), ct = r >> 8;
 			const int ctg = cg * CT + ct;
-			const bool ok = idx < 512 * CT && o < no && ctg < NCT;
+			const int gi = 2 * o + ((r >> 7) & 1);
+			const bool ok = idx < 512 * CT && o < no && ctg < NCT && gi >= glo && gi <= ghi;
 			const int m = o ? m1 : q;
 			v[i] = ok ? G[((long)(m * NKR + kb) * NCT + ctg) * 256 + (r & 255)] : uint4{0, 0, 0, 0};
 		}
@@ -681,13 +691,16 @@ int bt_ct(int NCT, int maxct)
 constexpr int BT_DS = 3;   // synthesis: steps of look-ahead of the loads
 constexpr int BT_DA = 3;   // analysis
 
+// two ring tiles per wavefront (64 rings per workgroup) at three wavefronts
+// per SIMD: 233 against 244 us (four tiles, two per SIMD) for 144 fields at T511
 template <int CT, typename FT>
 int launch_bt_synth(sphswe_ctx *ctx, const uint4 *SB, FT *four, int nf, int NCT)
 {
+	constexpr int RT = 2;
 	const int NJt = (ctx->NJ + 15) / 16;
-	const int nrg = (NJt + 7) / 8, ncg = (NCT + CT - 1) / CT;
+	const int nrg = (NJt + 2 * RT - 1) / (2 * RT), ncg = (NCT + CT - 1) / CT;
 	const int nblk = ((ctx->L + 2) / 2) * nrg * ncg;
-	k_bt_synth<CT, FT><<<(nblk + 7) / 8 * 8, 256, 0, ctx->stream>>>(ctx->d_bt_s, ctx->d_bt_soff, ctx->d_bt_kboff, SB,
+	k_bt_synth<CT, FT, 2, RT, 3><<<(nblk + 7) / 8 * 8, 256, 0, ctx->stream>>>(ctx->d_bt_s, ctx->d_bt_soff, ctx->d_bt_kboff, SB,
 									   four, ctx->L, ctx->nlat, ctx->NJ, nf, NCT, nrg,
 									   ncg);
 	SPHSWE_LAUNCH_CHECK(ctx);
@@ -695,12 +708,12 @@ int launch_bt_synth(sphswe_ctx *ctx, const uint4 *SB, FT *four, int nf, int NCT)
 }
 
 template <int CT, int RA, int AW, int D = BT_DA>
-int launch_bt_anal(sphswe_ctx *ctx, const uint4 *G, double2 *spec, int nf, int NCT)
+int launch_bt_anal(sphswe_ctx *ctx, const uint4 *G, double2 *spec, int nf, int NCT, int spl = 1)
 {
 	const int ncg = (NCT + CT - 1) / CT;
-	const int nblk = ((ctx->L + 2) / 2) * ncg;
+	const int nblk = ((ctx->L + 2) / 2) * ncg * spl;
 	k_bt_anal<CT, RA, AW, D><<<(nblk + 7) / 8 * 8, 64 * AW, 0, ctx->stream>>>(ctx->d_bt_a, ctx->d_bt_aoff, G, spec,
-										   ctx->L, ctx->NJ, nf, NCT, ncg);
+										   ctx->L, ctx->NJ, nf, NCT, ncg, spl);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }
@@ -763,8 +776,7 @@ int bt_synth_split(sphswe_ctx *ctx, const void *SB, FT *four, int nf)
 	case 2: return launch_bt_synth<2, FT>(ctx, sb, four, nf, NCT);
 	case 3: return launch_bt_synth<3, FT>(ctx, sb, four, nf, NCT);
 	case 4: return launch_bt_synth<4, FT>(ctx, sb, four, nf, NCT);
-	case 5: return launch_bt_synth<5, FT>(ctx, sb, four, nf, NCT);
-	default: return launch_bt_synth<6, FT>(ctx, sb, four, nf, NCT);
+	default: return launch_bt_synth<5, FT>(ctx, sb, four, nf, NCT);
 	}
 }
 int sphswe_bt_synth(sphswe_ctx *ctx, const void *SB, double2 *four, int nf) { return bt_synth_split(ctx, SB, four, nf); }
@@ -775,15 +787,17 @@ int sphswe_bt_anal(sphswe_ctx *ctx, const void *Gv, double2 *spec, int nf)
 	if (nf <= 0) return SPHSWE_OK;
 	const int NCT = (nf + 7) / 8;
 	const uint4 *G = (const uint4 *)Gv;
-	// (tiles per wavefront, wavefronts): every degree tile of the fullest order pair
+	// (tiles per wavefront, wavefronts): every degree tile of the fullest order
+	// pair in one workgroup for up to 32 tiles
 	const int tm = ctx->bt_tmax;
 	const int ct = bt_ct(NCT, BT_CTA);
 	if (tm <= 16) return ct == 1 ? launch_bt_anal<1, 1, 16>(ctx, G, spec, nf, NCT) : launch_bt_anal<2, 1, 16>(ctx, G, spec, nf, NCT);
 	if (tm <= 32) return ct == 1 ? launch_bt_anal<1, 2, 16>(ctx, G, spec, nf, NCT) : launch_bt_anal<2, 2, 16>(ctx, G, spec, nf, NCT);
-	if (tm <= 36) return ct == 1 ? launch_bt_anal<1, 3, 12>(ctx, G, spec, nf, NCT) : launch_bt_anal<2, 3, 12>(ctx, G, spec, nf, NCT);
-	if (tm <= 48) return ct == 1 ? launch_bt_anal<1, 3, 16>(ctx, G, spec, nf, NCT) : launch_bt_anal<2, 3, 16>(ctx, G, spec, nf, NCT);
-	if (tm <= 80) return launch_bt_anal<1, 5, 16>(ctx, G, spec, nf, NCT);
-	return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "bf16 table analysis: %d degree tiles per order pair", ctx->bt_tmax);
+	// more tiles: workgroups of four wavefronts with three tiles each, the
+	// pair's tiles split over ceil(tm / 12) workgroups (more, smaller
+	// workgroups in flight: 154 against 185 us for the 64-field analysis at T511)
+	const int spl = (tm + 11) / 12;
+	return ct == 1 ? launch_bt_anal<1, 3, 4, 2>(ctx, G, spec, nf, NCT, spl) : launch_bt_anal<2, 3, 4, 2>(ctx, G, spec, nf, NCT, spl);
 }
 
 // the tables, built on first use (allocates; not inside a stream capture)
