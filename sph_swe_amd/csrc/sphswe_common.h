@@ -182,6 +182,9 @@ int sphswe_bt_gprep(sphswe_ctx *ctx, const double2 *four, void *G, int nf);
 int sphswe_bt_gprep32(sphswe_ctx *ctx, const float2 *four, void *G, int nf);
 int sphswe_bt_synth(sphswe_ctx *ctx, const void *SB, double2 *four, int nf);
 int sphswe_bt_synth32(sphswe_ctx *ctx, const void *SB, float2 *four, int nf);
+// the divergence level of the mixed SWE tendency: synthesis of (i m P0, D' P1)
+// per member straight to the analysis operand of -(P0 / sq + P1 / (sq st)) scale
+int sphswe_bt_synth_div(sphswe_ctx *ctx, const void *SB, void *G1, int nmem, float scale);
 int sphswe_bt_anal(sphswe_ctx *ctx, const void *G, double2 *spec, int nf);
 int sphswe_leg_anal_tab(sphswe_ctx *ctx, const double2 *four, double2 *spec, int nf);
 int sphswe_leg_synth_pole_mx(sphswe_ctx *ctx, const double2 *spec, double2 *four, int nf, int P, const int *d_moff,

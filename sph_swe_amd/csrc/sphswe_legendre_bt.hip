@@ -355,12 +355,24 @@ __device__ __forceinline__ int bt_block(int nblk)
 // of an order the odd-parity wavefronts hand their sums to the even-parity
 // ones through the (drained) LDS buffers, which form F_N = E_0 + E_1 and
 // F_S = E_0 - E_1 and write both hemispheres.
-template <int CT, typename FT, int DA = 2, int RT = 4, int WPE = 2>
+//
+// EM = 1 (the divergence level of the SWE tendency, fields (P0, P1) per
+// member): instead of Fourier rows, every wavefront emits the analysis operand
+// of the per-ring combination O = -(P0 / sq + P1 / (sq st)) scale for its own
+// parity.  The combination has coefficients constant along a ring, so it
+// commutes with the longitude transforms: the analysis Fourier rows of O are
+// 2 pi times the combination of the synthesised rows (nphi > 2 L, no products),
+// and the analysis operand w_j (F_N + (-1)^p F_S) is 4 pi w_j times the
+// combination of the parity-p sums E_p.  It is written pre-split in the
+// analysis fragment order (one field per member, NCT1 = ceil(nmem / 8)).
+template <int CT, typename FT, int DA = 2, int RT = 4, int WPE = 2, int EM = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_bt_synth(const uint4 *__restrict__ TS, const int *__restrict__ soff, const int *__restrict__ kboff,
 		const uint4 *__restrict__ SB, FT *__restrict__ four, int L, int nlat, int NJ, int nf, int NCT, int nrg,
-		int ncg)
+		int ncg, uint4 *__restrict__ G1, const double *__restrict__ sq, const double *__restrict__ st,
+		const double *__restrict__ wg, float scale)
 {
+	static_assert(EM == 0 || RT == 2, "one 32-ring block per wavefront");
 	__shared__ uint4 Bs[2][2][CT][128];   // [buf][parity][ct][hi/lo, lane]
 	constexpr int NSTG = CT;              // 16-byte copies per thread per step (256 CT in all)
 	const int nblk = ((L + 2) / 2) * nrg * ncg;
@@ -433,7 +445,67 @@ void k_bt_synth(const uint4 *__restrict__ TS, const int *__restrict__ soff, cons
 	// even-parity wavefronts; lane n + 16 g holds rings 4 g + e of column n: the
 	// even lane of a (re, im) pair writes rings 4 g, 4 g + 1, the odd lane
 	// rings 4 g + 2, 4 g + 3
+	// EM = 1: the analysis operand of this wavefront's parity and 32-ring block
+	auto emit_g1 = [&](int m) {
+		// per wavefront: [column][ring] of one fragment half, then the ring
+		// factors (a, b, weight) of its 32 rings
+		float *T = (float *)&Bs[0][0][0][0] + wave * (8 * 33 + 96);
+		float *Fc = T + 8 * 33;
+		const int n = lane & 15, g = lane >> 4, ri = n & 1;
+		const int kb1 = rt0 >> 1, NKR = (NJ + 31) >> 5, nmem = nf >> 1, NCT1 = (nmem + 7) >> 3;
+		__syncthreads();
+		if (lane < 32) {
+			const int j = min(32 * kb1 + lane, NJ - 1);
+			const double ic = 1.0 / sq[j];
+			Fc[lane] = (float)(-scale * ic);
+			Fc[32 + lane] = (float)(-scale * ic / st[j]);
+			Fc[64 + lane] = 32 * kb1 + lane < NJ ? (float)(4.0 * M_PI * wg[j]) : 0.f;
+		}
+		wave_sync();
+#pragma unroll
+		for (int c = 0; c < CT; c++) {
+			const int ctg = cg * CT + c;   // columns 16 ctg + n: member 4 ctg + n / 4, field (n >> 1) & 1
+			if (ctg >= NCT) break;
+#pragma unroll
+			for (int r = 0; r < RT; r++)
+#pragma unroll
+				for (int e = 0; e < 4; e++) {
+					const float x = acc[r][c][e];
+					const float y = __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4e, 0xf, 0xf, false));
+					const int jj = 16 * r + 4 * g + e;
+					float v = (x * Fc[jj] + y * Fc[32 + jj]) * Fc[64 + jj];
+					if (m == 0 && ri) v = 0.f;
+					if (!(n & 2)) T[(2 * (n >> 2) + ri) * 33 + jj] = v;
+				}
+			wave_sync();
+			if (lane < 32) {
+				const int n1 = lane & 7, g1 = lane >> 3;
+				float v8[8];
+#pragma unroll
+				for (int i = 0; i < 8; i++) v8[i] = T[n1 * 33 + 8 * g1 + i];
+				uint4 h, l;
+				split8(v8, h, l);
+				const int mem1 = 4 * ctg + (n1 >> 1);
+				if (mem1 < nmem && kb1 < NKR) {   // ring groups rounded up past the last block write nothing
+					uint4 *o = G1 + (((long)(m * NKR + kb1) * NCT1 + (mem1 >> 3)) * 2 + wp) * 128 +
+						   2 * (mem1 & 7) + (n1 & 1) + 16 * g1;
+					o[0] = h;
+					o[64] = l;
+				}
+			}
+			wave_sync();
+		}
+#pragma unroll
+		for (int r = 0; r < RT; r++)
+#pragma unroll
+			for (int c = 0; c < CT; c++) acc[r][c] = f4v{0.f, 0.f, 0.f, 0.f};
+		__syncthreads();
+	};
 	auto emit = [&](int m) {
+		if constexpr (EM == 1) {
+			emit_g1(m);
+			return;
+		}
 		f4v *X = (f4v *)&Bs[0][0][0][0];   // [h][r][c][lane]: 2 * RT * CT * 64 * 16 bytes, within the LDS of Bs
 		__syncthreads();
 		if (wp) {
@@ -693,16 +765,17 @@ constexpr int BT_DA = 3;   // analysis
 
 // two ring tiles per wavefront (64 rings per workgroup) at three wavefronts
 // per SIMD: 233 against 244 us (four tiles, two per SIMD) for 144 fields at T511
-template <int CT, typename FT>
-int launch_bt_synth(sphswe_ctx *ctx, const uint4 *SB, FT *four, int nf, int NCT)
+template <int CT, typename FT, int EM = 0>
+int launch_bt_synth(sphswe_ctx *ctx, const uint4 *SB, FT *four, int nf, int NCT, uint4 *G1 = nullptr,
+		    float scale = 0.f)
 {
 	constexpr int RT = 2;
 	const int NJt = (ctx->NJ + 15) / 16;
 	const int nrg = (NJt + 2 * RT - 1) / (2 * RT), ncg = (NCT + CT - 1) / CT;
 	const int nblk = ((ctx->L + 2) / 2) * nrg * ncg;
-	k_bt_synth<CT, FT, 2, RT, 3><<<(nblk + 7) / 8 * 8, 256, 0, ctx->stream>>>(ctx->d_bt_s, ctx->d_bt_soff, ctx->d_bt_kboff, SB,
-									   four, ctx->L, ctx->nlat, ctx->NJ, nf, NCT, nrg,
-									   ncg);
+	k_bt_synth<CT, FT, 2, RT, 3, EM><<<(nblk + 7) / 8 * 8, 256, 0, ctx->stream>>>(
+		ctx->d_bt_s, ctx->d_bt_soff, ctx->d_bt_kboff, SB, four, ctx->L, ctx->nlat, ctx->NJ, nf, NCT, nrg, ncg, G1,
+		ctx->d_sq, ctx->d_st, ctx->d_wg, scale);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	return SPHSWE_OK;
 }
@@ -780,6 +853,22 @@ int bt_synth_split(sphswe_ctx *ctx, const void *SB, FT *four, int nf)
 	}
 }
 int sphswe_bt_synth(sphswe_ctx *ctx, const void *SB, double2 *four, int nf) { return bt_synth_split(ctx, SB, four, nf); }
+int sphswe_bt_synth_div(sphswe_ctx *ctx, const void *SB, void *G1, int nmem, float scale)
+{
+	if (nmem <= 0) return SPHSWE_OK;
+	if (ctx->nlat % 2 != 0 || 2 * ctx->L + 1 > ctx->plan.n)
+		return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "divergence level: needs an even ring count and nphi > 2 L");
+	const int NCT = (2 * nmem + 7) / 8;
+	const uint4 *sb = (const uint4 *)SB;
+	uint4 *g = (uint4 *)G1;
+	switch (bt_ct(NCT, BT_CTS)) {
+	case 1: return launch_bt_synth<1, float2, 1>(ctx, sb, nullptr, 2 * nmem, NCT, g, scale);
+	case 2: return launch_bt_synth<2, float2, 1>(ctx, sb, nullptr, 2 * nmem, NCT, g, scale);
+	case 3: return launch_bt_synth<3, float2, 1>(ctx, sb, nullptr, 2 * nmem, NCT, g, scale);
+	case 4: return launch_bt_synth<4, float2, 1>(ctx, sb, nullptr, 2 * nmem, NCT, g, scale);
+	default: return launch_bt_synth<5, float2, 1>(ctx, sb, nullptr, 2 * nmem, NCT, g, scale);
+	}
+}
 int sphswe_bt_synth32(sphswe_ctx *ctx, const void *SB, float2 *four, int nf) { return bt_synth_split(ctx, SB, four, nf); }
 
 int sphswe_bt_anal(sphswe_ctx *ctx, const void *Gv, double2 *spec, int nf)

@@ -152,7 +152,7 @@ __global__ void k_phys2(const double *__restrict__ P2, double *__restrict__ P1, 
 }
 
 // ---- the mixed form (SPHSWE_PREC_MIXED): FP32 ring-major physical fields ----
-// The same pointwise terms as k_phys1 / k_phys2 in FP32 on fields laid out
+// The same pointwise terms as k_phys1 in FP32 on fields laid out
 // [member][field][ring j][longitude], so that ring j of point k is k / nphi.
 __global__ void k_phys1_mx(const float *__restrict__ P9, float *__restrict__ P4, const double *__restrict__ ct,
 			   const double *__restrict__ st, const double *__restrict__ sq, int nphi, long nspat, int nmem,
@@ -194,26 +194,14 @@ __global__ void k_phys1_mx(const float *__restrict__ P9, float *__restrict__ P4,
 	}
 }
 
-__global__ void k_phys2_mx(const float *__restrict__ P2, float *__restrict__ P1, const double *__restrict__ st,
-			   const double *__restrict__ sq, int nphi, long nspat, int nmem, float scale)
-{
-	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-	if (i >= nspat * nmem) return;
-	int mem = (int)(i / nspat);
-	long k = i - (long)mem * nspat;
-	int j = (int)(k / nphi);
-	const float *p = P2 + (long)mem * 2 * nspat + k;
-	const float ic = (float)(1.0 / sq[j]);
-	P1[(long)mem * nspat + k] = -(p[0] * ic + (p[nspat] * ic) * (float)(1.0 / st[j])) * scale;
-}
-
-// The mixed form's physical stages without physical fields in HBM: one
+// The mixed form's physical stage without physical fields in HBM: one
 // workgroup per (member, ring pair j, j + 1) runs the inverse FFTs of the
-// level's input Fourier rows (9 at level 0, 2 at level 2), which leave each
-// thread the longitudes k + NT w + t NSL of its last-pass butterflies in
-// registers, forms the pointwise terms there (those of k_phys1_mx /
-// k_phys2_mx), and runs the forward FFTs of the outputs (4, 1) from LDS to the
-// output Fourier rows.  The two rings are the real and imaginary parts of one
+// 9 input fields' Fourier rows, which leave each thread the longitudes
+// k + NT w + t NSL of its last-pass butterflies in registers, forms the
+// pointwise terms there (those of k_phys1_mx), and runs the forward FFTs of
+// the 4 outputs from LDS to the output Fourier rows.  (The divergence level
+// needs no physical stage in this form: its pointwise step is a per-ring
+// linear combination, applied to the synthesis sums by the table synthesis.)  The two rings are the real and imaginary parts of one
 // complex sequence.  Fields go through the passes FI (inputs) or FO (outputs)
 // at a time as the sequences of one tile; one at a time measured fastest (at
 // T511 three inputs per round need twice the registers and run 1.9x slower),
@@ -236,15 +224,15 @@ struct FusedPlan {
 	static constexpr bool ok = NP >= 2 && NSL % 64 == 0 && NSL % NT == 0 && NV <= 8;
 };
 
-template <int LEV, int MODE, int N>
+template <int MODE, int N>
 __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	const float2 *__restrict__ Fin, float2 *__restrict__ Fout, const float2 *__restrict__ twg,
 	const double *__restrict__ ct, const double *__restrict__ st, const double *__restrict__ sq, int nlat, int L,
-	int nblk, int nblk_pad, float ga, float two_omega, float ia, float h0, float scale)
+	int nblk, int nblk_pad, float ga, float two_omega, float ia, float h0)
 {
 	constexpr int mode = MODE;
 	typedef FusedPlan<N> FP;
-	constexpr int NIN = LEV == 0 ? 9 : 2, NOUT = LEV == 0 ? 4 : 1;
+	constexpr int NIN = 9, NOUT = 4;
 	constexpr int FI = 1, FO = 1;   // fields per round, in and out
 	typedef WPad<N, FI> WI;
 	typedef WPad<N, FO> WO;
@@ -267,34 +255,27 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 
 	// Each input field's values (ring j in x, ring j + 1 in y, at this thread's
 	// longitudes) are folded into the outputs O as the field arrives, so that
-	// only U, V and the outputs stay live.  Level 0, in field order H, U, V,
-	// i m H, D'H, i m U, D'U, i m V, D'V:
+	// only U, V and the outputs stay live.  In field order H, U, V, i m H,
+	// D'H, i m U, D'U, i m V, D'V:
 	//   mode 1  O0 = U H, O1 = V H st, O2 = -ga Hl/c + 2 Omega V mu - (Ul/c U + Um/c V)/a,
 	//           O3 = -ga Hm/c - 2 Omega U mu - (Vl/c U + Vm/c V)/a;
 	//   mode 0  O0 = U, O1 = V st, O2, O3 without the advection terms;
 	//   mode 2  O0 = U (H - h0), O1 = V (H - h0) st, O2, O3 the advection terms only;
 	//   mode 3  O0 = -(Ul + Vm) h0 / st^2, O1 = 0, O2 = -ga Hl + 2 Omega V mu,
-	//           O3 = -ga Hm - 2 Omega U mu;
-	// level 2: O0 = -(P0/c + P1/(c st)) scale.
-	// per ring: 1 / sqrt(1 - mu^2), mu, cos phi and the reciprocals of cos phi
-	// and cos^2 phi, rounded once from FP64 (the terms multiply by them)
-	float ric[2], rmu[2], rst[2], ris[2], ris2[2];
+	//           O3 = -ga Hm - 2 Omega U mu.
+	// per ring: 1 / sqrt(1 - mu^2), mu, cos phi and the reciprocal of cos^2 phi,
+	// rounded once from FP64 (the terms multiply by them)
+	float ric[2], rmu[2], rst[2], ris2[2];
 #pragma unroll
 	for (int r = 0; r < 2; r++) {
 		ric[r] = (float)(1.0 / sq[j + r]);
 		rmu[r] = (float)ct[j + r];
 		rst[r] = (float)st[j + r];
-		ris[r] = (float)(1.0 / st[j + r]);
 		ris2[r] = (float)(1.0 / (st[j + r] * st[j + r]));
 	}
 	float Hs[NV][2], Us[NV][2], Vs[NV][2], O[NOUT][NV][2];
 	auto fold = [&](int q, int e, int r, float x) __attribute__((always_inline)) {
 		const float ic = ric[r], mu = rmu[r], sj = rst[r];
-		if (LEV == 1) {
-			if (q == 0) O[0][e][r] = x * ic;
-			else O[0][e][r] = -(O[0][e][r] + (x * ic) * ris[r]) * scale;
-			return;
-		}
 		switch (q) {
 		case 0: Hs[e][r] = mode == 2 ? x - h0 : x; break;
 		case 1:
@@ -350,7 +331,10 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	};
 	fetch(0);
 	// unrolled: every register array is indexed by compile-time numbers only
-#pragma unroll
+#ifndef SPHSWE_FUSED_UNR
+#define SPHSWE_FUSED_UNR 16
+#endif
+#pragma unroll SPHSWE_FUSED_UNR
 	for (int q0 = 0; q0 < NIN; q0 += FI) {
 #pragma unroll
 		for (int f = 0; f < FI; f++) {
@@ -514,7 +498,7 @@ struct SweBufs {
 	// rows and FP32 ring-major physical fields
 	void *SB, *G;
 	float2 *F32;
-	float *Q9, *Q4, *Q2, *Q1;
+	float *Q9, *Q4;
 };
 
 // one captured step per context, rebuilt when its arguments change
@@ -540,13 +524,15 @@ int swe_buffers(sphswe_ctx *ctx, int nmem, int prec, SweBufs &b)
 			return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: the mixed form has no FP32 FFT for %d longitudes",
 					      ctx->nphi);
 		if (int rc = sphswe_leg_bt_tables(ctx)) return rc;
+		if (ctx->nlat % 2 != 0 || 2 * ctx->L + 1 > ctx->nphi)
+			return sphswe_set_err(ctx, SPHSWE_ERR_ARG, "swe_rk4: the mixed form needs an even ring count and nphi > 2 L");
 		// Y, ACC, YN (3 nlm each), S4 (4), TH (1) spectral in FP64; operands of
-		// 9 and 4 fields; FP32 Fourier rows of 9 + 4 fields; FP32 fields 9 + 4 + 2 + 1
-		// (the last only for grids without a fused physical stage)
+		// 9 and 4 fields; FP32 Fourier rows of 9 + 4 fields; FP32 fields 9 + 4
+		// (only for grids without a fused physical stage)
 		sb_b = (sphswe_bt_sb_bytes(ctx, 9 * nmem) + 255) & ~(size_t)255;
 		g_b = (sphswe_bt_g_bytes(ctx, 4 * nmem) + 255) & ~(size_t)255;
 		bytes = (size_t)nlm * nmem * (3 + 3 + 3 + 4 + 1) * 16 + sb_b + g_b + (size_t)nfour * nmem * 13 * 8 +
-			(size_t)nspat * nmem * (9 + 4 + 2 + 1) * 4 + 1024;
+			(size_t)nspat * nmem * (9 + 4) * 4 + 1024;
 	} else {
 		// workspace: Y (3 nlm), ACC (3), YN (3), S9 (9), S4 (4), S2 (2), TH (1) spectral;
 		// P9 (9), P4 (4), P2 (2), P1 (1) physical; Fourier for 9 fields
@@ -590,8 +576,6 @@ int swe_buffers(sphswe_ctx *ctx, int nmem, int prec, SweBufs &b)
 		b.F32 = (float2 *)take((size_t)13 * nfour * nmem * 8);   // 9 synthesised + 4 analysed fields
 		b.Q9 = (float *)take((size_t)9 * nspat * nmem * 4);
 		b.Q4 = (float *)take((size_t)4 * nspat * nmem * 4);
-		b.Q2 = (float *)take((size_t)2 * nspat * nmem * 4);
-		b.Q1 = (float *)take((size_t)nspat * nmem * 4);
 		return SPHSWE_OK;
 	}
 	double2 *sp = (double2 *)ws.base;
@@ -669,11 +653,10 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 	return SPHSWE_OK;
 }
 
-// the fused physical stage of level LEV for the context's longitude count, or
-// 1 when there is no fused kernel for it (the caller runs the separate kernels)
-template <int LEV>
+// the fused physical stage for the context's longitude count, or 1 when there
+// is no fused kernel for it (the caller runs the separate kernels)
 int phys_fft_mx(sphswe_ctx *ctx, const float2 *Fin, float2 *Fout, int nm, float ga, float two_omega, float ia,
-		int mode, float h0, float scale)
+		int mode, float h0)
 {
 	const int nblk = nm * (ctx->nlat / 2);
 	const int nblk_pad = (nblk + 7) / 8 * 8;
@@ -681,18 +664,17 @@ int phys_fft_mx(sphswe_ctx *ctx, const float2 *Fin, float2 *Fout, int nm, float 
 		constexpr int M = decltype(md)::value;
 		auto go = [&](auto kern, int nt) -> int {
 			kern<<<nblk_pad, nt, 0, ctx->stream>>>(Fin, Fout, ctx->d_tw32, ctx->d_ct, ctx->d_st, ctx->d_sq,
-							       ctx->nlat, ctx->L, nblk, nblk_pad, ga, two_omega, ia, h0, scale);
+							       ctx->nlat, ctx->L, nblk, nblk_pad, ga, two_omega, ia, h0);
 			SPHSWE_LAUNCH_CHECK(ctx);
 			return SPHSWE_OK;
 		};
 		switch (ctx->plan.n) {
-		case 384: return go(k_phys_fft_mx<LEV, M, 384>, FusedPlan<384>::NT);
-		case 1536: return go(k_phys_fft_mx<LEV, M, 1536>, FusedPlan<1536>::NT);
+		case 384: return go(k_phys_fft_mx<M, 384>, FusedPlan<384>::NT);
+		case 1536: return go(k_phys_fft_mx<M, 1536>, FusedPlan<1536>::NT);
 		}
 		return 1;
 	};
 	if (ctx->nlat % 2 != 0 || 2 * ctx->L + 1 > ctx->plan.n) return 1;
-	if (LEV == 1) return by_n(std::integral_constant<int, 0>{});   // level 2 has one form
 	switch (mode) {
 	case 0: return by_n(std::integral_constant<int, 0>{});
 	case 1: return by_n(std::integral_constant<int, 1>{});
@@ -705,7 +687,10 @@ static_assert(FusedPlan<384>::ok && FusedPlan<1536>::ok, "fused plans");
 // one RK4 step in the mixed form: the Legendre contractions on the bf16 matrix
 // cores from the stored tables (operands split into bf16 hi + lo), Fourier rows,
 // longitude FFTs and pointwise terms in FP32, the state and the stage
-// combinations in FP64.  The projection levels are those of swe_step.
+// combinations in FP64.  The projection levels are those of swe_step; the
+// divergence level's pointwise step, a combination with coefficients constant
+// along each ring, goes straight from the synthesis sums to the analysis
+// operand (equal to the transform round trip up to rounding, since nphi > 2 L).
 int swe_step_mixed(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, const SweBufs &b)
 {
 	const long nlm = ctx->nlm, nspat = ctx->nspat;
@@ -727,7 +712,7 @@ int swe_step_mixed(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int 
 		if ((rc = sphswe_bt_synth32(ctx, b.SB, b.F32, 9 * nm))) return rc;
 		const float h0f = (float)(mode == 3 ? p->h0 / a : p->h0);
 		float2 *F4 = b.F32 + (long)9 * nfour * nm;   // level-1 rows after the 9 synthesised fields
-		rc = phys_fft_mx<0>(ctx, b.F32, F4, nm, (float)ga, (float)two_omega, (float)ia, mode, h0f, 0.f);
+		rc = phys_fft_mx(ctx, b.F32, F4, nm, (float)ga, (float)two_omega, (float)ia, mode, h0f);
 		if (rc == 1) {
 			if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q9, 9 * nm))) return rc;
 			k_phys1_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q9, b.Q4, ctx->d_ct, ctx->d_st, ctx->d_sq, nphi, nspat,
@@ -741,19 +726,12 @@ int swe_step_mixed(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int 
 		const double2 *TH = b.S4;
 		int th_stride = 4;
 		if (mode != 3) {
+			// the divergence level: its pointwise step is a per-ring linear
+			// combination, applied to the synthesis sums in place of a
+			// transform round trip
 			if ((rc = sphswe_bt_sprep_swe2(ctx, b.S4, b.SB, nm))) return rc;
-			if ((rc = sphswe_bt_synth32(ctx, b.SB, b.F32, 2 * nm))) return rc;
 			const float sc = (float)(p->nonlinear ? ia : p->h0 / a);
-			rc = phys_fft_mx<1>(ctx, b.F32, F4, nm, 0.f, 0.f, 0.f, mode, 0.f, sc);
-			if (rc == 1) {
-				if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q2, 2 * nm))) return rc;
-				k_phys2_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q2, b.Q1, ctx->d_st, ctx->d_sq, nphi, nspat, nm,
-									   sc);
-				if ((rc = sphswe_fft_r2c32(ctx, b.Q1, F4, nm))) return rc;
-			} else if (rc) {
-				return rc;
-			}
-			if ((rc = sphswe_bt_gprep32(ctx, F4, b.G, nm))) return rc;
+			if ((rc = sphswe_bt_synth_div(ctx, b.SB, b.G, nm, sc))) return rc;
 			if ((rc = sphswe_bt_anal(ctx, b.G, b.TH, nm))) return rc;
 			TH = b.TH;
 			th_stride = 1;
