@@ -565,24 +565,24 @@ void k_bt_synth(const uint4 *__restrict__ TS, const int *__restrict__ soff, cons
 			for (int c = 0; c < CT; c++) acc[r][c] = f4v{0.f, 0.f, 0.f, 0.f};
 		__syncthreads();
 	};
-	uint4 xa[RT][2], ya[RT][2];
+	// table fragments DA - 1 steps ahead in a ring of DA register sets (DA = 1:
+	// loaded at the start of their step, fewest registers)
+	uint4 qa[DA][RT][2];
 	uint4 sv[NSTG];
 	sload(0, sv);
-	if (DA == 2) aload(0, xa);
+#pragma unroll
+	for (int u = 0; u + 1 < DA; u++)
+		if (u < T) aload(u, qa[u]);
 	sstore(0, sv);
 	__syncthreads();
-	// DA = 2: the table fragments of step t + 1 are loaded during step t;
-	// DA = 1: at the start of step t (fewer registers, more workgroups per CU)
 	auto step = [&](int t, uint4 (&a)[RT][2], uint4 (&na)[RT][2]) {
 		const bool o = t >= T0;
 		const int kb = o ? t - T0 : t;
 		const int buf = t & 1;
 		const bool more = t + 1 < T;
 		if (DA == 1) aload(t, a);
-		if (more) {
-			sload(t + 1, sv);
-			if (DA == 2) aload(t + 1, na);
-		}
+		if (more) sload(t + 1, sv);
+		if (DA > 1 && t + DA - 1 < T) aload(t + DA - 1, na);
 		if (kb < (o ? O1.nkw : O0.nkw)) {
 #pragma unroll
 			for (int c = 0; c < CT; c++) {
@@ -595,9 +595,12 @@ void k_bt_synth(const uint4 *__restrict__ TS, const int *__restrict__ soff, cons
 		if (more) sstore(buf ^ 1, sv);
 		__syncthreads();
 	};
-	for (int t = 0; t < T; t += 2) {
-		step(t, xa, ya);
-		if (t + 1 < T) step(t + 1, ya, xa);
+	for (int t0 = 0; t0 < T; t0 += DA) {
+#pragma unroll
+		for (int u = 0; u < DA; u++) {
+			if (t0 + u >= T) break;
+			step(t0 + u, qa[u], qa[(u + DA - 1) % DA]);
+		}
 	}
 }
 
