@@ -53,8 +53,8 @@ enum {
  *           stepper this mode also keeps the Fourier rows, FFTs and physical
  *           fields in FP32 (the state and the RK4 combinations stay FP64).
  * The Fourier stage of the transform entry points is FP64 in every mode.  The
- * real and complex transforms take the modes FP64 .. BF16X3; the pole-layout
- * transforms inside the REXI step are FP64. */
+ * real transforms take every mode, the complex ones FP64 .. BF16X3; the
+ * pole-layout transforms inside the REXI step are FP64. */
 enum {
 	SPHSWE_PREC_FP64 = 0,
 	SPHSWE_PREC_FP32 = 1,
