@@ -177,6 +177,7 @@ int sphswe_bt_sprep(sphswe_ctx *ctx, const double2 *spec, void *SB, int nf);
 // (h, u, v) (the fields, i m and D' of each), 2 per member from the level-1
 // analyses (i m of the first, D' of the second of each member's four)
 int sphswe_bt_sprep_swe9(sphswe_ctx *ctx, const double2 *X, void *SB, int nmem);
+int sphswe_bt_sprep_swe6(sphswe_ctx *ctx, const double2 *X, void *SB, int nmem);
 int sphswe_bt_sprep_swe2(sphswe_ctx *ctx, const double2 *S4, void *SB, int nmem);
 int sphswe_bt_gprep(sphswe_ctx *ctx, const double2 *four, void *G, int nf);
 int sphswe_bt_gprep32(sphswe_ctx *ctx, const float2 *four, void *G, int nf);

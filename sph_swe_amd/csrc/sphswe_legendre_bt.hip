@@ -164,22 +164,26 @@ __device__ __forceinline__ void st2<double2>(double2 *p, float a, float b, float
 //   BT_RCP_COPY  FO = FS = 1, the field itself;
 //   BT_RCP_SWE9  FO = 9, FS = 3: (h, u, v) -> h, u, v, i m h, D'h, i m u, D'u,
 //                i m v, D'v (level 0 of the SWE tendency);
-//   BT_RCP_SWE2  FO = 2, FS = 4: i m of source 0, D' of source 1 (level 2).
-// One thread per (order m, 64-degree block kb, output field, 16-degree group
-// g): 16 consecutive degrees (8 of each parity), written as the hi/lo lanes of
-// both parities' fragments.
-enum { BT_RCP_COPY = 0, BT_RCP_SWE9 = 1, BT_RCP_SWE2 = 2 };
+//   BT_RCP_SWE2  FO = 2, FS = 4: i m of source 0, D' of source 1 (level 2);
+//   BT_RCP_SWE6  FO = 6, FS = 3: (h, u, v) -> h, u, v, D'h, D'u, D'v (level 0
+//                when the physical stage forms i m x from the rows of x: the
+//                synthesis is linear in each order, so the rows of i m x are
+//                i m times the rows of x).
+enum { BT_RCP_COPY = 0, BT_RCP_SWE9 = 1, BT_RCP_SWE2 = 2, BT_RCP_SWE6 = 3 };
 enum { BT_OP_ID = 0, BT_OP_IM = 1, BT_OP_DMU = 2 };
 template <int RCP>
 struct Recipe {
-	static constexpr int FO = RCP == BT_RCP_SWE9 ? 9 : RCP == BT_RCP_SWE2 ? 2 : 1;
-	static constexpr int FS = RCP == BT_RCP_SWE9 ? 3 : RCP == BT_RCP_SWE2 ? 4 : 1;
+	static constexpr int FO = RCP == BT_RCP_SWE9 ? 9 : RCP == BT_RCP_SWE6 ? 6 : RCP == BT_RCP_SWE2 ? 2 : 1;
+	static constexpr int FS = RCP == BT_RCP_SWE9 || RCP == BT_RCP_SWE6 ? 3 : RCP == BT_RCP_SWE2 ? 4 : 1;
 	static constexpr int FU = RCP == BT_RCP_SWE2 ? 2 : FS;   // the sources used: the first FU of FS
 	__device__ static void of(int q, int &s, int &op)
 	{
 		if (RCP == BT_RCP_SWE9) {
 			s = q < 3 ? q : (q - 3) >> 1;
 			op = q < 3 ? BT_OP_ID : ((q - 3) & 1) ? BT_OP_DMU : BT_OP_IM;
+		} else if (RCP == BT_RCP_SWE6) {
+			s = q < 3 ? q : q - 3;
+			op = q < 3 ? BT_OP_ID : BT_OP_DMU;
 		} else if (RCP == BT_RCP_SWE2) {
 			s = q;
 			op = q ? BT_OP_DMU : BT_OP_IM;
@@ -201,12 +205,12 @@ struct Recipe {
 // (three workgroups of the level-0 recipe per CU).
 template <int RCP>
 struct SprepShape {
-	static constexpr int MB = RCP == BT_RCP_SWE9 ? 8 : 16;   // members per workgroup
+	static constexpr int MB = RCP == BT_RCP_SWE9 || RCP == BT_RCP_SWE6 ? 8 : 16;   // members per workgroup
 	static constexpr int NSRC = MB * Recipe<RCP>::FU;        // source fields staged
 	static constexpr int NOUT = MB * Recipe<RCP>::FO;        // output fields written
 	static constexpr int NT = 8 * NOUT;
 	static constexpr int DW = 66;                             // staged degrees per field
-	static constexpr int WPE = RCP == BT_RCP_SWE9 ? 7 : 4;
+	static constexpr int WPE = RCP == BT_RCP_SWE9 || RCP == BT_RCP_SWE6 ? 7 : 4;
 	static_assert(NOUT % 8 == 0 && NT <= 1024, "fragment-aligned output fields");
 };
 
@@ -822,6 +826,10 @@ int sphswe_bt_sprep(sphswe_ctx *ctx, const double2 *spec, void *SB, int nf)
 int sphswe_bt_sprep_swe9(sphswe_ctx *ctx, const double2 *X, void *SB, int nmem)
 {
 	return bt_sprep<BT_RCP_SWE9>(ctx, X, SB, nmem);
+}
+int sphswe_bt_sprep_swe6(sphswe_ctx *ctx, const double2 *X, void *SB, int nmem)
+{
+	return bt_sprep<BT_RCP_SWE6>(ctx, X, SB, nmem);
 }
 int sphswe_bt_sprep_swe2(sphswe_ctx *ctx, const double2 *S4, void *SB, int nmem)
 {

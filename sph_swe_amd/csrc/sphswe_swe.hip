@@ -196,7 +196,7 @@ __global__ void k_phys1_mx(const float *__restrict__ P9, float *__restrict__ P4,
 
 // The mixed form's physical stage without physical fields in HBM: one
 // workgroup per (member, ring pair j, j + 1) runs the inverse FFTs of the
-// 9 input fields' Fourier rows, which leave each thread the longitudes
+// 9 input fields (from 6 row sets), which leave each thread the longitudes
 // k + NT w + t NSL of its last-pass butterflies in registers, forms the
 // pointwise terms there (those of k_phys1_mx), and runs the forward FFTs of
 // the 4 outputs from LDS to the output Fourier rows.  (The divergence level
@@ -224,6 +224,9 @@ struct FusedPlan {
 	static constexpr bool ok = NP >= 2 && NSL % 64 == 0 && NSL % NT == 0 && NV <= 8;
 };
 
+// The input holds 6 row sets per member (h, u, v, D'h, D'u, D'v); the rows of
+// the i m fields are formed from those of h, u, v as they are loaded (the
+// synthesis is linear in each order: rows of i m x = i m times rows of x).
 template <int MODE, int N>
 __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	const float2 *__restrict__ Fin, float2 *__restrict__ Fout, const float2 *__restrict__ twg,
@@ -232,7 +235,10 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 {
 	constexpr int mode = MODE;
 	typedef FusedPlan<N> FP;
-	constexpr int NIN = 9, NOUT = 4;
+	constexpr int NIN = 9, NOUT = 4, NSRC = 6;
+	// the input row set of field q, and whether it is multiplied by i m
+	auto src_of = [](int q) constexpr { return q < 3 ? q : (q & 1) ? (q - 3) >> 1 : 3 + ((q - 4) >> 1); };
+	auto im_of = [](int q) constexpr { return q >= 3 && (q & 1); };
 	constexpr int FI = 1, FO = 1;   // fields per round, in and out
 	typedef WPad<N, FI> WI;
 	typedef WPad<N, FO> WO;
@@ -320,7 +326,7 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	auto fetch = [&](int q0) __attribute__((always_inline)) {
 #pragma unroll
 		for (int f = 0; f < FI; f++) {
-			const float2 *src = Fin + (long)(mem * NIN + q0 + f) * frow;
+			const float2 *src = Fin + (long)(mem * NSRC + src_of(q0 + f)) * frow;
 #pragma unroll
 			for (int u = 0; u < NMU; u++) {
 				const int m = tid + u * NT;
@@ -331,17 +337,18 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	};
 	fetch(0);
 	// unrolled: every register array is indexed by compile-time numbers only
-#ifndef SPHSWE_FUSED_UNR
-#define SPHSWE_FUSED_UNR 16
-#endif
-#pragma unroll SPHSWE_FUSED_UNR
+#pragma unroll
 	for (int q0 = 0; q0 < NIN; q0 += FI) {
 #pragma unroll
 		for (int f = 0; f < FI; f++) {
 #pragma unroll
 			for (int u = 0; u < NMU; u++) {
 				const int m = tid + u * NT;
-				const float4 v = ab[f][u];
+				float4 v = ab[f][u];
+				if (im_of(q0 + f)) {   // i m A, i m B
+					const float fm = (float)m;
+					v = make_float4(-fm * v.y, fm * v.x, -fm * v.w, fm * v.z);
+				}
 				if (m == 0) {
 					sm[WI::at(f, 0)] = make_float2(v.x, v.z);
 				} else if (m <= L) {
@@ -653,8 +660,14 @@ int swe_step(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int nm, co
 	return SPHSWE_OK;
 }
 
-// the fused physical stage for the context's longitude count, or 1 when there
-// is no fused kernel for it (the caller runs the separate kernels)
+// whether the context's grid has a fused physical stage
+bool phys_fft_fused(const sphswe_ctx *ctx)
+{
+	return ctx->nlat % 2 == 0 && 2 * ctx->L + 1 <= ctx->plan.n && (ctx->plan.n == 384 || ctx->plan.n == 1536);
+}
+
+// the fused physical stage for the context's longitude count (the input holds
+// h, u, v, D'h, D'u, D'v per member), or 1 when there is no fused kernel for it
 int phys_fft_mx(sphswe_ctx *ctx, const float2 *Fin, float2 *Fout, int nm, float ga, float two_omega, float ia,
 		int mode, float h0)
 {
@@ -674,7 +687,7 @@ int phys_fft_mx(sphswe_ctx *ctx, const float2 *Fin, float2 *Fout, int nm, float 
 		}
 		return 1;
 	};
-	if (ctx->nlat % 2 != 0 || 2 * ctx->L + 1 > ctx->plan.n) return 1;
+	if (!phys_fft_fused(ctx)) return 1;
 	switch (mode) {
 	case 0: return by_n(std::integral_constant<int, 0>{});
 	case 1: return by_n(std::integral_constant<int, 1>{});
@@ -707,12 +720,15 @@ int swe_step_mixed(sphswe_ctx *ctx, const sphswe_swe_params *p, double2 *X, int 
 	int rc;
 	const double2 *Y = X;
 	double2 *YN = b.Ya;
+	// with the fused physical stage only h, u, v and their D' are synthesised:
+	// that kernel forms the rows of i m x from those of x
+	const bool six = phys_fft_fused(ctx);
 	for (int s = 0; s < 4; s++) {
-		if ((rc = sphswe_bt_sprep_swe9(ctx, Y, b.SB, nm))) return rc;
-		if ((rc = sphswe_bt_synth32(ctx, b.SB, b.F32, 9 * nm))) return rc;
+		if ((rc = six ? sphswe_bt_sprep_swe6(ctx, Y, b.SB, nm) : sphswe_bt_sprep_swe9(ctx, Y, b.SB, nm))) return rc;
+		if ((rc = sphswe_bt_synth32(ctx, b.SB, b.F32, (six ? 6 : 9) * nm))) return rc;
 		const float h0f = (float)(mode == 3 ? p->h0 / a : p->h0);
-		float2 *F4 = b.F32 + (long)9 * nfour * nm;   // level-1 rows after the 9 synthesised fields
-		rc = phys_fft_mx(ctx, b.F32, F4, nm, (float)ga, (float)two_omega, (float)ia, mode, h0f);
+		float2 *F4 = b.F32 + (long)9 * nfour * nm;   // output rows after the (at most 9) synthesised fields
+		rc = six ? phys_fft_mx(ctx, b.F32, F4, nm, (float)ga, (float)two_omega, (float)ia, mode, h0f) : 1;
 		if (rc == 1) {
 			if ((rc = sphswe_fft_c2r32(ctx, b.F32, b.Q9, 9 * nm))) return rc;
 			k_phys1_mx<<<g1(nspat * nm), 256, 0, st>>>(b.Q9, b.Q4, ctx->d_ct, ctx->d_st, ctx->d_sq, nphi, nspat,
