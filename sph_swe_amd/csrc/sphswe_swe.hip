@@ -239,6 +239,16 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	// the input row set of field q, and whether it is multiplied by i m
 	auto src_of = [](int q) constexpr { return q < 3 ? q : (q & 1) ? (q - 3) >> 1 : 3 + ((q - 4) >> 1); };
 	auto im_of = [](int q) constexpr { return q >= 3 && (q & 1); };
+	// LINF (modes 1, 2): the terms of O2 and O3 that are linear with per-ring
+	// coefficients (pressure gradient and Coriolis) are added to the output rows
+	// in Fourier space, where they are 2 pi times the same combination of the
+	// input rows (no products, so the transform pair is exact there); i m h and
+	// D'h are then not transformed: 7 of the 9 inverse FFTs.  At N = 384 the
+	// prefetched rows would cost a wave per SIMD (64-thread workgroups), so the
+	// short rings keep all 9.
+	constexpr bool LINF = (MODE == 1 || MODE == 2) && N >= 1024;
+	constexpr int NQ = LINF ? 7 : NIN;
+	auto field_at = [](int qi) constexpr { return !LINF ? qi : qi < 3 ? qi : qi + 2; };
 	constexpr int FI = 1, FO = 1;   // fields per round, in and out
 	typedef WPad<N, FI> WI;
 	typedef WPad<N, FO> WO;
@@ -291,8 +301,8 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 		case 2:
 			Vs[e][r] = x;
 			O[1][e][r] = mode == 3 ? 0.f : (mode == 0 ? x * sj : (x * Hs[e][r]) * sj);
-			O[2][e][r] = mode == 2 ? 0.f : two_omega * (x * mu);
-			O[3][e][r] = mode == 2 ? 0.f : -(two_omega * (Us[e][r] * mu));
+			O[2][e][r] = (mode == 2 || LINF) ? 0.f : two_omega * (x * mu);
+			O[3][e][r] = (mode == 2 || LINF) ? 0.f : -(two_omega * (Us[e][r] * mu));
 			break;
 		case 3:
 			if (mode != 2) O[2][e][r] = -ga * (mode == 3 ? x : x * ic) + O[2][e][r];
@@ -326,26 +336,26 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	auto fetch = [&](int q0) __attribute__((always_inline)) {
 #pragma unroll
 		for (int f = 0; f < FI; f++) {
-			const float2 *src = Fin + (long)(mem * NSRC + src_of(q0 + f)) * frow;
+			const float2 *src = Fin + (long)(mem * NSRC + src_of(field_at(q0 + f))) * frow;
 #pragma unroll
 			for (int u = 0; u < NMU; u++) {
 				const int m = tid + u * NT;
-				ab[f][u] = (q0 + f < NIN && m <= L) ? *(const float4 *)(src + (size_t)m * nlat + j)
-								    : make_float4(0.f, 0.f, 0.f, 0.f);
+				ab[f][u] = (q0 + f < NQ && m <= L) ? *(const float4 *)(src + (size_t)m * nlat + j)
+								   : make_float4(0.f, 0.f, 0.f, 0.f);
 			}
 		}
 	};
 	fetch(0);
 	// unrolled: every register array is indexed by compile-time numbers only
 #pragma unroll
-	for (int q0 = 0; q0 < NIN; q0 += FI) {
+	for (int q0 = 0; q0 < NQ; q0 += FI) {
 #pragma unroll
 		for (int f = 0; f < FI; f++) {
 #pragma unroll
 			for (int u = 0; u < NMU; u++) {
 				const int m = tid + u * NT;
 				float4 v = ab[f][u];
-				if (im_of(q0 + f)) {   // i m A, i m B
+				if (im_of(field_at(q0 + f))) {   // i m A, i m B
 					const float fm = (float)m;
 					v = make_float4(-fm * v.y, fm * v.x, -fm * v.w, fm * v.z);
 				}
@@ -358,13 +368,13 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 			}
 			for (int k = L + 1 + tid; k < N - L; k += NT) sm[WI::at(f, k)] = make_float2(0.f, 0.f);
 		}
-		if (q0 + FI < NIN) fetch(q0 + FI);
+		if (q0 + FI < NQ) fetch(q0 + FI);
 		__syncthreads();
 		passes_pp_range<N, FI, NT, +1, 0, NP - 1, float2, FP::RULE>(sm, sm2, tws);
 		// last pass into registers, folded into the outputs
 #pragma unroll
 		for (int f = 0; f < FI; f++) {
-			if (q0 + f >= NIN) break;
+			if (q0 + f >= NQ) break;
 #pragma unroll
 			for (int w = 0; w < NW; w++) {
 				const int k = tid + w * NT;
@@ -380,8 +390,8 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 				Dft<RL>::run(v, +1);
 #pragma unroll
 				for (int t = 0; t < RL; t++) {
-					fold(q0 + f, w * RL + t, 0, v[t].x);
-					fold(q0 + f, w * RL + t, 1, v[t].y);
+					fold(field_at(q0 + f), w * RL + t, 0, v[t].x);
+					fold(field_at(q0 + f), w * RL + t, 1, v[t].y);
 				}
 			}
 		}
@@ -392,6 +402,13 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 	// pass to the Fourier rows (butterflies k and NSL - k give both Z[m] and
 	// Z[N - m])
 	const float hn = (float)(M_PI / N);   // half the analysis normalisation 2 pi / N
+	// LINF: 2 pi times the per-ring coefficients of the linear terms
+	float lga[2], lom[2];
+#pragma unroll
+	for (int r = 0; r < 2; r++) {
+		lga[r] = (float)(-2.0 * M_PI * ga / sq[j + r]);
+		lom[r] = (float)(2.0 * M_PI * two_omega * ct[j + r]);
+	}
 	constexpr int NH = NSL / 2;
 #pragma unroll
 	for (int o0 = 0; o0 < NOUT; o0 += FO) {
@@ -403,6 +420,24 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 			for (int t = 0; t < RL; t++)
 				sm[WO::at(o, tid + w * NT + t * NSL)] =
 					make_float2(O[o0 + o][w * RL + t][0], O[o0 + o][w * RL + t][1]);
+	// LINF: the input rows of the linear terms of this output, loaded before its
+	// passes (thread tid takes butterflies tid and NSL - tid, orders k + t NSL)
+	float4 lp[2][RL], ls[2][RL];
+	if (LINF && MODE == 1 && (o0 == 2 || o0 == 3)) {
+		static_assert(FO * NH <= NT, "at most one last-pass butterfly pair per thread");
+		const float2 *rp = Fin + (long)(mem * NSRC + (o0 == 2 ? 0 : 3)) * frow + j;
+		const float2 *rs = Fin + (long)(mem * NSRC + (o0 == 2 ? 2 : 1)) * frow + j;
+#pragma unroll
+		for (int h = 0; h < 2; h++)
+#pragma unroll
+			for (int t = 0; t < RL; t++) {
+				const int k = h ? (tid == 0 ? NH : NSL - tid) : tid;
+				const int m = k + t * NSL;
+				const bool ok = tid < FO * NH && m <= L;
+				lp[h][t] = ok ? *(const float4 *)(rp + (size_t)m * nlat) : make_float4(0.f, 0.f, 0.f, 0.f);
+				ls[h][t] = ok ? *(const float4 *)(rs + (size_t)m * nlat) : make_float4(0.f, 0.f, 0.f, 0.f);
+			}
+	}
 	__syncthreads();
 	passes_pp_range<N, FO, NT, -1, 0, NP - 1, float2, FP::RULE>(sm, sm2, tws);
 	for (int it = tid; it < FO * NH; it += NT) {
@@ -422,11 +457,24 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 		}
 		Dft<RL>::run(z1, -1);
 		Dft<RL>::run(z2, -1);
-		auto emit = [&](int m, float2 Z, float2 Q) __attribute__((always_inline)) {
+		auto emit = [&](int m, float2 Z, float2 Q, const float4 &P, const float4 &S) __attribute__((always_inline)) {
 			if (m > L) return;
 			const float2 Zc = cconj(Q);
 			float2 a = make_float2(hn * (Z.x + Zc.x), hn * (Z.y + Zc.y));
 			float2 c = make_float2(hn * (Z.y - Zc.y), -hn * (Z.x - Zc.x));
+			if (LINF && MODE == 1 && (o0 == 2 || o0 == 3)) {
+				// O2 += 2 pi (-ga / c i m H + 2 Omega mu V), O3 += 2 pi (-ga / c D'H - 2 Omega mu U)
+				// from the rows P, S of ring j (x, y) and j + 1 (z, w)
+				const float fm = o0 == 2 ? (float)m : 1.f;
+				const float sg = o0 == 2 ? 1.f : -1.f;
+				// o0 == 2: i m (x + i y) = (-m y, m x); o0 == 3: D'H itself
+				const float p0x = o0 == 2 ? -fm * P.y : P.x, p0y = o0 == 2 ? fm * P.x : P.y;
+				const float p1x = o0 == 2 ? -fm * P.w : P.z, p1y = o0 == 2 ? fm * P.z : P.w;
+				a.x += lga[0] * p0x + sg * lom[0] * S.x;
+				a.y += lga[0] * p0y + sg * lom[0] * S.y;
+				c.x += lga[1] * p1x + sg * lom[1] * S.z;
+				c.y += lga[1] * p1y + sg * lom[1] * S.w;
+			}
 			if (m == 0) a.y = c.y = 0.f;
 			*(float4 *)(dst + (size_t)m * nlat + j) = make_float4(a.x, a.y, c.x, c.y);
 		};
@@ -434,8 +482,8 @@ __global__ __launch_bounds__(FusedPlan<N>::NT) void k_phys_fft_mx(
 		for (int t = 0; t < RL; t++) {
 			const float2 q1 = pp == 0 ? z1[(RL - t) % RL] : z2[RL - 1 - t];
 			const float2 q2 = pp == 0 ? z2[RL - 1 - t] : z1[RL - 1 - t];
-			emit(k1 + t * NSL, z1[t], q1);
-			emit(k2 + t * NSL, z2[t], q2);
+			emit(k1 + t * NSL, z1[t], q1, lp[0][t], ls[0][t]);
+			emit(k2 + t * NSL, z2[t], q2, lp[1][t], ls[1][t]);
 		}
 	}
 	__syncthreads();
