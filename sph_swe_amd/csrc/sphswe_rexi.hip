@@ -109,7 +109,9 @@ __device__ inline void wave_sum_n(double2 (&v)[R])
 
 // rows of a band sweep whose loads are issued together: the sweeps are
 // sequential in the row, so without this every row waits a full memory latency
-constexpr int RD = 4;
+// (8 rows: the phi sweeps at T256 / 200 poles 570 -> 502 us, against 4; more
+// rows add nothing)
+constexpr int RD = 8;
 
 // m of workgroup row y: 0, -1, 1, -2, 2, ... (longest systems first)
 __device__ inline int m_of(int y) { return (y & 1) ? -((y + 1) >> 1) : (y >> 1); }
