@@ -107,11 +107,13 @@ __device__ inline void wave_sum_n(double2 (&v)[R])
 		}
 }
 
-// rows of a band sweep whose loads are issued together: the sweeps are
-// sequential in the row, so without this every row waits a full memory latency
-// (8 rows: the phi sweeps at T256 / 200 poles 570 -> 502 us, against 4; more
-// rows add nothing)
-constexpr int RD = 8;
+// RD: rows of a band sweep whose loads are issued together. The sweeps are
+// sequential in the row, so without this every row waits a full memory
+// latency. 8 rows: the phi sweeps at T256 / 200 poles 570 -> 502 us against 4,
+// more rows add nothing. The refining forms hold 256 VGPRs with 8 rows (one
+// wave per SIMD, two with 4): they take 8 rows up to T511 and 4 above, where
+// the occupancy is worth more (T1023: 36 -> 31 ms per phi + vel pair)
+constexpr int sweep_rows(bool refine, int L) { return refine && L >= 512 ? 4 : 8; }
 
 // m of workgroup row y: 0, -1, 1, -2, 2, ... (longest systems first)
 __device__ inline int m_of(int y) { return (y & 1) ? -((y + 1) >> 1) : (y >> 1); }
@@ -347,7 +349,7 @@ struct PhiArgs {
 // beta*phi.  REF: the solve with the stored (FP32) factors is refined once:
 // x0 (kept in the first half of X), r = b - A x0 in FP64, the correction
 // solve, x = x0 + dx.
-template <int KL, typename FT, bool REF>
+template <int KL, typename FT, bool REF, int RD>
 __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 {
 	constexpr int KV = 2 * KL, NE = rexi_nec(KL);
@@ -558,7 +560,7 @@ struct VelArgs {
 	const double2 *Kv;
 };
 
-template <int KL, typename FT, bool REF>
+template <int KL, typename FT, bool REF, int RD>
 __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 {
 	constexpr int KV = 2 * KL, NE = rexi_nec(KL);
@@ -1509,8 +1511,9 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 		pa.two_omega = 2.0 * r->prm.coriolis_omega; pa.avg_geo = r->prm.h0 * g;
 		pa.coriolis = r->coriolis; pa.rows = r->rows;
 		pa.R = r->d_G; pa.K = r->d_kphi;
-		if (r->refine) k_rexi_phi<4, FT, true><<<grid, 64, 0, st>>>(pa);
-		else k_rexi_phi<4, FT, false><<<grid, 64, 0, st>>>(pa);
+		if (!r->refine) k_rexi_phi<4, FT, false, 8><<<grid, 64, 0, st>>>(pa);
+		else if (sweep_rows(true, L) == 8) k_rexi_phi<4, FT, true, 8><<<grid, 64, 0, st>>>(pa);
+		else k_rexi_phi<4, FT, true, 4><<<grid, 64, 0, st>>>(pa);
 	};
 	if (r->coriolis) {
 		// right-hand sides into Y (solved in place), or into the first half of G
@@ -1550,8 +1553,9 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 		va.two_omega = 2.0 * r->prm.coriolis_omega; va.ir = 1.0 / a;
 		va.coriolis = r->coriolis; va.rows = r->rows;
 		va.R = r->d_X; va.X = r->d_G; va.Kv = r->d_kvel;
-		if (r->refine) k_rexi_vel<2, FT, true><<<grid, 64, 0, st>>>(va);
-		else k_rexi_vel<2, FT, false><<<grid, 64, 0, st>>>(va);
+		if (!r->refine) k_rexi_vel<2, FT, false, 8><<<grid, 64, 0, st>>>(va);
+		else if (sweep_rows(true, L) == 8) k_rexi_vel<2, FT, true, 8><<<grid, 64, 0, st>>>(va);
+		else k_rexi_vel<2, FT, true, 4><<<grid, 64, 0, st>>>(va);
 	};
 	if (r->fac32) vel((const float2 *)r->d_fvel);
 	else vel((const double2 *)r->d_fvel);
