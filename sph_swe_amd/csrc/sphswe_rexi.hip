@@ -601,13 +601,12 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 		}
 		return;
 	}
-	// a, b at row j from G values already loaded
-	auto ab_pre = [&](int j, double2 gl, double2 gt, double2 &av, double2 &bv) __attribute__((always_inline)) {
+	// a, b at row j from G and basis values already loaded
+	auto ab_pre = [&](int j, double2 gl, double2 gt, double2 u0, double2 v0, double2 &av, double2 &bv)
+		__attribute__((always_inline)) {
 		if (j < 0 || j >= N) { av = bv = make_double2(0, 0); return; }
-		const int n = am + j;
-		const long k = (long)n * n + n + m;
-		av = ca(B[B_U0 * (long)a.ncplx + k], cs(gl, a.ir));
-		bv = ca(B[B_V0 * (long)a.ncplx + k], cs(gt, a.ir));
+		av = ca(u0, cs(gl, a.ir));
+		bv = ca(v0, cs(gt, a.ir));
 	};
 	// rhs_a = alpha a - W mu(b), rhs_b = W mu(a) + alpha b, mu = R(n-1) x_{n-1} + S(n+1) x_{n+1}
 	double2 aw[3], bw[3];   // (j-1, j, j+1)
@@ -652,7 +651,7 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 	for (int i = 0; i <= KL; i++)
 		if (i < N) keep(i, wa[i], wb[i]);
 	for (int jb = 0; jb < N; jb += RD) {
-		double2 lf[RD][KL / 2], gl[RD], gt[RD];
+		double2 lf[RD][KL / 2], gl[RD], gt[RD], u0[RD], v0[RD];
 		int kp[RD];
 #pragma unroll
 		for (int d = 0; d < RD; d++) {
@@ -664,6 +663,9 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 			const int jr = min(jb + d + KL + 3, N - 1);
 			gl[d] = a.G[(row0 + jr) * 2 * P + pc];
 			gt[d] = a.G[(row0 + jr) * 2 * P + P + pc];
+			const long nr = am + jr, kr = nr * nr + nr + m;
+			u0[d] = B[B_U0 * (long)a.ncplx + kr];
+			v0[d] = B[B_V0 * (long)a.ncplx + kr];
 		}
 #pragma unroll
 		for (int d = 0; d < RD; d++) {
@@ -683,7 +685,7 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 				const int jn = j + KL + 1;
 				if (jn < N) {
 					double2 an, bn;
-					ab_pre(jn + 2, gl[d], gt[d], an, bn);
+					ab_pre(jn + 2, gl[d], gt[d], u0[d], v0[d], an, bn);
 					rhs_next(jn, wa[KL], wb[KL], an, bn);
 					keep(jn, wa[KL], wb[KL]);
 				} else {
