@@ -428,9 +428,11 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 			}
 		}
 	};
-	// backward substitution with U (KV super-diagonals) from Y; emit(j, x)
-	// per row, j = N-1 .. 0, returns beta x for the pole sums when SUM
-	auto bwd = [&](auto emit, auto sum_tag) __attribute__((always_inline)) {
+	// backward substitution with U (KV super-diagonals) from Y; emit(j, x, d)
+	// per row, j = N-1 .. 0, returns beta x for the pole sums when SUM;
+	// pre(d, j) issues the emitter's loads for row j (j >= 0 or not) with the
+	// batch's, d = the row's place in the batch
+	auto bwd = [&](auto emit, auto sum_tag, auto pre) __attribute__((always_inline)) {
 		constexpr bool SUM = decltype(sum_tag)::value;
 		double2 v[KV + 1];
 #pragma unroll
@@ -445,6 +447,7 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 				for (int e = 0; e <= KL; e++) uf[d][e] = ldf(f + (size_t)e * P);
 				const int jy = jt - d - KV - 1;
 				yn[d] = (jy >= 0) ? a.Y[(row0 + jy) * P + pc] : make_double2(0, 0);
+				pre(d, jt - d);
 			}
 #pragma unroll
 			for (int d = 0; d < RD; d++) {
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 					const double2 x = cm(v[0], uf[d][0]);
 #pragma unroll
 					for (int e = 1; e <= KL; e++) v[2 * e] = cb(v[2 * e], cm(uf[d][e], x));
-					bx[d] = emit(j, x);
+					bx[d] = emit(j, x, d);
 #pragma unroll
 					for (int i = 0; i < KV; i++) v[i] = v[i + 1];
 					v[KV] = yn[d];
@@ -485,9 +488,10 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 		}
 		return pv ? cm(be, x) : make_double2(0, 0);
 	};
+	auto nopre = [](int, int) {};
 	if constexpr (!REF) {
 		fwd(a.Y, (size_t)P);
-		bwd(out, std::true_type{});
+		bwd([&](int j, double2 x, int) __attribute__((always_inline)) { return out(j, x); }, std::true_type{}, nopre);
 	} else {
 		const size_t rs = 2 * (size_t)P;
 		const double2 al2 = cm(al, al), al4 = cm(al2, al2), ial = cinv(al);
@@ -496,9 +500,10 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 		double2 xw[KV + 1];
 #pragma unroll
 		for (int k = 0; k <= KV; k++) xw[k] = make_double2(0, 0);
-		auto resid = [&](int q) __attribute__((always_inline)) {
+		// rq[d]: R of row q = j + KL for the row j at place d of a batch
+		double2 rq[RD];
+		auto resid = [&](int q, double2 r) __attribute__((always_inline)) {
 			if (q < 0 || q >= N) return;
-			double2 r = a.R[(row0 + q) * rs + pc];
 			const double2 *Kq = a.K + (row0 + q) * 20;
 #pragma unroll
 			for (int e = 0; e <= KL; e++) {
@@ -511,24 +516,32 @@ __global__ __launch_bounds__(64) void k_rexi_phi(PhiArgs<FT> a)
 			a.R[(row0 + q) * rs + pc] = r;
 		};
 		fwd(a.R, rs);
-		bwd([&](int j, double2 x) __attribute__((always_inline)) -> double2 {
+		bwd([&](int j, double2 x, int d) __attribute__((always_inline)) -> double2 {
 			if (pv) a.X[(row0 + j) * 2 * P + p] = x;
 #pragma unroll
 			for (int k = KV; k > 0; k--) xw[k] = xw[k - 1];
 			xw[0] = x;
-			resid(j + KL);
+			resid(j + KL, rq[d]);
 			return make_double2(0, 0);
-		}, std::false_type{});
+		}, std::false_type{}, [&](int d, int j) __attribute__((always_inline)) {
+			const int q = j + KL;
+			rq[d] = (j >= 0 && q < N) ? a.R[(row0 + q) * rs + pc] : make_double2(0, 0);
+		});
 		for (int j = -1; j >= -KL; j--) {
 #pragma unroll
 			for (int k = KV; k > 0; k--) xw[k] = xw[k - 1];
 			xw[0] = make_double2(0, 0);
-			resid(j + KL);
+			const int q = j + KL;
+			resid(q, q < N ? a.R[(row0 + q) * rs + pc] : make_double2(0, 0));
 		}
 		fwd(a.R, rs);
-		bwd([&](int j, double2 dx) __attribute__((always_inline)) -> double2 {
-			return out(j, ca(a.X[(row0 + j) * 2 * P + pc], dx));
-		}, std::true_type{});
+		// x0 of the batch's rows
+		double2 xq[RD];
+		bwd([&](int j, double2 dx, int d) __attribute__((always_inline)) -> double2 {
+			return out(j, ca(xq[d], dx));
+		}, std::true_type{}, [&](int d, int j) __attribute__((always_inline)) {
+			xq[d] = j >= 0 ? a.X[(row0 + j) * 2 * P + pc] : make_double2(0, 0);
+		});
 	}
 	// row 0: phi(-1) = 0, phi(1) = phi[1]
 	{
@@ -694,9 +707,10 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 			}
 		}
 	}
-	// backward substitution with U from Ya, Yb; emit(j, xa, xb, sa, sb) per row,
-	// j = N-1 .. 0, sets beta x for the pole sums when SUM
-	auto bwd = [&](auto emit, auto sum_tag) __attribute__((always_inline)) {
+	// backward substitution with U from Ya, Yb; emit(j, xa, xb, sa, sb, d) per
+	// row, j = N-1 .. 0, sets beta x for the pole sums when SUM; pre(d, j) as
+	// for phi
+	auto bwd = [&](auto emit, auto sum_tag, auto pre) __attribute__((always_inline)) {
 		constexpr bool SUM = decltype(sum_tag)::value;
 		double2 va[KV + 1], vb[KV + 1];
 #pragma unroll
@@ -716,6 +730,7 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 				const int jy = jt - d - KV - 1;
 				ya[d] = (jy >= 0) ? Ya[(row0 + jy) * P + pc] : make_double2(0, 0);
 				yb[d] = (jy >= 0) ? Yb[(row0 + jy) * P + pc] : make_double2(0, 0);
+				pre(d, jt - d);
 			}
 #pragma unroll
 			for (int d = 0; d < RD; d++) {
@@ -728,7 +743,7 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 						va[2 * e] = cb(va[2 * e], cm(uf[d][e], xa));
 						vb[2 * e] = cb(vb[2 * e], cm(uf[d][e], xb));
 					}
-					emit(j, xa, xb, sx[2 * d], sx[2 * d + 1]);
+					emit(j, xa, xb, sx[2 * d], sx[2 * d + 1], d);
 #pragma unroll
 					for (int i = 0; i < KV; i++) { va[i] = va[i + 1]; vb[i] = vb[i + 1]; }
 					va[KV] = ya[d];
@@ -748,14 +763,14 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 			}
 		}
 	};
-	auto sums = [&](int, double2 xa, double2 xb, double2 &sa, double2 &sb) __attribute__((always_inline)) {
+	auto sums = [&](int, double2 xa, double2 xb, double2 &sa, double2 &sb, int) __attribute__((always_inline)) {
 		if (pv) {
 			sa = cm(be, xa);
 			sb = cm(be, xb);
 		}
 	};
 	if constexpr (!REF) {
-		bwd(sums, std::true_type{});
+		bwd(sums, std::true_type{}, [](int, int) {});
 	} else {
 		const double2 al2 = cm(al, al);
 		// x0 into X; r = b - A x0 in place in R, row q once x0(q-2 .. q+2) is known;
@@ -763,10 +778,10 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 		double2 xwa[KV + 1], xwb[KV + 1];
 #pragma unroll
 		for (int k = 0; k <= KV; k++) xwa[k] = xwb[k] = make_double2(0, 0);
-		auto resid = [&](int q) __attribute__((always_inline)) {
+		// R and the K pieces of row q = j + KL for the row j at place d of a batch
+		double2 rqa[RD], rqb[RD], kq[RD][3];
+		auto resid = [&](int q, double2 ra, double2 rb, const double2 (&Kq)[3]) __attribute__((always_inline)) {
 			if (q < 0 || q >= N) return;
-			double2 ra = a.R[(row0 + q) * rs + pc], rb = a.R[(row0 + q) * rs + P + pc];
-			const double2 *Kq = a.Kv + (row0 + q) * 3;
 #pragma unroll
 			for (int e = 0; e <= 2; e++) {
 				const int c = q + 2 * e - KL;
@@ -778,7 +793,15 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 			a.R[(row0 + q) * rs + pc] = ra;
 			a.R[(row0 + q) * rs + P + pc] = rb;
 		};
-		bwd([&](int j, double2 xa, double2 xb, double2 &, double2 &) __attribute__((always_inline)) {
+		// loads of row q's R and K pieces (zero outside the band)
+		auto rk_load = [&](int q, double2 &ra, double2 &rb, double2 (&Kq)[3]) __attribute__((always_inline)) {
+			const bool in = q >= 0 && q < N;
+			ra = in ? a.R[(row0 + q) * rs + pc] : make_double2(0, 0);
+			rb = in ? a.R[(row0 + q) * rs + P + pc] : make_double2(0, 0);
+#pragma unroll
+			for (int e = 0; e <= 2; e++) Kq[e] = in ? a.Kv[(row0 + q) * 3 + e] : make_double2(0, 0);
+		};
+		bwd([&](int j, double2 xa, double2 xb, double2 &, double2 &, int d) __attribute__((always_inline)) {
 			if (pv) {
 				a.X[(row0 + j) * rs + p] = xa;
 				a.X[(row0 + j) * rs + P + p] = xb;
@@ -787,13 +810,17 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 			for (int k = KV; k > 0; k--) { xwa[k] = xwa[k - 1]; xwb[k] = xwb[k - 1]; }
 			xwa[0] = xa;
 			xwb[0] = xb;
-			resid(j + KL);
-		}, std::false_type{});
+			resid(j + KL, rqa[d], rqb[d], kq[d]);
+		}, std::false_type{}, [&](int d, int j) __attribute__((always_inline)) {
+			rk_load(j >= 0 ? j + KL : -1, rqa[d], rqb[d], kq[d]);
+		});
 		for (int j = -1; j >= -KL; j--) {
 #pragma unroll
 			for (int k = KV; k > 0; k--) { xwa[k] = xwa[k - 1]; xwb[k] = xwb[k - 1]; }
 			xwa[0] = xwb[0] = make_double2(0, 0);
-			resid(j + KL);
+			double2 ra, rb, Kq[3];
+			rk_load(j + KL, ra, rb, Kq);
+			resid(j + KL, ra, rb, Kq);
 		}
 		// correction: forward substitution of the residuals, then x = x0 + dx
 		{
@@ -838,10 +865,14 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 				}
 			}
 		}
-		bwd([&](int j, double2 dxa, double2 dxb, double2 &sa, double2 &sb) __attribute__((always_inline)) {
-			const double2 xa = ca(a.X[(row0 + j) * rs + pc], dxa), xb = ca(a.X[(row0 + j) * rs + P + pc], dxb);
-			sums(j, xa, xb, sa, sb);
-		}, std::true_type{});
+		// x0 of the batch's rows
+		double2 xqa[RD], xqb[RD];
+		bwd([&](int j, double2 dxa, double2 dxb, double2 &sa, double2 &sb, int d) __attribute__((always_inline)) {
+			sums(j, ca(xqa[d], dxa), ca(xqb[d], dxb), sa, sb, d);
+		}, std::true_type{}, [&](int d, int j) __attribute__((always_inline)) {
+			xqa[d] = j >= 0 ? a.X[(row0 + j) * rs + pc] : make_double2(0, 0);
+			xqb[d] = j >= 0 ? a.X[(row0 + j) * rs + P + pc] : make_double2(0, 0);
+		});
 	}
 }
 
