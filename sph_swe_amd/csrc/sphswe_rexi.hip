@@ -60,7 +60,7 @@ struct sphswe_rexi {
 	long *d_goff = nullptr;            // offset of the (|m|, parity) matrix, index 2|m| + parity
 	int4 *d_gtiles = nullptr;          // (m, parity, first row, -) of each output block of the apply kernel
 	int ngtiles = 0;
-	double2 *d_part = nullptr, *d_basis = nullptr, *d_acc = nullptr;
+	double2 *d_part = nullptr, *d_basis = nullptr;
 	double *d_phys = nullptr;
 	double2 *d_cphys = nullptr, *d_cspec = nullptr;
 	double2 *d_in = nullptr;           // phi = g h, u, v of a step (3 real spectra)
@@ -876,40 +876,75 @@ __global__ __launch_bounds__(64) void k_rexi_vel(VelArgs<FT> a)
 	}
 }
 
-// sum the pole-group partials (fixed order) into 3 complex spectra, standard layout
-__global__ void k_rexi_sum(const double2 *__restrict__ part, double2 *__restrict__ out, const int *__restrict__ lofc,
-			   const int *__restrict__ moff, int L, int ngroups, size_t rows, int ncplx)
+
+// the pole sums c (complex layout, as k_rexi_sum) reduced to the real spectrum
+// of Re(S(c)): the reference synthesises c, keeps the real part and analyses
+// it (AppSWE.hpp:519-521); on band-limited fields that is the spectral map
+// a(l, 0) = Re c(l, 0), a(l, m) = (c(l, m) + (-1)^m conj(c(l, -m))) / 2
+__global__ void k_rexi_sum_real(const double2 *__restrict__ part, double2 *__restrict__ out,
+				const int *__restrict__ mofr, const int *__restrict__ moff, int L, int ngroups,
+				size_t rows, long nlm)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= ncplx) return;
-	const int l = lofc[i], m = i - l * l - l;
-	const size_t row = moff[m + L] + (l - abs(m));
+	const long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (q >= nlm) return;
+	const int m = mofr[q], l = (int)(q - lm_index(L, 0, m));
+	const size_t rp = moff[m + L] + (l - m), rn = moff[-m + L] + (l - m);
+	const double sg = (m & 1) ? -1.0 : 1.0;
 #pragma unroll
 	for (int f = 0; f < 3; f++) {
-		double2 s = make_double2(0, 0);
+		double2 sp = make_double2(0, 0), sn = make_double2(0, 0);
 		for (int g = 0; g < ngroups; g++) {
-			const double2 v = part[((size_t)g * 3 + f) * rows + row];
-			s.x += v.x;
-			s.y += v.y;
+			const double2 v = part[((size_t)g * 3 + f) * rows + rp];
+			sp.x += v.x;
+			sp.y += v.y;
+			if (m > 0) {
+				const double2 w = part[((size_t)g * 3 + f) * rows + rn];
+				sn.x += w.x;
+				sn.y += w.y;
+			}
 		}
-		out[(size_t)f * ncplx + i] = s;
+		out[f * nlm + q] = m == 0 ? make_double2(sp.x, 0.0)
+					  : make_double2(0.5 * (sp.x + sg * sn.x), 0.5 * (sp.y - sg * sn.y));
 	}
 }
-
-__global__ void k_real_to_cplx(const double *__restrict__ a, double2 *__restrict__ o, long n)
+__global__ void k_fill_mu(double *__restrict__ o, const double *__restrict__ ct, int nlat, long n)
 {
 	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-	if (i < n) o[i] = make_double2(a[i], 0.0);
+	if (i < n) o[i] = ct[i % nlat];
 }
-__global__ void k_cplx_real_part(const double2 *__restrict__ a, double *__restrict__ o, long n)
+// The fields the REXI basis transforms are real: their complex spectra are
+// c(l, m) = a(l, m) and c(l, -m) = (-1)^m conj(a(l, m)), with a the real
+// spectrum (m >= 0), for the transform pair of SHTns' complex and real
+// interfaces (SPHDataComplex.hpp:179-216, SPHData.hpp:190-227), so they run as
+// real transforms between these two layout changes.
+// real layout (nf fields, stride nlm) -> complex layout (stride nc)
+__global__ void k_spec_r2cx(const double2 *__restrict__ a, double2 *__restrict__ c, const int *__restrict__ lofc,
+			    int L, long nlm, long nc, int nf)
 {
-	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-	if (i < n) o[i] = a[i].x;
+	const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= nc * nf) return;
+	const long f = i / nc, k = i - f * nc;
+	const int l = lofc[k], m = (int)(k - (long)l * l - l), am = abs(m);
+	const double2 v = a[f * nlm + lm_index(L, l, am)];
+	const double sg = (am & 1) ? -1.0 : 1.0;
+	c[i] = m >= 0 ? v : make_double2(sg * v.x, -sg * v.y);
 }
-__global__ void k_fill_mu(double2 *__restrict__ o, const double *__restrict__ ct, int nlat, long n)
+// complex layout of a real field's spectrum -> real layout (the m >= 0 half)
+__global__ void k_spec_c2rx(const double2 *__restrict__ c, double2 *__restrict__ a, const int *__restrict__ mofr,
+			    int L, long nlm, long nc, int nf)
+{
+	const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+	if (i >= nlm * nf) return;
+	const long f = i / nlm, q = i - f * nlm;
+	const int m = mofr[q], l = (int)(q - lm_index(L, 0, m));
+	a[i] = c[f * nc + (long)l * l + l + m];
+}
+// o[f] = g * x[f] (real pointwise, g one field, x nf fields)
+__global__ void k_mul_bcast(const double *__restrict__ g, const double *__restrict__ x, double *__restrict__ o,
+			    long nspat, int nf)
 {
 	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-	if (i < n) o[i] = make_double2(ct[i % nlat], 0.0);
+	if (i < nspat * nf) o[i] = g[i % nspat] * x[i];
 }
 // sum / difference of spectral pairs with a scale: o = s*(x + sgn*y)
 __global__ void k_comb(const double2 *__restrict__ x, const double2 *__restrict__ y, double2 *__restrict__ o, double sgn,
@@ -917,15 +952,6 @@ __global__ void k_comb(const double2 *__restrict__ x, const double2 *__restrict_
 {
 	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
 	if (i < n) o[i] = make_double2(s * (x[i].x + sgn * y[i].x), s * (x[i].y + sgn * y[i].y));
-}
-// o = g * x (complex pointwise, g one field, x nf fields)
-__global__ void k_cmul_bcast(const double2 *__restrict__ g, const double2 *__restrict__ x, double2 *__restrict__ o,
-			     long nspat, int nf)
-{
-	long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-	if (i >= nspat * nf) return;
-	const double2 a = g[i % nspat], b = x[i];
-	o[i] = make_double2(b.x * a.x - b.y * a.y, b.x * a.y + b.y * a.x);
 }
 
 inline int g256(long n) { return (int)((n + 255) / 256); }
@@ -1104,7 +1130,7 @@ extern "C" int sphswe_rexi_destroy(sphswe_rexi *r)
 			if (v[i] == r) { v.erase(v.begin() + i); break; }
 	}
 	void *ptrs[] = {r->d_alpha, r->d_beta, r->d_moff, r->d_fphi, r->d_fvel, r->d_pphi, r->d_pvel, r->d_oscale,
-			r->d_X, r->d_G, r->d_Y, r->d_part, r->d_basis, r->d_acc, r->d_phys, r->d_cphys, r->d_cspec,
+			r->d_X, r->d_G, r->d_Y, r->d_part, r->d_basis, r->d_phys, r->d_cphys, r->d_cspec,
 			r->d_gop, r->d_gop32, r->d_goff, r->d_gtiles, r->d_kphi, r->d_kvel,
 			r->d_in};
 	for (void *p : ptrs)
@@ -1304,8 +1330,8 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	    (rc = dev_alloc(ctx, &r->d_Y, r->rows * 2 * P)) ||
 	    (rc = dev_alloc(ctx, &r->d_part, (size_t)r->ngroups * 3 * r->rows)) ||
 	    (rc = dev_alloc(ctx, &r->d_basis, (size_t)NBASIS * ctx->ncplx)) ||
-	    (rc = dev_alloc(ctx, &r->d_acc, (size_t)3 * ctx->ncplx)) || (rc = dev_alloc(ctx, &r->d_phys, 3 * nsp)) ||
-	    (rc = dev_alloc(ctx, &r->d_cphys, 8 * nsp)) || (rc = dev_alloc(ctx, &r->d_cspec, (size_t)16 * ctx->ncplx)) ||
+	    (rc = dev_alloc(ctx, &r->d_phys, 3 * nsp)) ||
+	    (rc = dev_alloc(ctx, &r->d_cphys, 8 * nsp)) || (rc = dev_alloc(ctx, &r->d_cspec, (size_t)16 * ctx->ncplx + 8 * ctx->nlm)) ||
 	    (rc = dev_alloc(ctx, &r->d_in, (size_t)3 * ctx->nlm))) {
 		sphswe_rexi_destroy(r);
 		return rc;
@@ -1441,23 +1467,42 @@ extern "C" int sphswe_rexi_poles(const sphswe_rexi *r, int *npoles, double *alph
 }
 
 // pole-independent fields of one step from the complex physical phi0, u0, v0
+// r->d_phys: the synthesis of (phi0, u0, v0)
 static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 {
-	const long nsp = ctx->nspat, nc = ctx->ncplx;
-	double2 *cp = r->d_cphys;      // complex physical scratch, 8 fields
-	double2 *cs_ = r->d_cspec;     // complex spectral scratch, 16 fields
+	const long nsp = ctx->nspat, nc = ctx->ncplx, nlm = ctx->nlm;
+	const int L = ctx->L;
+	double2 *cs_ = r->d_cspec;     // complex spectral scratch, 16 fields, then real spectral scratch, 8 fields
+	double2 *rs_ = cs_ + 16 * nc;
+	double *rp_ = (double *)r->d_cphys;   // real physical scratch, 16 fields
 	double2 *B = r->d_basis;
 	const double ir = 1.0 / r->prm.earth_radius, W = 2.0 * r->prm.coriolis_omega;
 	hipStream_t st = ctx->stream;
 	int rc;
 	auto SPEC = [&](int k) { return (double *)(cs_ + (size_t)k * nc); };
-	auto PHYS = [&](int k) { return (double *)(cp + (size_t)k * nsp); };
+	auto RS = [&](int k) { return (double *)(rs_ + (size_t)k * nlm); };
+	auto RP = [&](int k) { return rp_ + (size_t)k * nsp; };
 	auto BF = [&](int k) { return (double *)(B + (size_t)k * nc); };
-	// cp[0..2] = phi0, u0, v0 (complex physical); cp[3] = v0 * st; cp[4] = mu
-	if ((rc = sphswe_spat_op_cplx(ctx, SPHSWE_SPAT_MUL_ST, PHYS(2), nullptr, PHYS(3), 1.0, 1))) return rc;
-	k_fill_mu<<<g256(nsp), 256, 0, st>>>(cp + 4 * nsp, ctx->d_ct, ctx->nlat, nsp);
-	// level A: spec[0..4] = A(phi0), A(u0), A(v0), A(v0 st), A(mu)
-	if ((rc = sphswe_sht_analysis_cplx(ctx, PHYS(0), SPEC(0), 5, SPHSWE_PREC_FP64))) return rc;
+	// real spectra (nf fields from RS(k0)) into the complex layout at c, and back
+	auto to_cplx = [&](int k0, double *c, int nf) -> int {
+		k_spec_r2cx<<<g256(nc * nf), 256, 0, st>>>(rs_ + (size_t)k0 * nlm, (double2 *)c, ctx->d_lofcidx, L, nlm,
+							  nc, nf);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		return SPHSWE_OK;
+	};
+	auto to_real = [&](const double *c, int k0, int nf) -> int {
+		k_spec_c2rx<<<g256(nlm * nf), 256, 0, st>>>((const double2 *)c, rs_ + (size_t)k0 * nlm, ctx->d_mofidx, L,
+							   nlm, nc, nf);
+		SPHSWE_LAUNCH_CHECK(ctx);
+		return SPHSWE_OK;
+	};
+	// level A: spec[0..4] = A(phi0), A(u0), A(v0), A(v0 st), A(mu) by the real
+	// analysis
+	SPHSWE_HIP(ctx, hipMemcpyAsync(RP(0), r->d_phys, sizeof(double) * 3 * nsp, hipMemcpyDeviceToDevice, st));
+	if ((rc = sphswe_spat_op(ctx, SPHSWE_SPAT_MUL_ST, r->d_phys + 2 * nsp, nullptr, RP(3), 1.0, 1))) return rc;
+	k_fill_mu<<<g256(nsp), 256, 0, st>>>(RP(4), ctx->d_ct, ctx->nlat, nsp);
+	if ((rc = sphswe_sht_analysis(ctx, RP(0), RS(0), 5, SPHSWE_PREC_FP64))) return rc;
+	if ((rc = to_cplx(0, SPEC(0), 5))) return rc;
 	// phi0c, u0c, v0c
 	SPHSWE_HIP(ctx, hipMemcpyAsync(BF(B_PHI), SPEC(0), sizeof(double2) * nc, hipMemcpyDeviceToDevice, st));
 	SPHSWE_HIP(ctx, hipMemcpyAsync(BF(B_U0), SPEC(1), sizeof(double2) * nc, hipMemcpyDeviceToDevice, st));
@@ -1476,9 +1521,11 @@ static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_ONE_MINUS_MU2_DMU, SPEC(2), SPEC(6), 1, 1, 0, 0, 0, 0))) return rc;
 		k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 5 * nc, cs_ + 6 * nc, cs_ + 10 * nc, 1.0, 1.0, nc);
 		SPHSWE_LAUNCH_CHECK(ctx);
-		if ((rc = sphswe_sht_synthesis_cplx(ctx, SPEC(10), PHYS(0), 1, SPHSWE_PREC_FP64))) return rc;
-		if ((rc = sphswe_spat_op_cplx(ctx, SPHSWE_SPAT_DIV_ST2, PHYS(0), nullptr, PHYS(0), 1.0, 1))) return rc;
-		if ((rc = sphswe_sht_analysis_cplx(ctx, PHYS(0), SPEC(11), 1, SPHSWE_PREC_FP64))) return rc;
+		if ((rc = to_real(SPEC(10), 0, 1))) return rc;
+		if ((rc = sphswe_sht_synthesis(ctx, RS(0), RP(0), 1, SPHSWE_PREC_FP64))) return rc;
+		if ((rc = sphswe_spat_op(ctx, SPHSWE_SPAT_DIV_ST2, RP(0), nullptr, RP(0), 1.0, 1))) return rc;
+		if ((rc = sphswe_sht_analysis(ctx, RP(0), RS(1), 1, SPHSWE_PREC_FP64))) return rc;
+		if ((rc = to_cplx(1, SPEC(11), 1))) return rc;
 		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_IDENTITY, SPEC(11), BF(B_DIV), 1, ir, 0, 0, 0, 0))) return rc;
 		k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 7 * nc, cs_ + 8 * nc, cs_ + 15 * nc, -1.0, ir, nc);   // eta0 in spec[15]
 		SPHSWE_LAUNCH_CHECK(ctx);
@@ -1486,11 +1533,13 @@ static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 	} else {
 	// synthesis, metric divisions: grad_lon / grad_lat divide by sqrt(1-mu^2); the
 	// complex div_lat further divides by cos(lat) (SPHOperatorsComplex.hpp:326-333)
-	if ((rc = sphswe_sht_synthesis_cplx(ctx, SPEC(5), PHYS(0), 5, SPHSWE_PREC_FP64))) return rc;
-	if ((rc = sphswe_spat_op_cplx(ctx, SPHSWE_SPAT_DIV_COSPHI, PHYS(0), nullptr, PHYS(0), 1.0, 5))) return rc;
-	if ((rc = sphswe_spat_op_cplx(ctx, SPHSWE_SPAT_DIV_COSLAT, PHYS(1), nullptr, PHYS(1), 1.0, 1))) return rc;
+	if ((rc = to_real(SPEC(5), 0, 5))) return rc;
+	if ((rc = sphswe_sht_synthesis(ctx, RS(0), RP(0), 5, SPHSWE_PREC_FP64))) return rc;
+	if ((rc = sphswe_spat_op(ctx, SPHSWE_SPAT_DIV_COSPHI, RP(0), nullptr, RP(0), 1.0, 5))) return rc;
+	if ((rc = sphswe_spat_op(ctx, SPHSWE_SPAT_DIV_COSLAT, RP(1), nullptr, RP(1), 1.0, 1))) return rc;
 	// level C: spec[10..14] = analyses
-	if ((rc = sphswe_sht_analysis_cplx(ctx, PHYS(0), SPEC(10), 5, SPHSWE_PREC_FP64))) return rc;
+	if ((rc = sphswe_sht_analysis(ctx, RP(0), RS(0), 5, SPHSWE_PREC_FP64))) return rc;
+	if ((rc = to_cplx(0, SPEC(10), 5))) return rc;
 	// div0 = ir (A1 + A2), eta0 = ir (A3 - A4), gms = (W ir) A5
 	k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 10 * nc, cs_ + 11 * nc, B + B_DIV * nc, 1.0, ir, nc);
 	k_comb<<<g256(nc), 256, 0, st>>>(cs_ + 12 * nc, cs_ + 13 * nc, cs_ + 15 * nc, -1.0, ir, nc);   // eta0 in spec[15]
@@ -1508,10 +1557,12 @@ static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 		SPHSWE_HIP(ctx, hipMemcpyAsync(SPEC(6), SPEC(1), sizeof(double2) * nc, hipMemcpyDeviceToDevice, st));
 		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU2, SPEC(1), SPEC(7), 1, 1, 0, 0, 0, 0))) return rc;
 		if ((rc = sphswe_spec_op_cplx(ctx, SPHSWE_OP_MU, SPEC(2), SPEC(8), 1, 1, 0, 0, 0, 0))) return rc;
-		if ((rc = sphswe_sht_synthesis_cplx(ctx, SPEC(5), PHYS(0), 4, SPHSWE_PREC_FP64))) return rc;
-		k_cmul_bcast<<<g256(3 * nsp), 256, 0, st>>>(cp, cp + nsp, cp + 4 * nsp, nsp, 3);
+		if ((rc = to_real(SPEC(5), 0, 4))) return rc;
+		if ((rc = sphswe_sht_synthesis(ctx, RS(0), RP(0), 4, SPHSWE_PREC_FP64))) return rc;
+		k_mul_bcast<<<g256(3 * nsp), 256, 0, st>>>(RP(0), RP(1), RP(4), nsp, 3);
 		SPHSWE_LAUNCH_CHECK(ctx);
-		if ((rc = sphswe_sht_analysis_cplx(ctx, PHYS(4), BF(B_W1), 3, SPHSWE_PREC_FP64))) return rc;
+		if ((rc = sphswe_sht_analysis(ctx, RP(4), RS(4), 3, SPHSWE_PREC_FP64))) return rc;
+		if ((rc = to_cplx(4, BF(B_W1), 3))) return rc;
 	} else {
 		SPHSWE_HIP(ctx, hipMemsetAsync(BF(B_W1), 0, sizeof(double2) * 3 * nc, st));
 	}
@@ -1529,8 +1580,6 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 	int rc;
 	// real synthesis of (phi, u, v); complex physical copies
 	if ((rc = sphswe_sht_synthesis(ctx, (const double *)in, r->d_phys, 3, SPHSWE_PREC_FP64))) return rc;
-	k_real_to_cplx<<<g256(3 * nsp), 256, 0, st>>>(r->d_phys, r->d_cphys, 3 * nsp);
-	SPHSWE_LAUNCH_CHECK(ctx);
 	if ((rc = rexi_basis(ctx, r))) return rc;
 	dim3 grid(r->ngroups, 2 * L + 1);
 	// phi solves over (m, pole)
@@ -1594,13 +1643,10 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 	else vel((const double2 *)r->d_fvel);
 	SPHSWE_LAUNCH_CHECK(ctx);
 	// pole sum, real part (spat_RealToSPHData + accumulation, AppSWE.hpp:519-521)
-	k_rexi_sum<<<g256(nc), 256, 0, st>>>(r->d_part, r->d_acc, ctx->d_lofcidx, r->d_moff, L, r->ngroups, r->rows,
-					      (int)nc);
+	k_rexi_sum_real<<<g256(ctx->nlm), 256, 0, st>>>(r->d_part, out, ctx->d_mofidx, r->d_moff, L, r->ngroups,
+							 r->rows, ctx->nlm);
 	SPHSWE_LAUNCH_CHECK(ctx);
-	if ((rc = sphswe_sht_synthesis_cplx(ctx, (double *)r->d_acc, (double *)r->d_cphys, 3, SPHSWE_PREC_FP64))) return rc;
-	k_cplx_real_part<<<g256(3 * nsp), 256, 0, st>>>(r->d_cphys, r->d_phys, 3 * nsp);
-	SPHSWE_LAUNCH_CHECK(ctx);
-	return sphswe_sht_analysis(ctx, r->d_phys, (double *)out, 3, SPHSWE_PREC_FP64);
+	return SPHSWE_OK;
 }
 
 extern "C" int sphswe_rexi_apply(sphswe_ctx *ctx, sphswe_rexi *r, const double *d_in, double *d_out)
