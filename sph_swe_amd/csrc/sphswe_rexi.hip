@@ -111,9 +111,10 @@ __device__ inline void wave_sum_n(double2 (&v)[R])
 // sequential in the row, so without this every row waits a full memory
 // latency. 8 rows: the phi sweeps at T256 / 200 poles 570 -> 502 us against 4,
 // more rows add nothing. The refining forms hold 256 VGPRs with 8 rows (one
-// wave per SIMD, two with 4): they take 8 rows up to T511 and 4 above, where
-// the occupancy is worth more (T1023: 36 -> 31 ms per phi + vel pair)
-constexpr int sweep_rows(bool refine, int L) { return refine && L >= 512 ? 4 : 8; }
+// wave per SIMD, two with 4): they take 8 rows up to T319 and 4 above, where
+// the occupancy is worth more (refined step at T383 150 -> 154 steps/s, at
+// T256 342 against 333 with 8 rows; T1023: 36 -> 31 ms per phi + vel pair)
+constexpr int sweep_rows(bool refine, int L) { return refine && L >= 320 ? 4 : 8; }
 
 // m of workgroup row y: 0, -1, 1, -2, 2, ... (longest systems first)
 __device__ inline int m_of(int y) { return (y & 1) ? -((y + 1) >> 1) : (y >> 1); }
