@@ -61,7 +61,6 @@ struct sphswe_rexi {
 	int4 *d_gtiles = nullptr;          // (m, parity, first row, -) of each output block of the apply kernel
 	int ngtiles = 0;
 	double2 *d_part = nullptr, *d_basis = nullptr;
-	double *d_phys = nullptr;
 	double2 *d_cphys = nullptr, *d_cspec = nullptr;
 	double2 *d_in = nullptr;           // phi = g h, u, v of a step (3 real spectra)
 	int ngroups = 0;
@@ -1131,7 +1130,7 @@ extern "C" int sphswe_rexi_destroy(sphswe_rexi *r)
 			if (v[i] == r) { v.erase(v.begin() + i); break; }
 	}
 	void *ptrs[] = {r->d_alpha, r->d_beta, r->d_moff, r->d_fphi, r->d_fvel, r->d_pphi, r->d_pvel, r->d_oscale,
-			r->d_X, r->d_G, r->d_Y, r->d_part, r->d_basis, r->d_phys, r->d_cphys, r->d_cspec,
+			r->d_X, r->d_G, r->d_Y, r->d_part, r->d_basis, r->d_cphys, r->d_cspec,
 			r->d_gop, r->d_gop32, r->d_goff, r->d_gtiles, r->d_kphi, r->d_kvel,
 			r->d_in};
 	for (void *p : ptrs)
@@ -1331,7 +1330,6 @@ static int rexi_build(sphswe_ctx *ctx, const sphswe_rexi_params *prm, const std:
 	    (rc = dev_alloc(ctx, &r->d_Y, r->rows * 2 * P)) ||
 	    (rc = dev_alloc(ctx, &r->d_part, (size_t)r->ngroups * 3 * r->rows)) ||
 	    (rc = dev_alloc(ctx, &r->d_basis, (size_t)NBASIS * ctx->ncplx)) ||
-	    (rc = dev_alloc(ctx, &r->d_phys, 3 * nsp)) ||
 	    (rc = dev_alloc(ctx, &r->d_cphys, 8 * nsp)) || (rc = dev_alloc(ctx, &r->d_cspec, (size_t)16 * ctx->ncplx + 8 * ctx->nlm)) ||
 	    (rc = dev_alloc(ctx, &r->d_in, (size_t)3 * ctx->nlm))) {
 		sphswe_rexi_destroy(r);
@@ -1468,7 +1466,7 @@ extern "C" int sphswe_rexi_poles(const sphswe_rexi *r, int *npoles, double *alph
 }
 
 // pole-independent fields of one step from the complex physical phi0, u0, v0
-// r->d_phys: the synthesis of (phi0, u0, v0)
+// the first 3 real fields of r->d_cphys: the synthesis of (phi0, u0, v0)
 static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 {
 	const long nsp = ctx->nspat, nc = ctx->ncplx, nlm = ctx->nlm;
@@ -1499,8 +1497,7 @@ static int rexi_basis(sphswe_ctx *ctx, sphswe_rexi *r)
 	};
 	// level A: spec[0..4] = A(phi0), A(u0), A(v0), A(v0 st), A(mu) by the real
 	// analysis
-	SPHSWE_HIP(ctx, hipMemcpyAsync(RP(0), r->d_phys, sizeof(double) * 3 * nsp, hipMemcpyDeviceToDevice, st));
-	if ((rc = sphswe_spat_op(ctx, SPHSWE_SPAT_MUL_ST, r->d_phys + 2 * nsp, nullptr, RP(3), 1.0, 1))) return rc;
+	if ((rc = sphswe_spat_op(ctx, SPHSWE_SPAT_MUL_ST, RP(2), nullptr, RP(3), 1.0, 1))) return rc;
 	k_fill_mu<<<g256(nsp), 256, 0, st>>>(RP(4), ctx->d_ct, ctx->nlat, nsp);
 	if ((rc = sphswe_sht_analysis(ctx, RP(0), RS(0), 5, SPHSWE_PREC_FP64))) return rc;
 	if ((rc = to_cplx(0, SPEC(0), 5))) return rc;
@@ -1580,7 +1577,7 @@ static int rexi_pole_sum(sphswe_ctx *ctx, sphswe_rexi *r, const double2 *in, dou
 	hipStream_t st = ctx->stream;
 	int rc;
 	// real synthesis of (phi, u, v); complex physical copies
-	if ((rc = sphswe_sht_synthesis(ctx, (const double *)in, r->d_phys, 3, SPHSWE_PREC_FP64))) return rc;
+	if ((rc = sphswe_sht_synthesis(ctx, (const double *)in, (double *)r->d_cphys, 3, SPHSWE_PREC_FP64))) return rc;
 	if ((rc = rexi_basis(ctx, r))) return rc;
 	dim3 grid(r->ngroups, 2 * L + 1);
 	// phi solves over (m, pole)
